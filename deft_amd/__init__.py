@@ -1,0 +1,381 @@
+"""deft_amd: MI355X-native White-Bear fundamental-measure functional and CG minimizer.
+
+Python binding over the C ABI in ``include/deft_amd.h`` (libdeft_amd.so, built
+in-tree by ``python -m deft_amd.build``).  The classes mirror deft's new-API
+generated functional and minimizer:
+
+* :class:`WhiteBearFluidVeff` - the generated ``WhiteBearFluidVeff``
+  (deft src/haskell/create_generators.py:21-22, src/haskell/NewCode.hs:120-242):
+  ``energy()``, ``grad()``, ``energy_grad_and_precond()``, ``Veff``, ``Vext``,
+  ``mu``, ``R``, ``kT``;
+* :class:`Minimize` - deft's ``Minimize`` (src/new/Minimize.h:13-199):
+  ``improve_energy()``, ``set_precision()``, ``set_relative_precision()``,
+  ``set_maxiter()``, ``set_miniter()``, ``precondition()``, ``energy()``,
+  ``get_iteration_count()``;
+* :func:`fft` / :func:`ifft` - the FFT seam (src/Grid.cpp:470-478,
+  src/ReciprocalGrid.cpp:52-65);
+* :func:`find_chemical_potential`, :func:`pressure` - the bulk path
+  (src/equation-of-state.cpp:164-192).
+
+There is no CPU fallback: without the built library or a HIP device every
+entry point raises.
+"""
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdeft_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "deft_amd.h")
+
+FFT_FP32 = 0
+FFT_FP64 = 1
+
+STOP_REASONS = {
+    0: "none", 1: "maxiter", 2: "nan-energy", 3: "infinite-energy", 4: "nan-slope",
+    5: "zero-slope", 6: "negative-curvature", 7: "energy-rose", 8: "no-change", 9: "converged",
+}
+
+
+class DeftError(RuntimeError):
+    pass
+
+
+class EnergyTerms(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("kTphi1", "kTphi2", "kTphi3", "ideal", "int_V_n", "minus_mu_N", "total")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class MinimizeStats(ctypes.Structure):
+    _fields_ = [("iter", ctypes.c_int), ("num_energy_calcs", ctypes.c_int), ("num_grad_calcs", ctypes.c_int),
+                ("stop_reason", ctypes.c_int), ("energy", ctypes.c_double), ("delta_energy", ctypes.c_double),
+                ("error_estimate", ctypes.c_double), ("step", ctypes.c_double)]
+
+
+_lib = None
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "deft_last_error": ([], ctypes.c_char_p),
+    "deft_version": ([], _I),
+    "deft_supported_length": ([_I], _I),
+    "deft_device_count": ([_IP], _I),
+    "deft_plan_create": ([_I, _I, _I, _DP, _D, _D, _D, _I, ctypes.POINTER(_P)], _I),
+    "deft_plan_destroy": ([_P], _I),
+    "deft_plan_set_fft_precision": ([_P, _I], _I),
+    "deft_plan_set_mu": ([_P, _D], _I),
+    "deft_plan_set_vext": ([_P, _DP], _I),
+    "deft_plan_set_veff": ([_P, _DP], _I),
+    "deft_plan_get_veff": ([_P, _DP], _I),
+    "deft_plan_get_density": ([_P, _DP], _I),
+    "deft_plan_veff_device_ptr": ([_P, ctypes.POINTER(_DP)], _I),
+    "deft_plan_sizes": ([_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)], _I),
+    "deft_functional_energy": ([_P, _DP, _DP, ctypes.POINTER(EnergyTerms)], _I),
+    "deft_functional_grad": ([_P, _DP, _DP, _DP, _DP], _I),
+    "deft_functional_grad_device": ([_P, _DP, _DP, _DP, _DP], _I),
+    "deft_weighted_densities": ([_P, _DP], _I),
+    "deft_fft": ([_P, _DP, _DP], _I),
+    "deft_ifft": ([_P, _DP, _DP], _I),
+    "deft_bulk_chemical_potential": ([_D, _D, _D, _DP], _I),
+    "deft_bulk_pressure": ([_D, _D, _D, _DP], _I),
+    "deft_bulk_free_energy_density": ([_D, _D, _D, _DP], _I),
+    "deft_minimize_create": ([_P, ctypes.POINTER(_P)], _I),
+    "deft_minimize_destroy": ([_P], _I),
+    "deft_minimize_set_precision": ([_P, _D], _I),
+    "deft_minimize_set_relative_precision": ([_P, _D], _I),
+    "deft_minimize_set_maxiter": ([_P, _I], _I),
+    "deft_minimize_set_miniter": ([_P, _I], _I),
+    "deft_minimize_precondition": ([_P, _I], _I),
+    "deft_minimize_improve_energy": ([_P, _I, _IP], _I),
+    "deft_minimize_run": ([_P, _I, _IP], _I),
+    "deft_minimize_energy": ([_P, _DP], _I),
+    "deft_minimize_stats_get": ([_P, ctypes.POINTER(MinimizeStats)], _I),
+    "deft_minimize_invalidate": ([_P], _I),
+}
+
+
+def lib():
+    """Loads libdeft_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DeftError(f"{LIB_PATH} is missing: run `python -m deft_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().deft_last_error()
+        raise DeftError(f"deft_amd error {rc}: {msg.decode() if msg else ''}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    _check(lib().deft_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def supported_length(n):
+    return bool(lib().deft_supported_length(int(n)))
+
+
+# ---------------------------------------------------------------------------
+# bulk path
+# ---------------------------------------------------------------------------
+
+def find_chemical_potential(n, R=1.0, kT=1.0):
+    """mu of ideal gas + White-Bear at bulk density n (src/equation-of-state.cpp:189-192)."""
+    out = ctypes.c_double()
+    _check(lib().deft_bulk_chemical_potential(n, R, kT, ctypes.byref(out)))
+    return out.value
+
+
+def pressure(n, R=1.0, kT=1.0):
+    """Bulk pressure (src/equation-of-state.cpp:164-170)."""
+    out = ctypes.c_double()
+    _check(lib().deft_bulk_pressure(n, R, kT, ctypes.byref(out)))
+    return out.value
+
+
+def bulk_free_energy_density(n, R=1.0, kT=1.0):
+    out = ctypes.c_double()
+    _check(lib().deft_bulk_free_energy_density(n, R, kT, ctypes.byref(out)))
+    return out.value
+
+
+# ---------------------------------------------------------------------------
+# functional
+# ---------------------------------------------------------------------------
+
+class WhiteBearFluidVeff:
+    """White-Bear fluid of Veff on one grid, resident on the GPU.
+
+    ``shape`` = (Nx, Ny, Nz) (even, supported FFT lengths); ``cell`` = 3x3 lattice
+    vectors as rows (or three edge lengths of an orthorhombic cell).  ``Vext`` is
+    FP64 on the host, stored FP32 on the device; +inf marks hard walls.
+    """
+
+    def __init__(self, shape, cell, R=1.0, kT=1.0, mu=0.0, fft_precision=FFT_FP32):
+        self.shape = tuple(int(s) for s in shape)
+        cell = np.asarray(cell, dtype=np.float64)
+        if cell.shape == (3,):
+            cell = np.diag(cell)
+        self.cell = np.ascontiguousarray(cell.reshape(3, 3))
+        self._R, self._kT, self._mu = float(R), float(kT), float(mu)
+        self._prec = int(fft_precision)
+        h = ctypes.c_void_p()
+        _check(lib().deft_plan_create(self.shape[0], self.shape[1], self.shape[2], _dptr(self.cell),
+                                      self._R, self._kT, self._mu, self._prec, ctypes.byref(h)))
+        self._h = h
+        self.N = int(np.prod(self.shape))
+        self.kshape = (self.shape[0], self.shape[1], self.shape[2] // 2 + 1)
+        self.volume = abs(float(np.linalg.det(self.cell)))
+        self.dV = self.volume / self.N
+        self._vext = np.zeros(self.shape)
+        self.last_terms = {}
+
+    @classmethod
+    def from_spacing(cls, ax, ay, az, dx, **kw):
+        """The generated (ax, ay, az, dx) constructor: N = ceil(a/dx/2)*2 (src/haskell/NewCode.hs:160-168)."""
+        shape = [int(math.ceil(a / dx / 2)) * 2 for a in (ax, ay, az)]
+        return cls(shape, np.diag([ax, ay, az]), **kw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().deft_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # parameters -----------------------------------------------------------
+    @property
+    def R(self):
+        return self._R
+
+    @property
+    def kT(self):
+        return self._kT
+
+    @property
+    def mu(self):
+        return self._mu
+
+    @mu.setter
+    def mu(self, v):
+        _check(lib().deft_plan_set_mu(self._h, float(v)))
+        self._mu = float(v)
+
+    @property
+    def fft_precision(self):
+        return self._prec
+
+    @fft_precision.setter
+    def fft_precision(self, v):
+        _check(lib().deft_plan_set_fft_precision(self._h, int(v)))
+        self._prec = int(v)
+
+    @property
+    def Vext(self):
+        return self._vext.copy()
+
+    @Vext.setter
+    def Vext(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), self.shape))
+        _check(lib().deft_plan_set_vext(self._h, _dptr(a)))
+        self._vext = a.copy()
+
+    @property
+    def Veff(self):
+        out = np.empty(self.shape)
+        _check(lib().deft_plan_get_veff(self._h, _dptr(out)))
+        return out
+
+    @Veff.setter
+    def Veff(self, v):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), self.shape))
+        _check(lib().deft_plan_set_veff(self._h, _dptr(a)))
+
+    def density(self):
+        out = np.empty(self.shape)
+        _check(lib().deft_plan_get_density(self._h, _dptr(out)))
+        return out
+
+    # functional seam ------------------------------------------------------
+    def energy(self, Veff=None):
+        if Veff is not None:
+            self.Veff = Veff
+        e = ctypes.c_double()
+        t = EnergyTerms()
+        _check(lib().deft_functional_energy(self._h, None, ctypes.byref(e), ctypes.byref(t)))
+        self.last_terms = t.as_dict()
+        return e.value
+
+    def grad(self, Veff=None):
+        return self.energy_grad_and_precond(Veff)[1]
+
+    def energy_grad_and_precond(self, Veff=None):
+        if Veff is not None:
+            self.Veff = Veff
+        g = np.zeros(self.shape)
+        pg = np.zeros(self.shape)
+        e = ctypes.c_double()
+        _check(lib().deft_functional_grad(self._h, None, _dptr(g), _dptr(pg), ctypes.byref(e)))
+        return e.value, g, pg
+
+    def weighted_densities(self):
+        out = np.empty((5,) + self.shape)
+        _check(lib().deft_weighted_densities(self._h, _dptr(out)))
+        return out
+
+    def fft(self, f):
+        """deft's fft(gd, f) = r2c * dV (src/Grid.cpp:470-478)."""
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(self.shape)
+        out = np.empty(self.kshape, dtype=np.complex128)
+        _check(lib().deft_fft(self._h, _dptr(a), out.ctypes.data_as(_DP)))
+        return out
+
+    def ifft(self, F):
+        """deft's ifft(gd, F) = c2r / V (src/ReciprocalGrid.cpp:52-65)."""
+        a = np.ascontiguousarray(F, dtype=np.complex128).reshape(self.kshape)
+        out = np.empty(self.shape)
+        _check(lib().deft_ifft(self._h, a.ctypes.data_as(_DP), _dptr(out)))
+        return out
+
+
+class Minimize:
+    """deft's new-API minimizer (src/new/Minimize.h:13-199) on a :class:`WhiteBearFluidVeff`."""
+
+    def __init__(self, f):
+        self.f = f
+        h = ctypes.c_void_p()
+        _check(lib().deft_minimize_create(f.handle, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().deft_minimize_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, p):
+        _check(lib().deft_minimize_set_precision(self._h, float(p)))
+
+    def set_relative_precision(self, p):
+        _check(lib().deft_minimize_set_relative_precision(self._h, float(p)))
+
+    def set_maxiter(self, m):
+        _check(lib().deft_minimize_set_maxiter(self._h, int(m)))
+
+    def set_miniter(self, m):
+        _check(lib().deft_minimize_set_miniter(self._h, int(m)))
+
+    def precondition(self, u):
+        _check(lib().deft_minimize_precondition(self._h, 1 if u else 0))
+
+    def improve_energy(self, verbose=False):
+        more = ctypes.c_int()
+        _check(lib().deft_minimize_improve_energy(self._h, 1 if verbose else 0, ctypes.byref(more)))
+        return bool(more.value)
+
+    def run(self, max_calls=1000000):
+        it = ctypes.c_int()
+        _check(lib().deft_minimize_run(self._h, int(max_calls), ctypes.byref(it)))
+        return it.value
+
+    def energy(self):
+        e = ctypes.c_double()
+        _check(lib().deft_minimize_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def stats(self):
+        s = MinimizeStats()
+        _check(lib().deft_minimize_stats_get(self._h, ctypes.byref(s)))
+        d = {n: getattr(s, n) for n, _ in s._fields_}
+        d["stop_reason"] = STOP_REASONS.get(d["stop_reason"], str(d["stop_reason"]))
+        return d
+
+    def get_iteration_count(self):
+        return self.stats()["iter"]
+
+    def invalidate_cache(self):
+        _check(lib().deft_minimize_invalidate(self._h))
+
+
+def declared_symbols(header=HEADER_PATH):
+    """Function names declared in include/deft_amd.h."""
+    import re
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(deft_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,749 @@
+// C ABI of the MI355X White-Bear functional: plans, the functional and FFT
+// seams, the bulk path, and the device-resident conjugate-gradient minimizer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+
+using namespace deft;
+
+namespace deft {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace deft
+
+#define TRY(expr)                    \
+  do {                               \
+    const int rc_ = (expr);          \
+    if (rc_ != DEFT_OK) return rc_;  \
+  } while (0)
+
+namespace {
+
+constexpr int kVersion = 1;
+
+bool det3(const double *c, double &det) {
+  det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
+  return std::isfinite(det) && det != 0.0;
+}
+
+// rows b_j of 2 pi inv(cell)^T, so that a_i . b_j = 2 pi delta_ij
+void reciprocal(const double *c, double det, double *b) {
+  double inv[9];
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) b[j * 3 + k] = 2 * M_PI * inv[k * 3 + j];
+}
+
+bool length_ok(int n) {
+  return with_length(n, [](auto) {});
+}
+
+int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
+  if (p->prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
+  return Launch<float>::evaluate(p, grad, have_old, precond);
+}
+
+// Per-term energies from the reduced sums, summed in a fixed order.
+deft_energy_terms terms_from_sums(const deft_plan_s *p) {
+  const double *s = p->hscal;
+  deft_energy_terms t;
+  t.kTphi1 = p->kT * s[SLOT_PHI1] * p->dV;
+  t.kTphi2 = p->kT * s[SLOT_PHI2] * p->dV;
+  t.kTphi3 = p->kT * s[SLOT_PHI3] * p->dV;
+  t.ideal = s[SLOT_IDEAL] * p->dV;
+  t.int_V_n = s[SLOT_EXT] * p->dV;
+  t.minus_mu_N = s[SLOT_MUN] * p->dV;
+  t.total = t.kTphi1 + t.kTphi2 + t.kTphi3 + t.ideal + t.int_V_n + t.minus_mu_N;
+  return t;
+}
+
+int ensure_tmp(deft_plan_s *p, size_t bytes) {
+  if (p->tmp_bytes >= bytes) return DEFT_OK;
+  if (p->tmp) (void)hipFree(p->tmp);
+  p->tmp = nullptr;
+  p->tmp_bytes = 0;
+  if (hipMalloc(&p->tmp, bytes) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "hipMalloc of scratch failed");
+  p->tmp_bytes = bytes;
+  return DEFT_OK;
+}
+
+int flush_moves(deft_plan_s *p) {
+  for (int i = 0; i < p->npend; ++i)
+    hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[i]);
+  p->npend = 0;
+  DEFT_HIP_CHECK(hipGetLastError());
+  return DEFT_OK;
+}
+
+void queue_move(deft_plan_s *p, double s) {
+  if (p->npend == 8) {
+    hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[0]);
+    for (int i = 1; i < 8; ++i) p->pend[i - 1] = p->pend[i];
+    p->npend = 7;
+  }
+  p->pend[p->npend++] = s;
+}
+
+int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
+  DEFT_HIP_CHECK(hipGetLastError());
+  DEFT_HIP_CHECK(hipMemcpyAsync(p->hscal + slot, p->dscal + slot, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  *out = p->hscal[slot];
+  return DEFT_OK;
+}
+
+void free_plan(deft_plan_s *p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(p->x);
+  (void)hipFree(p->d);
+  (void)hipFree(p->g);
+  (void)hipFree(p->pg);
+  (void)hipFree(p->og);
+  (void)hipFree(p->vext);
+  (void)hipFree(p->spec);
+  (void)hipFree(p->partial);
+  (void)hipFree(p->dscal);
+  (void)hipFree(p->twbuf);
+  (void)hipFree(p->tmp);
+  if (p->hscal) (void)hipHostFree(p->hscal);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+}  // namespace
+
+// ===========================================================================
+// minimizer state (src/new/Minimize.h:13-199)
+// ===========================================================================
+struct deft_minimize_s {
+  deft_plan_s *p = nullptr;
+  int iter = 0, maxiter = 10000000, miniter = 6;
+  double precision = 0.0, relative_precision = 1e-15;
+  bool use_precond = false;
+  double step = 0.1, orig_stepsize = 0.1, oldgradsqr = 1.0;
+  double deltaE = 0.0, dEdn = 0.0, log_avg = 0.0, error_estimate = 0.0;
+  bool have_dir = false, have_old = false;
+  bool E_valid = false;
+  double E = 0.0;
+  bool G_valid = false;
+  double pgg = 0.0, pgo = 0.0;
+  int num_energy = 0, num_grad = 0;
+  int stop = DEFT_STOP_NONE;
+};
+
+namespace {
+
+void invalidate(deft_minimize_s *m) {
+  m->E_valid = false;
+  m->G_valid = false;
+}
+
+int m_energy(deft_minimize_s *m, double *E) {
+  if (!m->E_valid) {
+    TRY(evaluate(m->p, false, false, false));
+    m->p->last_terms = terms_from_sums(m->p);
+    m->E = m->p->last_terms.total;
+    m->E_valid = true;
+    m->num_energy++;
+  }
+  *E = m->E;
+  return DEFT_OK;
+}
+
+// Minimize::pgrad (src/new/Minimize.h:136-165): with preconditioning the
+// energy, grad and pgrad come from one evaluation; without, pgrad = grad.
+int m_pgrad(deft_minimize_s *m) {
+  if (m->G_valid) return DEFT_OK;
+  if (m->use_precond) invalidate(m);
+  TRY(evaluate(m->p, true, m->have_old, m->use_precond));
+  m->p->last_terms = terms_from_sums(m->p);
+  if (m->use_precond || !m->E_valid) {
+    m->E = m->p->last_terms.total;
+    m->E_valid = true;
+  }
+  if (m->use_precond) m->num_energy++;
+  m->num_grad++;
+  m->pgg = m->p->hscal[SLOT_PGG];
+  m->pgo = m->p->hscal[SLOT_PGO];
+  m->G_valid = true;
+  return DEFT_OK;
+}
+
+inline bool better(double a, double b) { return a < b || b != b; }
+
+int stop_with(deft_minimize_s *m, int reason, int *more) {
+  m->stop = reason;
+  *more = 0;
+  return DEFT_OK;
+}
+
+// One Minimize::improve_energy (src/new/Minimize.cpp:9-313).
+int improve_energy(deft_minimize_s *m, int verbose, int *more) {
+  deft_plan_s *p = m->p;
+  m->iter++;
+  if (m->iter >= m->maxiter) return stop_with(m, DEFT_STOP_MAXITER, more);
+  double E0;
+  TRY(m_energy(m, &E0));
+  const double old_deltaE = m->deltaE;
+  if (E0 != E0) return stop_with(m, DEFT_STOP_NAN_ENERGY, more);
+
+  // Polak-Ribiere direction with automatic reset
+  TRY(m_pgrad(m));
+  invalidate(m);
+  double beta = m->have_old ? (m->pgg - m->pgo) / m->oldgradsqr : m->pgg / m->oldgradsqr;
+  if (beta < 0 || beta != beta || m->oldgradsqr == 0) beta = 0;
+  m->oldgradsqr = m->pgg;
+  hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
+                     beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
+  m->have_dir = true;
+  m->have_old = true;
+  double gdotd;
+  TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
+  if (gdotd > 0) {
+    hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
+                       p->pstride);
+    TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
+    if (verbose) printf("\t\treset to gradient\n");
+  }
+
+  // quadratic line minimization
+  const double slope = -gdotd;
+  if (E0 >= std::numeric_limits<double>::max() || E0 <= -std::numeric_limits<double>::max())
+    return stop_with(m, DEFT_STOP_INFINITE_ENERGY, more);
+  if (slope != slope) return stop_with(m, DEFT_STOP_NAN_SLOPE, more);
+  if (slope == 0) return stop_with(m, DEFT_STOP_ZERO_SLOPE, more);
+  if (m->step == m->orig_stepsize && std::fabs(E0) > 1e-200 && std::fabs(slope) > 1e-100)
+    m->step = 0.001 * std::fabs(E0) / slope;
+  if (slope * m->step < 0) m->step *= -1;
+  double step1 = m->step;
+  queue_move(p, step1);
+  double Etried = -137, E;
+  while (true) {
+    step1 *= 0.5;
+    queue_move(p, -step1);
+    invalidate(m);
+    TRY(m_energy(m, &E));
+    if (E == Etried) break;
+    Etried = E;
+    if (!better(E0, E)) break;
+  }
+  double E1;
+  TRY(m_energy(m, &E1));
+  const double curvature = 2.0 * (E1 - E0 + step1 * slope) / (step1 * step1);
+  if (curvature <= 0.0) {
+    if (better(E0, E1)) {
+      invalidate(m);
+      queue_move(p, -step1);
+      return stop_with(m, DEFT_STOP_NEGATIVE_CURVATURE, more);
+    }
+    double Ebest = E0;
+    while (true) {
+      TRY(m_energy(m, &E));
+      if (!(E <= Ebest)) break;
+      Ebest = E;
+      queue_move(p, step1);
+      invalidate(m);
+      step1 *= 2;
+    }
+    step1 *= 0.5;
+    queue_move(p, -step1);
+    m->step = step1;
+    invalidate(m);
+  } else if (E1 == E0) {
+    do {
+      invalidate(m);
+      queue_move(p, -step1);
+      step1 *= 2;
+      queue_move(p, step1);
+      TRY(m_energy(m, &E));
+    } while (E == E0);
+    if (E > E0) {
+      queue_move(p, -step1);
+      invalidate(m);
+      step1 = 0;
+    } else if (E != E) {
+      queue_move(p, -step1);
+      invalidate(m);
+      step1 = 0;
+    }
+  } else {
+    const double step2 = slope / curvature;
+    m->step = step2;
+    invalidate(m);
+    queue_move(p, step2 - step1);
+    double E2;
+    TRY(m_energy(m, &E2));
+    if (better(E1, E2) && better(E1, E0)) {
+      invalidate(m);
+      queue_move(p, -(step2 - step1));
+      m->step = step1;
+    } else if (E2 > E0) {
+      invalidate(m);
+      queue_move(p, -step2);
+      return stop_with(m, DEFT_STOP_ENERGY_ROSE, more);
+    }
+  }
+
+  // convergence estimate (src/new/Minimize.cpp:263-312)
+  double newE;
+  TRY(m_energy(m, &newE));
+  m->deltaE = newE - E0;
+  const double w = 0.01;
+  m->dEdn = (1 - w) * m->dEdn + w * m->deltaE;
+  m->dEdn = std::max(m->deltaE, old_deltaE);
+  const double new_log = (m->deltaE != 0 && old_deltaE != 0) ? std::log(std::fabs(m->deltaE / old_deltaE)) : 0.0;
+  m->log_avg = (1 - w) * m->log_avg + w * new_log;
+  const double ratio_avg = std::exp(m->log_avg);
+  double error_guess = std::fabs(std::max(m->dEdn, m->deltaE) / m->log_avg);
+  if (ratio_avg >= 1) error_guess = m->precision + std::fabs(newE);
+  m->error_estimate = 2 * error_guess;
+  if (verbose)
+    printf("iter %4d  E = %.15g  dE = %.3e  err = %.3e  step = %.3e  (#E %d, #G %d)\n", m->iter, newE, m->deltaE,
+           m->error_estimate, m->step, m->num_energy, m->num_grad);
+  if (m->deltaE == 0 && old_deltaE == 0) {
+    m->stop = DEFT_STOP_NO_CHANGE;
+    *more = m->iter < m->miniter;
+    return DEFT_OK;
+  }
+  if (!(m->error_estimate < m->precision) && !(m->error_estimate < m->relative_precision * std::fabs(newE))) {
+    *more = 1;
+    return DEFT_OK;
+  }
+  m->stop = DEFT_STOP_CONVERGED;
+  *more = m->iter < m->miniter;
+  return DEFT_OK;
+}
+
+int check_plan(deft_plan p) {
+  if (!p) return set_error(DEFT_ERR_ARG, "null plan");
+  return DEFT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *deft_last_error(void) { return g_last_error.c_str(); }
+
+int deft_version(void) { return kVersion; }
+
+int deft_supported_length(int n) { return (n > 0 && n % 2 == 0 && length_ok(n)) ? 1 : 0; }
+
+int deft_device_count(int *count) {
+  if (!count) return set_error(DEFT_ERR_ARG, "null count");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return DEFT_OK;
+}
+
+int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, double kT, double mu,
+                     int fft_precision, deft_plan *out) {
+  if (!out || !cell) return set_error(DEFT_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (!deft_supported_length(nx) || !deft_supported_length(ny) || !deft_supported_length(nz))
+    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must be even and in {16,20,32,48,64,96,128,256}");
+  double det;
+  if (!det3(cell, det)) return set_error(DEFT_ERR_ARG, "cell matrix is singular or not finite");
+  if (!(R > 0) || !(kT > 0) || !std::isfinite(R) || !std::isfinite(kT) || !std::isfinite(mu))
+    return set_error(DEFT_ERR_ARG, "R and kT must be positive and finite, mu finite");
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(DEFT_ERR_HIP, "no HIP device");
+
+  deft_plan_s *p = new deft_plan_s();
+  p->Nx = nx;
+  p->Ny = ny;
+  p->Nz = nz;
+  p->Nkz = nz / 2 + 1;
+  p->N = (long long)nx * ny * nz;
+  p->Nk = (long long)nx * ny * p->Nkz;
+  std::memcpy(p->cell, cell, sizeof(p->cell));
+  reciprocal(cell, det, p->recip);
+  p->volume = std::fabs(det);
+  p->dV = p->volume / (double)p->N;
+  p->R = R;
+  p->kT = kT;
+  p->mu = mu;
+  p->prec = fft_precision;
+  p->zblocks = (int)(((long long)nx * ny + 2 * zpairs(nz) - 1) / (2 * zpairs(nz)));
+  p->eblocks = (int)std::min<long long>(1024, (p->N + kThreads - 1) / kThreads);
+  p->pstride = std::max(p->zblocks, p->eblocks);
+
+  auto fail = [&](int code, const char *what) {
+    free_plan(p);
+    return set_error(code, what);
+  };
+  if (hipGetDevice(&p->device) != hipSuccess) return fail(DEFT_ERR_HIP, "hipGetDevice failed");
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "hipStreamCreate failed");
+  const size_t nb = sizeof(double) * (size_t)p->N;
+  if (hipMalloc(&p->x, nb) != hipSuccess || hipMalloc(&p->d, nb) != hipSuccess || hipMalloc(&p->g, nb) != hipSuccess ||
+      hipMalloc(&p->pg, nb) != hipSuccess || hipMalloc(&p->og, nb) != hipSuccess ||
+      hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
+      hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->Nk) != hipSuccess ||
+      hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
+      hipMalloc(&p->dscal, sizeof(double) * NUM_SLOTS) != hipSuccess)
+    return fail(DEFT_ERR_ALLOC, "device allocation failed");
+  if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS) != hipSuccess)
+    return fail(DEFT_ERR_ALLOC, "pinned allocation failed");
+  // twiddle tables exp(-2 pi i m / L), FP64 on the host
+  const int lens[3] = {nx, ny, nz};
+  size_t ntw = 0;
+  for (int a = 0; a < 3; ++a) ntw += lens[a];
+  if (hipMalloc(&p->twbuf, ntw * (sizeof(cx<float>) + sizeof(cx<double>))) != hipSuccess)
+    return fail(DEFT_ERR_ALLOC, "twiddle allocation failed");
+  std::vector<cx<float>> twf(ntw);
+  std::vector<cx<double>> twd(ntw);
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int L = lens[a];
+    for (int m = 0; m < L; ++m) {
+      const double ang = 2 * M_PI * (double)m / (double)L;
+      twd[off + m] = {std::cos(ang), -std::sin(ang)};
+      twf[off + m] = {(float)std::cos(ang), (float)-std::sin(ang)};
+    }
+    off += L;
+  }
+  cx<double> *dbase = reinterpret_cast<cx<double> *>(p->twbuf);
+  cx<float> *fbase = reinterpret_cast<cx<float> *>(dbase + ntw);
+  off = 0;
+  for (int a = 0; a < 3; ++a) {
+    p->twd[a] = dbase + off;
+    p->twf[a] = fbase + off;
+    off += lens[a];
+  }
+  if (hipMemcpy(dbase, twd.data(), ntw * sizeof(cx<double>), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(fbase, twf.data(), ntw * sizeof(cx<float>), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "twiddle upload failed");
+  if (hipMemset(p->x, 0, nb) != hipSuccess || hipMemset(p->d, 0, nb) != hipSuccess ||
+      hipMemset(p->og, 0, nb) != hipSuccess || hipMemset(p->g, 0, nb) != hipSuccess ||
+      hipMemset(p->pg, 0, nb) != hipSuccess || hipMemset(p->vext, 0, sizeof(float) * (size_t)p->N) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "initialisation failed");
+  *out = p;
+  return DEFT_OK;
+}
+
+int deft_plan_destroy(deft_plan p) {
+  free_plan(p);
+  return DEFT_OK;
+}
+
+int deft_plan_set_fft_precision(deft_plan p, int prec) {
+  TRY(check_plan(p));
+  if (prec != DEFT_FFT_FP32 && prec != DEFT_FFT_FP64) return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  p->prec = prec;
+  return DEFT_OK;
+}
+
+int deft_plan_set_mu(deft_plan p, double mu) {
+  TRY(check_plan(p));
+  if (!std::isfinite(mu)) return set_error(DEFT_ERR_ARG, "mu must be finite");
+  p->mu = mu;
+  return DEFT_OK;
+}
+
+int deft_plan_set_vext(deft_plan p, const double *vext) {
+  TRY(check_plan(p));
+  if (!vext) return set_error(DEFT_ERR_ARG, "null vext");
+  std::vector<float> f((size_t)p->N);
+  for (long long i = 0; i < p->N; ++i) f[i] = (float)vext[i];
+  DEFT_HIP_CHECK(hipMemcpyAsync(p->vext, f.data(), sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_plan_set_veff(deft_plan p, const double *veff) {
+  TRY(check_plan(p));
+  if (!veff) return set_error(DEFT_ERR_ARG, "null veff");
+  p->npend = 0;
+  DEFT_HIP_CHECK(hipMemcpyAsync(p->x, veff, sizeof(double) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_plan_get_veff(deft_plan p, double *veff) {
+  TRY(check_plan(p));
+  if (!veff) return set_error(DEFT_ERR_ARG, "null veff");
+  TRY(flush_moves(p));
+  DEFT_HIP_CHECK(hipMemcpyAsync(veff, p->x, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_plan_get_density(deft_plan p, double *n) {
+  TRY(check_plan(p));
+  if (!n) return set_error(DEFT_ERR_ARG, "null output");
+  TRY(flush_moves(p));
+  TRY(ensure_tmp(p, sizeof(double) * (size_t)p->N));
+  hipLaunchKernelGGL(k_density, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->vext, p->kT, p->tmp);
+  DEFT_HIP_CHECK(hipGetLastError());
+  DEFT_HIP_CHECK(hipMemcpyAsync(n, p->tmp, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_plan_veff_device_ptr(deft_plan p, double **ptr) {
+  TRY(check_plan(p));
+  if (!ptr) return set_error(DEFT_ERR_ARG, "null output");
+  TRY(flush_moves(p));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  *ptr = p->x;
+  return DEFT_OK;
+}
+
+int deft_plan_sizes(deft_plan p, long long *n_real, long long *n_spectral) {
+  TRY(check_plan(p));
+  if (n_real) *n_real = p->N;
+  if (n_spectral) *n_spectral = p->Nk;
+  return DEFT_OK;
+}
+
+int deft_functional_energy(deft_plan p, const double *veff, double *energy, deft_energy_terms *terms) {
+  TRY(check_plan(p));
+  if (veff) TRY(deft_plan_set_veff(p, veff));
+  TRY(evaluate(p, false, false, false));
+  p->last_terms = terms_from_sums(p);
+  if (energy) *energy = p->last_terms.total;
+  if (terms) *terms = p->last_terms;
+  return DEFT_OK;
+}
+
+int deft_functional_grad_device(deft_plan p, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
+                                double *energy) {
+  TRY(check_plan(p));
+  if (!outgrad_dev) return set_error(DEFT_ERR_ARG, "null outgrad");
+  if (veff_dev) {
+    p->npend = 0;
+    DEFT_HIP_CHECK(hipMemcpyAsync(p->x, veff_dev, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToDevice, p->stream));
+  }
+  TRY(evaluate(p, true, false, true));
+  p->last_terms = terms_from_sums(p);
+  if (energy) *energy = p->last_terms.total;
+  hipLaunchKernelGGL(k_accumulate, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, outgrad_dev, p->g);
+  if (outpgrad_dev)
+    hipLaunchKernelGGL(k_accumulate, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, outpgrad_dev, p->pg);
+  DEFT_HIP_CHECK(hipGetLastError());
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_functional_grad(deft_plan p, const double *veff, double *outgrad, double *outpgrad, double *energy) {
+  TRY(check_plan(p));
+  if (!outgrad) return set_error(DEFT_ERR_ARG, "null outgrad");
+  if (veff) TRY(deft_plan_set_veff(p, veff));
+  TRY(evaluate(p, true, false, true));
+  p->last_terms = terms_from_sums(p);
+  if (energy) *energy = p->last_terms.total;
+  std::vector<double> h((size_t)p->N);
+  DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->g, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  for (long long i = 0; i < p->N; ++i) outgrad[i] += h[i];
+  if (outpgrad) {
+    DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->pg, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    for (long long i = 0; i < p->N; ++i) outpgrad[i] += h[i];
+  }
+  return DEFT_OK;
+}
+
+int deft_weighted_densities(deft_plan p, double *out5) {
+  TRY(check_plan(p));
+  if (!out5) return set_error(DEFT_ERR_ARG, "null output");
+  TRY(ensure_tmp(p, 5 * sizeof(double) * (size_t)p->N));
+  if (p->prec == DEFT_FFT_FP64)
+    TRY(Launch<double>::weighted_densities(p, p->tmp));
+  else
+    TRY(Launch<float>::weighted_densities(p, p->tmp));
+  DEFT_HIP_CHECK(hipMemcpyAsync(out5, p->tmp, 5 * sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_fft(deft_plan p, const double *in, double *out) {
+  TRY(check_plan(p));
+  if (!in || !out) return set_error(DEFT_ERR_ARG, "null buffer");
+  TRY(ensure_tmp(p, sizeof(double) * (size_t)p->N));
+  DEFT_HIP_CHECK(hipMemcpyAsync(p->tmp, in, sizeof(double) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
+  if (p->prec == DEFT_FFT_FP64) {
+    TRY(Launch<double>::fft_forward(p, p->tmp, p->dV));
+    DEFT_HIP_CHECK(hipMemcpyAsync(out, p->spec, sizeof(cx<double>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
+    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  } else {
+    TRY(Launch<float>::fft_forward(p, p->tmp, p->dV));
+    std::vector<float> h(2 * (size_t)p->Nk);
+    DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->spec, sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
+    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  }
+  return DEFT_OK;
+}
+
+int deft_ifft(deft_plan p, const double *in, double *out) {
+  TRY(check_plan(p));
+  if (!in || !out) return set_error(DEFT_ERR_ARG, "null buffer");
+  TRY(ensure_tmp(p, sizeof(double) * (size_t)p->N));
+  if (p->prec == DEFT_FFT_FP64) {
+    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, in, sizeof(cx<double>) * (size_t)p->Nk, hipMemcpyHostToDevice, p->stream));
+    TRY(Launch<double>::fft_inverse(p, p->tmp, 1.0 / p->volume));
+  } else {
+    std::vector<float> h(2 * (size_t)p->Nk);
+    for (size_t i = 0; i < h.size(); ++i) h[i] = (float)in[i];
+    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, h.data(), sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyHostToDevice, p->stream));
+    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    TRY(Launch<float>::fft_inverse(p, p->tmp, 1.0 / p->volume));
+  }
+  DEFT_HIP_CHECK(hipMemcpyAsync(out, p->tmp, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+// --- bulk path (src/equation-of-state.cpp:164-192) ---------------------------
+static void bulk_terms(double n, double R, double kT, double *f_ex, double *df_ex) {
+  const WBConst c{4 * M_PI * R, 4 * M_PI * R * R};
+  const double n3 = 4 * M_PI / 3 * R * R * R * n, n2 = 4 * M_PI * R * R * n;
+  double p1, p2, p3, d[5];
+  wb_phi_partials(n3, n2, 0.0, 0.0, 0.0, c, p1, p2, p3, d);
+  *f_ex = kT * (p1 + p2 + p3);
+  *df_ex = kT * (d[0] * (4 * M_PI / 3 * R * R * R) + d[1] * (4 * M_PI * R * R));
+}
+
+int deft_bulk_chemical_potential(double n, double R, double kT, double *mu) {
+  if (!mu || !(n > 0) || !(R > 0) || !(kT > 0)) return set_error(DEFT_ERR_ARG, "bad bulk arguments");
+  double f, df;
+  bulk_terms(n, R, kT, &f, &df);
+  *mu = kT * std::log(n / water_nQ(kT)) + df;
+  return DEFT_OK;
+}
+
+int deft_bulk_free_energy_density(double n, double R, double kT, double *out) {
+  if (!out || !(n > 0) || !(R > 0) || !(kT > 0)) return set_error(DEFT_ERR_ARG, "bad bulk arguments");
+  double f, df;
+  bulk_terms(n, R, kT, &f, &df);
+  *out = kT * n * (std::log(n / water_nQ(kT)) - 1) + f;
+  return DEFT_OK;
+}
+
+int deft_bulk_pressure(double n, double R, double kT, double *pr) {
+  if (!pr) return set_error(DEFT_ERR_ARG, "null output");
+  double mu, f;
+  TRY(deft_bulk_chemical_potential(n, R, kT, &mu));
+  TRY(deft_bulk_free_energy_density(n, R, kT, &f));
+  *pr = n * mu - f;
+  return DEFT_OK;
+}
+
+// --- minimizer ----------------------------------------------------------------
+int deft_minimize_create(deft_plan p, deft_minimize *out) {
+  TRY(check_plan(p));
+  if (!out) return set_error(DEFT_ERR_ARG, "null output");
+  deft_minimize_s *m = new deft_minimize_s();
+  m->p = p;
+  *out = m;
+  return DEFT_OK;
+}
+
+int deft_minimize_destroy(deft_minimize m) {
+  delete m;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_precision(deft_minimize m, double v) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->precision = v;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_relative_precision(deft_minimize m, double v) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->relative_precision = v;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_maxiter(deft_minimize m, int v) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->maxiter = v;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_miniter(deft_minimize m, int v) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->miniter = v;
+  return DEFT_OK;
+}
+
+int deft_minimize_precondition(deft_minimize m, int v) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->use_precond = v != 0;
+  invalidate(m);
+  return DEFT_OK;
+}
+
+int deft_minimize_improve_energy(deft_minimize m, int verbose, int *more) {
+  if (!m || !more) return set_error(DEFT_ERR_ARG, "null argument");
+  return improve_energy(m, verbose, more);
+}
+
+int deft_minimize_run(deft_minimize m, int max_calls, int *iterations) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  int more = 1, n = 0;
+  while (more && n < max_calls) {
+    TRY(improve_energy(m, 0, &more));
+    ++n;
+  }
+  if (iterations) *iterations = n;
+  return DEFT_OK;
+}
+
+int deft_minimize_energy(deft_minimize m, double *energy) {
+  if (!m || !energy) return set_error(DEFT_ERR_ARG, "null argument");
+  return m_energy(m, energy);
+}
+
+int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *s) {
+  if (!m || !s) return set_error(DEFT_ERR_ARG, "null argument");
+  s->iter = m->iter;
+  s->num_energy_calcs = m->num_energy;
+  s->num_grad_calcs = m->num_grad;
+  s->stop_reason = m->stop;
+  s->energy = m->E;
+  s->delta_energy = m->deltaE;
+  s->error_estimate = m->error_estimate;
+  s->step = m->step;
+  return DEFT_OK;
+}
+
+int deft_minimize_invalidate(deft_minimize m) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  invalidate(m);
+  return DEFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Device-resident state of one White-Bear fluid grid (shared by the launch
+// units and the C ABI implementation).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/deft_amd.h"
+#include "passes.hpp"
+
+struct deft_plan_s {
+  int Nx = 0, Ny = 0, Nz = 0, Nkz = 0;
+  long long N = 0, Nk = 0;
+  double cell[9] = {0};
+  double recip[9] = {0};  // rows b1, b2, b3 with a_i . b_j = 2 pi delta_ij
+  double volume = 0, dV = 0;
+  double R = 1, kT = 1, mu = 0;
+  int prec = DEFT_FFT_FP32;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // FP64 vectors (N each): Veff (the minimization variable), CG direction,
+  // gradient, preconditioned gradient, previous gradient
+  double *x = nullptr, *d = nullptr, *g = nullptr, *pg = nullptr, *og = nullptr;
+  float *vext = nullptr;   // FP32, +inf marks hard walls
+  void *spec = nullptr;    // 6 half spectra of Nk complex (sized for FP64)
+  double *partial = nullptr;
+  int pstride = 0;         // partial-sum slot stride (>= blocks of any reducing kernel)
+  double *dscal = nullptr;  // NUM_SLOTS reduced sums (device)
+  double *hscal = nullptr;  // pinned host copy
+  void *twbuf = nullptr;
+  deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
+  deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
+  double *tmp = nullptr;   // scratch for host-seam calls (grows on demand)
+  size_t tmp_bytes = 0;
+
+  // moves x += s*d not yet applied (flushed into the next density pass)
+  double pend[8] = {0};
+  int npend = 0;
+
+  deft_energy_terms last_terms{};
+  int zblocks = 0;  // blocks of the z passes
+  int eblocks = 0;  // blocks of the elementwise FP64 kernels
+};
+
+namespace deft {
+
+inline double water_nQ(double kT) {
+  // nQ = (m kT / 2 pi)^(3/2), m = water mass in atomic units (deft src/IdealGas.cpp:54-59)
+  const double mass = 18.01528 * 1822.8885;
+  return std::pow(mass * kT / (2 * M_PI), 1.5);
+}
+
+inline GridArgs grid_args(const deft_plan_s *p) {
+  GridArgs a{};
+  a.Nx = p->Nx;
+  a.Ny = p->Ny;
+  a.Nz = p->Nz;
+  a.Nkz = p->Nkz;
+  a.N = p->N;
+  a.Nk = p->Nk;
+  for (int i = 0; i < 9; ++i) a.b[i] = p->recip[i];
+  a.R = p->R;
+  const double dr = std::cbrt(p->dV);
+  a.six_dr2 = 6.0 * dr * dr;
+  a.invN = 1.0 / (double)p->N;
+  a.kT = p->kT;
+  a.mu = p->mu;
+  a.logNQ = std::log(water_nQ(p->kT));
+  a.dV = p->dV;
+  a.wb.fpR = 4 * M_PI * p->R;
+  a.wb.fpR2 = 4 * M_PI * p->R * p->R;
+  return a;
+}
+
+template <typename T>
+inline cx<T> *spec_ptr(deft_plan_s *p, int field) {
+  return reinterpret_cast<cx<T> *>(p->spec) + (long long)field * p->Nk;
+}
+
+template <typename T>
+inline cx<T> *tw_ptr(deft_plan_s *p, int axis);
+template <>
+inline cx<float> *tw_ptr<float>(deft_plan_s *p, int axis) {
+  return p->twf[axis];
+}
+template <>
+inline cx<double> *tw_ptr<double>(deft_plan_s *p, int axis) {
+  return p->twd[axis];
+}
+
+int set_error(int code, const std::string &msg);
+
+// Kernel launch sequences, instantiated for T = float and T = double in
+// separate translation units.
+template <typename T>
+struct Launch {
+  // Energy (grad=false) or energy + grad + pgrad (grad=true) of the resident
+  // Veff after applying pending moves; reduced sums land in p->hscal.
+  static int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond);
+  // fft seam: real FP64 (device, N) -> spectrum in field 0 (scaled by scl)
+  static int fft_forward(deft_plan_s *p, const double *in_dev, double scl);
+  // ifft seam: spectrum in field 0 -> real FP64 (device, N) times scl
+  static int fft_inverse(deft_plan_s *p, double *out_dev, double scl);
+  // five weighted densities of the resident Veff into out_dev (5N doubles)
+  static int weighted_densities(deft_plan_s *p, double *out_dev);
+};
+
+extern template struct Launch<float>;
+extern template struct Launch<double>;
+
+}  // namespace deft

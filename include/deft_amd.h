@@ -1,0 +1,178 @@
+/*
+ * deft_amd: C ABI of the MI355X White-Bear fundamental-measure functional and
+ * its conjugate-gradient minimizer (one grid, resident on one GPU).
+ *
+ * The reference (deft, github.com/droundy/deft) reaches this path through three
+ * nested seams; each entry point below names the one it replaces:
+ *   - the FFT seam        fft()/ifft()                 src/Grid.cpp:470-478, src/ReciprocalGrid.cpp:52-65,
+ *                                                      src/new/Vector.h:327-373
+ *   - the functional seam FunctionalInterface          src/Functional.h:9-42 (integral, grad),
+ *                         generated NewFunctional      src/new/NewFunctional.h:27-87
+ *                                                      (energy, grad, energy_grad_and_precond)
+ *   - the minimizer seam  Minimize / MinimizerInterface src/new/Minimize.h:13-199, src/Minimizer.h:16-81
+ *
+ * Conventions (the reference's, kept):
+ *   - real fields are row-major, index (x*Ny + y)*Nz + z (src/Grid.h:37-42);
+ *     half spectra (x*Ny + y)*(Nz/2+1) + kz, interleaved re/im doubles;
+ *   - every grid edge is even (src/new/Vector.h:328-330) and a product of
+ *     2, 3 and 5 from the supported list (deft_supported_length);
+ *   - gradients are ACCUMULATED into caller buffers (+=), and the pgrad output
+ *     may be NULL (src/Functional.h:407-417, src/EffectivePotentialToDensity.cpp:62-79);
+ *   - gradients are derivatives of the discrete energy sum, so they carry the dV
+ *     factor (ingrad = dV, src/Functional.h:145-148);
+ *   - NaN and infinity are values, not errors; minimizers stop on them
+ *     (src/new/Minimize.cpp:22-30,75-111).
+ * Every function returns a deft_status; deft_last_error() describes the last
+ * failure on the calling thread.  Plans and minimizers are not thread-safe
+ * (the reference is single-threaded too); distinct plans may be used from
+ * distinct threads.
+ */
+#ifndef DEFT_AMD_H
+#define DEFT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DEFT_OK = 0,
+  DEFT_ERR_ARG = 1,          /* invalid argument */
+  DEFT_ERR_UNSUPPORTED = 2,  /* grid edge not supported by the FFT kernels */
+  DEFT_ERR_HIP = 3,          /* HIP runtime error (no device, launch failure, ...) */
+  DEFT_ERR_ALLOC = 4,        /* device or pinned allocation failed */
+  DEFT_ERR_STATE = 5         /* call not valid in the current state */
+} deft_status;
+
+/* FFT / convolution precision.  Pointwise Phi, log(1-n3), 1/(1-n3), all
+ * reductions and the CG scalars are FP64 in both modes. */
+typedef enum {
+  DEFT_FFT_FP32 = 0,
+  DEFT_FFT_FP64 = 1
+} deft_fft_precision;
+
+/* Why a minimizer stopped (mirrors the return-false branches of
+ * src/new/Minimize.cpp:9-313). */
+typedef enum {
+  DEFT_STOP_NONE = 0,
+  DEFT_STOP_MAXITER = 1,
+  DEFT_STOP_NAN_ENERGY = 2,
+  DEFT_STOP_INFINITE_ENERGY = 3,
+  DEFT_STOP_NAN_SLOPE = 4,
+  DEFT_STOP_ZERO_SLOPE = 5,
+  DEFT_STOP_NEGATIVE_CURVATURE = 6,
+  DEFT_STOP_ENERGY_ROSE = 7,
+  DEFT_STOP_NO_CHANGE = 8,
+  DEFT_STOP_CONVERGED = 9
+} deft_stop_reason;
+
+typedef struct deft_plan_s *deft_plan;
+typedef struct deft_minimize_s *deft_minimize;
+
+/* Per-term energies of the last evaluation, like the reference's printme /
+ * print_summary (src/Functional.cpp:281-316): kT*int Phi1, kT*int Phi2,
+ * kT*int Phi3, ideal gas, int Vext n, -mu N, and their sum. */
+typedef struct {
+  double kTphi1, kTphi2, kTphi3, ideal, int_V_n, minus_mu_N, total;
+} deft_energy_terms;
+
+typedef struct {
+  int iter;                 /* Minimize::get_iteration_count (src/new/Minimize.h:77) */
+  int num_energy_calcs;
+  int num_grad_calcs;
+  int stop_reason;          /* deft_stop_reason */
+  double energy;            /* last (cached) energy */
+  double delta_energy;      /* energy change of the last iteration */
+  double error_estimate;    /* src/new/Minimize.cpp:263-312 */
+  double step;              /* remembered line-search step */
+} deft_minimize_stats;
+
+const char *deft_last_error(void);
+int deft_version(void);
+/* 1 if an FFT edge of this length is supported, else 0. */
+int deft_supported_length(int n);
+/* Number of visible HIP devices (0 without a GPU). */
+int deft_device_count(int *count);
+
+/* --- plan: grid + White-Bear fluid parameters -------------------------------
+ * Replaces GridDescription (src/GridDescription.cpp:6-50) plus the generated
+ * WhiteBearFluidVeff constructor (src/haskell/NewCode.hs:150-178):
+ *   cell  : 3x3 row-major, rows are the lattice vectors a1, a2, a3 (src/lattice.h:79-112)
+ *   R, kT : hard-sphere radius and temperature; mu: chemical potential
+ * F[Veff] = kT int Phi_WB + int kT n (ln(n/nQ) - 1) + int Vext n - mu int n,
+ * n = exp(-Veff/kT) (src/haskell/WhiteBear.hs:29-31).  Vext defaults to 0;
+ * Vext = +inf marks a hard wall (density exactly 0 there).  The plan owns a
+ * device-resident Veff (the minimization variable), initialised to 0. */
+int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, double kT, double mu,
+                     int fft_precision, deft_plan *out);
+int deft_plan_destroy(deft_plan plan);
+int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
+int deft_plan_set_mu(deft_plan plan, double mu);
+/* Vext (N doubles, host); stored as FP32 on the device (+inf kept). */
+int deft_plan_set_vext(deft_plan plan, const double *vext);
+int deft_plan_set_veff(deft_plan plan, const double *veff);
+int deft_plan_get_veff(deft_plan plan, double *veff);
+/* n = exp(-Veff/kT), 0 on hard-wall points */
+int deft_plan_get_density(deft_plan plan, double *n);
+/* Device pointer of the resident Veff (N doubles) for zero-copy callers. */
+int deft_plan_veff_device_ptr(deft_plan plan, double **ptr);
+int deft_plan_sizes(deft_plan plan, long long *n_real, long long *n_spectral);
+
+/* --- functional seam -------------------------------------------------------
+ * FunctionalInterface::integral (src/Functional.h:131-143, src/Functional.cpp:67-69) /
+ * NewFunctional::energy.  veff may be NULL to use the resident Veff; otherwise
+ * it is uploaded and becomes the resident Veff. */
+int deft_functional_energy(deft_plan plan, const double *veff, double *energy, deft_energy_terms *terms);
+/* FunctionalInterface::grad (src/Functional.h:147-149,184-188) with the
+ * OfEffectivePotential chain rule (src/EffectivePotentialToDensity.cpp:62-79):
+ *   outgrad  += dF/dVeff = -(n/kT) dF/dn
+ *   outpgrad += grad/n = -dF/dn / kT  (new-API preconditioner, src/haskell/NewCode.hs:229-242)
+ * outpgrad and energy may be NULL.  Host buffers of N doubles. */
+int deft_functional_grad(deft_plan plan, const double *veff, double *outgrad, double *outpgrad, double *energy);
+/* Same with device buffers (veff may be NULL). */
+int deft_functional_grad_device(deft_plan plan, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
+                                double *energy);
+/* The five weighted densities n3, n2, n2v_x, n2v_y, n2v_z of the resident Veff
+ * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R). */
+int deft_weighted_densities(deft_plan plan, double *out5);
+
+/* --- FFT seam --------------------------------------------------------------
+ * fft(gd, f) = r2c * dV (src/Grid.cpp:470-478): in N doubles, out 2*Nk doubles.
+ * ifft(gd, F) = c2r / V (src/ReciprocalGrid.cpp:52-65): in 2*Nk doubles, out N.
+ * Run at the plan's FFT precision. */
+int deft_fft(deft_plan plan, const double *in, double *out);
+int deft_ifft(deft_plan plan, const double *in, double *out);
+
+/* --- bulk (homogeneous) path, host-side FP64 --------------------------------
+ * find_chemical_potential and pressure of ideal gas + White-Bear
+ * (src/equation-of-state.cpp:164-192), homogeneous f(n) per volume. */
+int deft_bulk_chemical_potential(double n, double R, double kT, double *mu);
+int deft_bulk_pressure(double n, double R, double kT, double *p);
+int deft_bulk_free_energy_density(double n, double R, double kT, double *f);
+
+/* --- minimizer seam ---------------------------------------------------------
+ * Minimize (src/new/Minimize.h:13-199): Polak-Ribiere CG with the quadratic
+ * line search and cached energies (src/new/Minimize.cpp:9-313), acting on the
+ * plan's resident Veff.  Defaults as the reference: miniter 6, precision 0,
+ * relative_precision 1e-15, no preconditioning, step 0.1. */
+int deft_minimize_create(deft_plan plan, deft_minimize *out);
+int deft_minimize_destroy(deft_minimize m);
+int deft_minimize_set_precision(deft_minimize m, double precision);
+int deft_minimize_set_relative_precision(deft_minimize m, double relative_precision);
+int deft_minimize_set_maxiter(deft_minimize m, int maxiter);
+int deft_minimize_set_miniter(deft_minimize m, int miniter);
+int deft_minimize_precondition(deft_minimize m, int use_preconditioning);
+/* Minimize::improve_energy: *more = 1 while the minimizer wants to continue. */
+int deft_minimize_improve_energy(deft_minimize m, int verbose, int *more);
+/* Loops improve_energy until it returns false or max_calls iterations ran. */
+int deft_minimize_run(deft_minimize m, int max_calls, int *iterations);
+/* Minimize::energy (cached). */
+int deft_minimize_energy(deft_minimize m, double *energy);
+int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *stats);
+/* Drops cached energy/gradient (after the caller changed Veff or the plan). */
+int deft_minimize_invalidate(deft_minimize m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEFT_AMD_H */

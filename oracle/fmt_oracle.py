@@ -1,0 +1,642 @@
+"""FP64 CPU oracle for the White-Bear hard-sphere fluid and its conjugate-gradient minimizer.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product path (``deft_amd``) never calls it and fails loudly
+when its HIP library is missing.
+
+It restates, in numpy/scipy FP64, the reference algorithm of deft
+(github.com/droundy/deft) for this path:
+
+* grid layout, FFT conventions and k-vectors
+  - row-major real fields ``i = x*Ny*Nz + y*Nz + z``            (src/Grid.h:37-42)
+  - forward transform r2c times dV                              (src/Grid.cpp:470-478, src/new/Vector.h:327-350)
+  - inverse transform c2r times 1/V                             (src/ReciprocalGrid.cpp:52-65, src/new/Vector.h:352-373)
+  - k-vector of a flat half-spectrum index, x and y wrapped
+    to (-N/2, N/2]                                              (src/ReciprocalOperators.h:20-41)
+* the fundamental-measure weights (step, shell, vector shell) with the
+  Gaussian smearing exp(-6 k^2 dr^2), dr = dV^(1/3), and their small-kR
+  series branches                                              (src/ReciprocalOperators.h:90-235, src/haskell/FMT.hs:21-65)
+* the White-Bear excess free energy density without tensor
+  (Phi1 + Phi2 + Phi3)                                          (src/HardSpheres.cpp:137-159, src/haskell/WhiteBear.hs:47-60)
+  and its closed-form partial derivatives                       (src/haskell/WhiteBear.hs:114-121)
+* the ideal gas with the water-mass thermal density nQ           (src/IdealGas.cpp:54-72, src/haskell/IdealGas.hs:7-20)
+* the external potential and chemical potential terms           (src/ExternalPotential.cpp:19-21, src/ChemicalPotential.cpp:19-22)
+* the effective-potential variable n = exp(-Veff/kT), its chain
+  rule and the new-API preconditioner grad/n                    (src/EffectivePotentialToDensity.cpp:28-90,
+                                                                 src/haskell/create_generators.py:21-22,
+                                                                 src/haskell/NewCode.hs:229-242,306-319)
+* the homogeneous path used for the chemical potential and
+  pressure                                                      (src/equation-of-state.cpp:164-192, src/Functional.h:382-392)
+* the new-API Polak-Ribiere conjugate gradient with quadratic
+  line search, cached energies and the geometric error
+  estimate                                                      (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
+
+One deliberate numerical difference from the reference: the Phi3 prefactor
+f3(n3) = (n3 + (1-n3)^2 ln(1-n3)) / (n3^2 (1-n3)^2) is evaluated with its Taylor
+series for |n3| < 0.05, where the closed form cancels catastrophically (it is
+0/0 at n3 = 0, e.g. inside a hard wall).  The series is the same function;
+tests check that both branches agree where they overlap.
+
+Hard walls are V_ext = +inf; there the density is exactly zero, the point
+contributes nothing to the energy and its gradient and preconditioned
+gradient are zero (the limit of the reference's V_ext = V_eff = 500 kT wall,
+papers/fuzzy-fmt/figs/new-bh-walls.cpp:136-145).
+"""
+
+import math
+
+import numpy as np
+from scipy import fft as _sfft
+
+# Molecular mass of water in atomic units, used by deft for nQ (src/IdealGas.cpp:54-59).
+WATER_MASS = 18.01528 * 1822.8885
+SPREADING = 6.0  # src/ReciprocalOperators.h:90
+F3_SERIES_CUTOFF = 0.05
+F3_SERIES_TERMS = 18
+
+
+def nQ(kT):
+    """Thermal density (m kT / 2 pi)^(3/2) (src/IdealGas.cpp:54-59)."""
+    return (WATER_MASS * kT / (2 * math.pi)) ** 1.5
+
+
+def _rfftn(a):
+    return _sfft.rfftn(a, workers=1)
+
+
+def _irfftn(a, shape):
+    return _sfft.irfftn(a, s=shape, workers=1)
+
+
+# ---------------------------------------------------------------------------
+# Grid and k-space weights
+# ---------------------------------------------------------------------------
+
+class Grid:
+    """Grid metadata: shape, cell, dV and half-spectrum k-vectors.
+
+    ``cell`` holds the lattice vectors a1, a2, a3 as rows; the real-space point
+    (x, y, z) sits at x*a1/Nx + y*a2/Ny + z*a3/Nz (src/GridDescription.cpp:6-16,
+    src/Grid.h:37-42).  dV = |det(cell)|/N.  k = 2 pi inv(cell)^T m with m the
+    wrapped integer frequency (src/lattice.h:105-107, src/ReciprocalOperators.h:23-38).
+    """
+
+    def __init__(self, shape, cell):
+        self.shape = tuple(int(s) for s in shape)
+        Nx, Ny, Nz = self.shape
+        if any(s % 2 for s in self.shape):
+            raise ValueError("every grid edge must be even (src/new/Vector.h:328-330)")
+        self.cell = np.array(cell, dtype=np.float64).reshape(3, 3)
+        self.volume = abs(float(np.linalg.det(self.cell)))
+        self.N = Nx * Ny * Nz
+        self.dV = self.volume / self.N
+        self.kshape = (Nx, Ny, Nz // 2 + 1)
+        recip = 2 * math.pi * np.linalg.inv(self.cell).T  # rows b1, b2, b3
+        self.recip = recip
+
+        def wrap(n):
+            m = np.arange(n)
+            return np.where(m > n // 2, m - n, m).astype(np.float64)
+
+        mx = wrap(Nx)[:, None, None]
+        my = wrap(Ny)[None, :, None]
+        mz = np.arange(Nz // 2 + 1, dtype=np.float64)[None, None, :]
+        self.kx = mx * recip[0, 0] + my * recip[1, 0] + mz * recip[2, 0]
+        self.ky = mx * recip[0, 1] + my * recip[1, 1] + mz * recip[2, 1]
+        self.kz = mx * recip[0, 2] + my * recip[1, 2] + mz * recip[2, 2]
+        self.k = np.sqrt(self.kx ** 2 + self.ky ** 2 + self.kz ** 2)
+
+    @classmethod
+    def cubic(cls, n, length):
+        return cls((n, n, n), np.diag([length] * 3))
+
+    def positions(self):
+        """Cartesian positions of every grid point, shape (Nx, Ny, Nz, 3), cell-origin based."""
+        Nx, Ny, Nz = self.shape
+        fx = (np.arange(Nx) / Nx)[:, None, None, None]
+        fy = (np.arange(Ny) / Ny)[None, :, None, None]
+        fz = (np.arange(Nz) / Nz)[None, None, :, None]
+        c = self.cell
+        return fx * c[0] + fy * c[1] + fz * c[2]
+
+
+def weights(grid, R):
+    """Fourier transforms of the fundamental-measure weights on the half spectrum.
+
+    Returns (w3, w2, wv) with
+      step          w3(k) = s 4 pi (sin kR - kR cos kR)/k^3        (src/ReciprocalOperators.h:94-118)
+      shell         w2(k) = s 4 pi R sin(kR)/k                     (src/ReciprocalOperators.h:120-142)
+      vector shell  w2v_j(k) = i wv(k) k_j,
+                    wv(k) = s 4 pi (R cos kR - sin kR/k)/k^2       (src/ReciprocalOperators.h:162-235)
+    s = exp(-6 k^2 dr^2), dr = dV^(1/3); for kR <= 1e-3 the reference's
+    power series (without smearing) is used.  n0 = n2/(4 pi R^2),
+    n1 = n2/(4 pi R), n1v = n2v/(4 pi R) (src/haskell/FMT.hs:34-39).
+    """
+    k = grid.k
+    dr = grid.dV ** (1.0 / 3.0)
+    kR = k * R
+    small = kR <= 1e-3
+    ks = np.where(small, 1.0, k)
+    kRs = ks * R
+    s = np.exp(-SPREADING * ks * ks * dr * dr)
+    kR2 = kR * kR
+    w3 = np.where(small,
+                  (4 * math.pi / 3) * R ** 3 * (kR2 ** 3 * (-1.0 / 15120) + kR2 ** 2 * (1.0 / 280) + kR2 * (-1.0 / 10) + 1),
+                  s * 4 * math.pi * (np.sin(kRs) - kRs * np.cos(kRs)) / ks ** 3)
+    w2 = np.where(small,
+                  4 * math.pi * R * R * (-kR2 ** 3 * (1.0 / 120 / 6 / 7) + kR2 ** 2 * (1.0 / 120) - kR2 * (1.0 / 6) + 1),
+                  s * 4 * math.pi * R * np.sin(kRs) / ks)
+    wv = np.where(small,
+                  4 * math.pi * R ** 3 * (kR2 ** 2 * (1.0 / 5040 - 1.0 / 720) + kR2 * (1.0 / 24 - 1.0 / 120) - 1.0 / 3),
+                  s * 4 * math.pi * (R * np.cos(kRs) - np.sin(kRs) / ks) / ks ** 2)
+    return w3, w2, wv
+
+
+def fft(grid, f):
+    """Forward transform with deft's normalisation: r2c times dV (src/Grid.cpp:470-478)."""
+    return _rfftn(f) * grid.dV
+
+
+def ifft(grid, F):
+    """Inverse transform with deft's normalisation: c2r times 1/V (src/ReciprocalGrid.cpp:52-65)."""
+    return _irfftn(F, grid.shape) / grid.dV
+
+
+# ---------------------------------------------------------------------------
+# White-Bear free-energy density and partials
+# ---------------------------------------------------------------------------
+
+def f3_and_derivative(n3):
+    """f3(n) = (n + (1-n)^2 ln(1-n)) / (n^2 (1-n)^2) and df3/dn.
+
+    Closed form as in src/haskell/WhiteBear.hs:50 (src/HardSpheres.cpp:153) for
+    |n| >= 0.05; below that the Taylor series f3 = g/(1-n)^2 with
+    g(n) = 3/2 - sum_{k>=3} 2 n^(k-2) / (k (k-1) (k-2)).
+    """
+    n = np.asarray(n3, dtype=np.float64)
+    small = np.abs(n) < F3_SERIES_CUTOFF
+    omn = 1.0 - n
+    # closed form (guard the small branch against 0/0)
+    nb = np.where(small, 0.5, n)
+    omb = 1.0 - nb
+    L = np.log1p(-nb)
+    num = nb + omb * omb * L
+    den = nb * nb * omb * omb
+    f3_big = num / den
+    dnum = nb - 2.0 * omb * L
+    dden = 2.0 * nb * omb * (1.0 - 2.0 * nb)
+    df3_big = dnum / den - num * dden / (den * den)
+    # series
+    ns = np.where(small, n, 0.0)
+    g = np.full_like(ns, 1.5)
+    gp = np.zeros_like(ns)
+    p = np.ones_like(ns)  # n^(k-3)
+    for kk in range(3, F3_SERIES_TERMS + 3):
+        # term of g: 2 n^(kk-2) / (kk (kk-1) (kk-2)); of g': 2 (kk-2) n^(kk-3) / (...)
+        c = 2.0 / (kk * (kk - 1) * (kk - 2))
+        gp -= c * (kk - 2) * p
+        p = p * ns
+        g -= c * p
+    oms = 1.0 - ns
+    f3_small = g / (oms * oms)
+    df3_small = gp / (oms * oms) + 2.0 * g / (oms * oms * oms)
+    return np.where(small, f3_small, f3_big), np.where(small, df3_small, df3_big)
+
+
+def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True):
+    """Phi = Phi1 + Phi2 + Phi3 of the no-tensor White-Bear functional and its partials.
+
+    Phi1 = -n0 ln(1-n3); Phi2 = (n1 n2 - n1v.n2v)/(1-n3);
+    Phi3 = f3(n3)/(36 pi) n2 (n2^2 - 3 n2v.n2v)   (src/haskell/WhiteBear.hs:47-60,
+    src/HardSpheres.cpp:137-159).  Partials with respect to (n3, n2, n2v_x, n2v_y, n2v_z),
+    n0, n1, n1v being folded in through n0 = n2/(4 pi R^2) etc. (src/haskell/WhiteBear.hs:114-121).
+    Returns (phi1, phi2, phi3, partials or None).
+    """
+    fpR = 4 * math.pi * R
+    fpR2 = 4 * math.pi * R * R
+    omn = 1.0 - n3
+    L = np.log1p(-n3)
+    v2 = vx * vx + vy * vy + vz * vz
+    f3, df3 = f3_and_derivative(n3)
+    t3 = n2 * (n2 * n2 - 3.0 * v2)
+    phi1 = -n2 * L / fpR2
+    phi2 = (n2 * n2 - v2) / (fpR * omn)
+    phi3 = f3 * t3 / (36 * math.pi)
+    if not want_partials:
+        return phi1, phi2, phi3, None
+    d3 = n2 / (fpR2 * omn) + (n2 * n2 - v2) / (fpR * omn * omn) + df3 * t3 / (36 * math.pi)
+    d2 = -L / fpR2 + 2.0 * n2 / (fpR * omn) + f3 * (3.0 * n2 * n2 - 3.0 * v2) / (36 * math.pi)
+    cv = -2.0 / (fpR * omn) - f3 * 6.0 * n2 / (36 * math.pi)
+    return phi1, phi2, phi3, (d3, d2, cv * vx, cv * vy, cv * vz)
+
+
+# ---------------------------------------------------------------------------
+# Homogeneous path: free energy per volume, chemical potential, pressure
+# ---------------------------------------------------------------------------
+
+def homogeneous_whitebear(n, R, kT=1.0):
+    """Excess free energy per volume kT Phi(n) of a uniform fluid (k=0 weights, src/Functional.h:382-392)."""
+    n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
+    n2 = np.float64(4 * math.pi * R * R * n)
+    z = np.float64(0.0)
+    p1, p2, p3, _ = whitebear_phi(n3, n2, z, z, z, R, want_partials=False)
+    return float(kT * (p1 + p2 + p3))
+
+
+def homogeneous_whitebear_dn(n, R, kT=1.0):
+    """d(kT Phi)/dn of a uniform fluid, through the partials (chain rule over n3, n2)."""
+    n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
+    n2 = np.float64(4 * math.pi * R * R * n)
+    z = np.float64(0.0)
+    _, _, _, d = whitebear_phi(n3, n2, z, z, z, R)
+    return float(kT * (d[0] * (4 * math.pi / 3 * R ** 3) + d[1] * (4 * math.pi * R * R)))
+
+
+def chemical_potential(n, R, kT=1.0):
+    """mu = d f/dn of ideal gas + White-Bear at bulk density n (src/equation-of-state.cpp:189-192)."""
+    return float(kT * math.log(n / nQ(kT)) + homogeneous_whitebear_dn(n, R, kT))
+
+
+def pressure(n, R, kT=1.0):
+    """p = n df/dn - f (src/equation-of-state.cpp:164-170)."""
+    f = kT * n * (math.log(n / nQ(kT)) - 1) + homogeneous_whitebear(n, R, kT)
+    return float(n * chemical_potential(n, R, kT) - f)
+
+
+def bulk_density(eta, R):
+    return eta / (4 * math.pi / 3 * R ** 3)
+
+
+# ---------------------------------------------------------------------------
+# The fluid functional of Veff: energy, gradient, preconditioned gradient
+# ---------------------------------------------------------------------------
+
+class WhiteBearFluidVeff:
+    """F[Veff] = kT int Phi + int kT n (ln(n/nQ) - 1) + int Vext n - mu int n, n = exp(-Veff/kT).
+
+    Mirrors deft's generated ``WhiteBearFluidVeff`` (src/haskell/WhiteBear.hs:29-31,
+    src/haskell/create_generators.py:21-22) and the old-API composition
+    OfEffectivePotential(HardSpheresWBnotensor + IdealGas + ChemicalPotential)
+    (papers/contact/figs/walls.cpp:86-89).  ``x`` is the Veff field (modified
+    in place by the minimizer, like the reference's data vector).
+    """
+
+    def __init__(self, grid, R, kT, mu, vext, x):
+        self.grid = grid
+        self.R = float(R)
+        self.kT = float(kT)
+        self.mu = float(mu)
+        self.vext = np.asarray(vext, dtype=np.float64).reshape(grid.shape)
+        self.finite = np.isfinite(self.vext)
+        self.vext_f = np.where(self.finite, self.vext, 0.0)
+        self.x = np.array(x, dtype=np.float64).reshape(grid.shape)
+        w3, w2, wv = weights(grid, self.R)
+        self.W = (w3, w2, 1j * wv * grid.kx, 1j * wv * grid.ky, 1j * wv * grid.kz)
+        self.parity = (1.0, 1.0, -1.0, -1.0, -1.0)  # w(-k) = +/- w(k) (src/Functional.h:407-417)
+        self.last_terms = {}
+        self.n_energy = 0
+        self.n_grad = 0
+
+    def density(self, x=None):
+        x = self.x if x is None else x
+        return np.where(self.finite, np.exp(-x / self.kT), 0.0)
+
+    def weighted_densities(self, n):
+        nk = _rfftn(n)
+        return [_irfftn(nk * W, self.grid.shape) for W in self.W]
+
+    def _local_terms(self, x, n):
+        kT = self.kT
+        ideal = np.where(self.finite, -n * (x + kT * math.log(nQ(kT)) + kT), 0.0)
+        ext = self.vext_f * n
+        mun = -self.mu * n
+        return ideal, ext, mun
+
+    def energy(self, x=None):
+        """Grand free energy (sum times dV); records each term like ``printme``."""
+        x = self.x if x is None else x
+        n = self.density(x)
+        nal = self.weighted_densities(n)
+        p1, p2, p3, _ = whitebear_phi(*nal, self.R, want_partials=False)
+        ideal, ext, mun = self._local_terms(x, n)
+        dV = self.grid.dV
+        kT = self.kT
+        t = {
+            "kTphi1": kT * float(np.sum(p1)) * dV,
+            "kTphi2": kT * float(np.sum(p2)) * dV,
+            "kTphi3": kT * float(np.sum(p3)) * dV,
+            "ideal": float(np.sum(ideal)) * dV,
+            "int_V_n": float(np.sum(ext)) * dV,
+            "minus_mu_N": float(np.sum(mun)) * dV,
+        }
+        self.last_terms = t
+        self.n_energy += 1
+        return (t["kTphi1"] + t["kTphi2"] + t["kTphi3"] + t["ideal"] + t["int_V_n"] + t["minus_mu_N"])
+
+    def dF_dn(self, x=None):
+        """Gradient of F with respect to the density values (including dV)."""
+        x = self.x if x is None else x
+        n = self.density(x)
+        nal = self.weighted_densities(n)
+        _, _, _, P = whitebear_phi(*nal, self.R)
+        acc = None
+        for Pa, W, par in zip(P, self.W, self.parity):
+            term = _rfftn(Pa) * (W * par)
+            acc = term if acc is None else acc + term
+        conv = _irfftn(acc, self.grid.shape)
+        kT = self.kT
+        local = -x - kT * math.log(nQ(kT)) + self.vext_f - self.mu
+        return np.where(self.finite, (kT * conv + local) * self.grid.dV, 0.0), n
+
+    def grad(self, x=None):
+        """dF/dVeff = dF/dn * (-n/kT) (src/EffectivePotentialToDensity.cpp:62-79)."""
+        g, n = self.dF_dn(x)
+        self.n_grad += 1
+        return -g * n / self.kT
+
+    def energy_grad_and_precond(self, x=None):
+        """(energy, grad, precond) with precond = grad / n = -dF/dn / kT
+        (src/haskell/NewCode.hs:229-242,306-319; zero on hard-wall points)."""
+        E = self.energy(x)
+        g, n = self.dF_dn(x)
+        self.n_grad += 1
+        return E, -g * n / self.kT, -g / self.kT
+
+    def have_preconditioner(self):
+        return True
+
+
+# ---------------------------------------------------------------------------
+# New-API minimizer (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
+# ---------------------------------------------------------------------------
+
+def _better(a, b):
+    """a is better than b (src/new/Minimize.cpp:5-7): a < b or b is NaN."""
+    return a < b or b != b
+
+
+class Minimize:
+    """Polak-Ribiere CG with deft's quadratic line search and cached energies.
+
+    ``f`` provides ``x`` (modified in place), ``energy()``, ``grad()``,
+    ``energy_grad_and_precond()`` and ``have_preconditioner()``.
+    """
+
+    def __init__(self, f):
+        self.f = f
+        self.iter = 0
+        self.maxiter = 10000000
+        self.miniter = 6
+        self.precision = 0.0
+        self.relative_precision = 1e-15
+        self.use_preconditioning = False
+        self.step = 0.1
+        self.orig_stepsize = 0.1
+        self.oldgradsqr = 1.0
+        self.deltaE = 0.0
+        self.dEdn = 0.0
+        self.log_dEdn_ratio_average = 0.0
+        self.error_estimate = 0.0
+        self.direction = None
+        self.oldgrad = None
+        self._E = None
+        self._g = None
+        self._pg = None
+        self.num_energy_calcs = 0
+        self.num_grad_calcs = 0
+        self.log = []
+        self.stop_reason = None
+
+    # configuration (src/new/Minimize.h:56-89)
+    def set_precision(self, p):
+        self.precision = p
+
+    def set_relative_precision(self, p):
+        self.relative_precision = p
+
+    def set_maxiter(self, m):
+        self.maxiter = m
+
+    def set_miniter(self, m):
+        self.miniter = m
+
+    def precondition(self, u):
+        self.use_preconditioning = bool(u)
+
+    def invalidate_cache(self):
+        self._E = None
+        self._g = None
+        self._pg = None
+
+    def energy(self):
+        if self._E is None:
+            self._E = self.f.energy()
+            self.num_energy_calcs += 1
+        return self._E
+
+    def grad(self):
+        if self._g is None:
+            self._g = self.f.grad()
+            self.num_grad_calcs += 1
+        return self._g
+
+    def pgrad(self):
+        if self._pg is None:
+            if self.use_preconditioning and self.f.have_preconditioner():
+                self.invalidate_cache()
+                E, g, pg = self.f.energy_grad_and_precond()
+                self.num_energy_calcs += 1
+                self.num_grad_calcs += 1
+                self._E, self._g, self._pg = E, g, pg
+            else:
+                self.grad()
+                self._pg = self._g
+        return self._pg
+
+    def _move(self, s):
+        """x += s*direction (invalidates nothing by itself, as in the reference)."""
+        self.f.x += s * self.direction
+
+    def improve_energy(self):
+        self.iter += 1
+        if self.iter >= self.maxiter:
+            self.stop_reason = "maxiter"
+            return False
+        E0 = self.energy()
+        old_deltaE = self.deltaE
+        if E0 != E0:
+            self.stop_reason = "nan-energy"
+            return False
+        pg = self.pgrad()
+        g = self.grad()
+        self.invalidate_cache()
+        if self.oldgrad is None:
+            beta = float(np.vdot(pg, g)) / self.oldgradsqr
+        else:
+            beta = float(np.vdot(pg, g - self.oldgrad)) / self.oldgradsqr
+        self.oldgrad = g
+        if beta < 0 or beta != beta or self.oldgradsqr == 0:
+            beta = 0.0
+        self.oldgradsqr = float(np.vdot(pg, g))
+        if self.direction is None:
+            self.direction = -pg
+        else:
+            self.direction = -pg + beta * self.direction
+        gdotd = float(np.vdot(g, self.direction))
+        if gdotd > 0:
+            self.direction = -g
+            gdotd = float(np.vdot(g, self.direction))
+        del g, pg
+
+        slope = -gdotd
+        if E0 != E0:
+            self.stop_reason = "nan-energy"
+            return False
+        if E0 >= np.finfo(np.float64).max or E0 <= -np.finfo(np.float64).max:
+            self.stop_reason = "infinite-energy"
+            return False
+        if slope != slope:
+            self.stop_reason = "nan-slope"
+            return False
+        if slope == 0:
+            self.stop_reason = "zero-slope"
+            return False
+        if self.step == self.orig_stepsize and abs(E0) > 1e-200 and abs(slope) > 1e-100:
+            self.step = 0.001 * abs(E0) / slope
+        if slope * self.step < 0:
+            self.step *= -1
+        step1 = self.step
+        self._move(step1)
+        Etried = -137.0
+        while True:
+            step1 *= 0.5
+            self._move(-step1)
+            self.invalidate_cache()
+            if self.energy() == Etried:
+                break
+            Etried = self.energy()
+            if not _better(E0, self.energy()):
+                break
+        E1 = self.energy()
+        curvature = 2.0 * (E1 - E0 + step1 * slope) / (step1 * step1)
+        step2 = slope / curvature if curvature != 0 else math.copysign(math.inf, slope) * math.copysign(1.0, curvature)
+        if curvature <= 0.0:
+            if _better(E0, E1):
+                self.invalidate_cache()
+                self._move(-step1)
+                self.stop_reason = "negative-curvature"
+                return False
+            Ebest = E0
+            while self.energy() <= Ebest:
+                Ebest = self.energy()
+                self._move(step1)
+                self.invalidate_cache()
+                step1 *= 2
+            step1 *= 0.5
+            self._move(-step1)
+            self.step = step1
+            self.invalidate_cache()
+        elif E1 == E0:
+            while True:
+                self.invalidate_cache()
+                self._move(-step1)
+                step1 *= 2
+                self._move(step1)
+                if self.energy() != E0:
+                    break
+            if self.energy() > E0:
+                self._move(-step1)
+                self.invalidate_cache()
+                step1 = 0
+            elif self.energy() != self.energy():
+                self._move(-step1)
+                self.invalidate_cache()
+                step1 = 0
+        else:
+            self.step = step2
+            self.invalidate_cache()
+            self._move(step2 - step1)
+            if _better(E1, self.energy()) and _better(E1, E0):
+                self.invalidate_cache()
+                self._move(-(step2 - step1))
+                self.step = step1
+            elif self.energy() > E0:
+                self.invalidate_cache()
+                self._move(-step2)
+                self.stop_reason = "energy-rose"
+                return False
+
+        newE = self.energy()
+        self.deltaE = newE - E0
+        w = 0.01
+        self.dEdn = (1 - w) * self.dEdn + w * self.deltaE
+        self.dEdn = max(self.deltaE, old_deltaE)
+        if self.deltaE and old_deltaE:
+            new_log = math.log(abs(self.deltaE / old_deltaE))
+        else:
+            new_log = 0.0
+        self.log_dEdn_ratio_average = (1 - w) * self.log_dEdn_ratio_average + w * new_log
+        ratio_avg = math.exp(self.log_dEdn_ratio_average)
+        if self.log_dEdn_ratio_average != 0:
+            error_guess = abs(max(self.dEdn, self.deltaE) / self.log_dEdn_ratio_average)
+        else:
+            error_guess = math.inf
+        if ratio_avg >= 1:
+            error_guess = self.precision + abs(newE)
+        self.error_estimate = 2 * error_guess
+        self.log.append((self.iter, newE, self.deltaE, self.error_estimate, self.step))
+        if self.deltaE == 0 and old_deltaE == 0:
+            self.stop_reason = "no-change"
+            return self.iter < self.miniter
+        if not (self.error_estimate < self.precision) and not (self.error_estimate < self.relative_precision * abs(newE)):
+            return True
+        self.stop_reason = "converged"
+        return self.iter < self.miniter
+
+
+# ---------------------------------------------------------------------------
+# Problem builders for the configurations this path is measured on
+# ---------------------------------------------------------------------------
+
+def minimum_image(grid, center):
+    """Distance of each grid point from ``center`` under periodic minimum image (orthorhombic cells)."""
+    pos = grid.positions() - np.asarray(center, dtype=np.float64)
+    L = np.diag(grid.cell)
+    pos = pos - L * np.round(pos / L)
+    return np.sqrt(np.sum(pos * pos, axis=-1))
+
+
+def planar_wall_vext(grid, R):
+    """V = inf for x < R (first cell axis), else 0."""
+    x = grid.positions()[..., 0]
+    return np.where(x < R, np.inf, 0.0)
+
+
+def spherical_solute_vext(grid, radius, center=(0.0, 0.0, 0.0)):
+    """V = inf for sphere centres within ``radius`` of ``center`` (periodic), else 0
+    (papers/contact/figs/inner-sphere.cpp:29-37)."""
+    r = minimum_image(grid, center)
+    return np.where(r < radius, np.inf, 0.0)
+
+
+def make_fluid(grid, eta, R=1.0, kT=1.0, vext=None):
+    """Fluid at bulk packing fraction eta, mu from the bulk equation of state, Veff = -kT ln n_b."""
+    nb = bulk_density(eta, R)
+    mu = chemical_potential(nb, R, kT)
+    if vext is None:
+        vext = np.zeros(grid.shape)
+    x = np.full(grid.shape, -kT * math.log(nb))
+    return WhiteBearFluidVeff(grid, R, kT, mu, vext, x)
+
+
+def minimize(f, relative_precision=1e-12, maxiter=100000, precondition=True, miniter=6, callback=None):
+    m = Minimize(f)
+    m.set_relative_precision(relative_precision)
+    m.set_maxiter(maxiter)
+    m.set_miniter(miniter)
+    m.precondition(precondition)
+    while m.improve_energy():
+        if callback is not None:
+            callback(m)
+    return m
