@@ -101,6 +101,12 @@ _SIGNATURES = {
     "deft_minimize_energy": ([_P, _DP], _I),
     "deft_minimize_stats_get": ([_P, ctypes.POINTER(MinimizeStats)], _I),
     "deft_minimize_invalidate": ([_P], _I),
+    "deft_minimize_set_mixed_precision": ([_P, _D, _I], _I),
+    "deft_minimize_fp32_iterations": ([_P, _IP], _I),
+    "deft_plan_set_profiling": ([_P, _I], _I),
+    "deft_plan_profile_reset": ([_P], _I),
+    "deft_kernel_count": ([], _I),
+    "deft_plan_profile_get": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong), _DP, _DP], _I),
 }
 
 
@@ -290,6 +296,25 @@ class WhiteBearFluidVeff:
         _check(lib().deft_functional_grad(self._h, None, _dptr(g), _dptr(pg), ctypes.byref(e)))
         return e.value, g, pg
 
+    def set_profiling(self, on=True):
+        _check(lib().deft_plan_set_profiling(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        _check(lib().deft_plan_profile_reset(self._h))
+
+    def profile(self):
+        """{kernel: (launches, total_ms, total_bytes)} accumulated while profiling was on."""
+        out = {}
+        for k in range(lib().deft_kernel_count()):
+            name = ctypes.c_char_p()
+            n = ctypes.c_longlong()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            _check(lib().deft_plan_profile_get(self._h, k, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms),
+                                               ctypes.byref(by)))
+            out[name.value.decode()] = (n.value, ms.value, by.value)
+        return out
+
     def weighted_densities(self):
         out = np.empty((5,) + self.shape)
         _check(lib().deft_weighted_densities(self._h, _dptr(out)))
@@ -369,6 +394,17 @@ class Minimize:
 
     def get_iteration_count(self):
         return self.stats()["iter"]
+
+    def set_mixed_precision(self, switch_relative_precision, polish_grad_precision=FFT_FP32):
+        """FP32 FFTs until this relative precision (or an FP32 stall), then FP64 line-search
+        energies with gradients at ``polish_grad_precision`` (0 disables)."""
+        _check(lib().deft_minimize_set_mixed_precision(self._h, float(switch_relative_precision),
+                                                       int(polish_grad_precision)))
+
+    def fp32_iterations(self):
+        it = ctypes.c_int()
+        _check(lib().deft_minimize_fp32_iterations(self._h, ctypes.byref(it)))
+        return it.value
 
     def invalidate_cache(self):
         _check(lib().deft_minimize_invalidate(self._h))

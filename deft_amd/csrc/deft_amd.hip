@@ -59,9 +59,13 @@ bool length_ok(int n) {
   return with_length(n, [](auto) {});
 }
 
-int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
-  if (p->prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
+int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+  if (prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
   return Launch<float>::evaluate(p, grad, have_old, precond);
+}
+
+int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
+  return evaluate_at(p, p->prec, grad, have_old, precond);
 }
 
 // Per-term energies from the reduced sums, summed in a fixed order.
@@ -89,8 +93,10 @@ int ensure_tmp(deft_plan_s *p, size_t bytes) {
 }
 
 int flush_moves(deft_plan_s *p) {
-  for (int i = 0; i < p->npend; ++i)
+  for (int i = 0; i < p->npend; ++i) {
+    ProfScope ps(p, K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[i]);
+  }
   p->npend = 0;
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
@@ -98,6 +104,7 @@ int flush_moves(deft_plan_s *p) {
 
 void queue_move(deft_plan_s *p, double s) {
   if (p->npend == 8) {
+    ProfScope ps(p, K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[0]);
     for (int i = 1; i < 8; ++i) p->pend[i - 1] = p->pend[i];
     p->npend = 7;
@@ -106,10 +113,13 @@ void queue_move(deft_plan_s *p, double s) {
 }
 
 int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
+  {
+    ProfScope ps(p, K_REDUCE, 8.0 * nb);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
+  }
   DEFT_HIP_CHECK(hipGetLastError());
   DEFT_HIP_CHECK(hipMemcpyAsync(p->hscal + slot, p->dscal + slot, sizeof(double), hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   *out = p->hscal[slot];
   return DEFT_OK;
 }
@@ -117,6 +127,10 @@ int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
 void free_plan(deft_plan_s *p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto &ev : p->event_pool) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
   (void)hipFree(p->x);
   (void)hipFree(p->d);
   (void)hipFree(p->g);
@@ -152,6 +166,12 @@ struct deft_minimize_s {
   double pgg = 0.0, pgo = 0.0;
   int num_energy = 0, num_grad = 0;
   int stop = DEFT_STOP_NONE;
+  // mixed precision: run FP32 FFTs until this relative precision (or until the
+  // FP32 line search stalls), then continue with FP64 FFTs; 0 disables
+  double switch_rp = 0.0;
+  int polish_grad_prec = DEFT_FFT_FP32;  // gradient precision after the switch
+  int e_prec = -1, g_prec = -1;          // evaluation precisions (-1: the plan's)
+  int fp32_iters = -1;  // iterations done before the switch (-1: no switch yet)
 };
 
 namespace {
@@ -161,9 +181,12 @@ void invalidate(deft_minimize_s *m) {
   m->G_valid = false;
 }
 
+int energy_prec(const deft_minimize_s *m) { return m->e_prec >= 0 ? m->e_prec : m->p->prec; }
+int grad_prec(const deft_minimize_s *m) { return m->g_prec >= 0 ? m->g_prec : m->p->prec; }
+
 int m_energy(deft_minimize_s *m, double *E) {
   if (!m->E_valid) {
-    TRY(evaluate(m->p, false, false, false));
+    TRY(evaluate_at(m->p, energy_prec(m), false, false, false));
     m->p->last_terms = terms_from_sums(m->p);
     m->E = m->p->last_terms.total;
     m->E_valid = true;
@@ -175,16 +198,26 @@ int m_energy(deft_minimize_s *m, double *E) {
 
 // Minimize::pgrad (src/new/Minimize.h:136-165): with preconditioning the
 // energy, grad and pgrad come from one evaluation; without, pgrad = grad.
+// When gradients run at a lower FFT precision than energies, the evaluation's
+// energy is not used: the cached energy of the same Veff is kept (or computed
+// at the energy precision), so the line search only ever compares energies of
+// one precision.
 int m_pgrad(deft_minimize_s *m) {
   if (m->G_valid) return DEFT_OK;
+  const bool same = energy_prec(m) == grad_prec(m);
+  double Ekeep = m->E;
+  bool keep = !same && m->E_valid;
   if (m->use_precond) invalidate(m);
-  TRY(evaluate(m->p, true, m->have_old, m->use_precond));
+  TRY(evaluate_at(m->p, grad_prec(m), true, m->have_old, m->use_precond));
   m->p->last_terms = terms_from_sums(m->p);
-  if (m->use_precond || !m->E_valid) {
+  if (same && (m->use_precond || !m->E_valid)) {
     m->E = m->p->last_terms.total;
     m->E_valid = true;
+  } else if (keep) {
+    m->E = Ekeep;
+    m->E_valid = true;
   }
-  if (m->use_precond) m->num_energy++;
+  if (m->use_precond && same) m->num_energy++;
   m->num_grad++;
   m->pgg = m->p->hscal[SLOT_PGG];
   m->pgo = m->p->hscal[SLOT_PGO];
@@ -216,15 +249,21 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   double beta = m->have_old ? (m->pgg - m->pgo) / m->oldgradsqr : m->pgg / m->oldgradsqr;
   if (beta < 0 || beta != beta || m->oldgradsqr == 0) beta = 0;
   m->oldgradsqr = m->pgg;
-  hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
+  {
+    ProfScope ps(p, K_CG_UPDATE, 40.0 * p->N);
+    hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
                      beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
+  }
   m->have_dir = true;
   m->have_old = true;
   double gdotd;
   TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
   if (gdotd > 0) {
-    hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
-                       p->pstride);
+    {
+      ProfScope ps(p, K_CG_RESET, 16.0 * p->N);
+      hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
+                         p->pstride);
+    }
     TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
     if (verbose) printf("\t\treset to gradient\n");
   }
@@ -328,12 +367,44 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
     *more = m->iter < m->miniter;
     return DEFT_OK;
   }
-  if (!(m->error_estimate < m->precision) && !(m->error_estimate < m->relative_precision * std::fabs(newE))) {
+  const double rp = (m->switch_rp > 0 && m->fp32_iters < 0) ? m->switch_rp : m->relative_precision;
+  if (!(m->error_estimate < m->precision) && !(m->error_estimate < rp * std::fabs(newE))) {
     *more = 1;
     return DEFT_OK;
   }
   m->stop = DEFT_STOP_CONVERGED;
   *more = m->iter < m->miniter;
+  return DEFT_OK;
+}
+
+// improve_energy plus the FP32 -> FP64 hand-over.  Phase 1 evaluates
+// everything with FP32 FFTs.  When it stops (its error estimate below
+// switch_rp*|E|, or its line search stalls on FP32 round-off), phase 2 computes
+// every line-search energy with FP64 FFTs while gradients stay at
+// polish_grad_prec.  The convergence history and the remembered step carry
+// over (the switch is meant to happen before FP32 round-off dominates the
+// energy differences); the CG direction restarts only if the gradient
+// precision changes.
+int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
+  TRY(improve_energy(m, verbose, more));
+  if (*more || m->switch_rp <= 0 || m->fp32_iters >= 0) return DEFT_OK;
+  const int s = m->stop;
+  if (s != DEFT_STOP_CONVERGED && s != DEFT_STOP_NEGATIVE_CURVATURE && s != DEFT_STOP_ENERGY_ROSE &&
+      s != DEFT_STOP_NO_CHANGE)
+    return DEFT_OK;
+  const int old_g = grad_prec(m);
+  m->e_prec = DEFT_FFT_FP64;
+  m->g_prec = m->polish_grad_prec;
+  invalidate(m);
+  if (m->g_prec != old_g) {
+    m->have_dir = false;
+    m->have_old = false;
+    m->oldgradsqr = 1.0;
+  }
+  m->fp32_iters = m->iter;
+  m->stop = DEFT_STOP_NONE;
+  if (verbose) printf("switching line-search energies to FP64 FFTs after %d iterations\n", m->iter);
+  *more = 1;
   return DEFT_OK;
 }
 
@@ -476,7 +547,7 @@ int deft_plan_set_vext(deft_plan p, const double *vext) {
   std::vector<float> f((size_t)p->N);
   for (long long i = 0; i < p->N; ++i) f[i] = (float)vext[i];
   DEFT_HIP_CHECK(hipMemcpyAsync(p->vext, f.data(), sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -485,7 +556,7 @@ int deft_plan_set_veff(deft_plan p, const double *veff) {
   if (!veff) return set_error(DEFT_ERR_ARG, "null veff");
   p->npend = 0;
   DEFT_HIP_CHECK(hipMemcpyAsync(p->x, veff, sizeof(double) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -494,7 +565,7 @@ int deft_plan_get_veff(deft_plan p, double *veff) {
   if (!veff) return set_error(DEFT_ERR_ARG, "null veff");
   TRY(flush_moves(p));
   DEFT_HIP_CHECK(hipMemcpyAsync(veff, p->x, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -506,7 +577,7 @@ int deft_plan_get_density(deft_plan p, double *n) {
   hipLaunchKernelGGL(k_density, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->vext, p->kT, p->tmp);
   DEFT_HIP_CHECK(hipGetLastError());
   DEFT_HIP_CHECK(hipMemcpyAsync(n, p->tmp, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -514,7 +585,7 @@ int deft_plan_veff_device_ptr(deft_plan p, double **ptr) {
   TRY(check_plan(p));
   if (!ptr) return set_error(DEFT_ERR_ARG, "null output");
   TRY(flush_moves(p));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   *ptr = p->x;
   return DEFT_OK;
 }
@@ -551,7 +622,7 @@ int deft_functional_grad_device(deft_plan p, const double *veff_dev, double *out
   if (outpgrad_dev)
     hipLaunchKernelGGL(k_accumulate, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, outpgrad_dev, p->pg);
   DEFT_HIP_CHECK(hipGetLastError());
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -564,11 +635,11 @@ int deft_functional_grad(deft_plan p, const double *veff, double *outgrad, doubl
   if (energy) *energy = p->last_terms.total;
   std::vector<double> h((size_t)p->N);
   DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->g, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   for (long long i = 0; i < p->N; ++i) outgrad[i] += h[i];
   if (outpgrad) {
     DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->pg, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    TRY(plan_sync(p));
     for (long long i = 0; i < p->N; ++i) outpgrad[i] += h[i];
   }
   return DEFT_OK;
@@ -583,7 +654,7 @@ int deft_weighted_densities(deft_plan p, double *out5) {
   else
     TRY(Launch<float>::weighted_densities(p, p->tmp));
   DEFT_HIP_CHECK(hipMemcpyAsync(out5, p->tmp, 5 * sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -595,12 +666,12 @@ int deft_fft(deft_plan p, const double *in, double *out) {
   if (p->prec == DEFT_FFT_FP64) {
     TRY(Launch<double>::fft_forward(p, p->tmp, p->dV));
     DEFT_HIP_CHECK(hipMemcpyAsync(out, p->spec, sizeof(cx<double>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
-    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    TRY(plan_sync(p));
   } else {
     TRY(Launch<float>::fft_forward(p, p->tmp, p->dV));
     std::vector<float> h(2 * (size_t)p->Nk);
     DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->spec, sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
-    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    TRY(plan_sync(p));
     for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
   }
   return DEFT_OK;
@@ -617,11 +688,11 @@ int deft_ifft(deft_plan p, const double *in, double *out) {
     std::vector<float> h(2 * (size_t)p->Nk);
     for (size_t i = 0; i < h.size(); ++i) h[i] = (float)in[i];
     DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, h.data(), sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyHostToDevice, p->stream));
-    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    TRY(plan_sync(p));
     TRY(Launch<float>::fft_inverse(p, p->tmp, 1.0 / p->volume));
   }
   DEFT_HIP_CHECK(hipMemcpyAsync(out, p->tmp, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
-  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  TRY(plan_sync(p));
   return DEFT_OK;
 }
 
@@ -708,14 +779,14 @@ int deft_minimize_precondition(deft_minimize m, int v) {
 
 int deft_minimize_improve_energy(deft_minimize m, int verbose, int *more) {
   if (!m || !more) return set_error(DEFT_ERR_ARG, "null argument");
-  return improve_energy(m, verbose, more);
+  return improve_energy_mixed(m, verbose, more);
 }
 
 int deft_minimize_run(deft_minimize m, int max_calls, int *iterations) {
   if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
   int more = 1, n = 0;
   while (more && n < max_calls) {
-    TRY(improve_energy(m, 0, &more));
+    TRY(improve_energy_mixed(m, 0, &more));
     ++n;
   }
   if (iterations) *iterations = n;
@@ -737,6 +808,55 @@ int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *s) {
   s->delta_energy = m->deltaE;
   s->error_estimate = m->error_estimate;
   s->step = m->step;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_precision, int polish_grad_precision) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (!(switch_relative_precision >= 0)) return set_error(DEFT_ERR_ARG, "switch precision must be >= 0");
+  if (polish_grad_precision != DEFT_FFT_FP32 && polish_grad_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  m->switch_rp = switch_relative_precision;
+  m->polish_grad_prec = polish_grad_precision;
+  if (switch_relative_precision > 0 && m->fp32_iters < 0) {
+    m->e_prec = DEFT_FFT_FP32;
+    m->g_prec = DEFT_FFT_FP32;
+  }
+  invalidate(m);
+  return DEFT_OK;
+}
+
+int deft_minimize_fp32_iterations(deft_minimize m, int *iterations) {
+  if (!m || !iterations) return set_error(DEFT_ERR_ARG, "null argument");
+  *iterations = m->fp32_iters;
+  return DEFT_OK;
+}
+
+int deft_plan_set_profiling(deft_plan p, int on) {
+  TRY(check_plan(p));
+  TRY(plan_sync(p));
+  p->prof = on != 0;
+  return DEFT_OK;
+}
+
+int deft_plan_profile_reset(deft_plan p) {
+  TRY(check_plan(p));
+  TRY(plan_sync(p));
+  for (auto &k : p->kstat) k = KernelStat();
+  return DEFT_OK;
+}
+
+int deft_kernel_count(void) { return NUM_KERNELS; }
+
+int deft_plan_profile_get(deft_plan p, int kernel, const char **name, long long *launches, double *total_ms,
+                          double *total_bytes) {
+  TRY(check_plan(p));
+  if (kernel < 0 || kernel >= NUM_KERNELS) return set_error(DEFT_ERR_ARG, "kernel index out of range");
+  TRY(plan_sync(p));
+  if (name) *name = kernel_name(kernel);
+  if (launches) *launches = p->kstat[kernel].launches;
+  if (total_ms) *total_ms = p->kstat[kernel].ms;
+  if (total_bytes) *total_bytes = p->kstat[kernel].bytes;
   return DEFT_OK;
 }
 

@@ -47,8 +47,10 @@ inline int x_blocks(const deft_plan_s *p) {
 // for fusion into the density pass.
 inline void take_moves(deft_plan_s *p, double &m1, double &m2, int &nm) {
   const int first = p->npend > 2 ? p->npend - 2 : 0;
-  for (int i = 0; i < first; ++i)
+  for (int i = 0; i < first; ++i) {
+    ProfScope ps(p, K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[i]);
+  }
   nm = p->npend - first;
   m1 = nm > 0 ? p->pend[first] : 0.0;
   m2 = nm > 1 ? p->pend[first + 1] : 0.0;
@@ -59,27 +61,32 @@ template <typename T>
 static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
   hipStream_t st = p->stream;
   const long long fs = p->Nk;
+  const double cb = (double)sizeof(cx<T>) * p->Nk;  // bytes of one half spectrum
   cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
   double m1, m2;
   int nm;
   take_moves(p, m1, m2, nm);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
+    ProfScope ps(p, K_ZFWD_DENSITY, (nm > 0 ? 28.0 : 12.0) * p->N + cb);
     hipLaunchKernelGGL((k_zfwd_density<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, p->x, p->d, m1, m2, nm,
                        p->vext, S0, p->partial, p->pstride, tw_ptr<T>(p, 2));
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
+    ProfScope ps(p, K_YPASS, 2 * cb);
     hipLaunchKernelGGL((k_ypass<T, L, false>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
                        tw_ptr<T>(p, 1));
   });
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
+    ProfScope ps(p, K_XCONV_FWD, 6 * cb);
     hipLaunchKernelGGL((k_xconv_fwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, S1, fs,
                        tw_ptr<T>(p, 0));
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
+    ProfScope ps(p, K_YPASS, 10 * cb);
     hipLaunchKernelGGL((k_ypass<T, L, true>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1, fs,
                        tw_ptr<T>(p, 1));
   });
@@ -90,49 +97,57 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
   const GridArgs g = grid_args(p);
   hipStream_t st = p->stream;
   const long long fs = p->Nk;
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
   cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
   launch_density_to_weighted<T>(p, g);
   if (!grad) {
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
+      ProfScope ps(p, K_ZPHI_E, 5 * cb);
       hipLaunchKernelGGL((k_zphi<T, L, false>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, fs, p->partial,
                          p->pstride, tw_ptr<T>(p, 2));
     });
+    ProfScope ps(p, K_REDUCE, 48.0 * p->zblocks);
     hipLaunchKernelGGL(k_reduce, dim3(6), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
                        p->zblocks, p->dscal);
   } else {
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
+      ProfScope ps(p, K_ZPHI_EG, 10 * cb);
       hipLaunchKernelGGL((k_zphi<T, L, true>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, fs, p->partial,
                          p->pstride, tw_ptr<T>(p, 2));
     });
     with_length(p->Ny, [&](auto LY) {
       constexpr int L = decltype(LY)::value;
+      ProfScope ps(p, K_YPASS, 10 * cb);
       hipLaunchKernelGGL((k_ypass<T, L, false>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1, fs,
                          tw_ptr<T>(p, 1));
     });
     with_length(p->Nx, [&](auto LX) {
       constexpr int L = decltype(LX)::value;
+      ProfScope ps(p, K_XCONV_BWD, 6 * cb);
       hipLaunchKernelGGL((k_xconv_bwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S1, fs, S0,
                          tw_ptr<T>(p, 0));
     });
     with_length(p->Ny, [&](auto LY) {
       constexpr int L = decltype(LY)::value;
+      ProfScope ps(p, K_YPASS, 2 * cb);
       hipLaunchKernelGGL((k_ypass<T, L, true>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
                          tw_ptr<T>(p, 1));
     });
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
+      ProfScope ps(p, K_ZGRAD, cb + (have_old ? 36.0 : 28.0) * p->N);
       hipLaunchKernelGGL((k_zgrad<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S0, p->x, p->vext, p->og,
                          have_old ? 1 : 0, precond ? 1 : 0, p->g, p->pg, p->partial, p->pstride, tw_ptr<T>(p, 2));
     });
+    ProfScope ps(p, K_REDUCE, 64.0 * p->zblocks);
     hipLaunchKernelGGL(k_reduce, dim3(8), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
                        p->zblocks, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
   DEFT_HIP_CHECK(hipMemcpyAsync(p->hscal, p->dscal, sizeof(double) * NUM_SLOTS, hipMemcpyDeviceToHost, st));
-  DEFT_HIP_CHECK(hipStreamSynchronize(st));
-  return DEFT_OK;
+  return plan_sync(p);
 }
 
 template <typename T>

@@ -6,9 +6,50 @@
 
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "../../include/deft_amd.h"
 #include "passes.hpp"
+
+namespace deft {
+
+// Kernels timed by the per-plan profiler (HIP events around each launch).
+enum KernelId : int {
+  K_ZFWD_DENSITY = 0,
+  K_YPASS,
+  K_XCONV_FWD,
+  K_ZPHI_E,
+  K_ZPHI_EG,
+  K_XCONV_BWD,
+  K_ZGRAD,
+  K_CG_UPDATE,
+  K_CG_RESET,
+  K_AXPY,
+  K_REDUCE,
+  K_FFT_SEAM,
+  NUM_KERNELS
+};
+
+inline const char *kernel_name(int id) {
+  static const char *names[NUM_KERNELS] = {"zfwd_density", "ypass", "xconv_fwd", "zphi_energy", "zphi_grad",
+                                           "xconv_bwd", "zgrad", "cg_update", "cg_reset", "axpy", "reduce",
+                                           "fft_seam"};
+  return (id >= 0 && id < NUM_KERNELS) ? names[id] : "?";
+}
+
+struct KernelStat {
+  long long launches = 0;
+  double ms = 0.0;
+  double bytes = 0.0;  // algorithmic bytes moved by all timed launches
+};
+
+struct PendingEvent {
+  hipEvent_t start, stop;
+  int id;
+  double bytes;
+};
+
+}  // namespace deft
 
 struct deft_plan_s {
   int Nx = 0, Ny = 0, Nz = 0, Nkz = 0;
@@ -41,6 +82,13 @@ struct deft_plan_s {
   int npend = 0;
 
   deft_energy_terms last_terms{};
+
+  // profiler: events recorded around launches while `prof` is set
+  bool prof = false;
+  std::vector<deft::PendingEvent> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+  size_t pool_used = 0;
+  deft::KernelStat kstat[deft::NUM_KERNELS];
   int zblocks = 0;  // blocks of the z passes
   int eblocks = 0;  // blocks of the elementwise FP64 kernels
 };
@@ -92,6 +140,50 @@ inline cx<double> *tw_ptr<double>(deft_plan_s *p, int axis) {
 }
 
 int set_error(int code, const std::string &msg);
+
+// Records HIP events around one launch when the plan's profiler is on.
+struct ProfScope {
+  deft_plan_s *p;
+  int id;
+  double bytes;
+  size_t slot = 0;
+  bool on;
+  ProfScope(deft_plan_s *p_, int id_, double bytes_) : p(p_), id(id_), bytes(bytes_), on(p_->prof) {
+    if (!on) return;
+    if (p->pool_used == p->event_pool.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        on = false;
+        return;
+      }
+      p->event_pool.push_back({a, b});
+    }
+    slot = p->pool_used++;
+    (void)hipEventRecord(p->event_pool[slot].first, p->stream);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(p->event_pool[slot].second, p->stream);
+    p->pending.push_back({p->event_pool[slot].first, p->event_pool[slot].second, id, bytes});
+  }
+};
+
+// Synchronises the plan's stream and folds finished profiler events into the stats.
+inline int plan_sync(deft_plan_s *p) {
+  const hipError_t e = hipStreamSynchronize(p->stream);
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  for (const PendingEvent &pe : p->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.start, pe.stop) == hipSuccess) {
+      p->kstat[pe.id].launches += 1;
+      p->kstat[pe.id].ms += ms;
+      p->kstat[pe.id].bytes += pe.bytes;
+    }
+  }
+  p->pending.clear();
+  p->pool_used = 0;
+  return DEFT_OK;
+}
 
 // Kernel launch sequences, instantiated for T = float and T = double in
 // separate translation units.

@@ -171,6 +171,32 @@ int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *stats);
 /* Drops cached energy/gradient (after the caller changed Veff or the plan). */
 int deft_minimize_invalidate(deft_minimize m);
 
+/* --- mixed precision (no counterpart in the reference, which is FP64 only) ---
+ * With switch_relative_precision > 0 the minimizer first evaluates energies
+ * and gradients with FP32 FFTs until its error estimate falls below
+ * switch_relative_precision*|E| or the FP32 line search stalls; from then on
+ * every line-search energy uses FP64 FFTs and gradients use
+ * polish_grad_precision (DEFT_FFT_FP32 keeps the CG direction, DEFT_FFT_FP64
+ * restarts it).  The convergence history carries over and the minimizer
+ * continues to its own precision.  Switch while FP32 energy differences are
+ * still well above FP32 round-off (about 1e-7 |E|): 1e-6 is a good value.  The plan's precision setting is
+ * not changed.  0 (default) disables the switch. */
+int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_precision, int polish_grad_precision);
+/* Iterations run in FP32 before the switch, -1 if no switch happened. */
+int deft_minimize_fp32_iterations(deft_minimize m, int *iterations);
+
+/* --- profiling (the reference only has ad-hoc clock() timers,
+ * src/new/Minimize.h:95-135) ------------------------------------------------
+ * With profiling on, every kernel launch of the plan is bracketed by HIP
+ * events on the plan's stream; per kernel family the library accumulates the
+ * launch count, the summed event time and the algorithmic bytes (compulsory
+ * reads + writes of that pass). */
+int deft_plan_set_profiling(deft_plan plan, int on);
+int deft_plan_profile_reset(deft_plan plan);
+int deft_kernel_count(void);
+int deft_plan_profile_get(deft_plan plan, int kernel, const char **name, long long *launches, double *total_ms,
+                          double *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,203 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: FP64 FFT mode must agree with the oracle to round-off
+(relative 1e-12 on energies, 1e-11 of the field scale on fields); FP32 FFT mode
+to FP32 round-off of the transforms (relative 2e-6 on fields, 1e-7 on energies).
+"""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import fmt_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _fluid(dm, shape, cell, eta=0.3, vext=None, prec=1, seed=None, amp=0.3):
+    g = O.Grid(shape, cell)
+    of = O.make_fluid(g, eta, vext=vext)
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        of.x += amp * rng.standard_normal(g.shape)
+    f = dm.WhiteBearFluidVeff(shape, cell, R=of.R, kT=of.kT, mu=of.mu, fft_precision=prec)
+    f.Vext = of.vext
+    f.Veff = of.x
+    return g, of, f
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+SHAPES = [((16, 16, 16), np.diag([3.2, 3.2, 3.2])),
+          ((32, 48, 20), np.diag([3.1, 4.9, 2.3])),
+          ((20, 16, 32), np.array([[2.0, 0.0, 0.1], [0.2, 1.9, 0.0], [0.0, 0.3, 3.4]])),
+          ((64, 64, 64), np.diag([6.4, 6.4, 6.4])),
+          ((96, 32, 128), np.diag([9.6, 3.2, 12.8]))]
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 3e-6)])
+@pytest.mark.parametrize("shape,cell", SHAPES[:4])
+def test_fft_seam_matches_numpy(gpu, shape, cell, prec, tol):
+    print("fft", shape, prec, flush=True)
+    g = O.Grid(shape, cell)
+    f = gpu.WhiteBearFluidVeff(shape, cell, fft_precision=prec)
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal(shape)
+    F = f.fft(a)
+    ref = O.fft(g, a)
+    assert _rel(F, ref) < tol
+    back = f.ifft(ref)
+    assert _rel(back, a) < tol
+    # round trip of the reference seam: ifft(fft(f)) = f (tests/new-fftinverse.cpp:21-43)
+    assert _rel(f.ifft(F), a) < 2 * tol
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 3e-6)])
+@pytest.mark.parametrize("shape,cell", SHAPES)
+def test_weighted_densities(gpu, shape, cell, prec, tol):
+    print("weighted", shape, prec, flush=True)
+    g, of, f = _fluid(gpu, shape, cell, prec=prec, seed=3)
+    got = f.weighted_densities()
+    ref = of.weighted_densities(of.density())
+    for a in range(5):
+        assert _rel(got[a], ref[a]) < tol, a
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+@pytest.mark.parametrize("shape,cell", SHAPES)
+def test_energy_and_gradient(gpu, shape, cell, prec, etol, gtol):
+    print("energy/grad", shape, prec, flush=True)
+    vext = None
+    g = O.Grid(shape, cell)
+    if shape[0] >= 32:
+        vext = O.planar_wall_vext(g, 1.0)
+        vext = np.where(np.isfinite(vext), 0.5 * np.sin(g.positions()[..., 1]), vext).astype(np.float32).astype(np.float64)
+    g, of, f = _fluid(gpu, shape, cell, vext=vext, prec=prec, seed=11)
+    E = f.energy()
+    Eo = of.energy()
+    assert abs(E - Eo) <= etol * abs(Eo)
+    for k, v in of.last_terms.items():
+        assert abs(f.last_terms[k] - v) <= etol * abs(Eo) + 1e-300, k
+    E2, gr, pg = f.energy_grad_and_precond()
+    Eo2, gro, pgo = of.energy_grad_and_precond()
+    assert abs(E2 - Eo2) <= etol * abs(Eo2)
+    assert _rel(gr, gro) < gtol
+    assert _rel(pg, pgo) < gtol
+    if vext is not None:
+        wall = ~np.isfinite(vext)
+        assert np.all(gr[wall] == 0) and np.all(pg[wall] == 0)
+
+
+def test_white_bear_known_answer(gpu):
+    """tests/new-hard-spheres.cpp:52-63: uniform n = 0.1, R = 1, cell 2^3, Nx = 20 -> 2.72225468848892."""
+    for prec, tol in ((1, 2e-11), (0, 1e-6)):
+        f = gpu.WhiteBearFluidVeff((20, 20, 20), np.diag([2.0, 2.0, 2.0]), R=1.0, kT=1.0, mu=0.0, fft_precision=prec)
+        f.Veff = -math.log(0.1)
+        f.energy()
+        t = f.last_terms
+        wb = t["kTphi1"] + t["kTphi2"] + t["kTphi3"]
+        assert abs(wb / 2.72225468848892 - 1) < tol
+
+
+def test_uniform_fluid_is_bulk(gpu):
+    """Omega = -p V and zero gradient for the bulk density at the bulk mu."""
+    eta = 0.4
+    nb = O.bulk_density(eta, 1.0)
+    mu = gpu.find_chemical_potential(nb)
+    assert abs(mu - O.chemical_potential(nb, 1.0)) < 1e-12 * abs(mu)
+    p = gpu.pressure(nb)
+    assert abs(p - O.pressure(nb, 1.0)) < 1e-12 * p
+    f = gpu.WhiteBearFluidVeff((32, 32, 32), np.diag([3.2] * 3), mu=mu, fft_precision=1)
+    f.Veff = -math.log(nb)
+    E, gr, pg = f.energy_grad_and_precond()
+    V = 3.2 ** 3
+    assert abs(E + p * V) < 1e-12 * p * V
+    assert np.max(np.abs(pg)) < 1e-12
+
+
+def _trajectory(gpu, prec):
+    shape, cell = (32, 32, 32), np.diag([3.2, 3.2, 3.2])
+    g = O.Grid(shape, cell)
+    vext = O.planar_wall_vext(g, 1.0)
+    g, of, f = _fluid(gpu, shape, cell, eta=0.3, vext=vext, prec=prec)
+    mo = O.Minimize(of)
+    mo.set_relative_precision(1e-12)
+    mo.precondition(True)
+    m = gpu.Minimize(f)
+    m.set_relative_precision(1e-12)
+    m.precondition(True)
+    return of, mo, f, m
+
+
+def test_cg_trajectory_fp64(gpu):
+    """FP64 mode follows the oracle's Minimize iterate by iterate."""
+    of, mo, f, m = _trajectory(gpu, 1)
+    for it in range(12):
+        a = mo.improve_energy()
+        b = m.improve_energy()
+        assert a == b
+        assert abs(m.energy() - mo.energy()) <= 1e-11 * abs(mo.energy()), it
+        if not a:
+            break
+    s = m.stats()
+    assert s["iter"] == mo.iter
+    assert abs(s["step"] - mo.step) <= 1e-6 * abs(mo.step)
+    assert _rel(f.Veff, of.x) < 1e-9
+
+
+@pytest.mark.parametrize("polish_grad", [0, 1])
+def test_cg_mixed_precision_reaches_fp64_answer(gpu, polish_grad):
+    """FP32 phase, then FP64 line-search energies (FP32 or FP64 gradients): |dF/F| < 1e-9."""
+    of, mo, f, m = _trajectory(gpu, 0)
+    while mo.improve_energy():
+        pass
+    m.set_relative_precision(1e-11)
+    m.set_mixed_precision(1e-6, polish_grad)
+    m.run()
+    assert m.fp32_iterations() > 0
+    assert f.fft_precision == 0  # the plan's own setting is untouched
+    assert abs(m.energy() / mo.energy() - 1) < 1e-9
+    # the reported energy is an FP64 energy of the final Veff
+    f.fft_precision = 1
+    assert abs(f.energy() / m.energy() - 1) < 1e-14
+
+
+@pytest.mark.slow
+def test_c1_wall_matches_oracle_golden(gpu):
+    gold = json.load(open(os.path.join(GOLDEN, "oracle_c1.json")))
+    g = O.Grid.cubic(64, 6.4)
+    vext = O.planar_wall_vext(g, 1.0)
+    g, of, f = _fluid(gpu, (64, 64, 64), np.diag([6.4] * 3), eta=0.3, vext=vext, prec=1)
+    m = gpu.Minimize(f)
+    m.set_relative_precision(1e-12)
+    m.precondition(True)
+    m.run()
+    E = m.energy()
+    print("c1", E, gold["energy"], m.stats(), flush=True)
+    assert abs(E / gold["energy"] - 1) < 1e-9
+    n = f.density()
+    assert np.max(np.abs(n[:, 0, 0] - np.array(gold["density_x_axis"]))) < 1e-6 * gold["density_max"]
+
+
+def test_bad_sizes_raise(gpu):
+    with pytest.raises(gpu.DeftError):
+        gpu.WhiteBearFluidVeff((18, 16, 16), np.diag([1.0] * 3))
+    with pytest.raises(gpu.DeftError):
+        gpu.WhiteBearFluidVeff((15, 16, 16), np.diag([1.0] * 3))
+
+
+def test_nan_veff_stops_minimizer(gpu):
+    f = gpu.WhiteBearFluidVeff((16, 16, 16), np.diag([3.2] * 3), mu=0.0, fft_precision=1)
+    v = np.full((16, 16, 16), 2.0)
+    v[3, 4, 5] = np.nan
+    f.Veff = v
+    m = gpu.Minimize(f)
+    assert m.improve_energy() is False
+    assert m.stats()["stop_reason"] == "nan-energy"

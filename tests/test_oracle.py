@@ -1,0 +1,269 @@
+"""Pins the FP64 oracle against the reference's known answers and against the mathematics.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign or
+normalisation, a transposed index) fails it.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fmt_oracle as O
+
+
+def test_white_bear_known_answer():
+    """tests/new-hard-spheres.cpp:52-63: uniform n = 0.1, R = 1, kT = 1, cell 2^3, Nx = 20."""
+    g = O.Grid.cubic(20, 2.0)
+    f = O.WhiteBearFluidVeff(g, 1.0, 1.0, 0.0, np.zeros(g.shape), np.full(g.shape, -math.log(0.1)))
+    f.energy()
+    t = f.last_terms
+    wb = t["kTphi1"] + t["kTphi2"] + t["kTphi3"]
+    assert abs(wb / 2.72225468848892 - 1) < 2e-11
+    # homogeneous value = energy / a^3 (tests/new-hard-spheres.cpp:73-77)
+    assert abs(O.homogeneous_whitebear(0.1, 1.0) / (2.72225468848892 / 8) - 1) < 2e-11
+
+
+@pytest.mark.parametrize("eta", [0.1, 0.3, 0.4, 0.45])
+def test_bulk_reduces_to_carnahan_starling(eta):
+    """White-Bear's bulk limit is Carnahan-Starling: f_ex, mu_ex and p in closed form."""
+    n = O.bulk_density(eta, 1.0)
+    f_cs = n * (4 * eta - 3 * eta ** 2) / (1 - eta) ** 2
+    mu_cs = (8 * eta - 9 * eta ** 2 + 3 * eta ** 3) / (1 - eta) ** 3
+    p_cs = n * (1 + eta + eta ** 2 - eta ** 3) / (1 - eta) ** 3
+    assert abs(O.homogeneous_whitebear(n, 1.0) / f_cs - 1) < 1e-13
+    assert abs((O.chemical_potential(n, 1.0) - math.log(n / O.nQ(1.0))) / mu_cs - 1) < 1e-13
+    assert abs(O.pressure(n, 1.0) / p_cs - 1) < 1e-12
+
+
+def test_golden_numbers_of_the_survey():
+    n3, n4 = O.bulk_density(0.3, 1.0), O.bulk_density(0.4, 1.0)
+    assert abs(O.homogeneous_whitebear(n3, 1.0) - 0.135931313640731) < 1e-14
+    assert abs(O.homogeneous_whitebear(n4, 1.0) - 0.297089227104872) < 1e-14
+    assert abs(O.chemical_potential(n3, 1.0) - math.log(n3 / O.nQ(1.0)) - 4.87172011661808) < 1e-12
+    assert abs(O.chemical_potential(n4, 1.0) - math.log(n4 / O.nQ(1.0)) - 9.03703703703704) < 1e-12
+    assert abs(O.pressure(n3, 1.0) - 0.284599662814618) < 1e-13
+    assert abs(O.pressure(n4, 1.0) - 0.661377207959654) < 1e-13
+    # 1/nQ at kT = 1 (src/new/SW_liquidFast.cpp:47)
+    assert abs(1 / O.nQ(1.0) / 2.646476976618268e-6 - 1) < 1e-14
+
+
+def test_fft_convention_against_brute_force_dft():
+    """fft = dV * sum_r f(r) exp(-i k.r) on the half spectrum; ifft = (1/V) sum_k (Hermitian)."""
+    g = O.Grid((4, 6, 8), np.diag([1.3, 2.1, 0.7]))
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(g.shape)
+    F = O.fft(g, f)
+    Nx, Ny, Nz = g.shape
+    x = np.arange(Nx)[:, None, None]
+    y = np.arange(Ny)[None, :, None]
+    z = np.arange(Nz)[None, None, :]
+    for (a, b, c) in [(0, 0, 0), (1, 2, 3), (3, 5, 4), (2, 1, 0)]:
+        ph = np.exp(-2j * np.pi * (a * x / Nx + b * y / Ny + c * z / Nz))
+        assert abs(F[a, b, c] - g.dV * np.sum(f * ph)) < 1e-13
+    back = O.ifft(g, F)
+    assert np.max(np.abs(back - f)) < 1e-14
+
+
+def test_fft_round_trip_reference_size():
+    """tests/new-fftinverse.cpp:21-43: 100 x 98 x 96, ifft(fft(f)) = f to 1e-15-ish."""
+    g = O.Grid((100, 98, 96), np.diag([5.0, 4.9, 4.8]))
+    rng = np.random.default_rng(1)
+    f = rng.random(g.shape)
+    assert np.max(np.abs(O.ifft(g, O.fft(g, f)) - f)) < 1e-13
+
+
+def test_weights_at_k_zero_and_series_branch():
+    g = O.Grid.cubic(16, 3.2)
+    for R in (0.7, 1.0, 2.0):
+        w3, w2, wv = O.weights(g, R)
+        assert abs(w3[0, 0, 0] - 4 * math.pi / 3 * R ** 3) < 1e-14 * R ** 3
+        assert abs(w2[0, 0, 0] - 4 * math.pi * R * R) < 1e-14 * R * R
+        # the vector weight is odd: i wv k_j vanishes at k = 0
+        assert abs(wv[0, 0, 0] * g.kx[0, 0, 0]) == 0.0
+    # series vs closed form on both sides of kR = 1e-3 (src/ReciprocalOperators.h:98-116)
+
+    class _G:
+        pass
+
+    for kR in (0.999e-3, 1.001e-3):
+        gg = _G()
+        gg.k = np.array([kR])
+        gg.dV = 1e-30  # no smearing
+        w3, w2, wv = O.weights(gg, 1.0)
+        k = kR
+        exact3 = 4 * math.pi * (math.sin(k) - k * math.cos(k)) / k ** 3
+        exact2 = 4 * math.pi * math.sin(k) / k
+        exactv = 4 * math.pi * (math.cos(k) - math.sin(k) / k) / k ** 2
+        assert abs(w3[0] / exact3 - 1) < 1e-9
+        assert abs(w2[0] / exact2 - 1) < 1e-12
+        assert abs(wv[0] / exactv - 1) < 1e-6
+
+
+def test_weight_integrals():
+    """tests/convolve.cpp:79-266: int step(f) = 4 pi R^3/3 int f, int shell(f) = 4 pi R^2 int f,
+    vector shell integrates to 0 and |n2v| <= n2."""
+    g = O.Grid.cubic(48, 6.0)
+    r = O.minimum_image(g, (3.0, 3.0, 3.0))
+    f = np.exp(-4.0 * r * r)
+    fk = np.fft.rfftn(f)
+    w3, w2, wv = O.weights(g, 1.0)
+    n3 = np.fft.irfftn(fk * w3, s=g.shape, axes=(0, 1, 2))
+    n2 = np.fft.irfftn(fk * w2, s=g.shape, axes=(0, 1, 2))
+    vx = np.fft.irfftn(fk * 1j * wv * g.kx, s=g.shape, axes=(0, 1, 2))
+    vy = np.fft.irfftn(fk * 1j * wv * g.ky, s=g.shape, axes=(0, 1, 2))
+    vz = np.fft.irfftn(fk * 1j * wv * g.kz, s=g.shape, axes=(0, 1, 2))
+    I = f.sum()
+    assert abs(n3.sum() / I - 4 * math.pi / 3) < 1e-10
+    assert abs(n2.sum() / I - 4 * math.pi) < 1e-10
+    assert abs(vx.sum()) < 1e-10 * I and abs(vy.sum()) < 1e-10 * I
+    assert np.all(np.sqrt(vx ** 2 + vy ** 2 + vz ** 2) <= n2 + 1e-9 * n2.max())
+    # the vector weight points from the blob towards the evaluation point: n2v_x < 0 for x > centre
+    # (w2v = i wv k, wv < 0 for small k: the gradient of the shell-smeared blob)
+    x = g.positions()[..., 0]
+    sel = (np.abs(x - 4.0) < 0.05) & (r < 1.2)
+    assert np.all(vx[sel] > 0) or np.all(vx[sel] < 0)
+
+
+def test_convolution_adjoint_parity():
+    """<a, W b> = <W~ a, b>, W~(k) = W(-k): even weights are self-adjoint, odd ones flip sign."""
+    g = O.Grid((16, 20, 32), np.diag([3.0, 3.5, 4.0]))
+    rng = np.random.default_rng(2)
+    a, b = rng.standard_normal(g.shape), rng.standard_normal(g.shape)
+    w3, w2, wv = O.weights(g, 1.0)
+    conv = lambda W, f: np.fft.irfftn(np.fft.rfftn(f) * W, s=g.shape, axes=(0, 1, 2))
+    for W, par in ((w3, 1), (w2, 1), (1j * wv * g.kx, -1), (1j * wv * g.kz, -1)):
+        lhs = np.sum(a * conv(W, b))
+        rhs = np.sum(conv(par * W, a) * b)
+        assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + 1)
+        if par < 0:  # and the sign does matter
+            assert abs(np.sum(conv(W, a) * b) + lhs) < 1e-12 * (abs(lhs) + 1)
+
+
+def test_f3_series_matches_closed_form_and_derivative():
+    ns = np.array([-0.04, -1e-3, 1e-8, 1e-3, 0.02, 0.0499, 0.0501, 0.2, 0.45])
+    f3, df3 = O.f3_and_derivative(ns)
+    # closed form where it is well conditioned
+    for n, v in zip(ns, f3):
+        if abs(n) >= 0.01:
+            L = math.log1p(-n)
+            exact = (n + (1 - n) ** 2 * L) / (n * n * (1 - n) ** 2)
+            assert abs(v / exact - 1) < 1e-12
+    assert abs(O.f3_and_derivative(np.array([0.0]))[0][0] - 1.5) < 1e-15
+    assert abs(O.f3_and_derivative(np.array([0.0]))[1][0] - 8.0 / 3.0) < 1e-14
+    h = 1e-6
+    for n, d in zip(ns, df3):
+        fp = O.f3_and_derivative(np.array([n + h]))[0][0]
+        fm = O.f3_and_derivative(np.array([n - h]))[0][0]
+        assert abs((fp - fm) / (2 * h) - d) < 1e-7 * max(1, abs(d))
+
+
+def test_phi_partials_by_finite_differences():
+    rng = np.random.default_rng(4)
+    R = 1.0
+    for _ in range(20):
+        n3 = rng.uniform(0.01, 0.5)
+        n2 = rng.uniform(0.1, 3.0)
+        v = rng.uniform(-0.5, 0.5, 3)
+        args = [n3, n2, *v]
+        _, _, _, P = O.whitebear_phi(*[np.array([a]) for a in args], R)
+        for i in range(5):
+            h = 1e-6
+            ap, am = list(args), list(args)
+            ap[i] += h
+            am[i] -= h
+            fp = sum(O.whitebear_phi(*[np.array([a]) for a in ap], R, want_partials=False)[:3])
+            fm = sum(O.whitebear_phi(*[np.array([a]) for a in am], R, want_partials=False)[:3])
+            assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0]))
+
+
+def test_gradient_by_finite_differences():
+    """Central differences along a random direction (src/Functional.cpp:159-212)."""
+    g = O.Grid((16, 16, 20), np.diag([3.2, 3.0, 3.4]))
+    rng = np.random.default_rng(1)
+    vext = O.planar_wall_vext(g, 1.0)
+    vext = np.where(np.isfinite(vext), 0.3 * np.cos(g.positions()[..., 2]), vext)
+    f = O.make_fluid(g, 0.3, vext=vext)
+    f.x += 0.3 * rng.standard_normal(g.shape)
+    E, gr, pg = f.energy_grad_and_precond()
+    d = rng.standard_normal(g.shape)
+    eps = 1e-4
+    fd = (f.energy(f.x + eps * d) - f.energy(f.x - eps * d)) / (2 * eps)
+    assert abs(fd - np.vdot(gr, d)) < 1e-7 * abs(fd)
+    n = f.density()
+    inside = np.isfinite(vext)
+    assert np.allclose(pg[inside] * n[inside], gr[inside], rtol=1e-13, atol=0)
+    assert np.all(gr[~inside] == 0) and np.all(pg[~inside] == 0)
+
+
+def test_uniform_fluid_is_bulk():
+    eta = 0.4
+    g = O.Grid.cubic(16, 3.2)
+    f = O.make_fluid(g, eta)
+    E, gr, pg = f.energy_grad_and_precond()
+    p = O.pressure(O.bulk_density(eta, 1.0), 1.0)
+    assert abs(E + p * g.volume) < 1e-12 * p * g.volume
+    assert np.max(np.abs(pg)) < 1e-12
+
+
+class SqrSum:
+    """tests/newcode.cpp:9-44: E = sum_i (i+1) x_i^2, preconditioner grad/(i+1)."""
+
+    def __init__(self, x):
+        self.x = np.array(x, dtype=np.float64)
+        self.k = np.arange(1, len(x) + 1, dtype=np.float64)
+
+    def energy(self):
+        return float(np.sum(self.k * self.x * self.x))
+
+    def grad(self):
+        return 2 * self.k * self.x
+
+    def energy_grad_and_precond(self):
+        g = self.grad()
+        return self.energy(), g, g / self.k
+
+    def have_preconditioner(self):
+        return True
+
+
+def test_minimize_exact_minimum_of_quadratic():
+    """tests/newcode.cpp:46-70: the default minimizer reaches E <= 1e-300."""
+    f = SqrSum(np.arange(1000) * 0.1)
+    m = O.Minimize(f)
+    while m.improve_energy():
+        pass
+    assert m.energy() <= 1e-300
+
+
+def test_minimize_precision_and_iteration_bound():
+    """tests/newcode.cpp:72-98: precision 1e-9 within N = 1000 iterations."""
+    f = SqrSum(np.arange(1000) * 0.1)
+    m = O.Minimize(f)
+    m.set_precision(1e-9)
+    m.set_miniter(0)
+    while m.improve_energy():
+        pass
+    assert m.energy() <= 1e-9
+    assert m.iter <= 1000
+
+
+def test_minimize_preconditioned_three_iterations():
+    """tests/newcode.cpp:100-132: exact preconditioning converges within 3 iterations."""
+    f = SqrSum(np.arange(1000) * 0.1)
+    m = O.Minimize(f)
+    m.set_precision(1e-9)
+    m.set_miniter(0)
+    m.precondition(True)
+    while m.improve_energy():
+        pass
+    assert m.energy() <= 1e-9
+    assert m.iter <= 3
+
+
+def test_minimize_stops_on_nan():
+    """src/new/Minimize.cpp:22-30: a NaN energy stops the minimizer."""
+    f = SqrSum([1.0, float("nan"), 2.0])
+    m = O.Minimize(f)
+    assert m.improve_energy() is False
+    assert m.stop_reason == "nan-energy"
