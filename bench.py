@@ -61,7 +61,7 @@ def gpu_run(steps, warmup):
         m.run()
         dt = time.perf_counter() - t0
         s = m.stats()
-        s["fp32_iterations"] = m.fp32_iterations()
+        s["fp32_iterations"], s["mixed_iterations"], _ = m.phase_iterations()
         s["seconds"] = dt
         s["energy"] = m.energy()
         m.close()
@@ -115,7 +115,20 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against a known byte count)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the minimizations of this workload are independent "
               f"and shard by instance); --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -139,6 +152,7 @@ def main():
                    "grid": [N, N, N], "cell": L, "eta": ETA, "R": R,
                    "switch_relative_precision": SWITCH_RP, "relative_precision": FINAL_RP,
                    "iterations": r0["iter"], "fp32_iterations": r0["fp32_iterations"],
+                   "fp64_energy_fp32_grad_iterations": r0["mixed_iterations"],
                    "energy_calcs": r0["num_energy_calcs"], "grad_calcs": r0["num_grad_calcs"],
                    "dF_over_F_vs_fp64_oracle": err, "max_dn_over_nmax_x_axis": dn,
                    "oracle_iterations": gold["iterations"]},

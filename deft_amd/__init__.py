@@ -103,6 +103,7 @@ _SIGNATURES = {
     "deft_minimize_invalidate": ([_P], _I),
     "deft_minimize_set_mixed_precision": ([_P, _D, _I], _I),
     "deft_minimize_fp32_iterations": ([_P, _IP], _I),
+    "deft_minimize_phase_iterations": ([_P, _IP, _IP, _IP], _I),
     "deft_plan_set_profiling": ([_P, _I], _I),
     "deft_plan_profile_reset": ([_P], _I),
     "deft_kernel_count": ([], _I),
@@ -405,6 +406,12 @@ class Minimize:
         it = ctypes.c_int()
         _check(lib().deft_minimize_fp32_iterations(self._h, ctypes.byref(it)))
         return it.value
+
+    def phase_iterations(self):
+        """(phase-0 FP32 iterations, phase-1 mixed iterations, current phase)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().deft_minimize_phase_iterations(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def invalidate_cache(self):
         _check(lib().deft_minimize_invalidate(self._h))

@@ -33,6 +33,7 @@ int set_error(int code, const std::string &msg) {
 namespace {
 
 constexpr int kVersion = 1;
+constexpr int kConfirmIterations = 3;
 
 bool det3(const double *c, double &det) {
   det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
@@ -94,7 +95,7 @@ int ensure_tmp(deft_plan_s *p, size_t bytes) {
 
 int flush_moves(deft_plan_s *p) {
   for (int i = 0; i < p->npend; ++i) {
-    ProfScope ps(p, K_AXPY, 24.0 * p->N);
+    ProfScope ps(p, 2 * K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[i]);
   }
   p->npend = 0;
@@ -104,7 +105,7 @@ int flush_moves(deft_plan_s *p) {
 
 void queue_move(deft_plan_s *p, double s) {
   if (p->npend == 8) {
-    ProfScope ps(p, K_AXPY, 24.0 * p->N);
+    ProfScope ps(p, 2 * K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[0]);
     for (int i = 1; i < 8; ++i) p->pend[i - 1] = p->pend[i];
     p->npend = 7;
@@ -114,11 +115,10 @@ void queue_move(deft_plan_s *p, double s) {
 
 int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
   {
-    ProfScope ps(p, K_REDUCE, 8.0 * nb);
+    ProfScope ps(p, 2 * K_REDUCE, 8.0 * nb);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
-  DEFT_HIP_CHECK(hipMemcpyAsync(p->hscal + slot, p->dscal + slot, sizeof(double), hipMemcpyDeviceToHost, p->stream));
   TRY(plan_sync(p));
   *out = p->hscal[slot];
   return DEFT_OK;
@@ -139,7 +139,6 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->vext);
   (void)hipFree(p->spec);
   (void)hipFree(p->partial);
-  (void)hipFree(p->dscal);
   (void)hipFree(p->twbuf);
   (void)hipFree(p->tmp);
   if (p->hscal) (void)hipHostFree(p->hscal);
@@ -171,7 +170,10 @@ struct deft_minimize_s {
   double switch_rp = 0.0;
   int polish_grad_prec = DEFT_FFT_FP32;  // gradient precision after the switch
   int e_prec = -1, g_prec = -1;          // evaluation precisions (-1: the plan's)
-  int fp32_iters = -1;  // iterations done before the switch (-1: no switch yet)
+  int fp32_iters = -1;      // iterations done in phase 0 (-1: no switch yet)
+  int iter_at_phase2 = -1;  // iteration count when phase 2 began (-1: not reached)
+  int phase = 0;            // precision phase of the mixed ladder
+  int confirm_until = 0;    // the final phase runs at least until this iteration
 };
 
 namespace {
@@ -250,7 +252,7 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   if (beta < 0 || beta != beta || m->oldgradsqr == 0) beta = 0;
   m->oldgradsqr = m->pgg;
   {
-    ProfScope ps(p, K_CG_UPDATE, 40.0 * p->N);
+    ProfScope ps(p, 2 * K_CG_UPDATE, 40.0 * p->N);
     hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
                      beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
   }
@@ -260,7 +262,7 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
   if (gdotd > 0) {
     {
-      ProfScope ps(p, K_CG_RESET, 16.0 * p->N);
+      ProfScope ps(p, 2 * K_CG_RESET, 16.0 * p->N);
       hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
                          p->pstride);
     }
@@ -362,50 +364,83 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   if (verbose)
     printf("iter %4d  E = %.15g  dE = %.3e  err = %.3e  step = %.3e  (#E %d, #G %d)\n", m->iter, newE, m->deltaE,
            m->error_estimate, m->step, m->num_energy, m->num_grad);
+  const int miniter = std::max(m->miniter, m->confirm_until);
   if (m->deltaE == 0 && old_deltaE == 0) {
     m->stop = DEFT_STOP_NO_CHANGE;
-    *more = m->iter < m->miniter;
+    *more = m->iter < miniter;
     return DEFT_OK;
   }
-  const double rp = (m->switch_rp > 0 && m->fp32_iters < 0) ? m->switch_rp : m->relative_precision;
+  const double rp = (m->switch_rp > 0 && m->phase == 0) ? m->switch_rp : m->relative_precision;
   if (!(m->error_estimate < m->precision) && !(m->error_estimate < rp * std::fabs(newE))) {
     *more = 1;
     return DEFT_OK;
   }
   m->stop = DEFT_STOP_CONVERGED;
-  *more = m->iter < m->miniter;
+  *more = m->iter < miniter;
   return DEFT_OK;
 }
 
-// improve_energy plus the FP32 -> FP64 hand-over.  Phase 1 evaluates
-// everything with FP32 FFTs.  When it stops (its error estimate below
-// switch_rp*|E|, or its line search stalls on FP32 round-off), phase 2 computes
-// every line-search energy with FP64 FFTs while gradients stay at
-// polish_grad_prec.  The convergence history and the remembered step carry
-// over (the switch is meant to happen before FP32 round-off dominates the
-// energy differences); the CG direction restarts only if the gradient
-// precision changes.
+// improve_energy plus the FP32 -> FP64 precision ladder:
+//   phase 0: energies and gradients with FP32 FFTs;
+//   phase 1: line-search energies with FP64 FFTs, gradients still FP32 (the
+//            CG direction and convergence history carry over);
+//   phase 2: everything FP64 (the CG direction restarts).
+// A phase ends whenever the reference algorithm would stop (converged at the
+// phase's precision, or a stalled line search); NaN, infinity and the
+// iteration cap stop the minimizer at any phase.  Only phase 2 may end the
+// minimization, so the final answer is always an FP64-converged one.
+// Phase 1 is skipped when the polishing gradient precision is FP64.
 int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   TRY(improve_energy(m, verbose, more));
-  if (*more || m->switch_rp <= 0 || m->fp32_iters >= 0) return DEFT_OK;
+  if (*more || m->switch_rp <= 0 || m->phase >= 2) return DEFT_OK;
   const int s = m->stop;
   if (s != DEFT_STOP_CONVERGED && s != DEFT_STOP_NEGATIVE_CURVATURE && s != DEFT_STOP_ENERGY_ROSE &&
       s != DEFT_STOP_NO_CHANGE)
     return DEFT_OK;
   const int old_g = grad_prec(m);
+  if (m->phase == 0) {
+    m->fp32_iters = m->iter;
+    m->phase = (m->polish_grad_prec == DEFT_FFT_FP32) ? 1 : 2;
+  } else {
+    m->phase = 2;
+  }
+  if (m->phase == 2) {
+    m->iter_at_phase2 = m->iter;
+    // a phase may hand over on one spuriously small energy change (the error
+    // estimate trusts a single iteration); the all-FP64 phase has to confirm
+    // convergence over a few iterations of its own
+    m->confirm_until = m->iter + kConfirmIterations;
+  }
   m->e_prec = DEFT_FFT_FP64;
-  m->g_prec = m->polish_grad_prec;
+  m->g_prec = (m->phase == 1) ? DEFT_FFT_FP32 : DEFT_FFT_FP64;
   invalidate(m);
   if (m->g_prec != old_g) {
     m->have_dir = false;
     m->have_old = false;
     m->oldgradsqr = 1.0;
   }
-  m->fp32_iters = m->iter;
   m->stop = DEFT_STOP_NONE;
-  if (verbose) printf("switching line-search energies to FP64 FFTs after %d iterations\n", m->iter);
+  if (verbose) printf("precision phase %d after %d iterations\n", m->phase, m->iter);
   *more = 1;
   return DEFT_OK;
+}
+
+// Copies between the compact half-spectrum layout of the seam and the padded
+// rows kept on the device.
+template <typename T>
+void compact_from_padded(const deft_plan_s *p, const T *pad, double *out) {
+  const long long rows = (long long)p->Nx * p->Ny;
+  for (long long r = 0; r < rows; ++r)
+    for (int k = 0; k < 2 * p->Nkz; ++k) out[r * 2 * p->Nkz + k] = (double)pad[r * 2 * p->ldk + k];
+}
+
+template <typename T>
+void padded_from_compact(const deft_plan_s *p, const double *in, T *pad) {
+  const long long rows = (long long)p->Nx * p->Ny;
+  for (long long r = 0; r < rows; ++r) {
+    for (int k = 0; k < 2 * p->Nkz; ++k) pad[r * 2 * p->ldk + k] = (T)in[r * 2 * p->Nkz + k];
+    for (int k = 2 * p->Nkz; k < 2 * p->ldk; ++k) pad[r * 2 * p->ldk + k] = (T)0;
+  }
 }
 
 int check_plan(deft_plan p) {
@@ -456,6 +491,8 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   p->Nkz = nz / 2 + 1;
   p->N = (long long)nx * ny * nz;
   p->Nk = (long long)nx * ny * p->Nkz;
+  p->ldk = padded_row(p->Nkz);
+  p->ks = (long long)nx * ny * p->ldk;
   std::memcpy(p->cell, cell, sizeof(p->cell));
   reciprocal(cell, det, p->recip);
   p->volume = std::fabs(det);
@@ -479,12 +516,14 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   if (hipMalloc(&p->x, nb) != hipSuccess || hipMalloc(&p->d, nb) != hipSuccess || hipMalloc(&p->g, nb) != hipSuccess ||
       hipMalloc(&p->pg, nb) != hipSuccess || hipMalloc(&p->og, nb) != hipSuccess ||
       hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
-      hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->Nk) != hipSuccess ||
-      hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
-      hipMalloc(&p->dscal, sizeof(double) * NUM_SLOTS) != hipSuccess)
+      hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
+      hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
-  if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS) != hipSuccess)
+  // reduced sums land directly in host-mapped pinned memory: no copy per evaluation
+  if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "pinned allocation failed");
+  if (hipHostGetDevicePointer((void **)&p->dscal, p->hscal, 0) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "hipHostGetDevicePointer failed");
   // twiddle tables exp(-2 pi i m / L), FP64 on the host
   const int lens[3] = {nx, ny, nz};
   size_t ntw = 0;
@@ -665,14 +704,16 @@ int deft_fft(deft_plan p, const double *in, double *out) {
   DEFT_HIP_CHECK(hipMemcpyAsync(p->tmp, in, sizeof(double) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
   if (p->prec == DEFT_FFT_FP64) {
     TRY(Launch<double>::fft_forward(p, p->tmp, p->dV));
-    DEFT_HIP_CHECK(hipMemcpyAsync(out, p->spec, sizeof(cx<double>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
+    std::vector<double> h(2 * (size_t)p->ks);
+    DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->spec, sizeof(cx<double>) * (size_t)p->ks, hipMemcpyDeviceToHost, p->stream));
     TRY(plan_sync(p));
+    compact_from_padded(p, h.data(), out);
   } else {
     TRY(Launch<float>::fft_forward(p, p->tmp, p->dV));
-    std::vector<float> h(2 * (size_t)p->Nk);
-    DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->spec, sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyDeviceToHost, p->stream));
+    std::vector<float> h(2 * (size_t)p->ks);
+    DEFT_HIP_CHECK(hipMemcpyAsync(h.data(), p->spec, sizeof(cx<float>) * (size_t)p->ks, hipMemcpyDeviceToHost, p->stream));
     TRY(plan_sync(p));
-    for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+    compact_from_padded(p, h.data(), out);
   }
   return DEFT_OK;
 }
@@ -682,12 +723,15 @@ int deft_ifft(deft_plan p, const double *in, double *out) {
   if (!in || !out) return set_error(DEFT_ERR_ARG, "null buffer");
   TRY(ensure_tmp(p, sizeof(double) * (size_t)p->N));
   if (p->prec == DEFT_FFT_FP64) {
-    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, in, sizeof(cx<double>) * (size_t)p->Nk, hipMemcpyHostToDevice, p->stream));
+    std::vector<double> h(2 * (size_t)p->ks);
+    padded_from_compact(p, in, h.data());
+    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, h.data(), sizeof(cx<double>) * (size_t)p->ks, hipMemcpyHostToDevice, p->stream));
+    TRY(plan_sync(p));
     TRY(Launch<double>::fft_inverse(p, p->tmp, 1.0 / p->volume));
   } else {
-    std::vector<float> h(2 * (size_t)p->Nk);
-    for (size_t i = 0; i < h.size(); ++i) h[i] = (float)in[i];
-    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, h.data(), sizeof(cx<float>) * (size_t)p->Nk, hipMemcpyHostToDevice, p->stream));
+    std::vector<float> h(2 * (size_t)p->ks);
+    padded_from_compact(p, in, h.data());
+    DEFT_HIP_CHECK(hipMemcpyAsync(p->spec, h.data(), sizeof(cx<float>) * (size_t)p->ks, hipMemcpyHostToDevice, p->stream));
     TRY(plan_sync(p));
     TRY(Launch<float>::fft_inverse(p, p->tmp, 1.0 / p->volume));
   }
@@ -698,7 +742,7 @@ int deft_ifft(deft_plan p, const double *in, double *out) {
 
 // --- bulk path (src/equation-of-state.cpp:164-192) ---------------------------
 static void bulk_terms(double n, double R, double kT, double *f_ex, double *df_ex) {
-  const WBConst c{4 * M_PI * R, 4 * M_PI * R * R};
+  const WBConst c = make_wb_const(R);
   const double n3 = 4 * M_PI / 3 * R * R * R * n, n2 = 4 * M_PI * R * R * n;
   double p1, p2, p3, d[5];
   wb_phi_partials(n3, n2, 0.0, 0.0, 0.0, c, p1, p2, p3, d);
@@ -818,7 +862,7 @@ int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_pr
     return set_error(DEFT_ERR_ARG, "unknown FFT precision");
   m->switch_rp = switch_relative_precision;
   m->polish_grad_prec = polish_grad_precision;
-  if (switch_relative_precision > 0 && m->fp32_iters < 0) {
+  if (switch_relative_precision > 0 && m->phase == 0) {
     m->e_prec = DEFT_FFT_FP32;
     m->g_prec = DEFT_FFT_FP32;
   }
@@ -829,6 +873,14 @@ int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_pr
 int deft_minimize_fp32_iterations(deft_minimize m, int *iterations) {
   if (!m || !iterations) return set_error(DEFT_ERR_ARG, "null argument");
   *iterations = m->fp32_iters;
+  return DEFT_OK;
+}
+
+int deft_minimize_phase_iterations(deft_minimize m, int *fp32_iterations, int *mixed_iterations, int *phase) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (fp32_iterations) *fp32_iterations = m->fp32_iters;
+  if (mixed_iterations) *mixed_iterations = m->phase == 2 ? m->iter_at_phase2 - m->fp32_iters : m->iter - m->fp32_iters;
+  if (phase) *phase = m->phase;
   return DEFT_OK;
 }
 

@@ -139,6 +139,26 @@ __device__ __forceinline__ void dft(cx<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS layout: element i of a sequence sits at skew(i) = i + (i >> SK), one pad
+// slot per 2^SK elements (SK = 4 for 8-byte FP32 complex, 3 for 16-byte FP64
+// complex).  Stockham stages write with strides R*NS (powers of two); the skew
+// spreads those writes over all LDS banks.  Sequences are lds_stride<T, L>()
+// apart (odd, so column-wise loads of several sequences do not collide either).
+// ---------------------------------------------------------------------------
+template <typename T>
+constexpr int lds_skew_shift() {
+  return sizeof(T) <= 4 ? 4 : 3;
+}
+template <typename T>
+__device__ __forceinline__ constexpr int skew(int i) {
+  return i + (i >> lds_skew_shift<T>());
+}
+template <typename T, int L>
+constexpr int lds_stride() {
+  return (L + (L >> lds_skew_shift<T>())) | 1;
+}
+
+// ---------------------------------------------------------------------------
 // Stockham stages
 // ---------------------------------------------------------------------------
 
@@ -177,7 +197,7 @@ __device__ __forceinline__ void stockham_stage(const cx<T>* __restrict__ src, cx
     const cx<T>* in = src + s * SS;
     cx<T> v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = in[j + r * M];
+    for (int r = 0; r < R; ++r) v[r] = in[skew<T>(j + r * M)];
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -186,9 +206,10 @@ __device__ __forceinline__ void stockham_stage(const cx<T>* __restrict__ src, cx
       }
     }
     dft<T, R, INV>(v);
-    cx<T>* out = dst + s * SS + (j / NS) * NS * R + k;
+    cx<T>* out = dst + s * SS;
+    const int o0 = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[r * NS] = v[r];
+    for (int r = 0; r < R; ++r) out[skew<T>(o0 + r * NS)] = v[r];
   }
 }
 
@@ -202,8 +223,8 @@ __device__ __forceinline__ void stockham_run(cx<T>* a, cx<T>* b, int nseq, const
   }
 }
 
-// Transforms the nseq sequences held in `a` (stride SS); `b` is scratch of the
-// same size.  Returns the buffer holding the result.  All threads of the block
+// Transforms the nseq sequences held in `a` (stride SS, elements at skew(i));
+// `b` is scratch of the same size.  Returns the buffer holding the result.  All threads of the block
 // must call it; the data in `a` must be visible (a __syncthreads() before).
 template <typename T, int L, int SS, bool INV>
 __device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw) {

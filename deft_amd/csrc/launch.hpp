@@ -37,7 +37,10 @@ inline bool with_length(int L, Fn &&fn) {
   }
 }
 
-inline int y_blocks(const deft_plan_s *p) { return p->Nx * ((p->Nkz + ccols(p->Ny) - 1) / ccols(p->Ny)); }
+inline int y_blocks(const deft_plan_s *p) {
+  const int C = ccols(p->Ny);
+  return y_main_blocks(p->Nx, p->Nkz, C) + y_tail_blocks(p->Nx, p->Nkz, C);
+}
 inline int x_blocks(const deft_plan_s *p) {
   const long long ncols = (long long)p->Ny * p->Nkz;
   return (int)((ncols + ccols(p->Nx) - 1) / ccols(p->Nx));
@@ -48,7 +51,7 @@ inline int x_blocks(const deft_plan_s *p) {
 inline void take_moves(deft_plan_s *p, double &m1, double &m2, int &nm) {
   const int first = p->npend > 2 ? p->npend - 2 : 0;
   for (int i = 0; i < first; ++i) {
-    ProfScope ps(p, K_AXPY, 24.0 * p->N);
+    ProfScope ps(p, 2 * K_AXPY, 24.0 * p->N);
     hipLaunchKernelGGL(k_axpy, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, p->d, p->pend[i]);
   }
   nm = p->npend - first;
@@ -60,7 +63,6 @@ inline void take_moves(deft_plan_s *p, double &m1, double &m2, int &nm) {
 template <typename T>
 static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
   hipStream_t st = p->stream;
-  const long long fs = p->Nk;
   const double cb = (double)sizeof(cx<T>) * p->Nk;  // bytes of one half spectrum
   cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
   double m1, m2;
@@ -68,26 +70,26 @@ static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
   take_moves(p, m1, m2, nm);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    ProfScope ps(p, K_ZFWD_DENSITY, (nm > 0 ? 28.0 : 12.0) * p->N + cb);
+    ProfScope ps(p, kid<T>(K_ZFWD_DENSITY), (nm > 0 ? 28.0 : 12.0) * p->N + cb);
     hipLaunchKernelGGL((k_zfwd_density<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, p->x, p->d, m1, m2, nm,
                        p->vext, S0, p->partial, p->pstride, tw_ptr<T>(p, 2));
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
-    ProfScope ps(p, K_YPASS, 2 * cb);
-    hipLaunchKernelGGL((k_ypass<T, L, false>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
+    ProfScope ps(p, kid<T>(K_YPASS1), 2 * cb);
+    hipLaunchKernelGGL((k_ypass<T, L, false, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
                        tw_ptr<T>(p, 1));
   });
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
-    ProfScope ps(p, K_XCONV_FWD, 6 * cb);
-    hipLaunchKernelGGL((k_xconv_fwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, S1, fs,
+    ProfScope ps(p, kid<T>(K_XCONV_FWD), 6 * cb);
+    hipLaunchKernelGGL((k_xconv_fwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, S1,
                        tw_ptr<T>(p, 0));
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
-    ProfScope ps(p, K_YPASS, 10 * cb);
-    hipLaunchKernelGGL((k_ypass<T, L, true>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1, fs,
+    ProfScope ps(p, kid<T>(K_YPASS5), 10 * cb);
+    hipLaunchKernelGGL((k_ypass<T, L, true, 5>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1,
                        tw_ptr<T>(p, 1));
   });
 }
@@ -96,57 +98,55 @@ template <typename T>
 int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
   const GridArgs g = grid_args(p);
   hipStream_t st = p->stream;
-  const long long fs = p->Nk;
   const double cb = (double)sizeof(cx<T>) * p->Nk;
   cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
   launch_density_to_weighted<T>(p, g);
   if (!grad) {
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, K_ZPHI_E, 5 * cb);
-      hipLaunchKernelGGL((k_zphi<T, L, false>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, fs, p->partial,
+      ProfScope ps(p, kid<T>(K_ZPHI_E), 5 * cb);
+      hipLaunchKernelGGL((k_zphi<T, L, false>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, p->partial,
                          p->pstride, tw_ptr<T>(p, 2));
     });
-    ProfScope ps(p, K_REDUCE, 48.0 * p->zblocks);
+    ProfScope ps(p, 2 * K_REDUCE, 48.0 * p->zblocks);
     hipLaunchKernelGGL(k_reduce, dim3(6), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
                        p->zblocks, p->dscal);
   } else {
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, K_ZPHI_EG, 10 * cb);
-      hipLaunchKernelGGL((k_zphi<T, L, true>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, fs, p->partial,
+      ProfScope ps(p, kid<T>(K_ZPHI_EG), 10 * cb);
+      hipLaunchKernelGGL((k_zphi<T, L, true>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, p->partial,
                          p->pstride, tw_ptr<T>(p, 2));
     });
     with_length(p->Ny, [&](auto LY) {
       constexpr int L = decltype(LY)::value;
-      ProfScope ps(p, K_YPASS, 10 * cb);
-      hipLaunchKernelGGL((k_ypass<T, L, false>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1, fs,
+      ProfScope ps(p, kid<T>(K_YPASS5), 10 * cb);
+      hipLaunchKernelGGL((k_ypass<T, L, false, 5>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1,
                          tw_ptr<T>(p, 1));
     });
     with_length(p->Nx, [&](auto LX) {
       constexpr int L = decltype(LX)::value;
-      ProfScope ps(p, K_XCONV_BWD, 6 * cb);
-      hipLaunchKernelGGL((k_xconv_bwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S1, fs, S0,
+      ProfScope ps(p, kid<T>(K_XCONV_BWD), 6 * cb);
+      hipLaunchKernelGGL((k_xconv_bwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S1, S0,
                          tw_ptr<T>(p, 0));
     });
     with_length(p->Ny, [&](auto LY) {
       constexpr int L = decltype(LY)::value;
-      ProfScope ps(p, K_YPASS, 2 * cb);
-      hipLaunchKernelGGL((k_ypass<T, L, true>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
+      ProfScope ps(p, kid<T>(K_YPASS1), 2 * cb);
+      hipLaunchKernelGGL((k_ypass<T, L, true, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
                          tw_ptr<T>(p, 1));
     });
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, K_ZGRAD, cb + (have_old ? 36.0 : 28.0) * p->N);
+      ProfScope ps(p, kid<T>(K_ZGRAD), cb + (have_old ? 36.0 : 28.0) * p->N);
       hipLaunchKernelGGL((k_zgrad<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S0, p->x, p->vext, p->og,
                          have_old ? 1 : 0, precond ? 1 : 0, p->g, p->pg, p->partial, p->pstride, tw_ptr<T>(p, 2));
     });
-    ProfScope ps(p, K_REDUCE, 64.0 * p->zblocks);
+    ProfScope ps(p, 2 * K_REDUCE, 64.0 * p->zblocks);
     hipLaunchKernelGGL(k_reduce, dim3(8), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
                        p->zblocks, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
-  DEFT_HIP_CHECK(hipMemcpyAsync(p->hscal, p->dscal, sizeof(double) * NUM_SLOTS, hipMemcpyDeviceToHost, st));
   return plan_sync(p);
 }
 
@@ -154,7 +154,6 @@ template <typename T>
 int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
   const GridArgs g = grid_args(p);
   hipStream_t st = p->stream;
-  const long long fs = p->Nk;
   cx<T> *S0 = spec_ptr<T>(p, 0);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
@@ -163,12 +162,12 @@ int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
-    hipLaunchKernelGGL((k_ypass<T, L, false>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
+    hipLaunchKernelGGL((k_ypass<T, L, false, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
                        tw_ptr<T>(p, 1));
   });
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
-    hipLaunchKernelGGL((k_xpass_plain<T, L, false>), dim3(x_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs, scl,
+    hipLaunchKernelGGL((k_xpass_plain<T, L, false>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, scl,
                        tw_ptr<T>(p, 0));
   });
   DEFT_HIP_CHECK(hipGetLastError());
@@ -179,21 +178,20 @@ template <typename T>
 int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
   const GridArgs g = grid_args(p);
   hipStream_t st = p->stream;
-  const long long fs = p->Nk;
   cx<T> *S0 = spec_ptr<T>(p, 0);
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
-    hipLaunchKernelGGL((k_xpass_plain<T, L, true>), dim3(x_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs, scl,
+    hipLaunchKernelGGL((k_xpass_plain<T, L, true>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, scl,
                        tw_ptr<T>(p, 0));
   });
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
-    hipLaunchKernelGGL((k_ypass<T, L, true>), dim3(y_blocks(p), 1), dim3(kThreads), 0, st, g, S0, fs,
+    hipLaunchKernelGGL((k_ypass<T, L, true, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
                        tw_ptr<T>(p, 1));
   });
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 1), dim3(kThreads), 0, st, g, S0, fs, out_dev,
+    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 1), dim3(kThreads), 0, st, g, S0, out_dev,
                        p->N, 1.0, tw_ptr<T>(p, 2));
   });
   DEFT_HIP_CHECK(hipGetLastError());
@@ -208,7 +206,7 @@ int Launch<T>::weighted_densities(deft_plan_s *p, double *out_dev) {
   launch_density_to_weighted<T>(p, g);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 5), dim3(kThreads), 0, st, g, S1, p->Nk, out_dev,
+    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 5), dim3(kThreads), 0, st, g, S1, out_dev,
                        p->N, 1.0, tw_ptr<T>(p, 2));
   });
   DEFT_HIP_CHECK(hipGetLastError());

@@ -2,12 +2,17 @@
 //
 // Layouts (deft src/Grid.h:37-42, FFTW r2c layout used by src/Grid.cpp:470-478):
 //   real field      r[x][y][z],  index (x*Ny + y)*Nz + z,        N  = Nx*Ny*Nz
-//   half spectrum   s[x][y][kz], index (x*Ny + y)*Nkz + kz,      Nk = Nx*Ny*(Nz/2+1)
+//   half spectrum   s[x][y][kz], index (x*Ny + y)*ldk + kz,      kz < Nkz = Nz/2+1
+// The device keeps half-spectrum rows padded to ldk = Nkz rounded up to 16
+// complex values, so every row starts on a 128-byte line; the padding is never
+// read or written.  (The FFT seam of the C ABI converts to the compact layout.)
 // A 3-D transform is three line passes, each staged through LDS:
 //   z pass: 2P contiguous real lines -> P complex FFTs of length Nz (two real
 //           lines packed as re/im), split into two half spectra
-//   y pass: C consecutive kz columns of one x plane, FFT along y (stride Nkz)
-//   x pass: C consecutive (y,kz) columns, FFT along x (stride Ny*Nkz)
+//   y pass: C consecutive kz columns of one x plane, FFT along y (stride ldk);
+//           the columns past the last full chunk of C (the Nyquist column for
+//           Nz = 128) are gathered from C different (x, kz) pairs per block
+//   x pass: C consecutive valid (y,kz) columns, FFT along x (stride Ny*ldk)
 // The fundamental-measure multiply is fused into the x pass (forward x FFT,
 // five weight products, five inverse x FFTs), Phi and its partials into the
 // z pass (five c2r lines -> FP64 pointwise -> five r2c lines), and the
@@ -41,16 +46,22 @@ enum Slot : int {
 
 struct GridArgs {
   int Nx, Ny, Nz, Nkz;
-  long long N, Nk;
-  double b[9];  // reciprocal lattice vectors as rows: k = mx*b0 + my*b1 + mz*b2
+  int ldk;        // padded half-spectrum row stride (complex values)
+  long long N;    // real points
+  long long ks;   // complex values per stored half spectrum, Nx*Ny*ldk
+  double b[9];    // reciprocal lattice vectors as rows: k = mx*b0 + my*b1 + mz*b2
   double R, six_dr2, invN;
   double kT, mu, logNQ, dV;
   WBConst wb;
 };
 
+constexpr int padded_row(int nkz) { return (nkz + 15) / 16 * 16; }
+
 // line pairs per z-pass block and columns per y/x-pass block
 constexpr int zpairs(int L) { return (1024 / L) > 0 ? (1024 / L) : 1; }
 constexpr int ccols(int L) { return (2048 / L) > 16 ? 16 : ((2048 / L) < 1 ? 1 : (2048 / L)); }
+static_assert(256 % ccols(20) == 0 && 256 % ccols(48) == 0 && 256 % ccols(96) == 0 && 256 % ccols(256) == 0,
+              "column tiles must divide the block");
 
 // Block-wide FP64 sums of NV values, stored to partial[(slot0+i)*stride + blockIdx.x].
 template <int NV>
@@ -76,37 +87,61 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restr
 
 static __device__ __forceinline__ int wrapf(int m, int n) { return (m > n / 2) ? m - n : m; }
 
-// k-vector length and weights (scaled by 1/N) of spectral element (x, y, kz).
+// Offset of valid column v (= y*Nkz + kz) inside one x plane.
+static __device__ __forceinline__ long long col_offset(const GridArgs& g, long long v) {
+  const long long y = v / g.Nkz;
+  return y * g.ldk + (v - y * g.Nkz);
+}
+
+// Weights (scaled by 1/N) of spectral element (x, y, kz): w3, w2 and the
+// vector weight's i-coefficient times k_j, which is -w3 k_j.
 template <typename T>
 __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y, int kz, T& w3, T& w2, T (&wk)[3]) {
-  const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
-  const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
-  const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
-  const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
-  const double k = sqrt(kx * kx + ky * ky + kz_ * kz_);
-  double a3, a2, av;
-  fm_weights(k, g.R, g.six_dr2, a3, a2, av);
-  w3 = (T)(a3 * g.invN);
-  w2 = (T)(a2 * g.invN);
-  const double s = av * g.invN;
-  wk[0] = (T)(s * kx);
-  wk[1] = (T)(s * ky);
-  wk[2] = (T)(s * kz_);
+  if constexpr (sizeof(T) <= 4) {
+    const float mx = (float)wrapf(x, g.Nx), my = (float)wrapf(y, g.Ny), mz = (float)kz;
+    const float kx = mx * (float)g.b[0] + my * (float)g.b[3] + mz * (float)g.b[6];
+    const float ky = mx * (float)g.b[1] + my * (float)g.b[4] + mz * (float)g.b[7];
+    const float kz_ = mx * (float)g.b[2] + my * (float)g.b[5] + mz * (float)g.b[8];
+    const float k = sqrtf(kx * kx + ky * ky + kz_ * kz_);
+    float a3, a2;
+    fm_weights_f32(k, (float)g.R, (float)g.six_dr2, a3, a2);
+    const float invN = (float)g.invN;
+    w3 = a3 * invN;
+    w2 = a2 * invN;
+    wk[0] = -w3 * kx;
+    wk[1] = -w3 * ky;
+    wk[2] = -w3 * kz_;
+  } else {
+    const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+    const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
+    const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
+    const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
+    const double k = sqrt(kx * kx + ky * ky + kz_ * kz_);
+    double a3, a2, av;
+    fm_weights(k, g.R, g.six_dr2, a3, a2, av);
+    w3 = (T)(a3 * g.invN);
+    w2 = (T)(a2 * g.invN);
+    const double s = av * g.invN;
+    wk[0] = (T)(s * kx);
+    wk[1] = (T)(s * ky);
+    wk[2] = (T)(s * kz_);
+  }
 }
 
 // Split P packed complex FFTs Z = FFT(a + i b) into the half spectra of a and b
-// and store lines [line0, line0+2P) of `dst` (Nkz = L/2+1 per line).
+// and store lines [line0, line0+2P) of `dst` (Nkz = L/2+1 values per line).
 template <typename T, int L, int P>
 __device__ __forceinline__ void store_r2c_halves(const cx<T>* Z, cx<T>* __restrict__ dst, long long line0,
-                                                 long long lines) {
+                                                 long long lines, int ldk) {
   constexpr int NKZ = L / 2 + 1;
+  constexpr int SS = lds_stride<T, L>();
   for (int e = threadIdx.x; e < 2 * P * NKZ; e += kThreads) {
     const int l = e / NKZ, k = e - l * NKZ;
     if (line0 + l < lines) {
-      const cx<T>* zl = Z + (l >> 1) * L;
-      const cx<T> a = zl[k];
-      const cx<T> b = cconj(zl[(L - k) % L]);
-      dst[(line0 + l) * NKZ + k] = (l & 1) ? scale(mul_mi(a - b), (T)0.5) : scale(a + b, (T)0.5);
+      const cx<T>* zl = Z + (l >> 1) * SS;
+      const cx<T> a = zl[skew<T>(k)];
+      const cx<T> b = cconj(zl[skew<T>((L - k) % L)]);
+      dst[(line0 + l) * ldk + k] = (l & 1) ? scale(mul_mi(a - b), (T)0.5) : scale(a + b, (T)0.5);
     }
   }
 }
@@ -114,18 +149,24 @@ __device__ __forceinline__ void store_r2c_halves(const cx<T>* Z, cx<T>* __restri
 // Load lines [line0, line0+2P) of half spectra and build the P packed complex
 // spectra Z = A + i B (Hermitian completion along z; the imaginary parts of the
 // kz = 0 and kz = Nz/2 elements are dropped, i.e. c2r returns the real part).
-// raw needs P*(L+2) elements, Z needs P*L.
+// Both buffers need zbuf_size<T, L, P>() elements.
+template <typename T, int L, int P>
+constexpr int zbuf_size() {
+  return P * ((L + 2) > lds_stride<T, L>() ? (L + 2) : lds_stride<T, L>());
+}
+
 template <typename T, int L, int P>
 __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, long long line0, long long lines,
-                                                cx<T>* raw, cx<T>* Z) {
+                                                int ldk, cx<T>* raw, cx<T>* Z) {
   constexpr int NKZ = L / 2 + 1;
   for (int e = threadIdx.x; e < 2 * P * NKZ; e += kThreads) {
-    const int l = e / NKZ;
+    const int l = e / NKZ, k = e - l * NKZ;
     cx<T> v = {(T)0, (T)0};
-    if (line0 + l < lines) v = src[line0 * NKZ + e];
+    if (line0 + l < lines) v = src[(line0 + l) * ldk + k];
     raw[e] = v;
   }
   __syncthreads();
+  constexpr int SS = lds_stride<T, L>();
   for (int e = threadIdx.x; e < P * L; e += kThreads) {
     const int p = e / L, z = e - p * L;
     cx<T> a, b;
@@ -140,7 +181,7 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
       a = cconj(raw[(2 * p) * NKZ + (L - z)]);
       b = cconj(raw[(2 * p + 1) * NKZ + (L - z)]);
     }
-    Z[e] = {a.re - b.im, a.im + b.re};
+    Z[p * SS + skew<T>(z)] = {a.re - b.im, a.im + b.re};
   }
   __syncthreads();
 }
@@ -156,8 +197,9 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* _
                                                            cx<T>* __restrict__ out, double* __restrict__ partial,
                                                            int pstride, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L);
-  __shared__ cx<T> A[P * L];
-  __shared__ cx<T> B[P * L];
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[P * SS];
+  __shared__ cx<T> B[P * SS];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -181,15 +223,15 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* _
         acc[2] += -g.mu * nv;
       }
     }
-    cx<T>* c = &A[(l >> 1) * L + z];
+    cx<T>* c = &A[(l >> 1) * SS + skew<T>(z)];
     if (l & 1)
       c->im = (T)nv;
     else
       c->re = (T)nv;
   }
   __syncthreads();
-  const cx<T>* Z = fft_lds<T, L, L, false>(A, B, P, tw);
-  store_r2c_halves<T, L, P>(Z, out, line0, lines);
+  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+  store_r2c_halves<T, L, P>(Z, out, line0, lines, g.ldk);
   block_sum_store<3>(acc, partial, SLOT_IDEAL, pstride);
 }
 
@@ -198,101 +240,145 @@ template <typename T, int L>
 __global__ __launch_bounds__(kThreads) void k_zfwd_plain(GridArgs g, const double* __restrict__ in,
                                                          cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L);
-  __shared__ cx<T> A[P * L];
-  __shared__ cx<T> B[P * L];
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[P * SS];
+  __shared__ cx<T> B[P * SS];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
     T v = (T)0;
     if (line0 + l < lines) v = (T)in[(line0 + l) * L + z];
-    cx<T>* c = &A[(l >> 1) * L + z];
+    cx<T>* c = &A[(l >> 1) * SS + skew<T>(z)];
     if (l & 1)
       c->im = v;
     else
       c->re = v;
   }
   __syncthreads();
-  const cx<T>* Z = fft_lds<T, L, L, false>(A, B, P, tw);
-  store_r2c_halves<T, L, P>(Z, out, line0, lines);
+  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+  store_r2c_halves<T, L, P>(Z, out, line0, lines, g.ldk);
 }
 
 // Plain c2r along z, out = scale * real part, for blockIdx.y fields.
 template <typename T, int L>
 __global__ __launch_bounds__(kThreads) void k_zinv_plain(GridArgs g, const cx<T>* __restrict__ in,
-                                                         long long fstride_in, double* __restrict__ out,
-                                                         long long fstride_out, double scl,
+                                                         double* __restrict__ out, long long fstride_out, double scl,
                                                          const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L);
-  __shared__ cx<T> A[P * (L + 2)];
-  __shared__ cx<T> B[P * (L + 2)];
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[zbuf_size<T, L, P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
-  load_c2r_packed<T, L, P>(in + blockIdx.y * fstride_in, line0, lines, A, B);
-  const cx<T>* R = fft_lds<T, L, L, true>(B, A, P, tw);
+  load_c2r_packed<T, L, P>(in + blockIdx.y * g.ks, line0, lines, g.ldk, A, B);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
   double* o = out + blockIdx.y * fstride_out;
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
     if (line0 + l < lines) {
-      const cx<T> v = R[(l >> 1) * L + z];
+      const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
       o[(line0 + l) * L + z] = scl * (double)((l & 1) ? v.im : v.re);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// y pass: FFT along y of blockIdx.y fields (in place).
+// y pass: FFT along y of NF consecutive fields (in place, field = blockIdx.y).  Blocks below
+// Nx*(Nkz/C) take C consecutive kz columns of one x plane (128-byte aligned
+// rows); the remaining blocks take the columns past the last full chunk, C
+// (x, kz) pairs per block.
 // ---------------------------------------------------------------------------
-template <typename T, int L, bool INV>
-__global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restrict__ data, long long fstride,
+inline int y_main_blocks(int Nx, int Nkz, int C) { return Nx * (Nkz / C); }
+inline int y_tail_blocks(int Nx, int Nkz, int C) {
+  const int ntail = Nkz - (Nkz / C) * C;
+  return (Nx * ntail + C - 1) / C;
+}
+
+template <typename T, int L, bool INV, int NF>
+__global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restrict__ data,
                                                     const cx<T>* __restrict__ tw) {
   constexpr int C = ccols(L);
-  constexpr int SS = L + 1;
+  constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
-  const int nch = (g.Nkz + C - 1) / C;
-  const int x = blockIdx.x / nch, ch = blockIdx.x - x * nch;
-  const int kz0 = ch * C;
-  const int ncol = min(C, g.Nkz - kz0);
-  cx<T>* base = data + blockIdx.y * fstride + (long long)x * L * g.Nkz + kz0;
+  const int nfull = g.Nkz / C;
+  const int ntail = g.Nkz - nfull * C;
+  const int nmain = g.Nx * nfull;
+  const long long plane = (long long)L * g.ldk;
+  int nseq;
+  // per-sequence base offsets (x plane + kz) for this thread's column slot
+  const int myc = threadIdx.x % C;  // kThreads is a multiple of C
+  long long off_c = 0;
+  if ((int)blockIdx.x < nmain) {
+    const int x = blockIdx.x / nfull, ch = blockIdx.x - x * nfull;
+    nseq = C;
+    off_c = (long long)x * plane + ch * C + myc;
+  } else {
+    const int q0 = (blockIdx.x - nmain) * C;
+    nseq = min(C, g.Nx * ntail - q0);
+    const int q = q0 + myc;
+    if (myc < nseq) off_c = (long long)(q / ntail) * plane + nfull * C + (q % ntail);
+  }
+  cx<T>* base = data + blockIdx.y * g.ks;  // gridDim.y == NF
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int y = e / C, c = e - y * C;
+    const int y = e / C;  // e % C == myc
     cx<T> v = {(T)0, (T)0};
-    if (c < ncol) v = base[(long long)y * g.Nkz + c];
-    A[c * SS + y] = v;
+    if (myc < nseq) v = base[off_c + (long long)y * g.ldk];
+    A[myc * SS + skew<T>(y)] = v;
   }
   __syncthreads();
-  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, ncol, tw);
+  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, nseq, tw);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int y = e / C, c = e - y * C;
-    if (c < ncol) base[(long long)y * g.Nkz + c] = R[c * SS + y];
+    const int y = e / C;
+    if (myc < nseq) base[off_c + (long long)y * g.ldk] = R[myc * SS + skew<T>(y)];
   }
 }
 
-// Plain FFT along x of blockIdx.y fields, in place, times `scl`.
+// Plain FFT along x of one field, in place, times `scl`.
 template <typename T, int L, bool INV>
-__global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __restrict__ data, long long fstride,
-                                                          double scl, const cx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __restrict__ data, double scl,
+                                                          const cx<T>* __restrict__ tw) {
   constexpr int C = ccols(L);
-  constexpr int SS = L + 1;
+  constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
   const long long ncols = (long long)g.Ny * g.Nkz;
+  const long long plane = (long long)g.Ny * g.ldk;
   const long long col0 = (long long)blockIdx.x * C;
   const int ncol = (int)min((long long)C, ncols - col0);
-  cx<T>* base = data + blockIdx.y * fstride + col0;
+  const int myc = threadIdx.x % C;
+  const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int xx = e / C, c = e - xx * C;
+    const int xx = e / C;
     cx<T> v = {(T)0, (T)0};
-    if (c < ncol) v = base[(long long)xx * ncols + c];
-    A[c * SS + xx] = v;
+    if (myc < ncol) v = data[(long long)xx * plane + off];
+    A[myc * SS + skew<T>(xx)] = v;
   }
   __syncthreads();
   const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, ncol, tw);
   const T s = (T)scl;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int xx = e / C, c = e - xx * C;
-    if (c < ncol) base[(long long)xx * ncols + c] = scale(R[c * SS + xx], s);
+    const int xx = e / C;
+    if (myc < ncol) data[(long long)xx * plane + off] = scale(R[myc * SS + skew<T>(xx)], s);
+  }
+}
+
+// Weight products of field f (0: w3, 1: w2, 2..4: i w2v_j) for a thread's elements.
+template <typename T, int PT>
+__device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const T (&w3)[PT], const T (&w2)[PT],
+                                            const T (&wk)[PT][3]) {
+  switch (f) {
+    case 0:
+      return scale(s, w3[i]);
+    case 1:
+      return scale(s, w2[i]);
+    case 2:
+      return mul_i(scale(s, wk[i][0]));
+    case 3:
+      return mul_i(scale(s, wk[i][1]));
+    default:
+      return mul_i(scale(s, wk[i][2]));
   }
 }
 
@@ -301,62 +387,55 @@ __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __r
 // ---------------------------------------------------------------------------
 template <typename T, int L>
 __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
-                                                        cx<T>* __restrict__ out, long long fstride,
-                                                        const cx<T>* __restrict__ tw) {
+                                                        cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols(L);
-  constexpr int SS = L + 1;
+  constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
   __shared__ cx<T> S2[C * SS];
   const long long ncols = (long long)g.Ny * g.Nkz;
+  const long long plane = (long long)g.Ny * g.ldk;
   const long long col0 = (long long)blockIdx.x * C;
   const int ncol = (int)min((long long)C, ncols - col0);
+  const int myc = threadIdx.x % C;
+  const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int xx = e / C, c = e - xx * C;
+    const int xx = e / C;
     cx<T> v = {(T)0, (T)0};
-    if (c < ncol) v = in[(long long)xx * ncols + col0 + c];
-    A[c * SS + xx] = v;
+    if (myc < ncol) v = in[(long long)xx * plane + off];
+    A[myc * SS + skew<T>(xx)] = v;
   }
   __syncthreads();
   cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
   cx<T>* W1 = (S == A) ? B : A;
   T w3[PT], w2[PT], wk[PT][3];
+  {
+    const long long col = col0 + myc;
+    const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
 #pragma unroll
-  for (int i = 0; i < PT; ++i) {
-    const int e = threadIdx.x + i * kThreads;
-    const int xx = e / C, c = e - xx * C;
-    w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
-    if (e < C * L && c < ncol) {
-      const long long col = col0 + c;
-      const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
-      element_weights<T>(g, xx, y, kz, w3[i], w2[i], wk[i]);
+    for (int i = 0; i < PT; ++i) {
+      const int e = threadIdx.x + i * kThreads;
+      w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
+      if (e < C * L && myc < ncol) element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
     }
   }
-#pragma unroll 1
+#pragma unroll
   for (int f = 0; f < 5; ++f) {
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       if (e < C * L) {
-        const int xx = e / C, c = e - xx * C;
-        const cx<T> s = S[c * SS + xx];
-        cx<T> r;
-        if (f == 0)
-          r = scale(s, w3[i]);
-        else if (f == 1)
-          r = scale(s, w2[i]);
-        else
-          r = mul_i(scale(s, wk[i][f - 2]));
-        W1[c * SS + xx] = r;
+        const int xx = e / C;
+        W1[myc * SS + skew<T>(xx)] = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(xx)], w3, w2, wk);
       }
     }
     __syncthreads();
     const cx<T>* R = fft_lds<T, L, SS, true>(W1, S2, ncol, tw);
-    cx<T>* o = out + f * fstride + col0;
+    cx<T>* o = out + f * g.ks;
     for (int e = threadIdx.x; e < C * L; e += kThreads) {
-      const int xx = e / C, c = e - xx * C;
-      if (c < ncol) o[(long long)xx * ncols + c] = R[c * SS + xx];
+      const int xx = e / C;
+      if (myc < ncol) o[(long long)xx * plane + off] = R[myc * SS + skew<T>(xx)];
     }
     __syncthreads();
   }
@@ -368,38 +447,40 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
 // src/Functional.h:407-417).
 // ---------------------------------------------------------------------------
 template <typename T, int L>
-__global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in, long long fstride,
+__global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols(L);
-  constexpr int SS = L + 1;
+  constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
   const long long ncols = (long long)g.Ny * g.Nkz;
+  const long long plane = (long long)g.Ny * g.ldk;
   const long long col0 = (long long)blockIdx.x * C;
   const int ncol = (int)min((long long)C, ncols - col0);
+  const int myc = threadIdx.x % C;
+  const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
   T w3[PT], w2[PT], wk[PT][3];
   cx<T> acc[PT];
+  {
+    const long long col = col0 + myc;
+    const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
 #pragma unroll
-  for (int i = 0; i < PT; ++i) {
-    const int e = threadIdx.x + i * kThreads;
-    const int xx = e / C, c = e - xx * C;
-    w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
-    acc[i] = {(T)0, (T)0};
-    if (e < C * L && c < ncol) {
-      const long long col = col0 + c;
-      const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
-      element_weights<T>(g, xx, y, kz, w3[i], w2[i], wk[i]);
+    for (int i = 0; i < PT; ++i) {
+      const int e = threadIdx.x + i * kThreads;
+      w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
+      acc[i] = {(T)0, (T)0};
+      if (e < C * L && myc < ncol) element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
     }
   }
-#pragma unroll 1
+#pragma unroll
   for (int f = 0; f < 5; ++f) {
-    const cx<T>* src = in + f * fstride + col0;
+    const cx<T>* src = in + f * g.ks;
     for (int e = threadIdx.x; e < C * L; e += kThreads) {
-      const int xx = e / C, c = e - xx * C;
+      const int xx = e / C;
       cx<T> v = {(T)0, (T)0};
-      if (c < ncol) v = src[(long long)xx * ncols + c];
-      A[c * SS + xx] = v;
+      if (myc < ncol) v = src[(long long)xx * plane + off];
+      A[myc * SS + skew<T>(xx)] = v;
     }
     __syncthreads();
     const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
@@ -407,14 +488,8 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       if (e < C * L) {
-        const int xx = e / C, c = e - xx * C;
-        const cx<T> s = S[c * SS + xx];
-        if (f == 0)
-          acc[i] = acc[i] + scale(s, w3[i]);
-        else if (f == 1)
-          acc[i] = acc[i] + scale(s, w2[i]);
-        else
-          acc[i] = acc[i] + mul_mi(scale(s, wk[i][f - 2]));
+        const cx<T> p = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(e / C)], w3, w2, wk);
+        acc[i] = (f < 2) ? acc[i] + p : acc[i] - p;
       }
     }
     __syncthreads();
@@ -422,16 +497,13 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
 #pragma unroll
   for (int i = 0; i < PT; ++i) {
     const int e = threadIdx.x + i * kThreads;
-    if (e < C * L) {
-      const int xx = e / C, c = e - xx * C;
-      A[c * SS + xx] = acc[i];
-    }
+    if (e < C * L) A[myc * SS + skew<T>(e / C)] = acc[i];
   }
   __syncthreads();
   const cx<T>* R = fft_lds<T, L, SS, true>(A, B, ncol, tw);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int xx = e / C, c = e - xx * C;
-    if (c < ncol) out[(long long)xx * ncols + col0 + c] = R[c * SS + xx];
+    const int xx = e / C;
+    if (myc < ncol) out[(long long)xx * plane + off] = R[myc * SS + skew<T>(xx)];
   }
 }
 
@@ -440,28 +512,29 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
 // sums) and, with GRAD, the five partials -> r2c back into the same buffers.
 // ---------------------------------------------------------------------------
 template <typename T, int L, bool GRAD>
-__global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields, long long fstride,
+__global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
                                                    double* __restrict__ partial, int pstride,
                                                    const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L);
   constexpr int NPT = 2 * P * L;
   constexpr int PT = (NPT + kThreads - 1) / kThreads;
-  __shared__ cx<T> A[P * (L + 2)];
-  __shared__ cx<T> B[P * (L + 2)];
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[zbuf_size<T, L, P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   T val[5][PT];
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
-    load_c2r_packed<T, L, P>(fields + f * fstride, line0, lines, A, B);
-    const cx<T>* R = fft_lds<T, L, L, true>(B, A, P, tw);
+    load_c2r_packed<T, L, P>(fields + f * g.ks, line0, lines, g.ldk, A, B);
+    const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       val[f][i] = (T)0;
       if (e < NPT) {
         const int l = e / L, z = e - l * L;
-        const cx<T> v = R[(l >> 1) * L + z];
+        const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
         val[f][i] = (l & 1) ? v.im : v.re;
       }
     }
@@ -497,7 +570,7 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
         const int e = threadIdx.x + i * kThreads;
         if (e < NPT) {
           const int l = e / L, z = e - l * L;
-          cx<T>* c = &A[(l >> 1) * L + z];
+          cx<T>* c = &A[(l >> 1) * SS + skew<T>(z)];
           if (l & 1)
             c->im = val[f][i];
           else
@@ -505,8 +578,8 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
         }
       }
       __syncthreads();
-      const cx<T>* Z = fft_lds<T, L, L, false>(A, B, P, tw);
-      store_r2c_halves<T, L, P>(Z, fields + f * fstride, line0, lines);
+      const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+      store_r2c_halves<T, L, P>(Z, fields + f * g.ks, line0, lines, g.ldk);
       __syncthreads();
     }
   }
@@ -525,18 +598,19 @@ __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __r
                                                     double* __restrict__ partial, int pstride,
                                                     const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L);
-  __shared__ cx<T> A[P * (L + 2)];
-  __shared__ cx<T> B[P * (L + 2)];
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[zbuf_size<T, L, P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
-  load_c2r_packed<T, L, P>(accs, line0, lines, A, B);
-  const cx<T>* R = fft_lds<T, L, L, true>(B, A, P, tw);
+  load_c2r_packed<T, L, P>(accs, line0, lines, g.ldk, A, B);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
   double dots[2] = {0.0, 0.0};
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
     if (line0 + l < lines) {
       const long long idx = (line0 + l) * L + z;
-      const cx<T> v = R[(l >> 1) * L + z];
+      const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
       const double conv = (double)((l & 1) ? v.im : v.re);
       const double xv = x[idx];
       const float ve = vext[idx];
@@ -556,7 +630,6 @@ __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __r
   }
   block_sum_store<2>(dots, partial, SLOT_PGG, pstride);
 }
-
 // ---------------------------------------------------------------------------
 // CG vector kernels (FP64)
 // ---------------------------------------------------------------------------

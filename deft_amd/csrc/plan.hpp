@@ -13,10 +13,13 @@
 
 namespace deft {
 
-// Kernels timed by the per-plan profiler (HIP events around each launch).
+// Kernel families timed by the per-plan profiler (HIP events around each
+// launch); each family is counted separately for FP32 and FP64 FFTs, so one
+// entry is one kernel instance for a given grid.
 enum KernelId : int {
   K_ZFWD_DENSITY = 0,
-  K_YPASS,
+  K_YPASS1,  // y pass over one spectrum
+  K_YPASS5,  // y pass over five spectra
   K_XCONV_FWD,
   K_ZPHI_E,
   K_ZPHI_EG,
@@ -27,13 +30,21 @@ enum KernelId : int {
   K_AXPY,
   K_REDUCE,
   K_FFT_SEAM,
-  NUM_KERNELS
+  NUM_FAMILIES
 };
+constexpr int NUM_KERNELS = 2 * NUM_FAMILIES;  // entry 2*family + (FP64 ? 1 : 0)
+
+template <typename T>
+constexpr int kid(int family) {
+  return 2 * family + (sizeof(T) == sizeof(double) ? 1 : 0);
+}
 
 inline const char *kernel_name(int id) {
-  static const char *names[NUM_KERNELS] = {"zfwd_density", "ypass", "xconv_fwd", "zphi_energy", "zphi_grad",
-                                           "xconv_bwd", "zgrad", "cg_update", "cg_reset", "axpy", "reduce",
-                                           "fft_seam"};
+  static const char *names[NUM_KERNELS] = {
+      "zfwd_density_f32", "zfwd_density_f64", "ypass1_f32", "ypass1_f64", "ypass5_f32",   "ypass5_f64",
+      "xconv_fwd_f32",    "xconv_fwd_f64",    "zphi_energy_f32", "zphi_energy_f64", "zphi_grad_f32", "zphi_grad_f64",
+      "xconv_bwd_f32",    "xconv_bwd_f64",    "zgrad_f32", "zgrad_f64", "cg_update", "cg_update_",
+      "cg_reset", "cg_reset_", "axpy", "axpy_", "reduce", "reduce_", "fft_seam_f32", "fft_seam_f64"};
   return (id >= 0 && id < NUM_KERNELS) ? names[id] : "?";
 }
 
@@ -53,7 +64,9 @@ struct PendingEvent {
 
 struct deft_plan_s {
   int Nx = 0, Ny = 0, Nz = 0, Nkz = 0;
+  int ldk = 0;        // padded half-spectrum row stride on the device
   long long N = 0, Nk = 0;
+  long long ks = 0;   // stored complex values per half spectrum, Nx*Ny*ldk
   double cell[9] = {0};
   double recip[9] = {0};  // rows b1, b2, b3 with a_i . b_j = 2 pi delta_ij
   double volume = 0, dV = 0;
@@ -69,8 +82,8 @@ struct deft_plan_s {
   void *spec = nullptr;    // 6 half spectra of Nk complex (sized for FP64)
   double *partial = nullptr;
   int pstride = 0;         // partial-sum slot stride (>= blocks of any reducing kernel)
-  double *dscal = nullptr;  // NUM_SLOTS reduced sums (device)
-  double *hscal = nullptr;  // pinned host copy
+  double *hscal = nullptr;  // NUM_SLOTS reduced sums in host-mapped pinned memory
+  double *dscal = nullptr;  // device address of hscal (written by k_reduce)
   void *twbuf = nullptr;
   deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
   deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
@@ -107,8 +120,9 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.Ny = p->Ny;
   a.Nz = p->Nz;
   a.Nkz = p->Nkz;
+  a.ldk = p->ldk;
   a.N = p->N;
-  a.Nk = p->Nk;
+  a.ks = p->ks;
   for (int i = 0; i < 9; ++i) a.b[i] = p->recip[i];
   a.R = p->R;
   const double dr = std::cbrt(p->dV);
@@ -118,14 +132,13 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.mu = p->mu;
   a.logNQ = std::log(water_nQ(p->kT));
   a.dV = p->dV;
-  a.wb.fpR = 4 * M_PI * p->R;
-  a.wb.fpR2 = 4 * M_PI * p->R * p->R;
+  a.wb = make_wb_const(p->R);
   return a;
 }
 
 template <typename T>
 inline cx<T> *spec_ptr(deft_plan_s *p, int field) {
-  return reinterpret_cast<cx<T> *>(p->spec) + (long long)field * p->Nk;
+  return reinterpret_cast<cx<T> *>(p->spec) + (long long)field * p->ks;
 }
 
 template <typename T>

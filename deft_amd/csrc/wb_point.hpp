@@ -18,15 +18,23 @@
 namespace deft {
 
 struct WBConst {
-  double fpR;    // 4 pi R
-  double fpR2;   // 4 pi R^2
+  double fpR;       // 4 pi R
+  double fpR2;      // 4 pi R^2
+  double inv_fpR;   // 1/(4 pi R)
+  double inv_fpR2;  // 1/(4 pi R^2)
 };
+
+inline WBConst make_wb_const(double R) {
+  const double pi = 3.14159265358979323846;
+  return WBConst{4 * pi * R, 4 * pi * R * R, 1.0 / (4 * pi * R), 1.0 / (4 * pi * R * R)};
+}
 
 constexpr double kInv36Pi = 1.0 / (36.0 * 3.14159265358979323846);
 constexpr double kF3SeriesCutoff = 0.05;
 constexpr int kF3SeriesTerms = 18;
 
-__host__ __device__ inline void f3_and_derivative(double n, double& f3, double& df3) {
+// f3 and f3' given om = 1-n, io = 1/(1-n) and L = ln(1-n).
+__host__ __device__ inline void f3_of(double n, double om, double io, double L, double& f3, double& df3) {
   if (fabs(n) < kF3SeriesCutoff) {
     double g = 1.5, gp = 0.0, p = 1.0;  // p = n^(k-3)
 #pragma unroll
@@ -36,51 +44,59 @@ __host__ __device__ inline void f3_and_derivative(double n, double& f3, double& 
       p = p * n;
       g -= c * p;
     }
-    const double om = 1.0 - n;
-    f3 = g / (om * om);
-    df3 = gp / (om * om) + 2.0 * g / (om * om * om);
+    const double io2 = io * io;
+    f3 = g * io2;
+    df3 = (gp + 2.0 * g * io) * io2;
   } else {
-    const double om = 1.0 - n;
-    const double L = log1p(-n);
     const double num = n + om * om * L;
-    const double den = n * n * om * om;
-    f3 = num / den;
+    const double iden = io * io / (n * n);
+    f3 = num * iden;
     const double dnum = n - 2.0 * om * L;
     const double dden = 2.0 * n * om * (1.0 - 2.0 * n);
-    df3 = dnum / den - num * dden / (den * den);
+    df3 = (dnum - num * dden * iden) * iden;
   }
+}
+
+__host__ __device__ inline void f3_and_derivative(double n, double& f3, double& df3) {
+  const double om = 1.0 - n;
+  f3_of(n, om, 1.0 / om, log1p(-n), f3, df3);
 }
 
 // Energy density terms only.
 __host__ __device__ inline void wb_phi(double n3, double n2, double vx, double vy, double vz, const WBConst& c,
                                        double& p1, double& p2, double& p3) {
-  const double omn = 1.0 - n3;
+  const double om = 1.0 - n3;
+  const double io = 1.0 / om;
   const double L = log1p(-n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
-  f3_and_derivative(n3, f3, df3);
-  const double t3 = n2 * (n2 * n2 - 3.0 * v2);
-  p1 = -n2 * L / c.fpR2;
-  p2 = (n2 * n2 - v2) / (c.fpR * omn);
-  p3 = f3 * t3 * kInv36Pi;
+  f3_of(n3, om, io, L, f3, df3);
+  const double n2s = n2 * n2;
+  p1 = -n2 * L * c.inv_fpR2;
+  p2 = (n2s - v2) * c.inv_fpR * io;
+  p3 = f3 * n2 * (n2s - 3.0 * v2) * kInv36Pi;
 }
 
 // Energy density terms and the five partials (d3, d2, dvx, dvy, dvz).
 __host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx, double vy, double vz,
                                                 const WBConst& c, double& p1, double& p2, double& p3,
                                                 double* d) {
-  const double omn = 1.0 - n3;
+  const double om = 1.0 - n3;
+  const double io = 1.0 / om;
   const double L = log1p(-n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
-  f3_and_derivative(n3, f3, df3);
-  const double t3 = n2 * (n2 * n2 - 3.0 * v2);
-  p1 = -n2 * L / c.fpR2;
-  p2 = (n2 * n2 - v2) / (c.fpR * omn);
+  f3_of(n3, om, io, L, f3, df3);
+  const double n2s = n2 * n2;
+  const double q = n2s - v2;
+  const double t3 = n2 * (n2s - 3.0 * v2);
+  const double a = c.inv_fpR * io;  // 1/(4 pi R (1-n3))
+  p1 = -n2 * L * c.inv_fpR2;
+  p2 = q * a;
   p3 = f3 * t3 * kInv36Pi;
-  d[0] = n2 / (c.fpR2 * omn) + (n2 * n2 - v2) / (c.fpR * omn * omn) + df3 * t3 * kInv36Pi;
-  d[1] = -L / c.fpR2 + 2.0 * n2 / (c.fpR * omn) + f3 * (3.0 * n2 * n2 - 3.0 * v2) * kInv36Pi;
-  const double cv = -2.0 / (c.fpR * omn) - f3 * 6.0 * n2 * kInv36Pi;
+  d[0] = n2 * c.inv_fpR2 * io + q * a * io + df3 * t3 * kInv36Pi;
+  d[1] = -L * c.inv_fpR2 + 2.0 * n2 * a + f3 * 3.0 * q * kInv36Pi;
+  const double cv = -2.0 * a - f3 * 6.0 * n2 * kInv36Pi;
   d[2] = cv * vx;
   d[3] = cv * vy;
   d[4] = cv * vz;
@@ -109,6 +125,41 @@ __device__ __forceinline__ void fm_weights(double k, double R, double six_dr2, d
     w2 = fp * (R * R) * (-kR2 * kR2 * kR2 * (1.0 / 120 / 6 / 7) + kR2 * kR2 * (1.0 / 120) - kR2 * (1.0 / 6) + 1);
     wv = fp * R3 * (kR2 * kR2 * (1.0 / 5040 - 1.0 / 720) + kR2 * (1.0 / 24 - 1.0 / 120) - 1.0 / 3);
   }
+}
+
+// FP32 evaluation of w3 and w2 for the FP32 FFT path.  The vector weight is
+// wv = s 4 pi (kR cos kR - sin kR)/k^3 = -w3 (also term by term in the series
+// of src/ReciprocalOperators.h:172-233), so only w3 and w2 are computed.  For
+// 1e-3 < kR < 1, (sin x - x cos x)/x^3 = sum_n (-1)^(n+1) 2n x^(2n-2)/(2n+1)!
+// replaces the closed form, which loses digits to cancellation in FP32.
+__device__ __forceinline__ void fm_weights_f32(float k, float R, float six_dr2, float& w3, float& w2) {
+  const float x = k * R;
+  const float fp = 12.566370614359172f;
+  const float R3 = R * R * R;
+  if (x <= 1e-3f) {
+    const float x2 = x * x;
+    w3 = (fp / 3.0f) * R3 * (x2 * x2 * x2 * (-1.0f / 15120) + x2 * x2 * (1.0f / 280) + x2 * (-1.0f / 10) + 1);
+    w2 = fp * (R * R) * (-x2 * x2 * x2 * (1.0f / 120 / 6 / 7) + x2 * x2 * (1.0f / 120) - x2 * (1.0f / 6) + 1);
+    return;
+  }
+  const float s = expf(-six_dr2 * k * k);
+  float sn, cs;
+  sincosf(x, &sn, &cs);
+  if (x < 1.0f) {
+    const float x2 = x * x;
+    const float poly =
+        ((((((x2 * (1.0f / 93405312000.0f) - 1.0f / 518918400.0f) * x2 + 1.0f / 3991680.0f) * x2 - 1.0f / 45360.0f) *
+               x2 +
+           1.0f / 840.0f) *
+              x2 -
+          1.0f / 30.0f) *
+             x2 +
+         1.0f / 3.0f);
+    w3 = s * fp * R3 * poly;
+  } else {
+    w3 = s * fp * (sn - x * cs) / (k * k * k);
+  }
+  w2 = s * fp * R * sn / k;
 }
 
 }  // namespace deft

@@ -172,18 +172,24 @@ int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *stats);
 int deft_minimize_invalidate(deft_minimize m);
 
 /* --- mixed precision (no counterpart in the reference, which is FP64 only) ---
- * With switch_relative_precision > 0 the minimizer first evaluates energies
- * and gradients with FP32 FFTs until its error estimate falls below
- * switch_relative_precision*|E| or the FP32 line search stalls; from then on
- * every line-search energy uses FP64 FFTs and gradients use
- * polish_grad_precision (DEFT_FFT_FP32 keeps the CG direction, DEFT_FFT_FP64
- * restarts it).  The convergence history carries over and the minimizer
- * continues to its own precision.  Switch while FP32 energy differences are
- * still well above FP32 round-off (about 1e-7 |E|): 1e-6 is a good value.  The plan's precision setting is
- * not changed.  0 (default) disables the switch. */
+ * With switch_relative_precision > 0 the minimizer climbs a precision ladder:
+ *   phase 0: energies and gradients with FP32 FFTs, until the error estimate
+ *            falls below switch_relative_precision*|E| or the line search stalls;
+ *   phase 1: line-search energies with FP64 FFTs, gradients FP32 (skipped when
+ *            polish_grad_precision is DEFT_FFT_FP64);
+ *   phase 2: everything FP64, to the minimizer's own precision, running at
+ *            least 3 iterations of its own before it may report convergence.
+ * Each phase hands over when the reference algorithm would stop; only phase 2
+ * ends the minimization.  The convergence history carries over; the CG
+ * direction restarts when the gradient precision changes.  Switch while FP32
+ * energy differences are still well above FP32 round-off (about 1e-7 |E|):
+ * 1e-6 is a good value.  The plan's precision setting is not changed.
+ * 0 (default) disables the ladder. */
 int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_precision, int polish_grad_precision);
-/* Iterations run in FP32 before the switch, -1 if no switch happened. */
+/* Iterations run in phase 0, -1 if the ladder was not used or not left. */
 int deft_minimize_fp32_iterations(deft_minimize m, int *iterations);
+/* Iterations of phase 0 and phase 1 and the current phase (0, 1 or 2). */
+int deft_minimize_phase_iterations(deft_minimize m, int *fp32_iterations, int *mixed_iterations, int *phase);
 
 /* --- profiling (the reference only has ad-hoc clock() timers,
  * src/new/Minimize.h:95-135) ------------------------------------------------

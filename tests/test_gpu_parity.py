@@ -162,6 +162,7 @@ def test_cg_mixed_precision_reaches_fp64_answer(gpu, polish_grad):
     m.set_mixed_precision(1e-6, polish_grad)
     m.run()
     assert m.fp32_iterations() > 0
+    assert m.phase_iterations()[2] == 2  # finished in the all-FP64 phase
     assert f.fft_precision == 0  # the plan's own setting is untouched
     assert abs(m.energy() / mo.energy() - 1) < 1e-9
     # the reported energy is an FP64 energy of the final Veff

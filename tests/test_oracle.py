@@ -267,3 +267,11 @@ def test_minimize_stops_on_nan():
     m = O.Minimize(f)
     assert m.improve_energy() is False
     assert m.stop_reason == "nan-energy"
+
+
+def test_vector_weight_is_minus_step_weight():
+    """wv = s 4 pi (R cos kR - sin kR/k)/k^2 equals -w3 = -s 4 pi (sin kR - kR cos kR)/k^3
+    (identity used by the device kernels to skip a third radial function)."""
+    g = O.Grid((32, 48, 20), np.diag([3.1, 4.9, 2.3]))
+    w3, w2, wv = O.weights(g, 1.0)
+    assert np.max(np.abs(wv + w3)) <= 1e-13 * np.max(np.abs(w3))
