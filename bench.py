@@ -84,6 +84,27 @@ def gpu_run(steps, warmup):
     return f, results, elapsed, prof, max(errs), dn, gold
 
 
+def rocprof_instance(family):
+    """rocprof name of the kernel instance behind a profiler family (None when ambiguous)."""
+    base, _, prec = family.rpartition("_")
+    T = {"f32": "float", "f64": "double"}.get(prec)
+    names = {"zfwd_density": f"deft::k_zfwd_density<{T}, {N}>", "xconv_fwd": f"deft::k_xconv_fwd<{T}, {N}>",
+             "xconv_bwd": f"deft::k_xconv_bwd<{T}, {N}>", "zphi_energy": f"deft::k_zphi<{T}, {N}, false>",
+             "zphi_grad": f"deft::k_zphi<{T}, {N}, true>", "zgrad": f"deft::k_zgrad<{T}, {N}>"}
+    return names.get(base) if T else None
+
+
+def committed_traffic(family):
+    """HBM bytes per dispatch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (profiles/c2_128_solute_pmc.json), or None."""
+    path = os.path.join(HERE, "profiles", "c2_128_solute_pmc.json")
+    inst = rocprof_instance(family)
+    if not inst or not os.path.exists(path):
+        return None
+    t = json.load(open(path))["per_dispatch_hbm_bytes"].get(inst)
+    return None if t is None else t["fetch_bytes"] + t["write_bytes"]
+
+
 def cpu_baseline(sample_seconds):
     """The FP64 oracle (numpy/scipy, one thread) on the same workload, timed over a
     bounded number of CG iterations and scaled by the oracle's own iteration count."""
@@ -157,7 +178,9 @@ def main():
                    "dF_over_F_vs_fp64_oracle": err, "max_dn_over_nmax_x_axis": dn,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname),
+                     "traffic_source": "profiles/c2_128_solute_pmc.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
+                                       "copy-calibrated, bytes per dispatch)",
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
                      "kernel_time_fraction_of_wall": total_ms * 1e-3 / elapsed},

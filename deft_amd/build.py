@@ -50,7 +50,24 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_examples(verbose)
     return LIB
+
+
+EXAMPLES = ["solute"]
+
+
+def build_examples(verbose=False):
+    """C++ drivers over include/deft_amd.hpp, linked against the in-tree library."""
+    exdir = os.path.join(HERE, "examples")
+    for name in EXAMPLES:
+        out = os.path.join(exdir, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+               os.path.join(exdir, name + ".cpp"), "-L" + HERE, "-ldeft_amd",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":
