@@ -47,6 +47,8 @@ def gpu_run(steps, warmup):
     mu = D.find_chemical_potential(nb, R, KT)
     f = D.WhiteBearFluidVeff((N, N, N), np.diag([L] * 3), R=R, kT=KT, mu=mu, fft_precision=D.FFT_FP32)
     f.Vext = solute_vext(N, L, SOLUTE)
+    tuning = f.autotune()
+    print("autotuned pass variants:", tuning, file=sys.stderr, flush=True)
     v0 = np.full((N, N, N), -KT * math.log(nb))
     gold = json.load(open(os.path.join(HERE, "tests", "golden", "oracle_c2.json")))
 

@@ -72,6 +72,9 @@ _SIGNATURES = {
     "deft_device_count": ([_IP], _I),
     "deft_plan_create": ([_I, _I, _I, _DP, _D, _D, _D, _I, ctypes.POINTER(_P)], _I),
     "deft_plan_destroy": ([_P], _I),
+    "deft_plan_autotune": ([_P], _I),
+    "deft_plan_tuning": ([_P, _I, _IP, _IP, _IP], _I),
+    "deft_plan_set_tuning": ([_P, _I, _I, _I, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
     "deft_plan_set_mu": ([_P, _D], _I),
     "deft_plan_set_vext": ([_P, _DP], _I),
@@ -296,6 +299,19 @@ class WhiteBearFluidVeff:
         e = ctypes.c_double()
         _check(lib().deft_functional_grad(self._h, None, _dptr(g), _dptr(pg), ctypes.byref(e)))
         return e.value, g, pg
+
+    def autotune(self):
+        """Times the pass variants on this grid and keeps the fastest (like FFTW_MEASURE)."""
+        _check(lib().deft_plan_autotune(self._h))
+        return {p: self.tuning(p) for p in (FFT_FP32, FFT_FP64)}
+
+    def tuning(self, fft_precision):
+        z, y, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().deft_plan_tuning(self._h, int(fft_precision), ctypes.byref(z), ctypes.byref(y), ctypes.byref(x)))
+        return {"zblock": z.value, "ytile": y.value, "xtile": x.value}
+
+    def set_tuning(self, fft_precision, zblock, ytile, xtile):
+        _check(lib().deft_plan_set_tuning(self._h, int(fft_precision), int(zblock), int(ytile), int(xtile)))
 
     def set_profiling(self, on=True):
         _check(lib().deft_plan_set_profiling(self._h, 1 if on else 0))

@@ -141,6 +141,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->partial);
   (void)hipFree(p->twbuf);
   (void)hipFree(p->tmp);
+  (void)hipFree(p->wtab);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -501,9 +502,8 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   p->kT = kT;
   p->mu = mu;
   p->prec = fft_precision;
-  p->zblocks = (int)(((long long)nx * ny + 2 * zpairs(nz) - 1) / (2 * zpairs(nz)));
   p->eblocks = (int)std::min<long long>(1024, (p->N + kThreads - 1) / kThreads);
-  p->pstride = std::max(p->zblocks, p->eblocks);
+  p->pstride = std::max(std::max(zblocks_for(p, kZBlocks[0]), zblocks_for(p, kZBlocks[1])), p->eblocks);
 
   auto fail = [&](int code, const char *what) {
     free_plan(p);
@@ -517,7 +517,8 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->pg, nb) != hipSuccess || hipMalloc(&p->og, nb) != hipSuccess ||
       hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
       hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
-      hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess)
+      hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
+      hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->Nk) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
   // reduced sums land directly in host-mapped pinned memory: no copy per evaluation
   if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
@@ -557,12 +558,49 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMemset(p->og, 0, nb) != hipSuccess || hipMemset(p->g, 0, nb) != hipSuccess ||
       hipMemset(p->pg, 0, nb) != hipSuccess || hipMemset(p->vext, 0, sizeof(float) * (size_t)p->N) != hipSuccess)
     return fail(DEFT_ERR_HIP, "initialisation failed");
+  if (hipMemset(p->spec, 0, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "initialisation failed");
+  hipLaunchKernelGGL(k_weight_table, dim3(p->eblocks), dim3(kThreads), 0, p->stream, grid_args(p), p->wtab);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "weight table failed");
   *out = p;
   return DEFT_OK;
 }
 
 int deft_plan_destroy(deft_plan p) {
   free_plan(p);
+  return DEFT_OK;
+}
+
+int deft_plan_autotune(deft_plan p) {
+  TRY(check_plan(p));
+  TRY(flush_moves(p));
+  TRY(plan_sync(p));
+  TRY(Launch<float>::autotune(p));
+  TRY(Launch<double>::autotune(p));
+  return DEFT_OK;
+}
+
+int deft_plan_tuning(deft_plan p, int fft_precision, int *zblock, int *ytile, int *xtile) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (zblock) *zblock = p->tz[fft_precision];
+  if (ytile) *ytile = p->ty[fft_precision];
+  if (xtile) *xtile = p->tx[fft_precision];
+  return DEFT_OK;
+}
+
+int deft_plan_set_tuning(deft_plan p, int fft_precision, int zblock, int ytile, int xtile) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  auto in = [](int v, const int (&opts)[2]) { return v == opts[0] || v == opts[1]; };
+  if (!in(zblock, kZBlocks) || !in(ytile, kTileBytes) || !in(xtile, kTileBytes))
+    return set_error(DEFT_ERR_ARG, "zblock must be 512 or 1024, tiles 8192 or 16384");
+  p->tz[fft_precision] = zblock;
+  p->ty[fft_precision] = ytile;
+  p->tx[fft_precision] = xtile;
   return DEFT_OK;
 }
 

@@ -16,7 +16,7 @@
 namespace deft {
 
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
   T re, im;
 };
 

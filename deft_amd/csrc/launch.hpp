@@ -1,6 +1,7 @@
 // Kernel launch sequences of one energy / energy+gradient evaluation and of
-// the FFT seam, templated on the FFT precision T.  Included by exactly one
-// translation unit per T, which instantiates Launch<T>.
+// the FFT seam, templated on the FFT precision T, plus the autotuner of the
+// pass variants.  Included by exactly one translation unit per T, which
+// instantiates Launch<T>.
 #pragma once
 
 #include <type_traits>
@@ -15,13 +16,16 @@ namespace deft {
     if (e_ != hipSuccess) return set_error(DEFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+template <int V>
+using ic = std::integral_constant<int, V>;
+
 // The grid edges the FFT kernels are instantiated for.
 template <typename Fn>
 inline bool with_length(int L, Fn &&fn) {
   switch (L) {
-#define DEFT_CASE(n)                        \
-  case n:                                   \
-    fn(std::integral_constant<int, n>{});   \
+#define DEFT_CASE(n) \
+  case n:            \
+    fn(ic<n>{});     \
     return true;
     DEFT_CASE(16)
     DEFT_CASE(20)
@@ -37,13 +41,32 @@ inline bool with_length(int L, Fn &&fn) {
   }
 }
 
-inline int y_blocks(const deft_plan_s *p) {
-  const int C = ccols(p->Ny);
+// tunable variants: z-pass block size and y/x tile bytes
+template <typename Fn>
+inline void with_zb(int zb, Fn &&fn) {
+  if (zb == kZBlocks[0])
+    fn(ic<kZBlocks[0]>{});
+  else
+    fn(ic<kZBlocks[1]>{});
+}
+template <typename Fn>
+inline void with_tb(int tb, Fn &&fn) {
+  if (tb == kTileBytes[0])
+    fn(ic<kTileBytes[0]>{});
+  else
+    fn(ic<kTileBytes[1]>{});
+}
+
+template <typename T>
+inline int y_blocks(const deft_plan_s *p, int tb) {
+  const int C = ccols<T>(p->Ny, tb);
   return y_main_blocks(p->Nx, p->Nkz, C) + y_tail_blocks(p->Nx, p->Nkz, C);
 }
-inline int x_blocks(const deft_plan_s *p) {
+template <typename T>
+inline int x_blocks(const deft_plan_s *p, int tb) {
   const long long ncols = (long long)p->Ny * p->Nkz;
-  return (int)((ncols + ccols(p->Nx) - 1) / ccols(p->Nx));
+  const int C = ccols<T>(p->Nx, tb);
+  return (int)((ncols + C - 1) / C);
 }
 
 // Applies all pending moves but the last two with axpy; returns the last two
@@ -60,91 +83,127 @@ inline void take_moves(deft_plan_s *p, double &m1, double &m2, int &nm) {
   p->npend = 0;
 }
 
+// --- single launches (each dispatches on the grid edge and the tuning) -----
+
+template <typename T>
+static void launch_ypass(deft_plan_s *p, const GridArgs &g, cx<T> *data, bool inv, int nf, int tb) {
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  const int fam = nf == 1 ? K_YPASS1 : K_YPASS5;
+  with_length(p->Ny, [&](auto LY) {
+    constexpr int L = decltype(LY)::value;
+    with_tb(tb, [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(fam), 2.0 * nf * cb);
+      const dim3 grid(y_blocks<T>(p, TB), nf);
+      if (inv && nf == 1)
+        hipLaunchKernelGGL((k_ypass<T, L, true, 1, TB>), grid, dim3(kThreads), 0, p->stream, g, data, tw_ptr<T>(p, 1));
+      else if (inv)
+        hipLaunchKernelGGL((k_ypass<T, L, true, 5, TB>), grid, dim3(kThreads), 0, p->stream, g, data, tw_ptr<T>(p, 1));
+      else if (nf == 1)
+        hipLaunchKernelGGL((k_ypass<T, L, false, 1, TB>), grid, dim3(kThreads), 0, p->stream, g, data, tw_ptr<T>(p, 1));
+      else
+        hipLaunchKernelGGL((k_ypass<T, L, false, 5, TB>), grid, dim3(kThreads), 0, p->stream, g, data, tw_ptr<T>(p, 1));
+    });
+  });
+}
+
+template <typename T>
+static void launch_xconv_fwd(deft_plan_s *p, const GridArgs &g, int tb) {
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  with_length(p->Nx, [&](auto LX) {
+    constexpr int L = decltype(LX)::value;
+    with_tb(tb, [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(K_XCONV_FWD), 6 * cb);
+      hipLaunchKernelGGL((k_xconv_fwd<T, L, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), spec_ptr<T>(p, 1), tw_ptr<T>(p, 0));
+    });
+  });
+}
+
+template <typename T>
+static void launch_xconv_bwd(deft_plan_s *p, const GridArgs &g, int tb) {
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  with_length(p->Nx, [&](auto LX) {
+    constexpr int L = decltype(LX)::value;
+    with_tb(tb, [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(K_XCONV_BWD), 6 * cb);
+      hipLaunchKernelGGL((k_xconv_bwd<T, L, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 1), spec_ptr<T>(p, 0), tw_ptr<T>(p, 0));
+    });
+  });
+}
+
+template <typename T>
+static void launch_zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb) {
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  with_length(p->Nz, [&](auto LZ) {
+    constexpr int L = decltype(LZ)::value;
+    with_zb(zb, [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      if (grad) {
+        ProfScope ps(p, kid<T>(K_ZPHI_EG), 10 * cb);
+        hipLaunchKernelGGL((k_zphi<T, L, true, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g,
+                           spec_ptr<T>(p, 1), p->partial, p->pstride, tw_ptr<T>(p, 2));
+      } else {
+        ProfScope ps(p, kid<T>(K_ZPHI_E), 5 * cb);
+        hipLaunchKernelGGL((k_zphi<T, L, false, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g,
+                           spec_ptr<T>(p, 1), p->partial, p->pstride, tw_ptr<T>(p, 2));
+      }
+    });
+  });
+}
+
 template <typename T>
 static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
-  hipStream_t st = p->stream;
+  const int ti = prec_index<T>();
   const double cb = (double)sizeof(cx<T>) * p->Nk;  // bytes of one half spectrum
-  cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
   double m1, m2;
   int nm;
   take_moves(p, m1, m2, nm);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    ProfScope ps(p, kid<T>(K_ZFWD_DENSITY), (nm > 0 ? 28.0 : 12.0) * p->N + cb);
-    hipLaunchKernelGGL((k_zfwd_density<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, p->x, p->d, m1, m2, nm,
-                       p->vext, S0, p->partial, p->pstride, tw_ptr<T>(p, 2));
+    with_zb(p->tz[ti], [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      ProfScope ps(p, kid<T>(K_ZFWD_DENSITY), (nm > 0 ? 28.0 : 12.0) * p->N + cb);
+      hipLaunchKernelGGL((k_zfwd_density<T, L, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g, p->x,
+                         p->d, m1, m2, nm, p->vext, spec_ptr<T>(p, 0), p->partial, p->pstride, tw_ptr<T>(p, 2));
+    });
   });
-  with_length(p->Ny, [&](auto LY) {
-    constexpr int L = decltype(LY)::value;
-    ProfScope ps(p, kid<T>(K_YPASS1), 2 * cb);
-    hipLaunchKernelGGL((k_ypass<T, L, false, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
-                       tw_ptr<T>(p, 1));
-  });
-  with_length(p->Nx, [&](auto LX) {
-    constexpr int L = decltype(LX)::value;
-    ProfScope ps(p, kid<T>(K_XCONV_FWD), 6 * cb);
-    hipLaunchKernelGGL((k_xconv_fwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, S1,
-                       tw_ptr<T>(p, 0));
-  });
-  with_length(p->Ny, [&](auto LY) {
-    constexpr int L = decltype(LY)::value;
-    ProfScope ps(p, kid<T>(K_YPASS5), 10 * cb);
-    hipLaunchKernelGGL((k_ypass<T, L, true, 5>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1,
-                       tw_ptr<T>(p, 1));
-  });
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 0), false, 1, p->ty[ti]);
+  launch_xconv_fwd<T>(p, g, p->tx[ti]);
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 1), true, 5, p->ty[ti]);
 }
 
 template <typename T>
 int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
   const GridArgs g = grid_args(p);
-  hipStream_t st = p->stream;
-  const double cb = (double)sizeof(cx<T>) * p->Nk;
-  cx<T> *S0 = spec_ptr<T>(p, 0), *S1 = spec_ptr<T>(p, 1);
+  const int ti = prec_index<T>();
+  const int zb = p->tz[ti];
+  const int nzb = zblocks_for(p, zb);
   launch_density_to_weighted<T>(p, g);
-  if (!grad) {
+  launch_zphi<T>(p, g, grad, zb);
+  if (grad) {
+    launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, 5, p->ty[ti]);
+    launch_xconv_bwd<T>(p, g, p->tx[ti]);
+    launch_ypass<T>(p, g, spec_ptr<T>(p, 0), true, 1, p->ty[ti]);
     with_length(p->Nz, [&](auto LZ) {
       constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, kid<T>(K_ZPHI_E), 5 * cb);
-      hipLaunchKernelGGL((k_zphi<T, L, false>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, p->partial,
-                         p->pstride, tw_ptr<T>(p, 2));
+      with_zb(zb, [&](auto ZBc) {
+        constexpr int ZB = decltype(ZBc)::value;
+        ProfScope ps(p, kid<T>(K_ZGRAD), (double)sizeof(cx<T>) * p->Nk + (have_old ? 36.0 : 28.0) * p->N);
+        hipLaunchKernelGGL((k_zgrad<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0), p->x,
+                           p->vext, p->og, have_old ? 1 : 0, precond ? 1 : 0, p->g, p->pg, p->partial, p->pstride,
+                           tw_ptr<T>(p, 2));
+      });
     });
-    ProfScope ps(p, 2 * K_REDUCE, 48.0 * p->zblocks);
-    hipLaunchKernelGGL(k_reduce, dim3(6), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
-                       p->zblocks, p->dscal);
-  } else {
-    with_length(p->Nz, [&](auto LZ) {
-      constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, kid<T>(K_ZPHI_EG), 10 * cb);
-      hipLaunchKernelGGL((k_zphi<T, L, true>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S1, p->partial,
-                         p->pstride, tw_ptr<T>(p, 2));
-    });
-    with_length(p->Ny, [&](auto LY) {
-      constexpr int L = decltype(LY)::value;
-      ProfScope ps(p, kid<T>(K_YPASS5), 10 * cb);
-      hipLaunchKernelGGL((k_ypass<T, L, false, 5>), dim3(y_blocks(p), 5), dim3(kThreads), 0, st, g, S1,
-                         tw_ptr<T>(p, 1));
-    });
-    with_length(p->Nx, [&](auto LX) {
-      constexpr int L = decltype(LX)::value;
-      ProfScope ps(p, kid<T>(K_XCONV_BWD), 6 * cb);
-      hipLaunchKernelGGL((k_xconv_bwd<T, L>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S1, S0,
-                         tw_ptr<T>(p, 0));
-    });
-    with_length(p->Ny, [&](auto LY) {
-      constexpr int L = decltype(LY)::value;
-      ProfScope ps(p, kid<T>(K_YPASS1), 2 * cb);
-      hipLaunchKernelGGL((k_ypass<T, L, true, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
-                         tw_ptr<T>(p, 1));
-    });
-    with_length(p->Nz, [&](auto LZ) {
-      constexpr int L = decltype(LZ)::value;
-      ProfScope ps(p, kid<T>(K_ZGRAD), cb + (have_old ? 36.0 : 28.0) * p->N);
-      hipLaunchKernelGGL((k_zgrad<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, S0, p->x, p->vext, p->og,
-                         have_old ? 1 : 0, precond ? 1 : 0, p->g, p->pg, p->partial, p->pstride, tw_ptr<T>(p, 2));
-    });
-    ProfScope ps(p, 2 * K_REDUCE, 64.0 * p->zblocks);
-    hipLaunchKernelGGL(k_reduce, dim3(8), dim3(kThreads), 0, st, p->partial, p->pstride, (int)SLOT_IDEAL,
-                       p->zblocks, p->dscal);
+  }
+  {
+    const int nslots = grad ? 8 : 6;
+    ProfScope ps(p, 2 * K_REDUCE, 8.0 * nslots * nzb);
+    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3(kThreads), 0, p->stream, p->partial, p->pstride, (int)SLOT_IDEAL,
+                       nzb, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
   return plan_sync(p);
@@ -153,22 +212,23 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
 template <typename T>
 int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
   const GridArgs g = grid_args(p);
-  hipStream_t st = p->stream;
-  cx<T> *S0 = spec_ptr<T>(p, 0);
+  const int ti = prec_index<T>();
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    hipLaunchKernelGGL((k_zfwd_plain<T, L>), dim3(p->zblocks), dim3(kThreads), 0, st, g, in_dev, S0,
-                       tw_ptr<T>(p, 2));
+    with_zb(p->tz[ti], [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      hipLaunchKernelGGL((k_zfwd_plain<T, L, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g, in_dev,
+                         spec_ptr<T>(p, 0), tw_ptr<T>(p, 2));
+    });
   });
-  with_length(p->Ny, [&](auto LY) {
-    constexpr int L = decltype(LY)::value;
-    hipLaunchKernelGGL((k_ypass<T, L, false, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
-                       tw_ptr<T>(p, 1));
-  });
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 0), false, 1, p->ty[ti]);
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
-    hipLaunchKernelGGL((k_xpass_plain<T, L, false>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, scl,
-                       tw_ptr<T>(p, 0));
+    with_tb(p->tx[ti], [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      hipLaunchKernelGGL((k_xpass_plain<T, L, false, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), scl, tw_ptr<T>(p, 0));
+    });
   });
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
@@ -177,22 +237,23 @@ int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
 template <typename T>
 int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
   const GridArgs g = grid_args(p);
-  hipStream_t st = p->stream;
-  cx<T> *S0 = spec_ptr<T>(p, 0);
+  const int ti = prec_index<T>();
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
-    hipLaunchKernelGGL((k_xpass_plain<T, L, true>), dim3(x_blocks(p)), dim3(kThreads), 0, st, g, S0, scl,
-                       tw_ptr<T>(p, 0));
+    with_tb(p->tx[ti], [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      hipLaunchKernelGGL((k_xpass_plain<T, L, true, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), scl, tw_ptr<T>(p, 0));
+    });
   });
-  with_length(p->Ny, [&](auto LY) {
-    constexpr int L = decltype(LY)::value;
-    hipLaunchKernelGGL((k_ypass<T, L, true, 1>), dim3(y_blocks(p)), dim3(kThreads), 0, st, g, S0,
-                       tw_ptr<T>(p, 1));
-  });
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 0), true, 1, p->ty[ti]);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 1), dim3(kThreads), 0, st, g, S0, out_dev,
-                       p->N, 1.0, tw_ptr<T>(p, 2));
+    with_zb(p->tz[ti], [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      hipLaunchKernelGGL((k_zinv_plain<T, L, ZB>), dim3(zblocks_for(p, ZB), 1), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), out_dev, p->N, 1.0, tw_ptr<T>(p, 2));
+    });
   });
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
@@ -201,16 +262,81 @@ int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
 template <typename T>
 int Launch<T>::weighted_densities(deft_plan_s *p, double *out_dev) {
   const GridArgs g = grid_args(p);
-  hipStream_t st = p->stream;
-  cx<T> *S1 = spec_ptr<T>(p, 1);
+  const int ti = prec_index<T>();
   launch_density_to_weighted<T>(p, g);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
-    hipLaunchKernelGGL((k_zinv_plain<T, L>), dim3(p->zblocks, 5), dim3(kThreads), 0, st, g, S1, out_dev,
-                       p->N, 1.0, tw_ptr<T>(p, 2));
+    with_zb(p->tz[ti], [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      hipLaunchKernelGGL((k_zinv_plain<T, L, ZB>), dim3(zblocks_for(p, ZB), 5), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 1), out_dev, p->N, 1.0, tw_ptr<T>(p, 2));
+    });
   });
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
+}
+
+// Mean event time (ms) of `reps` launches issued by fn() on the plan's stream,
+// after two warm-up launches.
+template <typename Fn>
+static float time_launches(deft_plan_s *p, Fn &&fn, int reps = 5) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return 1e30f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return 1e30f;
+  }
+  fn();
+  fn();
+  (void)hipEventRecord(a, p->stream);
+  for (int i = 0; i < reps; ++i) fn();
+  (void)hipEventRecord(b, p->stream);
+  float ms = 1e30f;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 1e30f;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+// Times every instantiated variant of the z, y and x passes on this grid
+// (the heaviest kernel of each pass) and keeps the fastest; spectra and
+// partial sums are overwritten, Veff and the CG vectors are not touched.
+template <typename T>
+int Launch<T>::autotune(deft_plan_s *p) {
+  const GridArgs g = grid_args(p);
+  const int ti = prec_index<T>();
+  const bool was_prof = p->prof;
+  p->prof = false;
+  float best = 1e30f;
+  for (int zb : kZBlocks) {
+    const float t = time_launches(p, [&] { launch_zphi<T>(p, g, true, zb); });
+    if (t < best) {
+      best = t;
+      p->tz[ti] = zb;
+    }
+  }
+  best = 1e30f;
+  for (int tb : kTileBytes) {
+    const float t = time_launches(p, [&] { launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, 5, tb); });
+    if (t < best) {
+      best = t;
+      p->ty[ti] = tb;
+    }
+  }
+  best = 1e30f;
+  for (int tb : kTileBytes) {
+    const float t = time_launches(p, [&] {
+      launch_xconv_fwd<T>(p, g, tb);
+      launch_xconv_bwd<T>(p, g, tb);
+    });
+    if (t < best) {
+      best = t;
+      p->tx[ti] = tb;
+    }
+  }
+  p->prof = was_prof;
+  DEFT_HIP_CHECK(hipGetLastError());
+  return plan_sync(p);
 }
 
 }  // namespace deft

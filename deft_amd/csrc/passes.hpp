@@ -45,6 +45,7 @@ enum Slot : int {
 };
 
 struct GridArgs {
+  const double* wtab;  // FP64 weight table: w3/N then w2/N per valid spectral element
   int Nx, Ny, Nz, Nkz;
   int ldk;        // padded half-spectrum row stride (complex values)
   long long N;    // real points
@@ -57,11 +58,19 @@ struct GridArgs {
 
 constexpr int padded_row(int nkz) { return (nkz + 15) / 16 * 16; }
 
-// line pairs per z-pass block and columns per y/x-pass block
-constexpr int zpairs(int L) { return (1024 / L) > 0 ? (1024 / L) : 1; }
-constexpr int ccols(int L) { return (2048 / L) > 16 ? 16 : ((2048 / L) < 1 ? 1 : (2048 / L)); }
-static_assert(256 % ccols(20) == 0 && 256 % ccols(48) == 0 && 256 % ccols(96) == 0 && 256 % ccols(256) == 0,
-              "column tiles must divide the block");
+// Line pairs per z-pass block (a block holds about ZB complex values) and
+// columns per y/x-pass block (a tile holds about TB bytes); both are chosen per
+// plan by the autotuner among the instantiated values.
+constexpr int kZBlocks[2] = {512, 1024};
+constexpr int kTileBytes[2] = {8192, 16384};
+constexpr int zpairs(int L, int ZB = 1024) { return (ZB / L) > 0 ? (ZB / L) : 1; }
+// (a y/x tile holds C columns of L complex values, about 16 KiB)
+template <typename T>
+constexpr int ccols(int L, int TB) {
+  const int c = TB / (L * 2 * (int)sizeof(T));
+  return c > 16 ? 16 : (c < 1 ? 1 : (c >= 8 ? (c >= 16 ? 16 : 8) : (c >= 4 ? 4 : (c >= 2 ? 2 : 1))));
+}
+static_assert(256 % ccols<float>(20, 16384) == 0 && ccols<double>(128, 8192) == 4, "column tiles");
 
 // Block-wide FP64 sums of NV values, stored to partial[(slot0+i)*stride + blockIdx.x].
 template <int NV>
@@ -87,6 +96,14 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restr
 
 static __device__ __forceinline__ int wrapf(int m, int n) { return (m > n / 2) ? m - n : m; }
 
+// p[i] if ok, else 0; the load itself always goes to a valid address (safe)
+// (a conditionally assigned complex<double> otherwise lands in scratch)
+template <typename T>
+__device__ __forceinline__ cx<T> load_or_zero(const cx<T>* __restrict__ p, long long i, long long safe, bool ok) {
+  const cx<T> v = p[ok ? i : safe];
+  return {ok ? v.re : (T)0, ok ? v.im : (T)0};
+}
+
 // Offset of valid column v (= y*Nkz + kz) inside one x plane.
 static __device__ __forceinline__ long long col_offset(const GridArgs& g, long long v) {
   const long long y = v / g.Nkz;
@@ -94,7 +111,9 @@ static __device__ __forceinline__ long long col_offset(const GridArgs& g, long l
 }
 
 // Weights (scaled by 1/N) of spectral element (x, y, kz): w3, w2 and the
-// vector weight's i-coefficient times k_j, which is -w3 k_j.
+// vector weight's i-coefficient times k_j, which is -w3 k_j.  FP32 evaluates
+// them in registers; FP64 reads the plan's precomputed table (FP64 sincos/exp
+// per element had dominated the x passes' VALU time).
 template <typename T>
 __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y, int kz, T& w3, T& w2, T (&wk)[3]) {
   if constexpr (sizeof(T) <= 4) {
@@ -116,15 +135,32 @@ __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y,
     const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
     const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
     const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
-    const double k = sqrt(kx * kx + ky * ky + kz_ * kz_);
+    const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+    const long long idx = ((long long)x * g.Ny + y) * g.Nkz + kz;
+    const double a3 = g.wtab[idx];
+    w3 = (T)a3;
+    w2 = (T)g.wtab[nk + idx];
+    wk[0] = (T)(-a3 * kx);
+    wk[1] = (T)(-a3 * ky);
+    wk[2] = (T)(-a3 * kz_);
+  }
+}
+
+// FP64 weight table (w3/N, w2/N) of every valid spectral element, built once per plan.
+static __global__ __launch_bounds__(kThreads) void k_weight_table(GridArgs g, double* __restrict__ tab) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    const int kz = (int)(i % g.Nkz);
+    const long long r = i / g.Nkz;
+    const int y = (int)(r % g.Ny), x = (int)(r / g.Ny);
+    const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+    const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
+    const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
+    const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
     double a3, a2, av;
-    fm_weights(k, g.R, g.six_dr2, a3, a2, av);
-    w3 = (T)(a3 * g.invN);
-    w2 = (T)(a2 * g.invN);
-    const double s = av * g.invN;
-    wk[0] = (T)(s * kx);
-    wk[1] = (T)(s * ky);
-    wk[2] = (T)(s * kz_);
+    fm_weights(sqrt(kx * kx + ky * ky + kz_ * kz_), g.R, g.six_dr2, a3, a2, av);
+    tab[i] = a3 * g.invN;
+    tab[nk + i] = a2 * g.invN;
   }
 }
 
@@ -161,9 +197,7 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
   constexpr int NKZ = L / 2 + 1;
   for (int e = threadIdx.x; e < 2 * P * NKZ; e += kThreads) {
     const int l = e / NKZ, k = e - l * NKZ;
-    cx<T> v = {(T)0, (T)0};
-    if (line0 + l < lines) v = src[(line0 + l) * ldk + k];
-    raw[e] = v;
+    raw[e] = load_or_zero(src, (line0 + l) * ldk + k, 0, line0 + l < lines);
   }
   __syncthreads();
   constexpr int SS = lds_stride<T, L>();
@@ -190,13 +224,13 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
 // z pass 1: (optional CG moves) Veff -> n = exp(-Veff/kT) -> r2c along z.
 // Also accumulates the local energy terms (ideal gas, Vext n, -mu n).
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* __restrict__ x,
                                                            const double* __restrict__ d, double m1, double m2,
                                                            int nmoves, const float* __restrict__ vext,
                                                            cx<T>* __restrict__ out, double* __restrict__ partial,
                                                            int pstride, const cx<T>* __restrict__ tw) {
-  constexpr int P = zpairs(L);
+  constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[P * SS];
   __shared__ cx<T> B[P * SS];
@@ -236,10 +270,10 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* _
 }
 
 // Plain r2c along z of a real FP64 field (FFT seam and tests).
-template <typename T, int L>
+template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zfwd_plain(GridArgs g, const double* __restrict__ in,
                                                          cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  constexpr int P = zpairs(L);
+  constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[P * SS];
   __shared__ cx<T> B[P * SS];
@@ -261,11 +295,11 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_plain(GridArgs g, const doubl
 }
 
 // Plain c2r along z, out = scale * real part, for blockIdx.y fields.
-template <typename T, int L>
+template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zinv_plain(GridArgs g, const cx<T>* __restrict__ in,
                                                          double* __restrict__ out, long long fstride_out, double scl,
                                                          const cx<T>* __restrict__ tw) {
-  constexpr int P = zpairs(L);
+  constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
@@ -295,10 +329,10 @@ inline int y_tail_blocks(int Nx, int Nkz, int C) {
   return (Nx * ntail + C - 1) / C;
 }
 
-template <typename T, int L, bool INV, int NF>
+template <typename T, int L, bool INV, int NF, int TB>
 __global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restrict__ data,
                                                     const cx<T>* __restrict__ tw) {
-  constexpr int C = ccols(L);
+  constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
@@ -323,9 +357,7 @@ __global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restric
   cx<T>* base = data + blockIdx.y * g.ks;  // gridDim.y == NF
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int y = e / C;  // e % C == myc
-    cx<T> v = {(T)0, (T)0};
-    if (myc < nseq) v = base[off_c + (long long)y * g.ldk];
-    A[myc * SS + skew<T>(y)] = v;
+    A[myc * SS + skew<T>(y)] = load_or_zero(base, off_c + (long long)y * g.ldk, (long long)y * g.ldk, myc < nseq);
   }
   __syncthreads();
   const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, nseq, tw);
@@ -336,10 +368,10 @@ __global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restric
 }
 
 // Plain FFT along x of one field, in place, times `scl`.
-template <typename T, int L, bool INV>
+template <typename T, int L, bool INV, int TB>
 __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __restrict__ data, double scl,
                                                           const cx<T>* __restrict__ tw) {
-  constexpr int C = ccols(L);
+  constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
@@ -351,9 +383,7 @@ __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __r
   const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
-    cx<T> v = {(T)0, (T)0};
-    if (myc < ncol) v = data[(long long)xx * plane + off];
-    A[myc * SS + skew<T>(xx)] = v;
+    A[myc * SS + skew<T>(xx)] = load_or_zero(data, (long long)xx * plane + off, 0, myc < ncol);
   }
   __syncthreads();
   const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, ncol, tw);
@@ -385,10 +415,10 @@ __device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const 
 // ---------------------------------------------------------------------------
 // x pass with the weight multiply: n~ -> n3, n2, n2v_x, n2v_y, n2v_z spectra.
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  constexpr int C = ccols(L);
+  constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
@@ -402,9 +432,7 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
   const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
-    cx<T> v = {(T)0, (T)0};
-    if (myc < ncol) v = in[(long long)xx * plane + off];
-    A[myc * SS + skew<T>(xx)] = v;
+    A[myc * SS + skew<T>(xx)] = load_or_zero(in, (long long)xx * plane + off, 0, myc < ncol);
   }
   __syncthreads();
   cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
@@ -446,10 +474,10 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
 // Even weights keep their sign, odd (vector) weights flip it (deft
 // src/Functional.h:407-417).
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  constexpr int C = ccols(L);
+  constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
@@ -478,9 +506,7 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
     const cx<T>* src = in + f * g.ks;
     for (int e = threadIdx.x; e < C * L; e += kThreads) {
       const int xx = e / C;
-      cx<T> v = {(T)0, (T)0};
-      if (myc < ncol) v = src[(long long)xx * plane + off];
-      A[myc * SS + skew<T>(xx)] = v;
+      A[myc * SS + skew<T>(xx)] = load_or_zero(src, (long long)xx * plane + off, 0, myc < ncol);
     }
     __syncthreads();
     const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
@@ -511,11 +537,11 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
 // z pass 2: c2r of the five weighted densities -> FP64 Phi (energy partial
 // sums) and, with GRAD, the five partials -> r2c back into the same buffers.
 // ---------------------------------------------------------------------------
-template <typename T, int L, bool GRAD>
+template <typename T, int L, bool GRAD, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
                                                    double* __restrict__ partial, int pstride,
                                                    const cx<T>* __restrict__ tw) {
-  constexpr int P = zpairs(L);
+  constexpr int P = zpairs(L, ZB);
   constexpr int NPT = 2 * P * L;
   constexpr int PT = (NPT + kThreads - 1) / kThreads;
   constexpr int SS = lds_stride<T, L>();
@@ -590,14 +616,14 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
 // z pass 3: c2r of the back-convolution -> dF/dn -> grad and pgrad (FP64),
 // with pgrad.grad and pgrad.oldgrad partial sums.
 // ---------------------------------------------------------------------------
-template <typename T, int L>
+template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __restrict__ accs,
                                                     const double* __restrict__ x, const float* __restrict__ vext,
                                                     const double* __restrict__ oldgrad, int have_old, int precond,
                                                     double* __restrict__ grad, double* __restrict__ pgrad,
                                                     double* __restrict__ partial, int pstride,
                                                     const cx<T>* __restrict__ tw) {
-  constexpr int P = zpairs(L);
+  constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];

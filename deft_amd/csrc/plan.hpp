@@ -87,6 +87,7 @@ struct deft_plan_s {
   void *twbuf = nullptr;
   deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
   deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
+  double *wtab = nullptr;  // FP64 weight table, 2 * Nx*Ny*Nkz doubles
   double *tmp = nullptr;   // scratch for host-seam calls (grows on demand)
   size_t tmp_bytes = 0;
 
@@ -102,8 +103,12 @@ struct deft_plan_s {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
   size_t pool_used = 0;
   deft::KernelStat kstat[deft::NUM_KERNELS];
-  int zblocks = 0;  // blocks of the z passes
   int eblocks = 0;  // blocks of the elementwise FP64 kernels
+  // tuning per FFT precision (index 0 FP32, 1 FP64): z-pass block size (complex
+  // values), y-pass and x-pass tile bytes; set by deft_plan_autotune
+  int tz[2] = {1024, 1024};
+  int ty[2] = {16384, 16384};
+  int tx[2] = {8192, 8192};
 };
 
 namespace deft {
@@ -116,6 +121,7 @@ inline double water_nQ(double kT) {
 
 inline GridArgs grid_args(const deft_plan_s *p) {
   GridArgs a{};
+  a.wtab = p->wtab;
   a.Nx = p->Nx;
   a.Ny = p->Ny;
   a.Nz = p->Nz;
@@ -153,6 +159,18 @@ inline cx<double> *tw_ptr<double>(deft_plan_s *p, int axis) {
 }
 
 int set_error(int code, const std::string &msg);
+
+template <typename T>
+constexpr int prec_index() {
+  return sizeof(T) == sizeof(double) ? 1 : 0;
+}
+
+// blocks of a z pass whose blocks hold about zb complex values
+inline int zblocks_for(const deft_plan_s *p, int zb) {
+  const long long lines = (long long)p->Nx * p->Ny;
+  const int pairs = zpairs(p->Nz, zb);
+  return (int)((lines + 2 * pairs - 1) / (2 * pairs));
+}
 
 // Records HIP events around one launch when the plan's profiler is on.
 struct ProfScope {
@@ -205,6 +223,8 @@ struct Launch {
   // Energy (grad=false) or energy + grad + pgrad (grad=true) of the resident
   // Veff after applying pending moves; reduced sums land in p->hscal.
   static int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond);
+  // times the tunable kernel variants on this grid and keeps the fastest
+  static int autotune(deft_plan_s *p);
   // fft seam: real FP64 (device, N) -> spectrum in field 0 (scaled by scl)
   static int fft_forward(deft_plan_s *p, const double *in_dev, double scl);
   // ifft seam: spectrum in field 0 -> real FP64 (device, N) times scl

@@ -105,6 +105,16 @@ int deft_device_count(int *count);
 int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, double kT, double mu,
                      int fft_precision, deft_plan *out);
 int deft_plan_destroy(deft_plan plan);
+/* Autotuner (no counterpart in the reference, whose FFTW_MEASURE planning,
+ * src/GridDescription.cpp:19-25, plays the same role): times the instantiated
+ * variants of the z pass (block of 512 or 1024 complex values), the y pass and
+ * the x passes (tiles of 8 or 16 KiB) on this grid for both FFT precisions and
+ * keeps the fastest.  Overwrites the plan's spectral scratch, not Veff. */
+int deft_plan_autotune(deft_plan plan);
+/* The chosen variants for one FFT precision, and an explicit choice
+ * (zblock 512 or 1024, ytile and xtile 8192 or 16384). */
+int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile, int *xtile);
+int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
 /* Vext (N doubles, host); stored as FP32 on the device (+inf kept). */

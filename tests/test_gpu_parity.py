@@ -202,3 +202,25 @@ def test_nan_veff_stops_minimizer(gpu):
     m = gpu.Minimize(f)
     assert m.improve_energy() is False
     assert m.stats()["stop_reason"] == "nan-energy"
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+def test_every_tuning_variant_matches_oracle(gpu, prec, etol, gtol):
+    """All z-block / y-tile / x-tile variants the autotuner may pick give the oracle's answer."""
+    shape, cell = (48, 32, 64), np.diag([4.8, 3.2, 6.4])
+    g = O.Grid(shape, cell)
+    vext = O.planar_wall_vext(g, 1.0)
+    g, of, f = _fluid(gpu, shape, cell, vext=vext, prec=prec, seed=5)
+    Eo, gro, pgo = of.energy_grad_and_precond()
+    for zb in (512, 1024):
+        for yt in (8192, 16384):
+            for xt in (8192, 16384):
+                f.set_tuning(prec, zb, yt, xt)
+                E, gr, pg = f.energy_grad_and_precond()
+                assert abs(E - Eo) <= etol * abs(Eo), (zb, yt, xt)
+                assert _rel(gr, gro) < gtol, (zb, yt, xt)
+    t = f.autotune()
+    assert t[prec]["zblock"] in (512, 1024) and t[prec]["xtile"] in (8192, 16384)
+    E, gr, pg = f.energy_grad_and_precond()
+    assert abs(E - Eo) <= etol * abs(Eo)
+    assert np.array_equal(f.Veff, of.x)  # tuning never touches the state
