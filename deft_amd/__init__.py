@@ -30,6 +30,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdeft_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "deft_amd.h")
+HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "deft_amd_nn.h")]
 
 FFT_FP32 = 0
 FFT_FP64 = 1
@@ -64,6 +65,7 @@ _D = ctypes.c_double
 _I = ctypes.c_int
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
 
 _SIGNATURES = {
     "deft_last_error": ([], ctypes.c_char_p),
@@ -111,6 +113,24 @@ _SIGNATURES = {
     "deft_plan_profile_reset": ([_P], _I),
     "deft_kernel_count": ([], _I),
     "deft_plan_profile_get": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong), _DP, _DP], _I),
+    # warm-start network (include/deft_amd_nn.h); device buffers are passed as integers
+    "deft_net_create": ([_IP, ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
+    "deft_net_destroy": ([_P], _I),
+    "deft_net_num_params": ([_P], ctypes.c_longlong),
+    "deft_net_layout": ([_P, _I, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
+                         ctypes.POINTER(ctypes.c_longlong)], _I),
+    "deft_net_get_params": ([_P, _FP], _I),
+    "deft_net_set_params": ([_P, _FP], _I),
+    "deft_net_forward": ([_P, _I, _I, _P, _FP, _P], _I),
+    "deft_net_loss_grad": ([_P, _I, _I, _P, _FP, _P, _P, _FP, _FP], _I),
+    "deft_net_train_step": ([_P, _I, _I, _P, _FP, _P, _P, ctypes.c_float, _FP], _I),
+    "deft_plan_warm_start": ([_P, _P, _D], _I),
+    "deft_buffer_create": ([ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
+    "deft_buffer_destroy": ([_P], _I),
+    "deft_buffer_upload": ([_P, ctypes.c_ulonglong, _P, ctypes.c_ulonglong], _I),
+    "deft_buffer_download": ([_P, ctypes.c_ulonglong, _P, ctypes.c_ulonglong], _I),
+    "deft_net_make_batch": ([_P, ctypes.c_longlong, _I, _I, _IP, _IP, _P], _I),
+    "deft_net_mfma_probe": ([_FP, _FP, _FP], _I),
 }
 
 
@@ -433,8 +453,10 @@ class Minimize:
         _check(lib().deft_minimize_invalidate(self._h))
 
 
-def declared_symbols(header=HEADER_PATH):
-    """Function names declared in include/deft_amd.h."""
+def declared_symbols(headers=None):
+    """Function names declared in include/deft_amd.h and include/deft_amd_nn.h."""
     import re
-    text = open(header).read()
-    return sorted(set(re.findall(r"\b(deft_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in headers or HEADER_PATHS:
+        names |= set(re.findall(r"\b(deft_[a-z0-9_]+)\s*\(", open(h).read()))
+    return sorted(names)

@@ -1,0 +1,981 @@
+// Warm-start network: a periodic 3-D U-Net predicting u = ln(n / n_b) of the
+// equilibrium density from exp(-Vext/kT) and the bulk packing fraction, with
+// hand-written BF16-MFMA forward and backward kernels (FP32 accumulation and
+// activations) and Adam training.  No counterpart in deft; the conjugate
+// gradient minimizer stays the judge of convergence.
+//
+// Layout: activations are FP32 channels-last, act[v * C + c], voxel
+// v = ((s*M + x)*M + y)*M + z for sample s of a batch; every level is periodic.
+// A 3x3x3 convolution is an implicit GEMM: rows = voxels, cols = output
+// channels, K = 27 taps x Cin (k = tap*Cin + ci, padded to a multiple of 32).
+// Weights are kept as FP32 masters W[co][k] and BF16 copies of the same layout;
+// the backward-data convolution runs the same kernel with the transposed,
+// tap-flipped copy Wt[ci][tap'*Cout + co] = W[co][26 - tap'][ci].
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/deft_amd_nn.h"
+#include "plan.hpp"
+
+namespace {
+
+using deft::set_error;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNT = 256;  // threads per block (4 waves)
+
+#define NN_HIP(expr)                                                                                \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess) return set_error(DEFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define NN_TRY(expr)                  \
+  do {                                \
+    const int rc_ = (expr);           \
+    if (rc_ != DEFT_OK) return rc_;   \
+  } while (0)
+
+// neighbour of voxel (s, x, y, z) at tap t (offsets t/9-1, t/3%3-1, t%3-1), periodic
+__device__ __forceinline__ long long nbr(long long s, int x, int y, int z, int t, int M) {
+  int xx = x + t / 9 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t % 3 - 1;
+  xx += (xx < 0) ? M : (xx >= M ? -M : 0);
+  yy += (yy < 0) ? M : (yy >= M ? -M : 0);
+  zz += (zz < 0) ? M : (zz >= M ? -M : 0);
+  return ((s * M + xx) * M + yy) * M + zz;
+}
+
+__device__ __forceinline__ void split(long long v, int M, long long &s, int &x, int &y, int &z) {
+  const long long M3 = (long long)M * M * M;
+  s = v / M3;
+  long long r = v - s * M3;
+  x = (int)(r / (M * M));
+  r -= (long long)x * M * M;
+  y = (int)(r / M);
+  z = (int)(r - (long long)y * M);
+}
+
+// ---------------------------------------------------------------------------
+// convolution: out[v][co] = act(bias[co] + sum_k A[v][k] W[co][k])
+// one wave computes 16 voxels x NT*16 output channels with
+// v_mfma_f32_16x16x32_bf16 (lane l: A row l%16, k (l/16)*8..+7; B col l%16, same
+// k; D rows (l/16)*4+j, col l%16).
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(kNT) void k_conv(const float *__restrict__ in, int cin, const __bf16 *__restrict__ W,
+                                              int kp, const float *__restrict__ bias, float *__restrict__ out, int M,
+                                              long long V, int relu) {
+  const int lane = threadIdx.x & 63;
+  const long long v0 = ((long long)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6)) * 16;
+  if (v0 >= V) return;
+  const long long vr = v0 + (lane & 15);
+  const bool row_ok = vr < V;
+  long long s = 0;
+  int x = 0, y = 0, z = 0;
+  if (row_ok) split(vr, M, s, x, y, z);
+  const int kmax = 27 * cin;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) acc[n] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < kp; kc += 32) {
+    const int k0 = kc + (lane >> 4) * 8;
+    bf16x8 a;
+    if (row_ok && k0 < kmax) {
+      const int t = k0 / cin, ci = k0 - t * cin;
+      const float *src = in + nbr(s, x, y, z, t, M) * cin + ci;
+      const float4 p = *reinterpret_cast<const float4 *>(src);
+      const float4 q = *reinterpret_cast<const float4 *>(src + 4);
+      a = bf16x8{(__bf16)p.x, (__bf16)p.y, (__bf16)p.z, (__bf16)p.w, (__bf16)q.x, (__bf16)q.y, (__bf16)q.z,
+                 (__bf16)q.w};
+    } else {
+      a = bf16x8{(__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f,
+                 (__bf16)0.f};
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8 *>(W + (long long)(n * 16 + (lane & 15)) * kp + k0);
+      acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[n], 0, 0, 0);
+    }
+  }
+  const int cout = NT * 16;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int co = n * 16 + (lane & 15);
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long v = v0 + (lane >> 4) * 4 + j;
+      if (v < V) {
+        float val = acc[n][j] + bb;
+        if (relu) val = fmaxf(val, 0.f);
+        out[v * cout + co] = val;
+      }
+    }
+  }
+}
+
+// weight gradient gw[co][k] += sum_v dpre[v][co] A[v][k] over a slice of voxels:
+// rows = output channels (16 per tile), cols = k (16 per tile), reduction over voxels
+__global__ __launch_bounds__(64) void k_conv_wgrad(const float *__restrict__ dpre, int cout,
+                                                   const float *__restrict__ in, int cin, int kp,
+                                                   float *__restrict__ gw, int M, long long V, long long vslice) {
+  const int lane = threadIdx.x;
+  const int ntk = kp / 16;
+  const int mt = blockIdx.x / ntk, nt = blockIdx.x - mt * ntk;
+  const long long vb0 = (long long)blockIdx.y * vslice;
+  const long long vb1 = min(V, vb0 + vslice);
+  const int co = mt * 16 + (lane & 15);
+  const int kcol = nt * 16 + (lane & 15);
+  const bool kok = kcol < 27 * cin;
+  const int t = kok ? kcol / cin : 0, ci = kok ? kcol - t * cin : 0;
+  floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (long long vb = vb0; vb < vb1; vb += 32) {
+    float av[8], bv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const long long v = vb + (lane >> 4) * 8 + i;
+      av[i] = 0.f;
+      bv[i] = 0.f;
+      if (v < vb1) {
+        av[i] = dpre[v * cout + co];
+        if (kok) {
+          long long s;
+          int x, y, z;
+          split(v, M, s, x, y, z);
+          bv[i] = in[nbr(s, x, y, z, t, M) * cin + ci];
+        }
+      }
+    }
+    const bf16x8 a = bf16x8{(__bf16)av[0], (__bf16)av[1], (__bf16)av[2], (__bf16)av[3],
+                            (__bf16)av[4], (__bf16)av[5], (__bf16)av[6], (__bf16)av[7]};
+    const bf16x8 b = bf16x8{(__bf16)bv[0], (__bf16)bv[1], (__bf16)bv[2], (__bf16)bv[3],
+                            (__bf16)bv[4], (__bf16)bv[5], (__bf16)bv[6], (__bf16)bv[7]};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) atomicAdd(&gw[(long long)(mt * 16 + (lane >> 4) * 4 + j) * kp + kcol], acc[j]);
+}
+
+// gb[c] += sum_v d[v][c]
+__global__ __launch_bounds__(kNT) void k_bias_grad(const float *__restrict__ d, int C, long long V,
+                                                   float *__restrict__ gb) {
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long long v = threadIdx.x; v < V; v += kNT) s += d[v * C + c];
+  __shared__ float red[kNT];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kNT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gb[c] += red[0];
+}
+
+// dpre = act > 0 ? d : 0
+__global__ __launch_bounds__(kNT) void k_relu_bwd(const float *__restrict__ d, const float *__restrict__ act,
+                                                  float *__restrict__ dpre, long long n) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
+    dpre[i] = act[i] > 0.f ? d[i] : 0.f;
+}
+
+__global__ __launch_bounds__(kNT) void k_add(const float *__restrict__ a, const float *__restrict__ b,
+                                             float *__restrict__ o, long long n) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
+    o[i] = a[i] + b[i];
+}
+
+// average pool 2x2x2: out (level M/2) from in (level M)
+__global__ __launch_bounds__(kNT) void k_pool(const float *__restrict__ in, float *__restrict__ out, int C, int Mo,
+                                              long long Vo) {
+  const int Mi = 2 * Mo;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < Vo * C; i += (long long)gridDim.x * kNT) {
+    const long long v = i / C;
+    const int c = (int)(i - v * C);
+    long long s;
+    int x, y, z;
+    split(v, Mo, s, x, y, z);
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const long long vi = ((s * Mi + 2 * x + (d >> 2)) * Mi + 2 * y + ((d >> 1) & 1)) * Mi + 2 * z + (d & 1);
+      acc += in[vi * C + c];
+    }
+    out[i] = 0.125f * acc;
+  }
+}
+
+// pool backward, accumulating: din[child] += dout[parent] / 8
+__global__ __launch_bounds__(kNT) void k_pool_bwd(const float *__restrict__ dout, float *__restrict__ din, int C,
+                                                  int Mi, long long Vi) {
+  const int Mo = Mi / 2;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < Vi * C; i += (long long)gridDim.x * kNT) {
+    const long long v = i / C;
+    const int c = (int)(i - v * C);
+    long long s;
+    int x, y, z;
+    split(v, Mi, s, x, y, z);
+    const long long vo = ((s * Mo + x / 2) * Mo + y / 2) * Mo + z / 2;
+    din[i] += 0.125f * dout[vo * C + c];
+  }
+}
+
+// nearest upsample: out (level M) from in (level M/2)
+__global__ __launch_bounds__(kNT) void k_up(const float *__restrict__ in, float *__restrict__ out, int C, int Mo,
+                                            long long Vo) {
+  const int Mi = Mo / 2;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < Vo * C; i += (long long)gridDim.x * kNT) {
+    const long long v = i / C;
+    const int c = (int)(i - v * C);
+    long long s;
+    int x, y, z;
+    split(v, Mo, s, x, y, z);
+    out[i] = in[(((s * Mi + x / 2) * Mi + y / 2) * Mi + z / 2) * C + c];
+  }
+}
+
+// upsample backward: din[parent] = sum of the 8 children of dout
+__global__ __launch_bounds__(kNT) void k_up_bwd(const float *__restrict__ dout, float *__restrict__ din, int C,
+                                                int Mi, long long Vi) {
+  const int Mo = 2 * Mi;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < Vi * C; i += (long long)gridDim.x * kNT) {
+    const long long v = i / C;
+    const int c = (int)(i - v * C);
+    long long s;
+    int x, y, z;
+    split(v, Mi, s, x, y, z);
+    float acc = 0.f;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const long long vo = ((s * Mo + 2 * x + (d >> 2)) * Mo + 2 * y + ((d >> 1) & 1)) * Mo + 2 * z + (d & 1);
+      acc += dout[vo * C + c];
+    }
+    din[i] = acc;
+  }
+}
+
+// x0[v][0] = exp(-Vext/kT) of the sample, x0[v][1] = its eta, channels 2..7 zero
+__global__ __launch_bounds__(kNT) void k_pack_input(const float *__restrict__ boltz, const float *__restrict__ eta,
+                                                    float *__restrict__ x0, long long M3, long long V) {
+  for (long long v = (long long)blockIdx.x * kNT + threadIdx.x; v < V; v += (long long)gridDim.x * kNT) {
+    const long long s = v / M3;
+    float *o = x0 + v * 8;
+    o[0] = boltz[v];
+    o[1] = eta[s];
+#pragma unroll
+    for (int c = 2; c < 8; ++c) o[c] = 0.f;
+  }
+}
+
+// head: u[v] = bh + sum_c wh[c] h[v][c]
+__global__ __launch_bounds__(kNT) void k_head(const float *__restrict__ h, int C, const float *__restrict__ wh,
+                                              const float *__restrict__ bh, float *__restrict__ u, long long V) {
+  for (long long v = (long long)blockIdx.x * kNT + threadIdx.x; v < V; v += (long long)gridDim.x * kNT) {
+    float acc = bh[0];
+    for (int c = 0; c < C; ++c) acc += wh[c] * h[v * C + c];
+    u[v] = acc;
+  }
+}
+
+// voxels that count in the loss: mask > 0, or boltz > 0 when no mask is given
+__device__ __forceinline__ float loss_mask(const float *mask, const float *x0, long long v) {
+  return mask ? mask[v] : x0[v * 8];
+}
+
+__global__ __launch_bounds__(kNT) void k_mask_count(const float *__restrict__ mask, const float *__restrict__ x0,
+                                                    unsigned long long *__restrict__ count, long long V) {
+  unsigned long long c = 0;
+  for (long long v = (long long)blockIdx.x * kNT + threadIdx.x; v < V; v += (long long)gridDim.x * kNT)
+    c += loss_mask(mask, x0, v) > 0.f;
+  __shared__ unsigned long long red[kNT];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = kNT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(count, red[0]);
+}
+
+// masked mean squared error: du = 2 (u - t) mask / count, partial sums of the loss
+__global__ __launch_bounds__(kNT) void k_loss(const float *__restrict__ u, const float *__restrict__ t,
+                                              const float *__restrict__ mask, const float *__restrict__ x0,
+                                              const unsigned long long *__restrict__ count,
+                                              float *__restrict__ du, float *__restrict__ loss_acc, long long V) {
+  const float inv_count = *count ? 1.f / (float)*count : 0.f;
+  float s = 0.f;
+  for (long long v = (long long)blockIdx.x * kNT + threadIdx.x; v < V; v += (long long)gridDim.x * kNT) {
+    const float m = loss_mask(mask, x0, v);
+    const float e = m > 0.f ? u[v] - t[v] : 0.f;
+    du[v] = 2.f * e * inv_count;
+    s += e * e;
+  }
+  __shared__ float red[kNT];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kNT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss_acc, red[0] * inv_count);
+}
+
+// head backward: dh[v][c] = du[v] wh[c]; gwh[c] += sum_v du h; gbh += sum_v du
+__global__ __launch_bounds__(kNT) void k_head_bwd_dh(const float *__restrict__ du, const float *__restrict__ wh,
+                                                     float *__restrict__ dh, int C, long long V) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < V * C; i += (long long)gridDim.x * kNT)
+    dh[i] = du[i / C] * wh[i % C];
+}
+
+__global__ __launch_bounds__(kNT) void k_head_bwd_w(const float *__restrict__ du, const float *__restrict__ h, int C,
+                                                    long long V, float *__restrict__ gwh, float *__restrict__ gbh) {
+  const int c = blockIdx.x;  // C+1 blocks: the last one sums du for the bias
+  float s = 0.f;
+  for (long long v = threadIdx.x; v < V; v += kNT) s += du[v] * (c < C ? h[v * C + c] : 1.f);
+  __shared__ float red[kNT];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kNT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (c < C)
+      gwh[c] += red[0];
+    else
+      gbh[0] += red[0];
+  }
+}
+
+// BF16 copy of the FP32 master weights, and the transposed, tap-flipped copy
+__global__ __launch_bounds__(kNT) void k_to_bf16(const float *__restrict__ w, __bf16 *__restrict__ wb, long long n) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
+    wb[i] = (__bf16)w[i];
+}
+
+__global__ __launch_bounds__(kNT) void k_transpose_w(const float *__restrict__ w, int cout, int cin, int kp,
+                                                     __bf16 *__restrict__ wt, int kpt) {
+  // wt[ci][t'*cout + co] = w[co][(26 - t')*cin + ci]; zero in the padding
+  const long long n = (long long)cin * kpt;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT) {
+    const int ci = (int)(i / kpt), k = (int)(i - (long long)ci * kpt);
+    float val = 0.f;
+    if (k < 27 * cout) {
+      const int tp = k / cout, co = k - tp * cout;
+      val = w[(long long)co * kp + (26 - tp) * cin + ci];
+    }
+    wt[i] = (__bf16)val;
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_adam(float *__restrict__ w, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, long long n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// out[b] = data[sample[b]] under the cubic symmetry + periodic shift xf[b] (9 ints:
+// axis permutation, flips, shifts): output coordinate o_a reads input axis perm[a] at
+// flip[a] ? M-1-((o_a - shift[a]) mod M) : (o_a - shift[a]) mod M
+__global__ __launch_bounds__(kNT) void k_make_batch(const float *__restrict__ data, const int *__restrict__ sample,
+                                                    const int *__restrict__ xf, float *__restrict__ out, int M,
+                                                    long long M3, long long V) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < V; i += (long long)gridDim.x * kNT) {
+    const long long b = i / M3;
+    long long r = i - b * M3;
+    int o[3];
+    o[0] = (int)(r / ((long long)M * M));
+    r -= (long long)o[0] * M * M;
+    o[1] = (int)(r / M);
+    o[2] = (int)(r - (long long)o[1] * M);
+    const int *f = xf + 9 * b;
+    int in[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      int c = o[a] - f[6 + a];
+      c += (c < 0) ? M : 0;
+      in[f[a]] = f[3 + a] ? M - 1 - c : c;
+    }
+    out[i] = data[(long long)sample[b] * M3 + ((long long)in[0] * M + in[1]) * M + in[2]];
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_boltz(const float *__restrict__ vext, float inv_kT,
+                                               float *__restrict__ boltz, long long n) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT) {
+    const float v = vext[i];
+    boltz[i] = (v == INFINITY) ? 0.f : expf(-v * inv_kT);
+  }
+}
+
+__global__ __launch_bounds__(kNT) void k_veff_from_u(const float *__restrict__ u, double kT, double log_nb,
+                                                     double *__restrict__ x, long long n) {
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
+    x[i] = -kT * (log_nb + (double)u[i]);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+int blocks_for(long long n) { return (int)std::max<long long>(1, std::min<long long>(4096, (n + kNT - 1) / kNT)); }
+
+struct Conv {
+  int cin = 0, cout = 0, kp = 0, kpt = 0, level = 0;
+  bool relu = true;
+  long long woff = 0, boff = 0;  // offsets into the parameter vector
+  __bf16 *wb = nullptr, *wt = nullptr;
+};
+
+}  // namespace
+
+struct deft_net_s {
+  int C[4] = {16, 32, 48, 64};
+  std::vector<Conv> conv;  // e0a e0b e1a e1b e2a e2b ba bb d2a d2b d1a d1b d0a d0b
+  long long head_w = 0, head_b = 0, nparams = 0;
+  float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  long long step = 0;
+  hipStream_t stream = nullptr;
+  // activation buffers for the current (batch, M)
+  int M = 0, B = 0;
+  std::vector<float *> bufs;
+  float *loss_acc = nullptr;   // host-mapped
+  unsigned long long *count_dev = nullptr;
+  float *eta_dev = nullptr;
+  float *ws = nullptr;  // warm-start staging: Boltzmann factor and prediction
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+enum Buf {
+  X0, A0, S0, P1, A1, S1, P2, A2, S2, P3, A3, B3, U2, R2, H2, D2, U1, R1, H1, D1, U0, R0, H0, D0, OUT,
+  // gradients / scratch
+  G_OUT, G_D0, G_H0, G_R0, G_U0, G_D1, G_H1, G_R1, G_U1, G_D2, G_H2, G_R2, G_U2, G_B3, G_A3, G_P3,
+  G_S2, G_A2, G_P2, G_S1, G_A1, G_P1, G_S0, G_A0, T0, T1, T2, T3, NUM_BUFS
+};
+
+struct BufSpec {
+  int level, ch;
+};
+
+BufSpec spec_of(const deft_net_s *n, int b) {
+  const int *C = n->C;
+  switch (b) {
+    case X0: return {0, 8};
+    case A0: case S0: case R0: case H0: case D0: case G_D0: case G_H0: case G_R0: case G_S0: case G_A0: return {0, C[0]};
+    case U0: case G_U0: return {0, C[1]};
+    case OUT: case G_OUT: return {0, 1};
+    case P1: case G_P1: return {1, C[0]};
+    case A1: case S1: case R1: case H1: case D1: case G_D1: case G_H1: case G_R1: case G_S1: case G_A1: return {1, C[1]};
+    case U1: case G_U1: return {1, C[2]};
+    case P2: case G_P2: return {2, C[1]};
+    case A2: case S2: case R2: case H2: case D2: case G_D2: case G_H2: case G_R2: case G_S2: case G_A2: return {2, C[2]};
+    case U2: case G_U2: return {2, C[3]};
+    case P3: case G_P3: return {3, C[2]};
+    case A3: case B3: case G_B3: case G_A3: return {3, C[3]};
+    case T0: return {0, 64};
+    case T1: return {1, 64};
+    case T2: return {2, 64};
+    case T3: return {3, 64};
+  }
+  return {0, 0};
+}
+
+long long voxels(const deft_net_s *n, int level) {
+  const long long m = n->M >> level;
+  return (long long)n->B * m * m * m;
+}
+
+int ensure_buffers(deft_net_s *n, int M, int B) {
+  if (n->M == M && n->B == B && !n->bufs.empty()) return DEFT_OK;
+  for (float *p : n->bufs) (void)hipFree(p);
+  n->bufs.assign(NUM_BUFS, nullptr);
+  n->M = M;
+  n->B = B;
+  for (int b = 0; b < NUM_BUFS; ++b) {
+    const BufSpec s = spec_of(n, b);
+    const size_t bytes = sizeof(float) * (size_t)voxels(n, s.level) * s.ch;
+    if (hipMalloc(&n->bufs[b], bytes) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "network buffer allocation failed");
+  }
+  (void)hipFree(n->eta_dev);
+  if (hipMalloc(&n->eta_dev, sizeof(float) * B) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "allocation failed");
+  return DEFT_OK;
+}
+
+void launch_conv(deft_net_s *n, const Conv &c, const float *in, const __bf16 *W, int kp, int cout, const float *bias,
+                 float *out, int level, bool relu) {
+  const long long V = voxels(n, level);
+  const int M = n->M >> level;
+  const long long tiles = (V + 15) / 16;
+  const dim3 grid((unsigned)((tiles + 3) / 4));
+  const int cin = (W == c.wb) ? c.cin : c.cout;
+  switch (cout / 16) {
+    case 1: hipLaunchKernelGGL((k_conv<1>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
+    case 2: hipLaunchKernelGGL((k_conv<2>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
+    case 3: hipLaunchKernelGGL((k_conv<3>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
+    default: hipLaunchKernelGGL((k_conv<4>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
+  }
+}
+
+void conv_forward(deft_net_s *n, int ci, const float *in, float *out) {
+  const Conv &c = n->conv[ci];
+  launch_conv(n, c, in, c.wb, c.kp, c.cout, n->params + c.boff, out, c.level, c.relu);
+}
+
+// backward through relu(conv(in)): dout is the gradient of the relu output `act`;
+// writes the gradient of `in` into din (skipped when din is null)
+void conv_backward(deft_net_s *n, int ci, const float *dout, const float *act, const float *in, float *din,
+                   float *scratch) {
+  const Conv &c = n->conv[ci];
+  const long long V = voxels(n, c.level);
+  const int M = n->M >> c.level;
+  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(V * c.cout)), dim3(kNT), 0, n->stream, dout, act, scratch,
+                     V * c.cout);
+  const long long vslice = 2048;
+  const dim3 wg((unsigned)((c.cout / 16) * (c.kp / 16)), (unsigned)((V + vslice - 1) / vslice));
+  hipLaunchKernelGGL(k_conv_wgrad, wg, dim3(64), 0, n->stream, scratch, c.cout, in, c.cin, c.kp,
+                     n->grads + c.woff, M, V, vslice);
+  hipLaunchKernelGGL(k_bias_grad, dim3(c.cout), dim3(kNT), 0, n->stream, scratch, c.cout, V, n->grads + c.boff);
+  if (din) launch_conv(n, c, scratch, c.wt, c.kpt, c.cin, nullptr, din, c.level, false);
+}
+
+void forward(deft_net_s *n) {
+  float **b = n->bufs.data();
+  auto blocks = [&](int buf) {
+    const BufSpec s = spec_of(n, buf);
+    return blocks_for(voxels(n, s.level) * s.ch);
+  };
+  conv_forward(n, 0, b[X0], b[A0]);
+  conv_forward(n, 1, b[A0], b[S0]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P1)), dim3(kNT), 0, n->stream, b[S0], b[P1], n->C[0], n->M >> 1, voxels(n, 1));
+  conv_forward(n, 2, b[P1], b[A1]);
+  conv_forward(n, 3, b[A1], b[S1]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P2)), dim3(kNT), 0, n->stream, b[S1], b[P2], n->C[1], n->M >> 2, voxels(n, 2));
+  conv_forward(n, 4, b[P2], b[A2]);
+  conv_forward(n, 5, b[A2], b[S2]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P3)), dim3(kNT), 0, n->stream, b[S2], b[P3], n->C[2], n->M >> 3, voxels(n, 3));
+  conv_forward(n, 6, b[P3], b[A3]);
+  conv_forward(n, 7, b[A3], b[B3]);
+  hipLaunchKernelGGL(k_up, dim3(blocks(U2)), dim3(kNT), 0, n->stream, b[B3], b[U2], n->C[3], n->M >> 2, voxels(n, 2));
+  conv_forward(n, 8, b[U2], b[R2]);
+  hipLaunchKernelGGL(k_add, dim3(blocks(H2)), dim3(kNT), 0, n->stream, b[R2], b[S2], b[H2], voxels(n, 2) * n->C[2]);
+  conv_forward(n, 9, b[H2], b[D2]);
+  hipLaunchKernelGGL(k_up, dim3(blocks(U1)), dim3(kNT), 0, n->stream, b[D2], b[U1], n->C[2], n->M >> 1, voxels(n, 1));
+  conv_forward(n, 10, b[U1], b[R1]);
+  hipLaunchKernelGGL(k_add, dim3(blocks(H1)), dim3(kNT), 0, n->stream, b[R1], b[S1], b[H1], voxels(n, 1) * n->C[1]);
+  conv_forward(n, 11, b[H1], b[D1]);
+  hipLaunchKernelGGL(k_up, dim3(blocks(U0)), dim3(kNT), 0, n->stream, b[D1], b[U0], n->C[1], n->M, voxels(n, 0));
+  conv_forward(n, 12, b[U0], b[R0]);
+  hipLaunchKernelGGL(k_add, dim3(blocks(H0)), dim3(kNT), 0, n->stream, b[R0], b[S0], b[H0], voxels(n, 0) * n->C[0]);
+  conv_forward(n, 13, b[H0], b[D0]);
+  hipLaunchKernelGGL(k_head, dim3(blocks_for(voxels(n, 0))), dim3(kNT), 0, n->stream, b[D0], n->C[0],
+                     n->params + n->head_w, n->params + n->head_b, b[OUT], voxels(n, 0));
+}
+
+// backward from G_OUT (gradient of the loss with respect to u) into n->grads
+void backward(deft_net_s *n) {
+  float **b = n->bufs.data();
+  const int *C = n->C;
+  auto blk = [&](long long cnt) { return dim3(blocks_for(cnt)); };
+  const long long V0 = voxels(n, 0), V1 = voxels(n, 1), V2 = voxels(n, 2), V3 = voxels(n, 3);
+  hipLaunchKernelGGL(k_head_bwd_w, dim3(C[0] + 1), dim3(kNT), 0, n->stream, b[G_OUT], b[D0], C[0], V0,
+                     n->grads + n->head_w, n->grads + n->head_b);
+  hipLaunchKernelGGL(k_head_bwd_dh, blk(V0 * C[0]), dim3(kNT), 0, n->stream, b[G_OUT], n->params + n->head_w,
+                     b[G_D0], C[0], V0);
+  // level 0 decoder: d0 = relu(conv13(h0)), h0 = r0 + s0, r0 = relu(conv12(u0))
+  conv_backward(n, 13, b[G_D0], b[D0], b[H0], b[G_H0], b[T0]);
+  (void)hipMemcpyAsync(b[G_S0], b[G_H0], sizeof(float) * V0 * C[0], hipMemcpyDeviceToDevice, n->stream);
+  conv_backward(n, 12, b[G_H0], b[R0], b[U0], b[G_U0], b[T0]);
+  hipLaunchKernelGGL(k_up_bwd, blk(V1 * C[1]), dim3(kNT), 0, n->stream, b[G_U0], b[G_D1], C[1], n->M >> 1, V1);
+  // level 1 decoder
+  conv_backward(n, 11, b[G_D1], b[D1], b[H1], b[G_H1], b[T1]);
+  (void)hipMemcpyAsync(b[G_S1], b[G_H1], sizeof(float) * V1 * C[1], hipMemcpyDeviceToDevice, n->stream);
+  conv_backward(n, 10, b[G_H1], b[R1], b[U1], b[G_U1], b[T1]);
+  hipLaunchKernelGGL(k_up_bwd, blk(V2 * C[2]), dim3(kNT), 0, n->stream, b[G_U1], b[G_D2], C[2], n->M >> 2, V2);
+  // level 2 decoder
+  conv_backward(n, 9, b[G_D2], b[D2], b[H2], b[G_H2], b[T2]);
+  (void)hipMemcpyAsync(b[G_S2], b[G_H2], sizeof(float) * V2 * C[2], hipMemcpyDeviceToDevice, n->stream);
+  conv_backward(n, 8, b[G_H2], b[R2], b[U2], b[G_U2], b[T2]);
+  hipLaunchKernelGGL(k_up_bwd, blk(V3 * C[3]), dim3(kNT), 0, n->stream, b[G_U2], b[G_B3], C[3], n->M >> 3, V3);
+  // bottleneck
+  conv_backward(n, 7, b[G_B3], b[B3], b[A3], b[G_A3], b[T3]);
+  conv_backward(n, 6, b[G_A3], b[A3], b[P3], b[G_P3], b[T3]);
+  hipLaunchKernelGGL(k_pool_bwd, blk(V2 * C[2]), dim3(kNT), 0, n->stream, b[G_P3], b[G_S2], C[2], n->M >> 2, V2);
+  // encoder level 2
+  conv_backward(n, 5, b[G_S2], b[S2], b[A2], b[G_A2], b[T2]);
+  conv_backward(n, 4, b[G_A2], b[A2], b[P2], b[G_P2], b[T2]);
+  hipLaunchKernelGGL(k_pool_bwd, blk(V1 * C[1]), dim3(kNT), 0, n->stream, b[G_P2], b[G_S1], C[1], n->M >> 1, V1);
+  // encoder level 1
+  conv_backward(n, 3, b[G_S1], b[S1], b[A1], b[G_A1], b[T1]);
+  conv_backward(n, 2, b[G_A1], b[A1], b[P1], b[G_P1], b[T1]);
+  hipLaunchKernelGGL(k_pool_bwd, blk(V0 * C[0]), dim3(kNT), 0, n->stream, b[G_P1], b[G_S0], C[0], n->M, V0);
+  // encoder level 0 (no gradient with respect to the input)
+  conv_backward(n, 1, b[G_S0], b[S0], b[A0], b[G_A0], b[T0]);
+  conv_backward(n, 0, b[G_A0], b[A0], b[X0], nullptr, b[T0]);
+}
+
+// refresh the BF16 copies after the FP32 masters changed
+void refresh_weights(deft_net_s *n) {
+  for (const Conv &c : n->conv) {
+    const long long nw = (long long)c.cout * c.kp;
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(nw)), dim3(kNT), 0, n->stream, n->params + c.woff, c.wb, nw);
+    if (c.wt)
+      hipLaunchKernelGGL(k_transpose_w, dim3(blocks_for((long long)c.cin * c.kpt)), dim3(kNT), 0, n->stream,
+                         n->params + c.woff, c.cout, c.cin, c.kp, c.wt, c.kpt);
+  }
+}
+
+int check_net(deft_net n) { return n ? DEFT_OK : set_error(DEFT_ERR_ARG, "null network"); }
+
+int check_shape(int M, int B) {
+  if (B < 1 || M < 16 || M % 8 != 0 || M > 512) return set_error(DEFT_ERR_ARG, "M must be a multiple of 8 in [16, 512], batch >= 1");
+  return DEFT_OK;
+}
+
+int prepare(deft_net_s *n, int M, int B, const float *boltz_dev, const float *eta_host) {
+  NN_TRY(ensure_buffers(n, M, B));
+  NN_HIP(hipMemcpyAsync(n->eta_dev, eta_host, sizeof(float) * B, hipMemcpyHostToDevice, n->stream));
+  const long long M3 = (long long)M * M * M;
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(M3 * B)), dim3(kNT), 0, n->stream, boltz_dev, n->eta_dev,
+                     n->bufs[X0], M3, M3 * B);
+  return DEFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int deft_net_create(const int channels[4], unsigned long long seed, deft_net *out) {
+  if (!out) return set_error(DEFT_ERR_ARG, "null output");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(DEFT_ERR_HIP, "no HIP device");
+  deft_net_s *n = new deft_net_s();
+  if (channels)
+    for (int i = 0; i < 4; ++i) {
+      if (channels[i] < 16 || channels[i] > 64 || channels[i] % 16) {
+        delete n;
+        return set_error(DEFT_ERR_ARG, "channels must be multiples of 16 in [16, 64]");
+      }
+      n->C[i] = channels[i];
+    }
+  const int *C = n->C;
+  // (cin, cout, level, relu) of the 14 convolutions
+  const int spec[14][3] = {{8, C[0], 0},    {C[0], C[0], 0}, {C[0], C[1], 1}, {C[1], C[1], 1}, {C[1], C[2], 2},
+                           {C[2], C[2], 2}, {C[2], C[3], 3}, {C[3], C[3], 3}, {C[3], C[2], 2}, {C[2], C[2], 2},
+                           {C[2], C[1], 1}, {C[1], C[1], 1}, {C[1], C[0], 0}, {C[0], C[0], 0}};
+  long long off = 0;
+  for (int i = 0; i < 14; ++i) {
+    Conv c;
+    c.cin = spec[i][0];
+    c.cout = spec[i][1];
+    c.level = spec[i][2];
+    c.kp = (27 * c.cin + 31) / 32 * 32;
+    c.kpt = (27 * c.cout + 31) / 32 * 32;
+    c.woff = off;
+    off += (long long)c.cout * c.kp;
+    c.boff = off;
+    off += c.cout;
+    n->conv.push_back(c);
+  }
+  n->head_w = off;
+  off += C[0];
+  n->head_b = off;
+  off += 1;
+  n->nparams = off;
+  auto fail = [&](const char *what) {
+    deft_net_destroy(n);
+    return set_error(DEFT_ERR_ALLOC, what);
+  };
+  if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
+  const size_t pb = sizeof(float) * (size_t)off;
+  if (hipMalloc(&n->params, pb) != hipSuccess || hipMalloc(&n->grads, pb) != hipSuccess ||
+      hipMalloc(&n->adam_m, pb) != hipSuccess || hipMalloc(&n->adam_v, pb) != hipSuccess ||
+      hipMalloc(&n->count_dev, sizeof(unsigned long long)) != hipSuccess)
+    return fail("parameter allocation failed");
+  if (hipHostMalloc(&n->loss_acc, sizeof(float), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return fail("pinned allocation failed");
+  for (Conv &c : n->conv) {
+    if (hipMalloc(&c.wb, sizeof(__bf16) * (size_t)c.cout * c.kp) != hipSuccess) return fail("weight allocation failed");
+    if (c.cin % 16 == 0 && hipMalloc(&c.wt, sizeof(__bf16) * (size_t)c.cin * c.kpt) != hipSuccess)
+      return fail("weight allocation failed");
+  }
+  // He-style initialisation from a fixed-seed generator (host), zero biases and padding
+  std::vector<float> h((size_t)off, 0.f);
+  unsigned long long st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  auto uni = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (double)((st >> 11) & ((1ULL << 53) - 1)) / (double)(1ULL << 53);
+  };
+  for (const Conv &c : n->conv) {
+    const int fan_in = 27 * (c.woff == 0 ? 2 : c.cin);
+    const double a = std::sqrt(6.0 / fan_in);
+    for (int co = 0; co < c.cout; ++co)
+      for (int k = 0; k < 27 * c.cin; ++k) {
+        const int ci = k % c.cin;
+        if (c.woff == 0 && ci >= 2) continue;  // padded input channels
+        h[(size_t)(c.woff + (long long)co * c.kp + k)] = (float)((2 * uni() - 1) * a);
+      }
+  }
+  for (int c = 0; c < C[0]; ++c) h[(size_t)(n->head_w + c)] = (float)((2 * uni() - 1) * std::sqrt(3.0 / C[0]));
+  if (hipMemcpy(n->params, h.data(), pb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(n->grads, 0, pb) != hipSuccess || hipMemset(n->adam_m, 0, pb) != hipSuccess ||
+      hipMemset(n->adam_v, 0, pb) != hipSuccess)
+    return fail("initialisation failed");
+  refresh_weights(n);
+  if (hipStreamSynchronize(n->stream) != hipSuccess) return fail("initialisation failed");
+  *out = n;
+  return DEFT_OK;
+}
+
+int deft_net_destroy(deft_net n) {
+  if (!n) return DEFT_OK;
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  for (float *p : n->bufs) (void)hipFree(p);
+  for (Conv &c : n->conv) {
+    (void)hipFree(c.wb);
+    (void)hipFree(c.wt);
+  }
+  (void)hipFree(n->params);
+  (void)hipFree(n->grads);
+  (void)hipFree(n->adam_m);
+  (void)hipFree(n->adam_v);
+  (void)hipFree(n->count_dev);
+  (void)hipFree(n->eta_dev);
+  (void)hipFree(n->ws);
+  if (n->loss_acc) (void)hipHostFree(n->loss_acc);
+  if (n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+  return DEFT_OK;
+}
+
+long long deft_net_num_params(deft_net n) { return n ? n->nparams : -1; }
+
+int deft_net_layout(deft_net n, int conv, int *cin, int *cout, int *kp, long long *woff, long long *boff) {
+  NN_TRY(check_net(n));
+  if (conv == -1) {  // head
+    if (cin) *cin = n->C[0];
+    if (cout) *cout = 1;
+    if (kp) *kp = n->C[0];
+    if (woff) *woff = n->head_w;
+    if (boff) *boff = n->head_b;
+    return DEFT_OK;
+  }
+  if (conv < 0 || conv >= (int)n->conv.size()) return set_error(DEFT_ERR_ARG, "conv index out of range");
+  const Conv &c = n->conv[conv];
+  if (cin) *cin = c.cin;
+  if (cout) *cout = c.cout;
+  if (kp) *kp = c.kp;
+  if (woff) *woff = c.woff;
+  if (boff) *boff = c.boff;
+  return DEFT_OK;
+}
+
+int deft_net_get_params(deft_net n, float *out) {
+  NN_TRY(check_net(n));
+  if (!out) return set_error(DEFT_ERR_ARG, "null output");
+  NN_HIP(hipMemcpyAsync(out, n->params, sizeof(float) * n->nparams, hipMemcpyDeviceToHost, n->stream));
+  NN_HIP(hipStreamSynchronize(n->stream));
+  return DEFT_OK;
+}
+
+int deft_net_set_params(deft_net n, const float *in) {
+  NN_TRY(check_net(n));
+  if (!in) return set_error(DEFT_ERR_ARG, "null input");
+  NN_HIP(hipMemcpyAsync(n->params, in, sizeof(float) * n->nparams, hipMemcpyHostToDevice, n->stream));
+  refresh_weights(n);
+  NN_HIP(hipStreamSynchronize(n->stream));
+  n->step = 0;
+  NN_HIP(hipMemset(n->adam_m, 0, sizeof(float) * n->nparams));
+  NN_HIP(hipMemset(n->adam_v, 0, sizeof(float) * n->nparams));
+  return DEFT_OK;
+}
+
+int deft_net_forward(deft_net n, int M, int batch, const float *boltz_dev, const float *eta, float *u_dev) {
+  NN_TRY(check_net(n));
+  NN_TRY(check_shape(M, batch));
+  if (!boltz_dev || !eta || !u_dev) return set_error(DEFT_ERR_ARG, "null buffer");
+  NN_TRY(prepare(n, M, batch, boltz_dev, eta));
+  forward(n);
+  NN_HIP(hipMemcpyAsync(u_dev, n->bufs[OUT], sizeof(float) * voxels(n, 0), hipMemcpyDeviceToDevice, n->stream));
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipStreamSynchronize(n->stream));
+  return DEFT_OK;
+}
+
+int deft_net_loss_grad(deft_net n, int M, int batch, const float *boltz_dev, const float *eta,
+                       const float *target_dev, const float *mask_dev, float *loss, float *grads_host) {
+  NN_TRY(check_net(n));
+  NN_TRY(check_shape(M, batch));
+  if (!boltz_dev || !eta || !target_dev) return set_error(DEFT_ERR_ARG, "null buffer");
+  NN_TRY(prepare(n, M, batch, boltz_dev, eta));
+  forward(n);
+  const long long V = voxels(n, 0);
+  NN_HIP(hipMemsetAsync(n->grads, 0, sizeof(float) * n->nparams, n->stream));
+  NN_HIP(hipMemsetAsync(n->count_dev, 0, sizeof(unsigned long long), n->stream));
+  *n->loss_acc = 0.f;
+  float *acc_dev = nullptr;
+  NN_HIP(hipHostGetDevicePointer((void **)&acc_dev, n->loss_acc, 0));
+  hipLaunchKernelGGL(k_mask_count, dim3(blocks_for(V)), dim3(kNT), 0, n->stream, mask_dev, n->bufs[X0], n->count_dev,
+                     V);
+  hipLaunchKernelGGL(k_loss, dim3(blocks_for(V)), dim3(kNT), 0, n->stream, n->bufs[OUT], target_dev, mask_dev,
+                     n->bufs[X0], n->count_dev, n->bufs[G_OUT], acc_dev, V);
+  backward(n);
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipStreamSynchronize(n->stream));
+  if (loss) *loss = *n->loss_acc;
+  if (grads_host) NN_HIP(hipMemcpy(grads_host, n->grads, sizeof(float) * n->nparams, hipMemcpyDeviceToHost));
+  return DEFT_OK;
+}
+
+int deft_net_train_step(deft_net n, int M, int batch, const float *boltz_dev, const float *eta,
+                        const float *target_dev, const float *mask_dev, float lr, float *loss) {
+  NN_TRY(deft_net_loss_grad(n, M, batch, boltz_dev, eta, target_dev, mask_dev, loss, nullptr));
+  n->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)n->step), c2 = 1.f - std::pow(b2, (float)n->step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n->nparams)), dim3(kNT), 0, n->stream, n->params, n->grads, n->adam_m,
+                     n->adam_v, n->nparams, lr, b1, b2, 1e-8f, c1, c2);
+  refresh_weights(n);
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipStreamSynchronize(n->stream));
+  return DEFT_OK;
+}
+
+int deft_buffer_create(unsigned long long bytes, void **dev) {
+  if (!dev) return set_error(DEFT_ERR_ARG, "null output");
+  *dev = nullptr;
+  if (bytes == 0) return set_error(DEFT_ERR_ARG, "empty buffer");
+  if (hipMalloc(dev, bytes) != hipSuccess) {
+    *dev = nullptr;
+    return set_error(DEFT_ERR_ALLOC, "device buffer allocation failed");
+  }
+  return DEFT_OK;
+}
+
+int deft_buffer_destroy(void *dev) {
+  if (dev) NN_HIP(hipFree(dev));
+  return DEFT_OK;
+}
+
+int deft_buffer_upload(void *dev, unsigned long long offset, const void *host, unsigned long long bytes) {
+  if (!dev || !host) return set_error(DEFT_ERR_ARG, "null buffer");
+  NN_HIP(hipMemcpy(static_cast<char *>(dev) + offset, host, bytes, hipMemcpyHostToDevice));
+  return DEFT_OK;
+}
+
+int deft_buffer_download(const void *dev, unsigned long long offset, void *host, unsigned long long bytes) {
+  if (!dev || !host) return set_error(DEFT_ERR_ARG, "null buffer");
+  NN_HIP(hipMemcpy(host, static_cast<const char *>(dev) + offset, bytes, hipMemcpyDeviceToHost));
+  return DEFT_OK;
+}
+
+int deft_net_make_batch(const float *data_dev, long long count, int M, int batch, const int *samples,
+                        const int *transforms, float *out_dev) {
+  NN_TRY(check_shape(M, batch));
+  if (!data_dev || !samples || !transforms || !out_dev) return set_error(DEFT_ERR_ARG, "null buffer");
+  for (int b = 0; b < batch; ++b) {
+    if (samples[b] < 0 || samples[b] >= count) return set_error(DEFT_ERR_ARG, "sample index out of range");
+    const int *f = transforms + 9 * b;
+    if ((1 << f[0] | 1 << f[1] | 1 << f[2]) != 7) return set_error(DEFT_ERR_ARG, "transform axes must permute 0, 1, 2");
+    for (int a = 0; a < 3; ++a)
+      if (f[3 + a] < 0 || f[3 + a] > 1 || f[6 + a] < 0 || f[6 + a] >= M)
+        return set_error(DEFT_ERR_ARG, "flip must be 0/1 and shift in [0, M)");
+  }
+  int *dmeta = nullptr;
+  NN_HIP(hipMalloc(&dmeta, sizeof(int) * 10 * batch));
+  std::vector<int> meta((size_t)10 * batch);
+  std::memcpy(meta.data(), samples, sizeof(int) * batch);
+  std::memcpy(meta.data() + batch, transforms, sizeof(int) * 9 * batch);
+  const long long M3 = (long long)M * M * M;
+  hipError_t e = hipMemcpy(dmeta, meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_make_batch, dim3(blocks_for(M3 * batch)), dim3(kNT), 0, 0, data_dev, dmeta, dmeta + batch,
+                       out_dev, M, M3, M3 * batch);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
+  (void)hipFree(dmeta);
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("make_batch: ") + hipGetErrorString(e));
+  return DEFT_OK;
+}
+
+int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
+  if (!p) return set_error(DEFT_ERR_ARG, "null plan");
+  NN_TRY(check_net(n));
+  if (!(eta > 0 && eta < 0.74)) return set_error(DEFT_ERR_ARG, "eta must lie in (0, 0.74)");
+  if (p->Nx != p->Ny || p->Nx != p->Nz) return set_error(DEFT_ERR_UNSUPPORTED, "warm start needs a cubic grid");
+  const int M = p->Nx;
+  NN_TRY(check_shape(M, 1));
+  const size_t need = sizeof(float) * 2 * (size_t)p->N;
+  if (n->ws_bytes < need) {
+    (void)hipFree(n->ws);
+    n->ws = nullptr;
+    n->ws_bytes = 0;
+    NN_HIP(hipMalloc(&n->ws, need));
+    n->ws_bytes = need;
+  }
+  float *boltz = n->ws, *u = n->ws + p->N;
+  hipLaunchKernelGGL(k_boltz, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT), boltz,
+                     p->N);
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipStreamSynchronize(p->stream));
+  const float e = (float)eta;
+  NN_TRY(deft_net_forward(n, M, 1, boltz, &e, u));
+  const double nb = eta / (4 * M_PI / 3 * p->R * p->R * p->R);
+  p->npend = 0;
+  hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb), p->x,
+                     p->N);
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+// Plain BF16 MFMA GEMM probe: C[m][n] = sum_k A[m][k] B[n][k] for one 16x16
+// tile and K = 32 (A, B in FP32, rounded to BF16), to check the lane layout.
+int deft_net_mfma_probe(const float *a_host, const float *b_host, float *c_host) {
+  if (!a_host || !b_host || !c_host) return set_error(DEFT_ERR_ARG, "null buffer");
+  // runs k_conv on a 16^3 grid whose input is non-zero on voxels 0..15 only and whose
+  // weights are non-zero on the centre tap only, so output voxels 0..15 are the GEMM
+  float *in = nullptr, *out = nullptr;
+  __bf16 *w = nullptr;
+  const int M = 16, cin = 32, kp = 27 * 32;
+  const long long V = (long long)M * M * M;
+  NN_HIP(hipMalloc(&in, sizeof(float) * V * cin));
+  NN_HIP(hipMalloc(&out, sizeof(float) * V * 16));
+  NN_HIP(hipMalloc(&w, sizeof(__bf16) * 16 * kp));
+  std::vector<float> hin((size_t)(V * cin), 0.f);
+  std::vector<__bf16> hw((size_t)16 * kp, (__bf16)0.f);
+  for (int m = 0; m < 16; ++m)
+    for (int k = 0; k < 32; ++k) hin[(size_t)m * cin + k] = a_host[m * 32 + k];  // voxels 0..15 (z line of x=y=0)
+  for (int n = 0; n < 16; ++n)
+    for (int k = 0; k < 32; ++k) hw[(size_t)n * kp + 13 * 32 + k] = (__bf16)b_host[n * 32 + k];  // centre tap
+  NN_HIP(hipMemcpy(in, hin.data(), sizeof(float) * hin.size(), hipMemcpyHostToDevice));
+  NN_HIP(hipMemcpy(w, hw.data(), sizeof(__bf16) * hw.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL((k_conv<1>), dim3(1), dim3(kNT), 0, 0, in, cin, w, kp, nullptr, out, M, V, 0);
+  NN_HIP(hipGetLastError());
+  NN_HIP(hipDeviceSynchronize());
+  std::vector<float> hout((size_t)16 * 16);
+  NN_HIP(hipMemcpy(hout.data(), out, sizeof(float) * 16 * 16, hipMemcpyDeviceToHost));
+  for (int i = 0; i < 256; ++i) c_host[i] = hout[i];
+  (void)hipFree(in);
+  (void)hipFree(out);
+  (void)hipFree(w);
+  return DEFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,306 @@
+"""Warm-start network: the float64 oracle pinned by finite differences and exact
+bfloat16 rounding cases (CPU), then the device forward, backward, training and
+warm start against it (GPU)."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import deft_amd
+import deft_amd.nn
+from oracle import unet_oracle as U
+
+SMALL = (16, 16, 16, 16)
+
+
+def random_params(channels, seed):
+    convs, hw, hb, n = U.layout(channels)
+    rng = np.random.default_rng(seed)
+    p = np.zeros(n)
+    for c in convs:
+        W = rng.uniform(-1, 1, (c["cout"], c["kp"])) * math.sqrt(6.0 / (27 * c["cin"]))
+        W[:, 27 * c["cin"]:] = 0
+        if c["woff"] == 0:  # input channels 2..7 carry zeros
+            k = np.arange(c["kp"])
+            W[:, (k < 27 * 8) & (k % 8 >= 2)] = 0
+        p[c["woff"]:c["woff"] + W.size] = W.ravel()
+        p[c["boff"]:c["boff"] + c["cout"]] = rng.uniform(0.0, 0.1, c["cout"])
+    p[hw:hb] = rng.uniform(-0.5, 0.5, channels[0])
+    p[hb] = 0.1
+    return p.astype(np.float32)
+
+
+def random_inputs(B, M, seed):
+    rng = np.random.default_rng(seed)
+    boltz = np.where(rng.random((B, M, M, M)) > 0.2, rng.random((B, M, M, M)), 0.0).astype(np.float32)
+    eta = rng.uniform(0.1, 0.45, B).astype(np.float32)
+    target = rng.normal(size=(B, M, M, M)).astype(np.float32)
+    mask = (boltz > 0).astype(np.float32)
+    return boltz, eta, target, mask
+
+
+# ----------------------------------------------------------------- CPU: oracle
+
+def test_bf16_rounding_known_values():
+    x = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 1 + 2 ** -7, -2.5, 1 + 2 ** -8 + 2 ** -20, 3.0e38],
+                 dtype=np.float32)
+    # ties go to the even mantissa; anything past the tie rounds up
+    want = [1.0, 1.0, 1 + 2 ** -6, 1 + 2 ** -7, -2.5, 1 + 2 ** -7, 2.9979e38]
+    got = U.bf16(x)
+    assert np.allclose(got[:6], want[:6], rtol=0, atol=0)
+    assert abs(got[6] / 3.0e38 - 1) < 2 ** -8
+
+
+def test_layout_counts():
+    convs, hw, hb, n = U.layout((16, 32, 48, 64))
+    assert [c["kp"] for c in convs[:4]] == [224, 448, 448, 864]
+    assert n == sum(c["cout"] * c["kp"] + c["cout"] for c in convs) + 16 + 1
+    assert convs[1]["woff"] == 16 * 224 + 16
+
+
+def test_pool_up_are_adjoint():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(2, 8, 8, 8, 3))
+    y = rng.normal(size=(2, 4, 4, 4, 3))
+    assert abs((U.pool(x) * y).sum() - (x * U.pool_backward(y)).sum()) < 1e-12
+    assert abs((U.up(y) * x).sum() - (y * U.up_backward(x)).sum()) < 1e-12
+    assert np.allclose(U.pool(U.up(y)), y)
+
+
+def test_conv_is_a_periodic_correlation():
+    """A single tap of the convolution shifts the input by that tap's offset."""
+    c = dict(cin=8, cout=16, kp=224, woff=0, boff=16 * 224)
+    p = np.zeros(16 * 224 + 16)
+    t = 9 * 2 + 3 * 0 + 1  # offset (+1, -1, 0)
+    p[3 * 224 + t * 8 + 0] = 1.0  # out channel 3 reads input channel 0 at that tap
+    x = np.zeros((1, 8, 8, 8, 8))
+    x[0, 5, 0, 4, 0] = 2.0
+    out = U.conv(x, p, c, False)
+    assert out[0, 4, 1, 4, 3] == 2.0  # v + (1, -1, 0) = (5, 0, 4)
+    assert out.sum() == 2.0
+
+
+def test_oracle_gradient_matches_finite_differences():
+    ch = SMALL
+    p = random_params(ch, 1).astype(np.float64)
+    boltz, eta, target, mask = random_inputs(2, 16, 2)
+    L0, g = U.loss_and_grad(p, boltz, eta, target, mask, ch)
+    rng = np.random.default_rng(3)
+    convs, hw, hb, n = U.layout(ch)
+    blocks = [(c["woff"], c["woff"] + c["cout"] * c["kp"]) for c in convs]
+    blocks += [(c["boff"], c["boff"] + c["cout"]) for c in convs] + [(hw, hb + 1)]
+    # central differences along a random direction inside each parameter block; the
+    # step is small enough that no ReLU input changes sign
+    e = 1e-7
+    for a, b in blocks:
+        d = np.zeros(n)
+        d[a:b] = rng.normal(size=b - a)
+        for c in convs:  # keep the zero padding zero
+            d[c["woff"]:c["woff"] + c["cout"] * c["kp"]].reshape(c["cout"], c["kp"])[:, 27 * c["cin"]:] = 0
+        fd = (U.loss_and_grad(p + e * d, boltz, eta, target, mask, ch)[0]
+              - U.loss_and_grad(p - e * d, boltz, eta, target, mask, ch)[0]) / (2 * e)
+        assert abs(fd - g @ d) < 1e-6 * abs(g @ d), (a, b, fd, g @ d)
+
+
+def test_random_vext_shapes():
+    rng = np.random.default_rng(0)
+    seen_inf = False
+    for _ in range(12):
+        V = deft_amd.nn.random_vext(32, 3.2, rng)
+        assert V.shape == (32, 32, 32)
+        assert np.isfinite(V).mean() > 0.3
+        assert not np.any(np.isnan(V))
+        seen_inf |= bool(np.isinf(V).any())
+    assert seen_inf
+
+
+# ----------------------------------------------------------------- GPU
+
+def _dev(a):
+    return deft_amd.nn.DeviceArray.from_numpy(a)
+
+
+def transform_numpy(x, t):
+    """deft_net_make_batch's mapping restated: output axis a reads input axis t[a] at
+    flip ? M-1-((o-s) mod M) : (o-s) mod M."""
+    M = x.shape[0]
+    o = np.indices((M, M, M))
+    src = [None] * 3
+    for a in range(3):
+        c = (o[a] - t[6 + a]) % M
+        src[t[a]] = M - 1 - c if t[3 + a] else c
+    return x[src[0], src[1], src[2]]
+
+
+@pytest.mark.gpu
+def test_mfma_lane_layout(gpu):
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=(16, 32)).astype(np.float32)
+    b = rng.normal(size=(16, 32)).astype(np.float32)
+    c = deft_amd.nn.mfma_probe(a, b)
+    want = U.bf16(a) @ U.bf16(b).T
+    assert np.max(np.abs(c - want)) < 1e-5 * np.max(np.abs(want))
+
+
+@pytest.mark.gpu
+def test_layout_matches_oracle(gpu):
+    net = deft_amd.nn.WarmStartNet()
+    convs, hw, hb, n = U.layout(net.channels)
+    assert net.num_params == n
+    for i, c in enumerate(convs):
+        assert net.layout(i) == (c["cin"], c["cout"], c["kp"], c["woff"], c["boff"])
+    assert net.layout(-1)[3:] == (hw, hb)
+    p = net.params
+    for c in convs:  # padding of the initial weights is zero
+        W = p[c["woff"]:c["woff"] + c["cout"] * c["kp"]].reshape(c["cout"], c["kp"])
+        assert not np.any(W[:, 27 * c["cin"]:])
+    net.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("channels", [SMALL, (16, 32, 48, 64)])
+def test_forward_matches_oracle(gpu, channels):
+    net = deft_amd.nn.WarmStartNet(channels)
+    p = random_params(channels, 7)
+    net.params = p
+    boltz, eta, _, _ = random_inputs(2, 16, 8)
+    u = net.forward(_dev(boltz), eta).numpy()
+    want, _ = U.forward(p, boltz, eta, channels, emulate_bf16=True)
+
+    def rms_rel(a, b):
+        return np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2))
+
+    # Residual differences are FP32 accumulation order, which now and then moves an
+    # activation across a BF16 rounding boundary (2^-8 relative) and propagates.
+    # Without BF16 emulation, or with truncating conversions, the RMS is 1e-2 or more.
+    assert rms_rel(u, want) < 3e-3, rms_rel(u, want)
+    assert np.max(np.abs(u - want)) < 2e-2 * np.max(np.abs(want))
+    exact, _ = U.forward(p, boltz, eta, channels, emulate_bf16=False)
+    assert rms_rel(u, exact) < 5e-2
+    net.close()
+
+
+@pytest.mark.gpu
+def test_loss_grad_matches_oracle(gpu):
+    ch = (16, 32, 48, 64)
+    net = deft_amd.nn.WarmStartNet(ch)
+    p = random_params(ch, 11)
+    net.params = p
+    boltz, eta, target, mask = random_inputs(2, 16, 12)
+    loss, g = net.loss_grad(_dev(boltz), eta, _dev(target), _dev(mask))
+    lo, go = U.loss_and_grad(p, boltz, eta, target, mask, ch, emulate_bf16=True)
+    assert abs(loss - lo) < 5e-3 * lo
+    convs, hw, hb, n = U.layout(ch)
+    blocks = [(c["woff"], c["woff"] + c["cout"] * c["kp"]) for c in convs]
+    blocks += [(c["boff"], c["boff"] + c["cout"]) for c in convs] + [(hw, hb), (hb, hb + 1)]
+    for a, b in blocks:
+        ref = go[a:b]
+        assert np.linalg.norm(g[a:b] - ref) <= 2e-2 * np.linalg.norm(ref) + 1e-7, (a, b)
+    net.close()
+
+
+@pytest.mark.gpu
+def test_loss_without_mask_uses_boltz(gpu):
+    net = deft_amd.nn.WarmStartNet(SMALL, seed=2)
+    boltz, eta, target, mask = random_inputs(2, 16, 13)
+    b, t = _dev(boltz), _dev(target)
+    l1, g1 = net.loss_grad(b, eta, t, _dev(mask))
+    l2, g2 = net.loss_grad(b, eta, t)
+    # the weight gradients are summed with atomics, so agreement is to FP32 rounding
+    assert abs(l1 - l2) < 1e-6 * l1
+    assert np.linalg.norm(g1 - g2) < 1e-5 * np.linalg.norm(g1)
+    net.close()
+
+
+@pytest.mark.gpu
+def test_make_batch_matches_numpy(gpu):
+    rng = np.random.default_rng(6)
+    M, n = 16, 3
+    data = rng.normal(size=(n, M, M, M)).astype(np.float32)
+    d = _dev(data)
+    samples = np.array([2, 0, 2, 1], dtype=np.int32)
+    xf = deft_amd.nn.random_transforms(rng, 4, M)
+    xf[0] = [0, 1, 2, 0, 0, 0, 0, 0, 0]  # identity
+    out = deft_amd.nn.DeviceArray((4, M, M, M))
+    deft_amd.nn.make_batch(d, n, M, samples, xf, out)
+    got = out.numpy()
+    assert np.array_equal(got[0], data[2])
+    for b in range(4):
+        assert np.array_equal(got[b], transform_numpy(data[samples[b]], xf[b])), b
+    with pytest.raises(deft_amd.DeftError):
+        deft_amd.nn.make_batch(d, n, M, [3], xf[:1], out)
+    bad = xf[:1].copy()
+    bad[0, :3] = [0, 0, 2]
+    with pytest.raises(deft_amd.DeftError):
+        deft_amd.nn.make_batch(d, n, M, [0], bad, out)
+
+
+@pytest.mark.gpu
+def test_training_steps_reduce_loss(gpu):
+    net = deft_amd.nn.WarmStartNet(SMALL, seed=3)
+    boltz, eta, _, mask = random_inputs(2, 16, 4)
+    # a smooth target the network can fit: a function of the local Boltzmann factor
+    target = np.where(mask > 0, 0.5 * boltz - 0.2, 0).astype(np.float32)
+    b, t = _dev(boltz), _dev(target)
+    first = net.train_step(b, eta, t, 3e-3)
+    for _ in range(100):
+        last = net.train_step(b, eta, t, 3e-3)
+    assert last < 0.5 * first, (first, last)
+    assert np.all(np.isfinite(net.forward(b, eta).numpy()))
+    net.close()
+
+
+@pytest.mark.gpu
+def test_warm_start_rejects_bad_plans(gpu):
+    net = deft_amd.nn.WarmStartNet(SMALL)
+    f = deft_amd.WhiteBearFluidVeff((32, 32, 20), np.diag([3.2, 3.2, 2.0]), R=1, kT=1, mu=0.0)
+    with pytest.raises(deft_amd.DeftError):
+        net.warm_start(f, 0.3)
+    g = deft_amd.WhiteBearFluidVeff((32, 32, 32), np.diag([3.2] * 3), R=1, kT=1, mu=0.0)
+    with pytest.raises(deft_amd.DeftError):
+        net.warm_start(g, 0.9)
+    f.close()
+    g.close()
+    net.close()
+
+
+@pytest.mark.gpu
+def test_warm_start_sets_veff_from_prediction(gpu):
+    net = deft_amd.nn.WarmStartNet(SMALL, seed=9)
+    M, L, eta = 32, 3.2, 0.3
+    V = deft_amd.nn.random_vext(M, L, np.random.default_rng(1))
+    f = deft_amd.WhiteBearFluidVeff((M, M, M), np.diag([L] * 3), R=1, kT=1.0, mu=0.0)
+    f.Vext = V
+    net.warm_start(f, eta)
+    ok = np.isfinite(V)
+    boltz = np.where(ok, np.exp(-np.where(ok, V, 0)), 0).astype(np.float32)
+    u = net.forward(_dev(boltz[None]), [eta]).numpy()[0]
+    nb = eta / (4 * math.pi / 3)
+    want = -(math.log(nb) + u.astype(np.float64))
+    assert np.max(np.abs(f.Veff - want)) < 1e-4 * np.max(np.abs(want))
+    f.close()
+    net.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(deft_amd.nn.WEIGHTS_PATH), reason="no trained weights committed")
+def test_trained_warm_start_saves_iterations(gpu):
+    """Held-out geometry (seed not used in training): the trained network's start converges
+    to the same free energy as the bulk start in fewer CG iterations."""
+    net = deft_amd.nn.WarmStartNet.load()
+    M, L = 64, 6.4
+    f = deft_amd.WhiteBearFluidVeff((M, M, M), np.diag([L] * 3), R=1, kT=1.0, mu=0.0)
+    rng = np.random.default_rng(2024)
+    gains = []
+    for eta in (0.25, 0.40):
+        V = deft_amd.nn.random_vext(M, L, rng)
+        _, sp = deft_amd.nn.solve(f, V, eta, relative_precision=1e-11)
+        _, sw = deft_amd.nn.solve(f, V, eta, relative_precision=1e-11, warm=net)
+        assert sp["stop_reason"] == sw["stop_reason"] == "converged"
+        assert abs(sw["energy"] - sp["energy"]) < 1e-9 * abs(sp["energy"])
+        gains.append(sp["iter"] - sw["iter"])
+    assert min(gains) > 0, gains
+    f.close()
+    net.close()
