@@ -117,6 +117,7 @@ _SIGNATURES = {
     "deft_net_create": ([_IP, ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_net_destroy": ([_P], _I),
     "deft_net_num_params": ([_P], ctypes.c_longlong),
+    "deft_net_set_conv_kernel": ([_P, _I], _I),
     "deft_net_layout": ([_P, _I, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
                          ctypes.POINTER(ctypes.c_longlong)], _I),
     "deft_net_get_params": ([_P, _FP], _I),
@@ -131,6 +132,7 @@ _SIGNATURES = {
     "deft_buffer_download": ([_P, ctypes.c_ulonglong, _P, ctypes.c_ulonglong], _I),
     "deft_net_make_batch": ([_P, ctypes.c_longlong, _I, _I, _IP, _IP, _P], _I),
     "deft_net_mfma_probe": ([_FP, _FP, _FP], _I),
+    "deft_net_layer_probe": ([_I, _I, _I, _I, _I, _FP, _FP, _FP, _FP, _I, _I, _FP], _I),
 }
 
 

@@ -120,6 +120,86 @@ __global__ __launch_bounds__(kNT) void k_conv(const float *__restrict__ in, int 
   }
 }
 
+// LDS-tiled convolution for grids whose edge is a multiple of 16: a block owns a
+// 4 x 4 x 16 (x, y, z) tile of output voxels; the 6 x 6 x 18 periodic halo of the
+// input is staged once in LDS as BF16 (voxel stride CIN + 8 elements, so the 16-byte
+// fragment reads of 8 consecutive lanes fall in distinct banks); wave w computes
+// x = x0 + w, its four y rows as four 16-voxel MFMA row tiles that share each
+// weight fragment.
+template <int NT, int CIN>
+__global__ __launch_bounds__(kNT) void k_conv_tile(const float *__restrict__ in, const __bf16 *__restrict__ W, int kp,
+                                                   const float *__restrict__ bias, float *__restrict__ out, int M,
+                                                   int relu) {
+  constexpr int CS = CIN + 8;
+  constexpr int HY = 6, HZ = 18, HV = 6 * HY * HZ;
+  constexpr int CH8 = CIN / 8;
+  __shared__ __align__(16) __bf16 tile[HV * CS];
+  const int tilesZ = M >> 4, tilesY = M >> 2, tilesX = M >> 2;
+  int bid = blockIdx.x;
+  const int tz = bid % tilesZ;
+  bid /= tilesZ;
+  const int ty = bid % tilesY;
+  bid /= tilesY;
+  const int tx = bid % tilesX;
+  const long long s = bid / tilesX;
+  const int x0 = tx * 4, y0 = ty * 4, z0 = tz * 16;
+  for (int i = threadIdx.x; i < HV * CH8; i += kNT) {
+    const int c8 = i % CH8, hv = i / CH8;
+    const int hz = hv % HZ, hy = (hv / HZ) % HY, hx = hv / (HZ * HY);
+    int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+    gx += (gx < 0) ? M : (gx >= M ? -M : 0);
+    gy += (gy < 0) ? M : (gy >= M ? -M : 0);
+    gz += (gz < 0) ? M : (gz >= M ? -M : 0);
+    const float *src = in + (((s * M + gx) * M + gy) * M + gz) * CIN + c8 * 8;
+    const float4 p = *reinterpret_cast<const float4 *>(src);
+    const float4 q = *reinterpret_cast<const float4 *>(src + 4);
+    *reinterpret_cast<bf16x8 *>(&tile[hv * CS + c8 * 8]) =
+        bf16x8{(__bf16)p.x, (__bf16)p.y, (__bf16)p.z, (__bf16)p.w, (__bf16)q.x, (__bf16)q.y, (__bf16)q.z, (__bf16)q.w};
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  floatx4 acc[4][NT];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[r][n] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = bf16x8{(__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f,
+                              (__bf16)0.f, (__bf16)0.f, (__bf16)0.f, (__bf16)0.f};
+  for (int kc = 0; kc < kp; kc += 32) {
+    const int k0 = kc + (lane >> 4) * 8;
+    const bool kin = k0 < 27 * CIN;
+    const int t = kin ? k0 / CIN : 0;
+    const int ci = kin ? k0 - t * CIN : 0;
+    const int base = (((w + t / 9) * HY + (t / 3) % 3) * HZ + (lane & 15) + t % 3) * CS + ci;
+    bf16x8 bfr[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) bfr[n] = *reinterpret_cast<const bf16x8 *>(W + (long long)(n * 16 + (lane & 15)) * kp + k0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bf16x8 raw = *reinterpret_cast<const bf16x8 *>(&tile[base + r * HZ * CS]);
+      const bf16x8 a = kin ? raw : zero8;
+#pragma unroll
+      for (int n = 0; n < NT; ++n) acc[r][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[n], acc[r][n], 0, 0, 0);
+    }
+  }
+  constexpr int COUT = NT * 16;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const int co = n * 16 + (lane & 15);
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long long row = ((s * M + x0 + w) * M + y0 + r) * M + z0 + (lane >> 4) * 4;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float val = acc[r][n][j] + bb;
+        if (relu) val = fmaxf(val, 0.f);
+        out[(row + j) * COUT + co] = val;
+      }
+    }
+  }
+}
+
 // weight gradient gw[co][k] += sum_v dpre[v][co] A[v][k] over a slice of voxels:
 // rows = output channels (16 per tile), cols = k (16 per tile), reduction over voxels
 __global__ __launch_bounds__(64) void k_conv_wgrad(const float *__restrict__ dpre, int cout,
@@ -451,6 +531,7 @@ struct deft_net_s {
   hipStream_t stream = nullptr;
   // activation buffers for the current (batch, M)
   int M = 0, B = 0;
+  bool generic_conv = false;  // use the untiled convolution everywhere (kernel cross-check)
   std::vector<float *> bufs;
   float *loss_acc = nullptr;   // host-mapped
   unsigned long long *count_dev = nullptr;
@@ -516,19 +597,53 @@ int ensure_buffers(deft_net_s *n, int M, int B) {
   return DEFT_OK;
 }
 
-void launch_conv(deft_net_s *n, const Conv &c, const float *in, const __bf16 *W, int kp, int cout, const float *bias,
-                 float *out, int level, bool relu) {
-  const long long V = voxels(n, level);
-  const int M = n->M >> level;
+template <int CIN>
+void launch_conv_tile(int nt, dim3 grid, hipStream_t st, const float *in, const __bf16 *W, int kp, const float *bias,
+                      float *out, int M, int relu) {
+  switch (nt) {
+    case 1: hipLaunchKernelGGL((k_conv_tile<1, CIN>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, relu); break;
+    case 2: hipLaunchKernelGGL((k_conv_tile<2, CIN>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, relu); break;
+    case 3: hipLaunchKernelGGL((k_conv_tile<3, CIN>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, relu); break;
+    default: hipLaunchKernelGGL((k_conv_tile<4, CIN>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, relu); break;
+  }
+}
+
+void launch_conv_raw(hipStream_t st, bool tiled, int M, long long V, int cin, int cout, const float *in,
+                     const __bf16 *W, int kp, const float *bias, float *out, bool relu) {
+  const int r = relu ? 1 : 0;
+  if (M % 16 == 0 && tiled) {
+    const dim3 grid((unsigned)(V / 256));
+    switch (cin) {
+      case 8: launch_conv_tile<8>(cout / 16, grid, st, in, W, kp, bias, out, M, r); return;
+      case 16: launch_conv_tile<16>(cout / 16, grid, st, in, W, kp, bias, out, M, r); return;
+      case 32: launch_conv_tile<32>(cout / 16, grid, st, in, W, kp, bias, out, M, r); return;
+      case 48: launch_conv_tile<48>(cout / 16, grid, st, in, W, kp, bias, out, M, r); return;
+      case 64: launch_conv_tile<64>(cout / 16, grid, st, in, W, kp, bias, out, M, r); return;
+      default: break;
+    }
+  }
   const long long tiles = (V + 15) / 16;
   const dim3 grid((unsigned)((tiles + 3) / 4));
-  const int cin = (W == c.wb) ? c.cin : c.cout;
   switch (cout / 16) {
-    case 1: hipLaunchKernelGGL((k_conv<1>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
-    case 2: hipLaunchKernelGGL((k_conv<2>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
-    case 3: hipLaunchKernelGGL((k_conv<3>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
-    default: hipLaunchKernelGGL((k_conv<4>), grid, dim3(kNT), 0, n->stream, in, cin, W, kp, bias, out, M, V, relu ? 1 : 0); break;
+    case 1: hipLaunchKernelGGL((k_conv<1>), grid, dim3(kNT), 0, st, in, cin, W, kp, bias, out, M, V, r); break;
+    case 2: hipLaunchKernelGGL((k_conv<2>), grid, dim3(kNT), 0, st, in, cin, W, kp, bias, out, M, V, r); break;
+    case 3: hipLaunchKernelGGL((k_conv<3>), grid, dim3(kNT), 0, st, in, cin, W, kp, bias, out, M, V, r); break;
+    default: hipLaunchKernelGGL((k_conv<4>), grid, dim3(kNT), 0, st, in, cin, W, kp, bias, out, M, V, r); break;
   }
+}
+
+// gw (cout x kp, zeroed by the caller) += weight gradient of a convolution
+void launch_wgrad_raw(hipStream_t st, int M, long long V, int cin, int cout, int kp, const float *dpre,
+                      const float *in, float *gw) {
+  const long long vslice = 2048;
+  const dim3 wg((unsigned)((cout / 16) * (kp / 16)), (unsigned)((V + vslice - 1) / vslice));
+  hipLaunchKernelGGL(k_conv_wgrad, wg, dim3(64), 0, st, dpre, cout, in, cin, kp, gw, M, V, vslice);
+}
+
+void launch_conv(deft_net_s *n, const Conv &c, const float *in, const __bf16 *W, int kp, int cout, const float *bias,
+                 float *out, int level, bool relu) {
+  const int cin = (W == c.wb) ? c.cin : c.cout;
+  launch_conv_raw(n->stream, !n->generic_conv, n->M >> level, voxels(n, level), cin, cout, in, W, kp, bias, out, relu);
 }
 
 void conv_forward(deft_net_s *n, int ci, const float *in, float *out) {
@@ -545,10 +660,7 @@ void conv_backward(deft_net_s *n, int ci, const float *dout, const float *act, c
   const int M = n->M >> c.level;
   hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(V * c.cout)), dim3(kNT), 0, n->stream, dout, act, scratch,
                      V * c.cout);
-  const long long vslice = 2048;
-  const dim3 wg((unsigned)((c.cout / 16) * (c.kp / 16)), (unsigned)((V + vslice - 1) / vslice));
-  hipLaunchKernelGGL(k_conv_wgrad, wg, dim3(64), 0, n->stream, scratch, c.cout, in, c.cin, c.kp,
-                     n->grads + c.woff, M, V, vslice);
+  launch_wgrad_raw(n->stream, M, V, c.cin, c.cout, c.kp, scratch, in, n->grads + c.woff);
   hipLaunchKernelGGL(k_bias_grad, dim3(c.cout), dim3(kNT), 0, n->stream, scratch, c.cout, V, n->grads + c.boff);
   if (din) launch_conv(n, c, scratch, c.wt, c.kpt, c.cin, nullptr, din, c.level, false);
 }
@@ -763,6 +875,12 @@ int deft_net_destroy(deft_net n) {
   return DEFT_OK;
 }
 
+int deft_net_set_conv_kernel(deft_net n, int tiled) {
+  NN_TRY(check_net(n));
+  n->generic_conv = tiled == 0;
+  return DEFT_OK;
+}
+
 long long deft_net_num_params(deft_net n) { return n ? n->nparams : -1; }
 
 int deft_net_layout(deft_net n, int conv, int *cin, int *cout, int *kp, long long *woff, long long *boff) {
@@ -942,6 +1060,82 @@ int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
                      p->N);
   NN_HIP(hipGetLastError());
   NN_HIP(hipStreamSynchronize(p->stream));
+  return DEFT_OK;
+}
+
+int deft_net_layer_probe(int op, int M, int batch, int cin, int cout, const float *in_host, const float *in2_host,
+                         const float *w_host, const float *bias_host, int relu, int tiled, float *out_host) {
+  if (batch < 1 || M < 2 || M > 256 || M % 2) return set_error(DEFT_ERR_ARG, "M must be even in [2, 256], batch >= 1");
+  if (op < 0 || op > 4) return set_error(DEFT_ERR_ARG, "op must be 0..4");
+  if (!in_host || !out_host) return set_error(DEFT_ERR_ARG, "null buffer");
+  const bool conv = op <= 2;
+  if (conv && (cin % 8 || cin < 8 || cin > 64 || cout % 16 || cout < 16 || cout > 64 || (op == 1 && cin % 16)))
+    return set_error(DEFT_ERR_ARG, "cin must be 8..64 (multiple of 8, of 16 for op 1), cout 16..64 (multiple of 16)");
+  const bool needw = op <= 1;
+  if (needw && !w_host) return set_error(DEFT_ERR_ARG, "null weights");
+  if (op == 2 && !in2_host) return set_error(DEFT_ERR_ARG, "null second input");
+  const long long M3 = (long long)M * M * M, V = M3 * batch, Vh = V / 8;
+  const int kp = (27 * cin + 31) / 32 * 32, kpt = (27 * cout + 31) / 32 * 32;
+  long long n_in = 0, n_in2 = 0, n_out = 0;
+  switch (op) {
+    case 0: n_in = V * cin; n_out = V * cout; break;
+    case 1: n_in = V * cout; n_out = V * cin; break;
+    case 2: n_in = V * cin; n_in2 = V * cout; n_out = (long long)cout * kp; break;
+    case 3: n_in = V * cin; n_out = Vh * cin; break;       // pool: M -> M/2
+    case 4: n_in = Vh * cin; n_out = V * cin; break;       // upsample: M/2 -> M
+  }
+  std::vector<void *> mem;
+  auto alloc = [&](size_t bytes, void **p) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) mem.push_back(*p);
+    return e;
+  };
+  auto release = [&]() {
+    for (void *p : mem) (void)hipFree(p);
+  };
+  float *din = nullptr, *din2 = nullptr, *dout = nullptr, *dw = nullptr, *dbias = nullptr;
+  __bf16 *wb = nullptr;
+  hipError_t e = alloc(sizeof(float) * n_in, (void **)&din);
+  if (e == hipSuccess) e = alloc(sizeof(float) * n_out, (void **)&dout);
+  if (e == hipSuccess && n_in2) e = alloc(sizeof(float) * n_in2, (void **)&din2);
+  if (e == hipSuccess && needw) e = alloc(sizeof(float) * (size_t)cout * kp, (void **)&dw);
+  if (e == hipSuccess && needw) e = alloc(sizeof(__bf16) * (size_t)std::max<long long>((long long)cout * kp, (long long)cin * kpt), (void **)&wb);
+  if (e == hipSuccess && bias_host && op == 0) e = alloc(sizeof(float) * cout, (void **)&dbias);
+  if (e == hipSuccess) e = hipMemcpy(din, in_host, sizeof(float) * n_in, hipMemcpyHostToDevice);
+  if (e == hipSuccess && din2) e = hipMemcpy(din2, in2_host, sizeof(float) * n_in2, hipMemcpyHostToDevice);
+  if (e == hipSuccess && dw) e = hipMemcpy(dw, w_host, sizeof(float) * (size_t)cout * kp, hipMemcpyHostToDevice);
+  if (e == hipSuccess && dbias) e = hipMemcpy(dbias, bias_host, sizeof(float) * cout, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release();
+    return set_error(DEFT_ERR_HIP, std::string("layer probe setup: ") + hipGetErrorString(e));
+  }
+  switch (op) {
+    case 0:
+      hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for((long long)cout * kp)), dim3(kNT), 0, 0, dw, wb,
+                         (long long)cout * kp);
+      launch_conv_raw(0, tiled != 0, M, V, cin, cout, din, wb, kp, dbias, dout, relu != 0);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_transpose_w, dim3(blocks_for((long long)cin * kpt)), dim3(kNT), 0, 0, dw, cout, cin, kp, wb,
+                         kpt);
+      launch_conv_raw(0, tiled != 0, M, V, cout, cin, din, wb, kpt, nullptr, dout, false);
+      break;
+    case 2:
+      e = hipMemset(dout, 0, sizeof(float) * n_out);
+      if (e == hipSuccess) launch_wgrad_raw(0, M, V, cin, cout, kp, din2, din, dout);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_pool, dim3(blocks_for(n_out)), dim3(kNT), 0, 0, din, dout, cin, M / 2, Vh);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_up, dim3(blocks_for(n_out)), dim3(kNT), 0, 0, din, dout, cin, M, V);
+      break;
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out_host, dout, sizeof(float) * n_out, hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("layer probe: ") + hipGetErrorString(e));
   return DEFT_OK;
 }
 

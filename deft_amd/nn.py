@@ -115,6 +115,10 @@ class WarmStartNet:
         except Exception:
             pass
 
+    def set_conv_kernel(self, tiled):
+        """True: LDS-tiled convolutions where the grid edge allows (default); False: untiled."""
+        _check(lib().deft_net_set_conv_kernel(self._h, 1 if tiled else 0))
+
     @property
     def num_params(self):
         return int(lib().deft_net_num_params(self._h))
@@ -192,6 +196,34 @@ def mfma_probe(a, b):
     return c
 
 
+def layer_probe(op, x, channels=None, w=None, bias=None, relu=False, tiled=True, x2=None):
+    """Runs one layer kernel on host arrays (deft_net_layer_probe).  op: "conv" (x of
+    shape (B, M, M, M, cin), w (channels, kp), output channels `channels`), "dgrad"
+    (x = d(pre-activation) with the forward's cout channels, w the forward weights,
+    `channels` = the forward's cin), "wgrad" (x = input, x2 = d(pre-activation)),
+    "pool" or "up" (x of shape (B, M/2, M/2, M/2, C))."""
+    codes = {"conv": 0, "dgrad": 1, "wgrad": 2, "pool": 3, "up": 4}
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    B, C = x.shape[0], x.shape[-1]
+    M = x.shape[1] * (2 if op == "up" else 1)
+    if op == "conv":
+        cin, cout, shape = C, int(channels), (B, M, M, M, int(channels))
+    elif op == "dgrad":
+        cin, cout, shape = int(channels), C, (B, M, M, M, int(channels))
+    elif op == "wgrad":
+        cin, cout = C, int(x2.shape[-1])
+        shape = (cout, (27 * cin + 31) // 32 * 32)
+    elif op == "pool":
+        cin, cout, shape = C, C, (B, M // 2, M // 2, M // 2, C)
+    else:
+        cin, cout, shape = C, C, (B, M, M, M, C)
+    out = np.empty(shape, dtype=np.float32)
+    keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (x2, w, bias)]
+    _check(lib().deft_net_layer_probe(codes[op], M, B, cin, cout, _fp(x), *(None if a is None else _fp(a) for a in keep),
+                                      1 if relu else 0, 1 if tiled else 0, _fp(out)))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # training data: random external potentials solved by the library's minimizer
 # ---------------------------------------------------------------------------
@@ -212,7 +244,7 @@ def random_vext(M, L, rng):
         if kind in (0, 3):
             for _ in range(int(rng.integers(1, 4))):
                 c = rng.uniform(0, L, 3)
-                rad = rng.uniform(0.3, 2.5)
+                rad = rng.uniform(0.3, 3.0)
                 dx, dy, dz = (_periodic_offsets(M, L, ci) for ci in c)
                 r2 = dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2
                 V[r2 < rad * rad] = np.inf
@@ -254,6 +286,7 @@ def solve(fluid, vext, eta, R=1.0, kT=1.0, relative_precision=1e-10, warm=None):
     m.run()
     s = m.stats()
     s["energy"] = m.energy()
+    s["fp32_iterations"], s["mixed_iterations"], _ = m.phase_iterations()
     m.close()
     return fluid.density(), s
 

@@ -35,6 +35,11 @@ typedef struct deft_net_s *deft_net;
 int deft_net_create(const int channels[4], unsigned long long seed, deft_net *out);
 int deft_net_destroy(deft_net net);
 
+/* tiled = 1 (default): LDS-tiled convolutions on levels whose edge is a
+ * multiple of 16, the untiled kernel elsewhere; tiled = 0: the untiled kernel
+ * everywhere.  Both compute identical sums in the same order. */
+int deft_net_set_conv_kernel(deft_net net, int tiled);
+
 /* number of FP32 parameters, and where convolution `conv` (0..13, or -1 for
  * the head) keeps them: weights W[co][k] at woff with row length kp
  * (k = tap * cin + ci, tap = 9 (dx+1) + 3 (dy+1) + (dz+1), zero padding past
@@ -80,6 +85,21 @@ int deft_net_make_batch(const float *data_dev, long long count, int M, int batch
  * The plan must be cubic with an edge that is a multiple of 8; points with
  * Vext = +inf keep density zero whatever Veff holds. */
 int deft_plan_warm_start(deft_plan plan, deft_net net, double eta);
+
+/* One layer on its own, for checking each kernel against a reference (host
+ * buffers, channels-last, M^3 grids, batch of them):
+ *   op 0: out = act(conv(in) + bias), in B M^3 x cin, w cout x kp (FP32,
+ *         rounded to BF16 on the device), out B M^3 x cout; relu 0/1
+ *   op 1: backward-data of that convolution: in = d(pre-activation) B M^3 x cout,
+ *         out B M^3 x cin (no bias, no activation)
+ *   op 2: weight gradient: in = x (B M^3 x cin), in2 = d(pre-activation)
+ *         (B M^3 x cout), out cout x kp
+ *   op 3: 2x2x2 average pool of in (B M^3 x cin) -> out (B (M/2)^3 x cin)
+ *   op 4: nearest upsample of in (B (M/2)^3 x cin) -> out (B M^3 x cin)
+ * kp = 27 cin rounded up to 32; tiled selects the LDS-tiled convolution where
+ * M is a multiple of 16. */
+int deft_net_layer_probe(int op, int M, int batch, int cin, int cout, const float *in_host, const float *in2_host,
+                         const float *w_host, const float *bias_host, int relu, int tiled, float *out_host);
 
 /* One 16x16 output tile of the convolution kernel's BF16 MFMA GEMM with
  * K = 32: c[m][n] = sum_k a[m][k] b[n][k] (inputs rounded to BF16). */

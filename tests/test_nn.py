@@ -116,6 +116,27 @@ def test_random_vext_shapes():
     assert seen_inf
 
 
+def oracle_envelope(p, boltz, eta, channels, want, seed=0):
+    """RMS change of the BF16-emulated oracle output under a 1e-6 relative perturbation
+    of the weights: how far FP32 accumulation-order differences alone can move u for
+    this network (BF16 rounding flips cascade through the layers of some networks)."""
+    rng = np.random.default_rng(seed)
+    q = np.asarray(p, dtype=np.float64) * (1 + 1e-6 * rng.standard_normal(len(p)))
+    pert, _ = U.forward(q, boltz, eta, channels, emulate_bf16=True)
+    return np.sqrt(np.mean((pert - want) ** 2))
+
+
+def conv_case(rng, B, M, cin, cout):
+    x = np.maximum(rng.normal(size=(B, M, M, M, cin)), 0).astype(np.float32)
+    kp = (27 * cin + 31) // 32 * 32
+    w = np.zeros((cout, kp), dtype=np.float32)
+    w[:, :27 * cin] = rng.normal(size=(cout, 27 * cin)) / math.sqrt(27 * cin)
+    bias = rng.normal(size=cout).astype(np.float32) * 0.1
+    c = dict(cin=cin, cout=cout, kp=kp, woff=0, boff=cout * kp)
+    params = np.concatenate([w.ravel(), bias])
+    return x, w, bias, c, params
+
+
 # ----------------------------------------------------------------- GPU
 
 def _dev(a):
@@ -144,6 +165,53 @@ def test_mfma_lane_layout(gpu):
     assert np.max(np.abs(c - want)) < 1e-5 * np.max(np.abs(want))
 
 
+CONV_CASES = [  # (B, M, cin, cout, tiled): tiled applies where M % 16 == 0
+    (2, 16, 8, 16, True), (1, 16, 16, 32, True), (1, 16, 16, 32, False), (1, 32, 48, 48, True),
+    (1, 16, 64, 48, True), (2, 8, 64, 64, False), (1, 16, 32, 64, True), (2, 4, 32, 16, False),
+    (2, 2, 16, 16, False)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,M,cin,cout,tiled", CONV_CASES)
+def test_conv_layer_matches_oracle(gpu, B, M, cin, cout, tiled):
+    """One convolution (forward, backward-data, weight gradient) against the oracle's with
+    the same BF16 operand rounding: only FP32 accumulation order differs."""
+    rng = np.random.default_rng(B * 1000 + M * 10 + cin + cout)
+    x, w, bias, c, params = conv_case(rng, B, M, cin, cout)
+    for relu in (False, True):
+        got = deft_amd.nn.layer_probe("conv", x, cout, w, bias, relu=relu, tiled=tiled)
+        W = U._weights(params, c, True)
+        pre = np.zeros(x.shape[:4] + (cout,))
+        mag = np.zeros_like(pre)
+        xb = U.bf16(x)
+        for t in range(27):
+            dx, dy, dz = U._tap(t)
+            xs = np.roll(xb, (-dx, -dy, -dz), axis=(1, 2, 3))
+            pre += xs @ W[:, t, :].T
+            mag += np.abs(xs) @ np.abs(W[:, t, :]).T
+        want = pre + bias if not relu else np.maximum(pre + bias, 0)
+        assert np.all(np.abs(got - want) <= 1e-6 * (mag + 1)), (relu, np.max(np.abs(got - want)))
+    d = rng.normal(size=(B, M, M, M, cout)).astype(np.float32)
+    gW, _, din = U.conv_backward(d, np.ones_like(d), x, params, c, True)
+    got_din = deft_amd.nn.layer_probe("dgrad", d, cin, w, tiled=tiled) if cin % 16 == 0 else None
+    if got_din is not None:
+        assert np.max(np.abs(got_din - din)) <= 1e-5 * np.max(np.abs(din))
+    got_gw = deft_amd.nn.layer_probe("wgrad", x, x2=d)
+    assert np.max(np.abs(got_gw - gW)) <= 1e-5 * np.max(np.abs(gW))
+    assert not np.any(got_gw[:, 27 * cin:])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M", [4, 16])
+def test_pool_and_upsample_layers(gpu, M):
+    rng = np.random.default_rng(M)
+    x = rng.normal(size=(2, M, M, M, 16)).astype(np.float32)
+    got = deft_amd.nn.layer_probe("pool", x)
+    assert np.allclose(got, U.pool(x.astype(np.float64)), rtol=0, atol=1e-6)
+    y = rng.normal(size=(2, M // 2, M // 2, M // 2, 32)).astype(np.float32)
+    assert np.array_equal(deft_amd.nn.layer_probe("up", y), U.up(y).astype(np.float32))
+
+
 @pytest.mark.gpu
 def test_layout_matches_oracle(gpu):
     net = deft_amd.nn.WarmStartNet()
@@ -168,17 +236,12 @@ def test_forward_matches_oracle(gpu, channels):
     boltz, eta, _, _ = random_inputs(2, 16, 8)
     u = net.forward(_dev(boltz), eta).numpy()
     want, _ = U.forward(p, boltz, eta, channels, emulate_bf16=True)
-
-    def rms_rel(a, b):
-        return np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2))
-
-    # Residual differences are FP32 accumulation order, which now and then moves an
-    # activation across a BF16 rounding boundary (2^-8 relative) and propagates.
-    # Without BF16 emulation, or with truncating conversions, the RMS is 1e-2 or more.
-    assert rms_rel(u, want) < 3e-3, rms_rel(u, want)
-    assert np.max(np.abs(u - want)) < 2e-2 * np.max(np.abs(want))
-    exact, _ = U.forward(p, boltz, eta, channels, emulate_bf16=False)
-    assert rms_rel(u, exact) < 5e-2
+    # FP32 accumulation order now and then moves an activation across a BF16 rounding
+    # boundary (2^-8 relative), and such flips propagate: the device must stay within
+    # the oracle's own envelope for that effect (each layer is checked exactly above).
+    env = oracle_envelope(p, boltz, eta, channels, want)
+    rms = np.sqrt(np.mean((u - want) ** 2))
+    assert rms < 3 * env + 1e-4 * np.sqrt(np.mean(want ** 2)), (rms, env)
     net.close()
 
 
@@ -198,6 +261,26 @@ def test_loss_grad_matches_oracle(gpu):
     for a, b in blocks:
         ref = go[a:b]
         assert np.linalg.norm(g[a:b] - ref) <= 2e-2 * np.linalg.norm(ref) + 1e-7, (a, b)
+    net.close()
+
+
+@pytest.mark.gpu
+def test_tiled_and_untiled_convolutions_agree(gpu):
+    """M = 32: levels 0 and 1 take the LDS-tiled kernel by default; both kernels form
+    the same BF16 products and accumulate them in the same order."""
+    ch = (16, 32, 48, 64)
+    net = deft_amd.nn.WarmStartNet(ch)
+    net.params = random_params(ch, 21)
+    boltz, eta, target, _ = random_inputs(2, 32, 22)
+    b, t = _dev(boltz), _dev(target)
+    u_tiled = net.forward(b, eta).numpy()
+    l_tiled, g_tiled = net.loss_grad(b, eta, t)
+    net.set_conv_kernel(False)
+    u_plain = net.forward(b, eta).numpy()
+    l_plain, g_plain = net.loss_grad(b, eta, t)
+    assert np.array_equal(u_tiled, u_plain)
+    assert abs(l_tiled - l_plain) < 1e-6 * l_plain
+    assert np.linalg.norm(g_tiled - g_plain) < 1e-5 * np.linalg.norm(g_plain)
     net.close()
 
 
