@@ -107,6 +107,7 @@ _SIGNATURES = {
     "deft_minimize_stats_get": ([_P, ctypes.POINTER(MinimizeStats)], _I),
     "deft_minimize_invalidate": ([_P], _I),
     "deft_minimize_set_mixed_precision": ([_P, _D, _I], _I),
+    "deft_minimize_set_fused_line_search": ([_P, _I], _I),
     "deft_minimize_fp32_iterations": ([_P, _IP], _I),
     "deft_minimize_phase_iterations": ([_P, _IP, _IP, _IP], _I),
     "deft_plan_set_profiling": ([_P, _I], _I),
@@ -126,6 +127,7 @@ _SIGNATURES = {
     "deft_net_loss_grad": ([_P, _I, _I, _P, _FP, _P, _P, _FP, _FP], _I),
     "deft_net_train_step": ([_P, _I, _I, _P, _FP, _P, _P, ctypes.c_float, _FP], _I),
     "deft_plan_warm_start": ([_P, _P, _D], _I),
+    "deft_net_last_blend": ([_P, _DP], _I),
     "deft_buffer_create": ([ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_buffer_destroy": ([_P], _I),
     "deft_buffer_upload": ([_P, ctypes.c_ulonglong, _P, ctypes.c_ulonglong], _I),
@@ -450,6 +452,11 @@ class Minimize:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().deft_minimize_phase_iterations(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def set_fused_line_search(self, mode):
+        """1: energy+gradient at the quadratic step's trial point, 0: energy only,
+        -1: on with the mixed-precision ladder (default)."""
+        _check(lib().deft_minimize_set_fused_line_search(self._h, int(mode)))
 
     def invalidate_cache(self):
         _check(lib().deft_minimize_invalidate(self._h))

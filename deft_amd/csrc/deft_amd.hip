@@ -34,6 +34,7 @@ namespace {
 
 constexpr int kVersion = 1;
 constexpr int kConfirmIterations = 3;
+constexpr int kMaxPhaseHalvings = 12;
 
 bool det3(const double *c, double &det) {
   det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
@@ -175,6 +176,13 @@ struct deft_minimize_s {
   int iter_at_phase2 = -1;  // iteration count when phase 2 began (-1: not reached)
   int phase = 0;            // precision phase of the mixed ladder
   int confirm_until = 0;    // the final phase runs at least until this iteration
+  // fused line search: the trial point s2 of the quadratic step gets energy and
+  // gradient from one evaluation (when both use one FFT precision), so an
+  // accepted s2 needs no second evaluation at the start of the next iteration;
+  // -1: on exactly when the mixed-precision ladder is on
+  int fuse_ls = -1;
+  bool s2_accepted = true;  // the last quadratic step kept its trial point s2
+  double good_step = 0.0;   // step of the last iteration that clearly lowered the energy
 };
 
 namespace {
@@ -283,6 +291,11 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   double step1 = m->step;
   queue_move(p, step1);
   double Etried = -137, E;
+  // Below the last precision phase the halving is capped: a line search that
+  // needs more than kMaxPhaseHalvings halvings has reached the round-off of the
+  // phase's energies, and the phase ends where the iteration began.
+  const bool capped = m->switch_rp > 0 && m->phase < 2;
+  int halvings = 0;
   while (true) {
     step1 *= 0.5;
     queue_move(p, -step1);
@@ -291,6 +304,12 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
     if (E == Etried) break;
     Etried = E;
     if (!better(E0, E)) break;
+    if (capped && ++halvings > kMaxPhaseHalvings) {
+      queue_move(p, -step1);
+      invalidate(m);
+      m->deltaE = 0;
+      return stop_with(m, DEFT_STOP_NO_CHANGE, more);
+    }
   }
   double E1;
   TRY(m_energy(m, &E1));
@@ -337,8 +356,21 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
     invalidate(m);
     queue_move(p, step2 - step1);
     double E2;
-    TRY(m_energy(m, &E2));
-    if (better(E1, E2) && better(E1, E0)) {
+    // fused only while trial points are being kept: a rejected s2 wastes its gradient
+    const bool fuse = (m->fuse_ls > 0 || (m->fuse_ls < 0 && m->switch_rp > 0)) && m->s2_accepted &&
+                      energy_prec(m) == grad_prec(m);
+    if (fuse) {
+      // E(s2) from the evaluation that also yields the gradient the next
+      // iteration starts from; the energies and the trajectory are unchanged
+      const int ne = m->num_energy;
+      TRY(m_pgrad(m));
+      if (m->num_energy == ne) m->num_energy++;
+      E2 = m->E;
+    } else {
+      TRY(m_energy(m, &E2));
+    }
+    m->s2_accepted = !(better(E1, E2) && better(E1, E0));
+    if (!m->s2_accepted) {
       invalidate(m);
       queue_move(p, -(step2 - step1));
       m->step = step1;
@@ -362,6 +394,9 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   double error_guess = std::fabs(std::max(m->dEdn, m->deltaE) / m->log_avg);
   if (ratio_avg >= 1) error_guess = m->precision + std::fabs(newE);
   m->error_estimate = 2 * error_guess;
+  // the step of the last iteration whose decrease stands clear of the round-off
+  // of its energies: the next precision phase starts from it
+  if (m->deltaE < -(energy_prec(m) == DEFT_FFT_FP32 ? 1e-6 : 1e-13) * std::fabs(newE)) m->good_step = m->step;
   if (verbose)
     printf("iter %4d  E = %.15g  dE = %.3e  err = %.3e  step = %.3e  (#E %d, #G %d)\n", m->iter, newE, m->deltaE,
            m->error_estimate, m->step, m->num_energy, m->num_grad);
@@ -415,6 +450,10 @@ int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   m->e_prec = DEFT_FFT_FP64;
   m->g_prec = (m->phase == 1) ? DEFT_FFT_FP32 : DEFT_FFT_FP64;
   invalidate(m);
+  // A phase usually ends on a line search that ran into the round-off of its
+  // energies and shrank the step to nothing; the next phase starts from the
+  // step of the last iteration that clearly lowered the energy.
+  if (m->good_step > 0 && std::fabs(m->step) < 1e-3 * std::fabs(m->good_step)) m->step = m->good_step;
   if (m->g_prec != old_g) {
     m->have_dir = false;
     m->have_old = false;
@@ -890,6 +929,13 @@ int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *s) {
   s->delta_energy = m->deltaE;
   s->error_estimate = m->error_estimate;
   s->step = m->step;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_fused_line_search(deft_minimize m, int mode) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (mode < -1 || mode > 1) return set_error(DEFT_ERR_ARG, "mode must be -1, 0 or 1");
+  m->fuse_ls = mode;
   return DEFT_OK;
 }
 

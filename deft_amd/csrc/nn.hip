@@ -502,9 +502,9 @@ __global__ __launch_bounds__(kNT) void k_boltz(const float *__restrict__ vext, f
 }
 
 __global__ __launch_bounds__(kNT) void k_veff_from_u(const float *__restrict__ u, double kT, double log_nb,
-                                                     double *__restrict__ x, long long n) {
+                                                     double lambda, double *__restrict__ x, long long n) {
   for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
-    x[i] = -kT * (log_nb + (double)u[i]);
+    x[i] = -kT * (log_nb + lambda * (double)u[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,6 +538,7 @@ struct deft_net_s {
   float *eta_dev = nullptr;
   float *ws = nullptr;  // warm-start staging: Boltzmann factor and prediction
   size_t ws_bytes = 0;
+  double last_lambda = 1.0;  // blend factor of the last warm start
 };
 
 namespace {
@@ -1055,11 +1056,31 @@ int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
   const float e = (float)eta;
   NN_TRY(deft_net_forward(n, M, 1, boltz, &e, u));
   const double nb = eta / (4 * M_PI / 3 * p->R * p->R * p->R);
-  p->npend = 0;
-  hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb), p->x,
-                     p->N);
-  NN_HIP(hipGetLastError());
-  NN_HIP(hipStreamSynchronize(p->stream));
+  // Veff = -kT (ln n_b + lambda u): lambda = 1 unless the prediction puts the
+  // weighted packing fraction at or past 1 somewhere (infinite or NaN energy);
+  // then the start is pulled toward the bulk one (lambda = 0, always finite)
+  double lambda = 1.0;
+  for (int attempt = 0;; ++attempt) {
+    p->npend = 0;
+    hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb), lambda,
+                       p->x, p->N);
+    NN_HIP(hipGetLastError());
+    NN_HIP(hipStreamSynchronize(p->stream));
+    if (lambda == 0.0) break;
+    double E = 0;
+    deft_energy_terms terms;
+    NN_TRY(deft_functional_energy(p, nullptr, &E, &terms));
+    if (std::isfinite(E)) break;
+    lambda = (attempt < 8) ? 0.5 * lambda : 0.0;
+  }
+  n->last_lambda = lambda;
+  return DEFT_OK;
+}
+
+int deft_net_last_blend(deft_net n, double *lambda) {
+  NN_TRY(check_net(n));
+  if (!lambda) return set_error(DEFT_ERR_ARG, "null output");
+  *lambda = n->last_lambda;
   return DEFT_OK;
 }
 

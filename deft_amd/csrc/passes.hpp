@@ -574,9 +574,14 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
     double p1 = 0.0, p2 = 0.0, p3 = 0.0;
     if constexpr (GRAD) {
       double dd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (valid)
+      if (valid) {
+        // the energy terms come from wb_phi, as in the energy-only variant
+        double q1, q2, q3;
+        wb_phi((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i], (double)val[4][i], g.wb,
+               p1, p2, p3);
         wb_phi_partials((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i],
-                        (double)val[4][i], g.wb, p1, p2, p3, dd);
+                        (double)val[4][i], g.wb, q1, q2, q3, dd);
+      }
 #pragma unroll
       for (int f = 0; f < 5; ++f) val[f][i] = (T)dd[f];
     } else {

@@ -35,6 +35,9 @@ constexpr int kF3SeriesTerms = 18;
 
 // f3 and f3' given om = 1-n, io = 1/(1-n) and L = ln(1-n).
 __host__ __device__ inline void f3_of(double n, double om, double io, double L, double& f3, double& df3) {
+  // no FMA contraction: the energy-only and the energy+gradient kernels must
+  // round the energy density identically (the line search compares them)
+#pragma clang fp contract(off)
   if (fabs(n) < kF3SeriesCutoff) {
     double g = 1.5, gp = 0.0, p = 1.0;  // p = n^(k-3)
 #pragma unroll
@@ -65,6 +68,9 @@ __host__ __device__ inline void f3_and_derivative(double n, double& f3, double& 
 // Energy density terms only.
 __host__ __device__ inline void wb_phi(double n3, double n2, double vx, double vy, double vz, const WBConst& c,
                                        double& p1, double& p2, double& p3) {
+  // no FMA contraction: every kernel that sums the energy density calls this
+  // function, so energies of the same Veff agree to the last bit
+#pragma clang fp contract(off)
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
   const double L = log1p(-n3);

@@ -181,8 +181,13 @@ class WarmStartNet:
         return loss.value
 
     def warm_start(self, fluid, eta):
-        """Sets fluid's Veff from the network's prediction for its Vext."""
+        """Sets fluid's Veff from the network's prediction for its Vext; returns the blend
+        factor lambda (1: prediction used as is; < 1: pulled toward the bulk start
+        because the prediction's energy was not finite)."""
         _check(lib().deft_plan_warm_start(fluid.handle, self._h, float(eta)))
+        lam = ctypes.c_double()
+        _check(lib().deft_net_last_blend(self._h, ctypes.byref(lam)))
+        return lam.value
 
 
 def mfma_probe(a, b):
@@ -268,6 +273,15 @@ def random_vext(M, L, rng):
     return np.zeros((M, M, M))
 
 
+def training_vext(M, L, rng):
+    """Training potentials: half from random_vext, half from the C3 generator
+    (8 Gaussians and a planar hard wall, deft_amd.workloads)."""
+    from . import workloads
+    if rng.random() < 0.5:
+        return random_vext(M, L, rng)
+    return workloads.c3_cubic_vext(M, L, rng)
+
+
 def solve(fluid, vext, eta, R=1.0, kT=1.0, relative_precision=1e-10, warm=None):
     """Converges the White-Bear fluid in `vext` at bulk packing fraction eta; returns
     (density, minimizer stats).  `warm` = a WarmStartNet to start from its prediction."""
@@ -303,7 +317,7 @@ def make_dataset(count, M=64, L=6.4, seed=0, eta_range=(0.1, 0.45), R=1.0, kT=1.
     i = 0
     while i < count:
         eta = float(rng.uniform(*eta_range))
-        V = random_vext(M, L, rng)
+        V = training_vext(M, L, rng)
         n, s = solve(fluid, V, eta, R, kT)
         if s["stop_reason"] != "converged" or not np.all(np.isfinite(n)):
             continue

@@ -20,17 +20,11 @@ import numpy as np
 
 from . import WhiteBearFluidVeff
 from . import nn as NN
+from . import workloads as W
 
 
 def log(msg):
     print(msg, flush=True)
-
-
-def solute_vext(M, L, radius):
-    x = np.arange(M) * L / M
-    x = np.where(x >= L / 2, x - L, x)
-    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
-    return np.where(r2 < radius * radius, np.inf, 0.0)
 
 
 def compare_starts(net, M, L, cases, relative_precision=1e-11):
@@ -92,7 +86,7 @@ def main(argv=None):
     small = compare_starts(net, args.M, args.L, [(NN.random_vext(args.M, args.L, rng), float(rng.uniform(0.15, 0.45)))
                                                  for _ in range(6)])
     log("radius-3 hard solute, eta 0.40, 128^3, 12.8^3 cell:")
-    big = compare_starts(net, 128, 12.8, [(solute_vext(128, 12.8, 3.0), 0.40)])
+    big = compare_starts(net, 128, 12.8, [(W.c2_solute_vext(128, 12.8, 3.0), 0.40)])
     report = {"holdout_rms_u": float(np.mean(rms)), "final_loss": float(np.mean(hist[-100:])),
               "seconds": time.perf_counter() - t0, "heldout_64": small, "solute_128": big}
     if args.report:

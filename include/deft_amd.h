@@ -196,6 +196,15 @@ int deft_minimize_invalidate(deft_minimize m);
  * 1e-6 is a good value.  The plan's precision setting is not changed.
  * 0 (default) disables the ladder. */
 int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_precision, int polish_grad_precision);
+/* Fused line search (no counterpart in the reference): the trial point s2 of
+ * the quadratic line step (src/new/Minimize.cpp:160-175) is evaluated for
+ * energy and gradient at once when both use the same FFT precision, so an
+ * accepted s2 costs no extra evaluation at the next iteration's start; it is
+ * used while the previous quadratic step kept its s2 (a rejected s2 wastes the
+ * gradient).  The energies and the iterates are those of the reference
+ * algorithm; only the evaluation counts change.  mode 1 on, 0 off, -1 (default) on exactly when
+ * the mixed-precision ladder is on. */
+int deft_minimize_set_fused_line_search(deft_minimize m, int mode);
 /* Iterations run in phase 0, -1 if the ladder was not used or not left. */
 int deft_minimize_fp32_iterations(deft_minimize m, int *iterations);
 /* Iterations of phase 0 and phase 1 and the current phase (0, 1 or 2). */

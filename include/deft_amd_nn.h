@@ -83,8 +83,13 @@ int deft_net_make_batch(const float *data_dev, long long count, int M, int batch
 /* Sets the plan's Veff from the network's prediction for its external
  * potential: Veff = -kT (ln n_bulk + u), n_bulk = eta / (4 pi R^3 / 3).
  * The plan must be cubic with an edge that is a multiple of 8; points with
- * Vext = +inf keep density zero whatever Veff holds. */
+ * Vext = +inf keep density zero whatever Veff holds.  One energy evaluation
+ * checks the start: where the prediction makes it infinite or NaN (a weighted
+ * packing fraction at or past 1), u is scaled by lambda = 1/2, 1/4, ... and
+ * finally 0 (the bulk start) until the energy is finite. */
 int deft_plan_warm_start(deft_plan plan, deft_net net, double eta);
+/* lambda of the last warm start (1 = the network's prediction unchanged). */
+int deft_net_last_blend(deft_net net, double *lambda);
 
 /* One layer on its own, for checking each kernel against a reference (host
  * buffers, channels-last, M^3 grids, batch of them):

@@ -152,6 +152,78 @@ def test_cg_trajectory_fp64(gpu):
     assert _rel(f.Veff, of.x) < 1e-9
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_energy_only_and_gradient_evaluations_agree_bitwise(gpu, prec):
+    """The line search compares energies from both evaluation kinds, so they must round
+    identically (a difference of FP32 round-off size stalls the line search)."""
+    shape, cell = (32, 32, 32), np.diag([3.2, 3.2, 3.2])
+    g = O.Grid(shape, cell)
+    for seed in range(3):
+        _, of, f = _fluid(gpu, shape, cell, eta=0.35, vext=O.planar_wall_vext(g, 1.0), prec=prec, seed=seed)
+        e = f.energy()
+        eg, _, _ = f.energy_grad_and_precond()
+        assert e == eg, (seed, e, eg, e - eg)
+
+
+@pytest.mark.parametrize("polish_grad", [0, 1])
+@pytest.mark.parametrize("amp", [1e-3, 1e-2])
+def test_mixed_precision_from_near_converged_starts(gpu, polish_grad, amp):
+    """Warm starts close to the answer: the FP32 phase hits its round-off floor within a
+    few iterations; the FP64 phases must still converge to the FP64 minimum."""
+    shape, cell = (32, 32, 32), np.diag([3.2, 3.2, 3.2])
+    g = O.Grid(shape, cell)
+    _, of, f = _fluid(gpu, shape, cell, eta=0.35, vext=O.planar_wall_vext(g, 1.0), prec=1)
+    m = gpu.Minimize(f)
+    m.precondition(True)
+    m.set_relative_precision(1e-13)
+    m.run()
+    E_star, v_star = m.energy(), f.Veff.copy()
+    m.close()
+    rng = np.random.default_rng(int(amp * 1e4) + polish_grad)
+    for trial in range(3):
+        f.fft_precision = 0
+        f.Veff = v_star + amp * rng.standard_normal(shape)
+        m = gpu.Minimize(f)
+        m.precondition(True)
+        m.set_relative_precision(1e-11)
+        m.set_mixed_precision(1e-6, polish_grad)
+        m.run()
+        s = m.stats()
+        assert s["stop_reason"] == "converged", (trial, s)
+        assert abs(m.energy() / E_star - 1) < 1e-9, (trial, m.energy(), E_star)
+        m.close()
+
+
+def test_cg_trajectory_fp64_fused_line_search(gpu):
+    """Energy and gradient at the quadratic step's trial point in one evaluation: the
+    iterates stay the oracle's; only the evaluation counts drop."""
+    of, mo, f, m = _trajectory(gpu, 1)
+    m.set_fused_line_search(1)
+    for it in range(12):
+        a = mo.improve_energy()
+        b = m.improve_energy()
+        assert a == b
+        assert abs(m.energy() - mo.energy()) <= 1e-11 * abs(mo.energy()), it
+        if not a:
+            break
+    s = m.stats()
+    assert s["iter"] == mo.iter
+    assert _rel(f.Veff, of.x) < 1e-9
+    # against the unfused device run from the same start: same energies, fewer evaluations
+    _, _, f2, m2 = _trajectory(gpu, 1)
+    m2.set_fused_line_search(0)
+    for it in range(s["iter"]):
+        m2.improve_energy()
+    s2 = m2.stats()
+    assert s2["iter"] == s["iter"]
+    assert abs(m2.energy() - m.energy()) <= 1e-13 * abs(m.energy())
+    # Every gradient evaluation also counts as an energy evaluation, so the work in
+    # energy-only units is num_energy + num_grad.  In this wall case the quadratic
+    # trial point is rejected at every iteration, so fusing stops after the first
+    # wasted gradient.
+    assert s["num_energy_calcs"] + s["num_grad_calcs"] <= s2["num_energy_calcs"] + s2["num_grad_calcs"] + 1
+
+
 @pytest.mark.parametrize("polish_grad", [0, 1])
 def test_cg_mixed_precision_reaches_fp64_answer(gpu, polish_grad):
     """FP32 phase, then FP64 line-search energies (FP32 or FP64 gradients): |dF/F| < 1e-9."""
