@@ -114,6 +114,11 @@ _SIGNATURES = {
     "deft_plan_profile_reset": ([_P], _I),
     "deft_kernel_count": ([], _I),
     "deft_plan_profile_get": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong), _DP, _DP], _I),
+    "deft_batch_create": ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
+    "deft_batch_minimize": ([_P, ctypes.POINTER(_P), _I], _I),
+    "deft_batch_stats": ([_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                          ctypes.POINTER(ctypes.c_longlong)], _I),
+    "deft_batch_destroy": ([_P], _I),
     # warm-start network (include/deft_amd_nn.h); device buffers are passed as integers
     "deft_net_create": ([_IP, ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_net_destroy": ([_P], _I),
@@ -460,6 +465,45 @@ class Minimize:
 
     def invalidate_cache(self):
         _check(lib().deft_minimize_invalidate(self._h))
+
+
+class Batch:
+    """Several fluids minimized together (deft_batch_*): every evaluation the
+    minimizers ask for runs in a lockstep round of batched launches."""
+
+    def __init__(self, fluids):
+        self.fluids = list(fluids)
+        arr = (_P * len(self.fluids))(*[f.handle for f in self.fluids])
+        h = ctypes.c_void_p()
+        _check(lib().deft_batch_create(arr, len(self.fluids), ctypes.byref(h)))
+        self._h = h
+
+    def minimize(self, minimizers):
+        """Runs each minimizer (one per fluid of the batch) to its own stop."""
+        arr = (_P * len(minimizers))(*[m._h for m in minimizers])
+        _check(lib().deft_batch_minimize(self._h, arr, len(minimizers)))
+
+    def stats(self):
+        r, l, q = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().deft_batch_stats(self._h, ctypes.byref(r), ctypes.byref(l), ctypes.byref(q)))
+        return {"rounds": r.value, "launches": l.value, "requests": q.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().deft_batch_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def declared_symbols(headers=None):

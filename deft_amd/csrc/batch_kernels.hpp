@@ -1,0 +1,111 @@
+// Batched launches of the evaluation passes: one launch covers every grid of a
+// batch whose pass axis has the same length and tuning.  Block b of a launch
+// belongs to the grid i with off[i] <= b < off[i+1] and runs that grid's block
+// b - off[i] through the same device body as the single-grid kernel, so every
+// grid gets bit-for-bit the result of its own single launch.
+#pragma once
+
+#include "passes.hpp"
+
+namespace deft {
+
+// Everything one grid's evaluation passes need (one entry per grid and round).
+template <typename T>
+struct BatchArgs {
+  GridArgs g;
+  double* x;
+  const double* d;
+  double m1, m2;
+  int nmoves;
+  int have_old, precond;
+  int nzb;  // z-pass blocks (reduction length)
+  const float* vext;
+  cx<T>* spec0;  // the density spectrum / back-convolution
+  cx<T>* spec1;  // the five weighted-density spectra
+  double* partial;
+  int pstride;
+  const cx<T>* twx;
+  const cx<T>* twy;
+  const cx<T>* twz;
+  const double* og;
+  double* grad;
+  double* pgrad;
+  double* dscal;
+};
+
+// One launch group: n grids, block offsets off[0..n] (off[0] = 0) and the
+// index of each grid's BatchArgs entry.
+struct BatchGroup {
+  const int* off;
+  const int* idx;
+  int n;
+};
+
+__device__ __forceinline__ int batch_find(const BatchGroup& grp, int b) {
+  int lo = 0, hi = grp.n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (grp.off[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void kb_zfwd_density(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  zfwd_density_body<T, L, ZB>(blockIdx.x - grp.off[i], a.g, a.x, a.d, a.m1, a.m2, a.nmoves, a.vext, a.spec0,
+                              a.partial, a.pstride, a.twz);
+}
+
+// field set: 0 = spec0, 1 = spec1 (gridDim.y fields of it)
+template <typename T, int L, bool INV, int TB>
+__global__ __launch_bounds__(kThreads) void kb_ypass(BatchGroup grp, const BatchArgs<T>* __restrict__ args,
+                                                     int which) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  ypass_body<T, L, INV, TB>(blockIdx.x - grp.off[i], blockIdx.y, a.g, which ? a.spec1 : a.spec0, a.twy);
+}
+
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void kb_xconv_fwd(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  xconv_fwd_body<T, L, TB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.spec1, a.twx);
+}
+
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void kb_xconv_bwd(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  xconv_bwd_body<T, L, TB>(blockIdx.x - grp.off[i], a.g, a.spec1, a.spec0, a.twx);
+}
+
+template <typename T, int L, bool GRAD, int ZB>
+__global__ __launch_bounds__(kThreads) void kb_zphi(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  zphi_body<T, L, GRAD, ZB>(blockIdx.x - grp.off[i], a.g, a.spec1, a.partial, a.pstride, a.twz);
+}
+
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void kb_zgrad(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  zgrad_body<T, L, ZB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.x, a.vext, a.og, a.have_old, a.precond, a.grad,
+                       a.pgrad, a.partial, a.pstride, a.twz);
+}
+
+// nslots blocks per grid: block b reduces slot b % nslots of grid b / nslots
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kb_reduce(BatchGroup grp, const BatchArgs<T>* __restrict__ args,
+                                                      int nslots) {
+  const int i = blockIdx.x / nslots;
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  reduce_body(SLOT_IDEAL + (int)(blockIdx.x - i * nslots), a.partial, a.pstride, a.nzb, a.dscal);
+}
+
+}  // namespace deft

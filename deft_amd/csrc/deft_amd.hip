@@ -3,10 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "launch.hpp"
@@ -61,7 +64,10 @@ bool length_ok(int n) {
   return with_length(n, [](auto) {});
 }
 
+int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond);
+
 int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+  if (p->batch) return batch_evaluate(p, prec, grad, have_old, precond);
   if (prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
   return Launch<float>::evaluate(p, grad, have_old, precond);
 }
@@ -183,6 +189,28 @@ struct deft_minimize_s {
   int fuse_ls = -1;
   bool s2_accepted = true;  // the last quadratic step kept its trial point s2
   double good_step = 0.0;   // step of the last iteration that clearly lowered the energy
+};
+
+// ===========================================================================
+// batches: several plans whose evaluations run in lockstep rounds; every
+// minimizer of a batch runs deft's algorithm unchanged in a thread of its own,
+// each evaluation it asks for joins the current round, and the round is
+// launched (Launch<T>::evaluate_batch, one launch per pass and group of equal
+// axis lengths) once every running minimizer has posted a request
+// ===========================================================================
+struct deft_batch_s {
+  std::vector<deft_plan_s *> plans;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0;                  // minimizer threads still running
+  std::vector<BatchReq> posted[2]; // this round's requests: FP32, FP64
+  int nposted = 0;
+  long long round = 0;
+  int round_rc = DEFT_OK;
+  std::string round_err;
+  BatchStage stage;
+  long long rounds_run = 0, launches = 0, requests = 0;
 };
 
 namespace {
@@ -465,6 +493,53 @@ int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   return DEFT_OK;
 }
 
+// Launches the posted requests as one round and wakes the waiting minimizers;
+// the caller holds b->mu.
+int run_round(deft_batch_s *b) {
+  b->stage.used = 0;
+  int rc = DEFT_OK;
+  if (!b->posted[0].empty())
+    rc = Launch<float>::evaluate_batch(b->stream, b->stage, b->posted[0].data(), (int)b->posted[0].size(),
+                                       &b->launches);
+  if (rc == DEFT_OK && !b->posted[1].empty())
+    rc = Launch<double>::evaluate_batch(b->stream, b->stage, b->posted[1].data(), (int)b->posted[1].size(),
+                                        &b->launches);
+  if (rc == DEFT_OK) {
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) rc = set_error(DEFT_ERR_HIP, std::string("batch round: ") + hipGetErrorString(e));
+  }
+  b->round_rc = rc;
+  b->round_err = rc == DEFT_OK ? std::string() : g_last_error;
+  b->requests += b->nposted;
+  b->posted[0].clear();
+  b->posted[1].clear();
+  b->nposted = 0;
+  b->round++;
+  b->rounds_run++;
+  b->cv.notify_all();
+  return rc;
+}
+
+int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+  deft_batch_s *b = p->batch;
+  std::unique_lock<std::mutex> lk(b->mu);
+  b->posted[prec == DEFT_FFT_FP64 ? 1 : 0].push_back(BatchReq{p, grad, have_old, precond});
+  b->nposted++;
+  const long long mine = b->round;
+  if (b->nposted >= std::max(b->active, 1))
+    run_round(b);
+  else
+    b->cv.wait(lk, [&] { return b->round != mine; });
+  if (b->round_rc != DEFT_OK) return set_error(b->round_rc, b->round_err);
+  return DEFT_OK;
+}
+
+void batch_thread_done(deft_batch_s *b) {
+  std::unique_lock<std::mutex> lk(b->mu);
+  b->active--;
+  if (b->nposted > 0 && b->nposted >= b->active) run_round(b);
+}
+
 // Copies between the compact half-spectrum layout of the seam and the padded
 // rows kept on the device.
 template <typename T>
@@ -607,7 +682,68 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
 }
 
 int deft_plan_destroy(deft_plan p) {
+  if (p && p->batch) return set_error(DEFT_ERR_STATE, "the plan is in a batch: destroy the batch first");
   free_plan(p);
+  return DEFT_OK;
+}
+
+int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
+  if (!out || !plans || n < 1) return set_error(DEFT_ERR_ARG, "need plans and an output");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i]) return set_error(DEFT_ERR_ARG, "null plan");
+    if (plans[i]->batch) return set_error(DEFT_ERR_STATE, "a plan is already in a batch");
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return set_error(DEFT_ERR_ARG, "a plan appears twice");
+  }
+  deft_batch_s *b = new deft_batch_s();
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return set_error(DEFT_ERR_HIP, "stream creation failed");
+  }
+  b->stage.cap = 65536 + (size_t)n * 4096;
+  if (hipHostMalloc((void **)&b->stage.host, b->stage.cap, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void **)&b->stage.dev, b->stage.cap) != hipSuccess) {
+    if (b->stage.host) (void)hipHostFree(b->stage.host);
+    if (b->stage.dev) (void)hipFree(b->stage.dev);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return set_error(DEFT_ERR_ALLOC, "batch staging allocation failed");
+  }
+  for (int i = 0; i < n; ++i) {
+    deft_plan_s *p = plans[i];
+    TRY(flush_moves(p));
+    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+    p->own_stream = p->stream;
+    p->stream = b->stream;
+    p->batch = b;
+    b->plans.push_back(p);
+  }
+  *out = b;
+  return DEFT_OK;
+}
+
+int deft_batch_destroy(deft_batch b) {
+  if (!b) return DEFT_OK;
+  if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is minimizing");
+  (void)hipStreamSynchronize(b->stream);
+  for (deft_plan_s *p : b->plans) {
+    p->stream = p->own_stream;
+    p->own_stream = nullptr;
+    p->batch = nullptr;
+  }
+  (void)hipHostFree(b->stage.host);
+  (void)hipFree(b->stage.dev);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return DEFT_OK;
+}
+
+int deft_batch_stats(deft_batch b, long long *rounds, long long *launches, long long *requests) {
+  if (!b) return set_error(DEFT_ERR_ARG, "null batch");
+  if (rounds) *rounds = b->rounds_run;
+  if (launches) *launches = b->launches;
+  if (requests) *requests = b->requests;
   return DEFT_OK;
 }
 
@@ -901,6 +1037,36 @@ int deft_minimize_precondition(deft_minimize m, int v) {
 int deft_minimize_improve_energy(deft_minimize m, int verbose, int *more) {
   if (!m || !more) return set_error(DEFT_ERR_ARG, "null argument");
   return improve_energy_mixed(m, verbose, more);
+}
+
+int deft_batch_minimize(deft_batch b, deft_minimize *mins, int n) {
+  if (!b || !mins || n < 1) return set_error(DEFT_ERR_ARG, "need a batch and minimizers");
+  for (int i = 0; i < n; ++i) {
+    if (!mins[i] || mins[i]->p->batch != b) return set_error(DEFT_ERR_ARG, "every minimizer's plan must be in the batch");
+    for (int j = 0; j < i; ++j)
+      if (mins[j]->p == mins[i]->p) return set_error(DEFT_ERR_ARG, "two minimizers share a plan");
+  }
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is already minimizing");
+    b->active = n;
+  }
+  std::vector<int> rcs(n, DEFT_OK);
+  std::vector<std::string> errs(n);
+  std::vector<std::thread> threads;
+  threads.reserve(n);
+  for (int i = 0; i < n; ++i)
+    threads.emplace_back([&, i] {
+      int more = 1, rc = DEFT_OK;
+      while (more && rc == DEFT_OK) rc = improve_energy_mixed(mins[i], 0, &more);
+      rcs[i] = rc;
+      if (rc != DEFT_OK) errs[i] = g_last_error;
+      batch_thread_done(b);
+    });
+  for (std::thread &t : threads) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rcs[i] != DEFT_OK) return set_error(rcs[i], errs[i]);
+  return DEFT_OK;
 }
 
 int deft_minimize_run(deft_minimize m, int max_calls, int *iterations) {

@@ -4,8 +4,13 @@
 // instantiates Launch<T>.
 #pragma once
 
+#include <cstring>
+#include <map>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
+#include "batch_kernels.hpp"
 #include "plan.hpp"
 
 namespace deft {
@@ -207,6 +212,185 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
   }
   DEFT_HIP_CHECK(hipGetLastError());
   return plan_sync(p);
+}
+
+// --- batched evaluation ----------------------------------------------------
+
+namespace batch_detail {
+
+struct Group {
+  int L = 0, tune = 0, flag = 0;
+  std::vector<int> members;  // request indices
+  std::vector<int> off;      // block offsets, size members + 1
+  size_t off_at = 0, idx_at = 0;
+  int total() const { return off.back(); }
+};
+
+// groups of the requests accepted by `pred`, keyed by (axis length, tuning, flag)
+template <typename KeyFn, typename BlocksFn, typename PredFn>
+std::vector<Group> make_groups(int n, KeyFn key, BlocksFn blocks, PredFn pred) {
+  std::map<std::tuple<int, int, int>, Group> m;
+  for (int i = 0; i < n; ++i) {
+    if (!pred(i)) continue;
+    const auto k = key(i);
+    Group &g = m[k];
+    g.L = std::get<0>(k);
+    g.tune = std::get<1>(k);
+    g.flag = std::get<2>(k);
+    if (g.off.empty()) g.off.push_back(0);
+    g.members.push_back(i);
+    g.off.push_back(g.off.back() + blocks(i));
+  }
+  std::vector<Group> out;
+  for (auto &kv : m) out.push_back(std::move(kv.second));
+  return out;
+}
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace batch_detail
+
+template <typename T>
+int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *reqs, int n, long long *launches) {
+  using namespace batch_detail;
+  if (n <= 0) return DEFT_OK;
+  const int ti = prec_index<T>();
+  std::vector<BatchArgs<T>> args(n);
+  for (int i = 0; i < n; ++i) {
+    deft_plan_s *p = reqs[i].p;
+    BatchArgs<T> &a = args[i];
+    take_moves(p, a.m1, a.m2, a.nmoves);
+    a.g = grid_args(p);
+    a.x = p->x;
+    a.d = p->d;
+    a.have_old = reqs[i].have_old ? 1 : 0;
+    a.precond = reqs[i].precond ? 1 : 0;
+    a.nzb = zblocks_for(p, p->tz[ti]);
+    a.vext = p->vext;
+    a.spec0 = spec_ptr<T>(p, 0);
+    a.spec1 = spec_ptr<T>(p, 1);
+    a.partial = p->partial;
+    a.pstride = p->pstride;
+    a.twx = tw_ptr<T>(p, 0);
+    a.twy = tw_ptr<T>(p, 1);
+    a.twz = tw_ptr<T>(p, 2);
+    a.og = p->og;
+    a.grad = p->g;
+    a.pgrad = p->pg;
+    a.dscal = p->dscal;
+  }
+  auto P = [&](int i) { return reqs[i].p; };
+  auto all = [&](int) { return true; };
+  auto gr = [&](int i) { return reqs[i].grad; };
+  auto zkey = [&](int i) { return std::make_tuple(P(i)->Nz, P(i)->tz[ti], 0); };
+  auto zkey_g = [&](int i) { return std::make_tuple(P(i)->Nz, P(i)->tz[ti], reqs[i].grad ? 1 : 0); };
+  auto ykey = [&](int i) { return std::make_tuple(P(i)->Ny, P(i)->ty[ti], 0); };
+  auto xkey = [&](int i) { return std::make_tuple(P(i)->Nx, P(i)->tx[ti], 0); };
+  auto zbl = [&](int i) { return zblocks_for(P(i), P(i)->tz[ti]); };
+  auto ybl = [&](int i) { return y_blocks<T>(P(i), P(i)->ty[ti]); };
+  auto xbl = [&](int i) { return x_blocks<T>(P(i), P(i)->tx[ti]); };
+  auto rkey = [&](int i) { return std::make_tuple(0, 0, reqs[i].grad ? 8 : 6); };
+  std::vector<Group> g_dens = make_groups(n, zkey, zbl, all);
+  std::vector<Group> g_y1 = make_groups(n, ykey, ybl, all);
+  std::vector<Group> g_xf = make_groups(n, xkey, xbl, all);
+  std::vector<Group> g_phi = make_groups(n, zkey_g, zbl, all);
+  std::vector<Group> g_yg = make_groups(n, ykey, ybl, gr);
+  std::vector<Group> g_xb = make_groups(n, xkey, xbl, gr);
+  std::vector<Group> g_zg = make_groups(n, zkey, zbl, gr);
+  std::vector<Group> g_red = make_groups(n, rkey, [&](int i) { return reqs[i].grad ? 8 : 6; }, all);
+  std::vector<Group> *stages[] = {&g_dens, &g_y1, &g_xf, &g_phi, &g_yg, &g_xb, &g_zg, &g_red};
+  // staging layout: args, then offsets and member indices of every group
+  size_t at = align_up(S.used, 256);
+  const size_t args_at = at;
+  at += sizeof(BatchArgs<T>) * n;
+  for (auto *gv : stages)
+    for (Group &g : *gv) {
+      at = align_up(at, 16);
+      g.off_at = at;
+      at += sizeof(int) * g.off.size();
+      g.idx_at = at;
+      at += sizeof(int) * g.members.size();
+    }
+  if (at > S.cap) return set_error(DEFT_ERR_ALLOC, "batch staging buffer too small");
+  std::memcpy(S.host + args_at, args.data(), sizeof(BatchArgs<T>) * n);
+  for (auto *gv : stages)
+    for (Group &g : *gv) {
+      std::memcpy(S.host + g.off_at, g.off.data(), sizeof(int) * g.off.size());
+      std::memcpy(S.host + g.idx_at, g.members.data(), sizeof(int) * g.members.size());
+    }
+  DEFT_HIP_CHECK(hipMemcpyAsync(S.dev + S.used, S.host + S.used, at - S.used, hipMemcpyHostToDevice, st));
+  S.used = at;
+  const BatchArgs<T> *dargs = reinterpret_cast<const BatchArgs<T> *>(S.dev + args_at);
+  auto grp = [&](const Group &g) {
+    return BatchGroup{reinterpret_cast<const int *>(S.dev + g.off_at), reinterpret_cast<const int *>(S.dev + g.idx_at),
+                      (int)g.members.size()};
+  };
+  long long nl = 0;
+  auto zpass = [&](const std::vector<Group> &gs, auto launch) {
+    for (const Group &g : gs)
+      with_length(g.L, [&](auto LZ) {
+        with_zb(g.tune, [&](auto ZBc) {
+          launch(LZ, ZBc, g);
+          ++nl;
+        });
+      });
+  };
+  auto tpass = [&](const std::vector<Group> &gs, auto launch) {
+    for (const Group &g : gs)
+      with_length(g.L, [&](auto LA) {
+        with_tb(g.tune, [&](auto TBc) {
+          launch(LA, TBc, g);
+          ++nl;
+        });
+      });
+  };
+  const dim3 blk(kThreads);
+  zpass(g_dens, [&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zfwd_density<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_y1, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0);
+  });
+  tpass(g_xf, [&](auto LX, auto TBc, const Group &g) {
+    constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_xconv_fwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_y1, [&](auto LY, auto TBc, const Group &g) {  // same grouping: inverse y of the 5 fields
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1);
+  });
+  zpass(g_phi, [&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    if (g.flag)
+      hipLaunchKernelGGL((kb_zphi<T, L, true, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+    else
+      hipLaunchKernelGGL((kb_zphi<T, L, false, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_yg, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1);
+  });
+  tpass(g_xb, [&](auto LX, auto TBc, const Group &g) {
+    constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_xconv_bwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_yg, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0);
+  });
+  zpass(g_zg, [&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zgrad<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  for (const Group &g : g_red) {
+    hipLaunchKernelGGL(kb_reduce<T>, dim3(g.total()), blk, 0, st, grp(g), dargs, g.flag);
+    ++nl;
+  }
+  DEFT_HIP_CHECK(hipGetLastError());
+  if (launches) *launches += nl;
+  return DEFT_OK;
 }
 
 template <typename T>

@@ -72,10 +72,10 @@ constexpr int ccols(int L, int TB) {
 }
 static_assert(256 % ccols<float>(20, 16384) == 0 && ccols<double>(128, 8192) == 4, "column tiles");
 
-// Block-wide FP64 sums of NV values, stored to partial[(slot0+i)*stride + blockIdx.x].
+// Block-wide FP64 sums of NV values, stored to partial[(slot0+i)*stride + bx].
 template <int NV>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restrict__ partial, int slot0,
-                                                int stride) {
+                                                int stride, int bx) {
   __shared__ double red[NV][kThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -90,7 +90,7 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restr
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) s += red[threadIdx.x][w];
-    partial[(long long)(slot0 + threadIdx.x) * stride + blockIdx.x] = s;
+    partial[(long long)(slot0 + threadIdx.x) * stride + bx] = s;
   }
 }
 
@@ -225,17 +225,17 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
 // Also accumulates the local energy terms (ideal gas, Vext n, -mu n).
 // ---------------------------------------------------------------------------
 template <typename T, int L, int ZB>
-__global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* __restrict__ x,
-                                                           const double* __restrict__ d, double m1, double m2,
-                                                           int nmoves, const float* __restrict__ vext,
-                                                           cx<T>* __restrict__ out, double* __restrict__ partial,
-                                                           int pstride, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, double* __restrict__ x,
+                                                  const double* __restrict__ d, double m1, double m2, int nmoves,
+                                                  const float* __restrict__ vext, cx<T>* __restrict__ out,
+                                                  double* __restrict__ partial, int pstride,
+                                                  const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[P * SS];
   __shared__ cx<T> B[P * SS];
   const long long lines = (long long)g.Nx * g.Ny;
-  const long long line0 = (long long)blockIdx.x * 2 * P;
+  const long long line0 = (long long)bx * 2 * P;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
@@ -266,7 +266,16 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* _
   __syncthreads();
   const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
   store_r2c_halves<T, L, P>(Z, out, line0, lines, g.ldk);
-  block_sum_store<3>(acc, partial, SLOT_IDEAL, pstride);
+  block_sum_store<3>(acc, partial, SLOT_IDEAL, pstride, bx);
+}
+
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* __restrict__ x,
+                                                           const double* __restrict__ d, double m1, double m2,
+                                                           int nmoves, const float* __restrict__ vext,
+                                                           cx<T>* __restrict__ out, double* __restrict__ partial,
+                                                           int pstride, const cx<T>* __restrict__ tw) {
+  zfwd_density_body<T, L, ZB>(blockIdx.x, g, x, d, m1, m2, nmoves, vext, out, partial, pstride, tw);
 }
 
 // Plain r2c along z of a real FP64 field (FFT seam and tests).
@@ -329,9 +338,9 @@ inline int y_tail_blocks(int Nx, int Nkz, int C) {
   return (Nx * ntail + C - 1) / C;
 }
 
-template <typename T, int L, bool INV, int NF, int TB>
-__global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restrict__ data,
-                                                    const cx<T>* __restrict__ tw) {
+template <typename T, int L, bool INV, int TB>
+__device__ __forceinline__ void ypass_body(int bx, int by, const GridArgs& g, cx<T>* __restrict__ data,
+                                           const cx<T>* __restrict__ tw) {
   constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
@@ -344,17 +353,17 @@ __global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restric
   // per-sequence base offsets (x plane + kz) for this thread's column slot
   const int myc = threadIdx.x % C;  // kThreads is a multiple of C
   long long off_c = 0;
-  if ((int)blockIdx.x < nmain) {
-    const int x = blockIdx.x / nfull, ch = blockIdx.x - x * nfull;
+  if (bx < nmain) {
+    const int x = bx / nfull, ch = bx - x * nfull;
     nseq = C;
     off_c = (long long)x * plane + ch * C + myc;
   } else {
-    const int q0 = (blockIdx.x - nmain) * C;
+    const int q0 = (bx - nmain) * C;
     nseq = min(C, g.Nx * ntail - q0);
     const int q = q0 + myc;
     if (myc < nseq) off_c = (long long)(q / ntail) * plane + nfull * C + (q % ntail);
   }
-  cx<T>* base = data + blockIdx.y * g.ks;  // gridDim.y == NF
+  cx<T>* base = data + by * g.ks;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int y = e / C;  // e % C == myc
     A[myc * SS + skew<T>(y)] = load_or_zero(base, off_c + (long long)y * g.ldk, (long long)y * g.ldk, myc < nseq);
@@ -365,6 +374,12 @@ __global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restric
     const int y = e / C;
     if (myc < nseq) base[off_c + (long long)y * g.ldk] = R[myc * SS + skew<T>(y)];
   }
+}
+
+template <typename T, int L, bool INV, int NF, int TB>
+__global__ __launch_bounds__(kThreads) void k_ypass(GridArgs g, cx<T>* __restrict__ data,
+                                                    const cx<T>* __restrict__ tw) {
+  ypass_body<T, L, INV, TB>(blockIdx.x, blockIdx.y, g, data, tw);  // gridDim.y == NF
 }
 
 // Plain FFT along x of one field, in place, times `scl`.
@@ -416,8 +431,8 @@ __device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const 
 // x pass with the weight multiply: n~ -> n3, n2, n2v_x, n2v_y, n2v_z spectra.
 // ---------------------------------------------------------------------------
 template <typename T, int L, int TB>
-__global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
-                                                        cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const cx<T>* __restrict__ in,
+                                               cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
@@ -426,7 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
   __shared__ cx<T> S2[C * SS];
   const long long ncols = (long long)g.Ny * g.Nkz;
   const long long plane = (long long)g.Ny * g.ldk;
-  const long long col0 = (long long)blockIdx.x * C;
+  const long long col0 = (long long)bx * C;
   const int ncol = (int)min((long long)C, ncols - col0);
   const int myc = threadIdx.x % C;
   const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
@@ -469,14 +484,20 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
   }
 }
 
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
+                                                        cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
+  xconv_fwd_body<T, L, TB>(blockIdx.x, g, in, out, tw);
+}
+
 // ---------------------------------------------------------------------------
 // x pass of the back-convolution: sum_a w_a(-k) FFT_x(P_a) -> inverse FFT_x.
 // Even weights keep their sign, odd (vector) weights flip it (deft
 // src/Functional.h:407-417).
 // ---------------------------------------------------------------------------
 template <typename T, int L, int TB>
-__global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
-                                                        cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const cx<T>* __restrict__ in,
+                                               cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols<T>(L, TB);
   constexpr int SS = lds_stride<T, L>();
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
@@ -484,7 +505,7 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
   __shared__ cx<T> B[C * SS];
   const long long ncols = (long long)g.Ny * g.Nkz;
   const long long plane = (long long)g.Ny * g.ldk;
-  const long long col0 = (long long)blockIdx.x * C;
+  const long long col0 = (long long)bx * C;
   const int ncol = (int)min((long long)C, ncols - col0);
   const int myc = threadIdx.x % C;
   const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
@@ -533,14 +554,19 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
   }
 }
 
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
+                                                        cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
+  xconv_bwd_body<T, L, TB>(blockIdx.x, g, in, out, tw);
+}
+
 // ---------------------------------------------------------------------------
 // z pass 2: c2r of the five weighted densities -> FP64 Phi (energy partial
 // sums) and, with GRAD, the five partials -> r2c back into the same buffers.
 // ---------------------------------------------------------------------------
 template <typename T, int L, bool GRAD, int ZB>
-__global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
-                                                   double* __restrict__ partial, int pstride,
-                                                   const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __restrict__ fields,
+                                          double* __restrict__ partial, int pstride, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
   constexpr int NPT = 2 * P * L;
   constexpr int PT = (NPT + kThreads - 1) / kThreads;
@@ -548,7 +574,7 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
-  const long long line0 = (long long)blockIdx.x * 2 * P;
+  const long long line0 = (long long)bx * 2 * P;
   T val[5][PT];
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
@@ -614,7 +640,14 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
       __syncthreads();
     }
   }
-  block_sum_store<3>(acc, partial, SLOT_PHI1, pstride);
+  block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, bx);
+}
+
+template <typename T, int L, bool GRAD, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
+                                                   double* __restrict__ partial, int pstride,
+                                                   const cx<T>* __restrict__ tw) {
+  zphi_body<T, L, GRAD, ZB>(blockIdx.x, g, fields, partial, pstride, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,18 +655,18 @@ __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict
 // with pgrad.grad and pgrad.oldgrad partial sums.
 // ---------------------------------------------------------------------------
 template <typename T, int L, int ZB>
-__global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __restrict__ accs,
-                                                    const double* __restrict__ x, const float* __restrict__ vext,
-                                                    const double* __restrict__ oldgrad, int have_old, int precond,
-                                                    double* __restrict__ grad, double* __restrict__ pgrad,
-                                                    double* __restrict__ partial, int pstride,
-                                                    const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T>* __restrict__ accs,
+                                           const double* __restrict__ x, const float* __restrict__ vext,
+                                           const double* __restrict__ oldgrad, int have_old, int precond,
+                                           double* __restrict__ grad, double* __restrict__ pgrad,
+                                           double* __restrict__ partial, int pstride,
+                                           const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
-  const long long line0 = (long long)blockIdx.x * 2 * P;
+  const long long line0 = (long long)bx * 2 * P;
   load_c2r_packed<T, L, P>(accs, line0, lines, g.ldk, A, B);
   const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
   double dots[2] = {0.0, 0.0};
@@ -659,8 +692,18 @@ __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __r
       if (have_old) dots[1] += pg * oldgrad[idx];
     }
   }
-  block_sum_store<2>(dots, partial, SLOT_PGG, pstride);
+  block_sum_store<2>(dots, partial, SLOT_PGG, pstride, bx);
 }
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __restrict__ accs,
+                                                    const double* __restrict__ x, const float* __restrict__ vext,
+                                                    const double* __restrict__ oldgrad, int have_old, int precond,
+                                                    double* __restrict__ grad, double* __restrict__ pgrad,
+                                                    double* __restrict__ partial, int pstride,
+                                                    const cx<T>* __restrict__ tw) {
+  zgrad_body<T, L, ZB>(blockIdx.x, g, accs, x, vext, oldgrad, have_old, precond, grad, pgrad, partial, pstride, tw);
+}
+
 // ---------------------------------------------------------------------------
 // CG vector kernels (FP64)
 // ---------------------------------------------------------------------------
@@ -679,7 +722,7 @@ static __global__ __launch_bounds__(kThreads) void k_cg_update(long long N, cons
     oldgrad[i] = gi;
     acc[0] += gi * di;
   }
-  block_sum_store<1>(acc, partial, SLOT_GD, pstride);
+  block_sum_store<1>(acc, partial, SLOT_GD, pstride, blockIdx.x);
 }
 
 // direction = -grad, partial grad.direction
@@ -692,7 +735,7 @@ static __global__ __launch_bounds__(kThreads) void k_cg_reset(long long N, const
     dir[i] = -gi;
     acc[0] += gi * -gi;
   }
-  block_sum_store<1>(acc, partial, SLOT_GD, pstride);
+  block_sum_store<1>(acc, partial, SLOT_GD, pstride, blockIdx.x);
 }
 
 // x = x + s*d, rounded like the host expression x += s*d
@@ -709,7 +752,7 @@ static __global__ __launch_bounds__(kThreads) void k_dot(long long N, const doub
   double acc[1] = {0.0};
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)
     acc[0] += a[i] * b[i];
-  block_sum_store<1>(acc, partial, SLOT_NORM, pstride);
+  block_sum_store<1>(acc, partial, SLOT_NORM, pstride, blockIdx.x);
 }
 
 // n = exp(-Veff/kT), 0 on hard walls
@@ -727,10 +770,9 @@ static __global__ __launch_bounds__(kThreads) void k_accumulate(long long N, dou
     out[i] += a[i];
 }
 
-// out[slot] = sum_{b < nb} partial[slot*pstride + b] for slots [slot0, slot0 + gridDim.x)
-static __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partial, int pstride, int slot0,
-                                                     int nb, double* __restrict__ out) {
-  const int slot = slot0 + blockIdx.x;
+// out[slot] = sum_{b < nb} partial[slot*pstride + b], in a fixed order
+__device__ __forceinline__ void reduce_body(int slot, const double* __restrict__ partial, int pstride, int nb,
+                                            double* __restrict__ out) {
   const double* p = partial + (long long)slot * pstride;
   double acc[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += kThreads) acc[0] += p[b];
@@ -747,6 +789,12 @@ static __global__ __launch_bounds__(kThreads) void k_reduce(const double* __rest
     for (int w = 0; w < kThreads / 64; ++w) t += red[w];
     out[slot] = t;
   }
+}
+
+// out[slot] for slots [slot0, slot0 + gridDim.x)
+static __global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partial, int pstride, int slot0,
+                                                     int nb, double* __restrict__ out) {
+  reduce_body(slot0 + blockIdx.x, partial, pstride, nb, out);
 }
 
 }  // namespace deft

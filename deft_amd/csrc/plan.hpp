@@ -62,6 +62,8 @@ struct PendingEvent {
 
 }  // namespace deft
 
+struct deft_batch_s;
+
 struct deft_plan_s {
   int Nx = 0, Ny = 0, Nz = 0, Nkz = 0;
   int ldk = 0;        // padded half-spectrum row stride on the device
@@ -104,6 +106,8 @@ struct deft_plan_s {
   size_t pool_used = 0;
   deft::KernelStat kstat[deft::NUM_KERNELS];
   int eblocks = 0;  // blocks of the elementwise FP64 kernels
+  deft_batch_s *batch = nullptr;  // evaluations go through this batch's rounds
+  hipStream_t own_stream = nullptr;  // the plan's stream while it is in a batch
   // tuning per FFT precision (index 0 FP32, 1 FP64): z-pass block size (complex
   // values), y-pass and x-pass tile bytes; set by deft_plan_autotune
   int tz[2] = {1024, 1024};
@@ -216,10 +220,28 @@ inline int plan_sync(deft_plan_s *p) {
   return DEFT_OK;
 }
 
+// One grid's evaluation request in a batched round.
+struct BatchReq {
+  deft_plan_s *p;
+  bool grad, have_old, precond;
+};
+
+// Pinned host staging and its device copy for one round's argument tables.
+struct BatchStage {
+  char *host = nullptr;
+  char *dev = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+};
+
 // Kernel launch sequences, instantiated for T = float and T = double in
 // separate translation units.
 template <typename T>
 struct Launch {
+  // The same evaluation for several grids at once (all at precision T): the
+  // passes are launched once per group of grids with equal axis length and
+  // tuning; enqueued on `st` without synchronising.  launches += kernels run.
+  static int evaluate_batch(hipStream_t st, BatchStage &stage, const BatchReq *reqs, int n, long long *launches);
   // Energy (grad=false) or energy + grad + pgrad (grad=true) of the resident
   // Veff after applying pending moves; reduced sums land in p->hscal.
   static int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond);

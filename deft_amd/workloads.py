@@ -52,47 +52,41 @@ def c2_solute_vext(n=128, length=12.8, radius=3.0):
     return sphere_vext((n, n, n), (length,) * 3, (0.0, 0.0, 0.0), radius)
 
 
-def c3_instance(rng, R=1.0):
-    """One C3 instance: (shape, cell, vext, eta)."""
-    shape = tuple(int(rng.choice(C3_EDGES)) for _ in range(3))
-    cell = tuple(n / 10.0 for n in shape)  # dx = 0.1
+def c3_vext(shape, cell, rng, R=1.0):
+    """The C3 potential on a grid: 8 Gaussians A exp(-|r-c|^2 / 2w^2) with
+    A ~ U(-2, 2) kT, w ~ U(0.5, 2), c uniform, plus one planar hard wall of
+    thickness R on a random axis at a random offset."""
     V = np.zeros(shape)
     for _ in range(8):
         A = rng.uniform(-2.0, 2.0)
         w = rng.uniform(0.5, 2.0)
         c = [rng.uniform(0, L) for L in cell]
         dx, dy, dz = (_axis(n, L, ci) for n, L, ci in zip(shape, cell, c))
-        r2 = dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2
-        V += A * np.exp(-r2 / (2 * w * w))
-    axis = int(rng.integers(0, 3))
-    d = np.abs(_axis(shape[axis], cell[axis], rng.uniform(0, cell[axis])))
-    wall = d < R / 2
-    sl = [None, None, None]
-    sl[axis] = slice(None)
-    V = np.where(np.broadcast_to(wall[tuple(sl)], shape), np.inf, V)
-    eta = float(rng.uniform(0.1, 0.45))
-    return shape, cell, V, eta
-
-
-def c3_cubic_vext(M, L, rng, R=1.0):
-    """The C3 potential (8 Gaussians and one planar hard wall) on a cubic M^3 grid of edge L."""
-    shape, cell = (M, M, M), (L, L, L)
-    V = np.zeros(shape)
-    for _ in range(8):
-        A = rng.uniform(-2.0, 2.0)
-        w = rng.uniform(0.5, 2.0)
-        dx, dy, dz = (_axis(M, L, rng.uniform(0, L)) for _ in range(3))
         V += A * np.exp(-(dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2) / (2 * w * w))
     axis = int(rng.integers(0, 3))
-    wall = np.abs(_axis(M, L, rng.uniform(0, L))) < R / 2
+    wall = np.abs(_axis(shape[axis], cell[axis], rng.uniform(0, cell[axis]))) < R / 2
     sl = [None, None, None]
     sl[axis] = slice(None)
     return np.where(np.broadcast_to(wall[tuple(sl)], shape), np.inf, V)
 
 
-def c3_batch(count=64, seed=12345):
+def c3_instance(rng, edges=C3_EDGES, R=1.0):
+    """One C3 instance: (shape, cell, vext, eta); edges drawn per axis from `edges`."""
+    shape = tuple(int(rng.choice(edges)) for _ in range(3))
+    cell = tuple(n / 10.0 for n in shape)  # dx = 0.1
+    V = c3_vext(shape, cell, rng, R)
+    eta = float(rng.uniform(0.1, 0.45))
+    return shape, cell, V, eta
+
+
+def c3_cubic_vext(M, L, rng, R=1.0):
+    """The C3 potential on a cubic M^3 grid of edge L."""
+    return c3_vext((M, M, M), (L, L, L), rng, R)
+
+
+def c3_batch(count=64, seed=12345, edges=C3_EDGES):
     rng = np.random.default_rng(seed)
-    return [c3_instance(rng) for _ in range(count)]
+    return [c3_instance(rng, edges) for _ in range(count)]
 
 
 def c5_instance(rng, n=256, length=12.8, obstacles=30):

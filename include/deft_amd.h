@@ -210,6 +210,26 @@ int deft_minimize_fp32_iterations(deft_minimize m, int *iterations);
 /* Iterations of phase 0 and phase 1 and the current phase (0, 1 or 2). */
 int deft_minimize_phase_iterations(deft_minimize m, int *fp32_iterations, int *mixed_iterations, int *phase);
 
+/* --- batches (no counterpart in the reference, which runs one minimization
+ * per process, e.g. papers/contact/figs/inner-sphere.cpp:177-211) -----------
+ * A batch joins plans of any grid shapes (uneven edges such as {32, 48, 64,
+ * 96} per axis).  deft_batch_minimize runs one minimizer per plan, each with
+ * the unchanged algorithm above (and its mixed-precision ladder if set), in a
+ * host thread of its own; every evaluation they ask for joins a lockstep
+ * round, launched as one kernel per pass and group of grids with equal axis
+ * length, with one device synchronisation per round.  Each grid gets bit for
+ * bit the energies and iterates of its own unbatched run.  While in a batch a
+ * plan's work runs on the batch's stream, and the plan cannot be destroyed. */
+typedef struct deft_batch_s *deft_batch;
+int deft_batch_create(deft_plan *plans, int n, deft_batch *out);
+/* Runs every minimizer to its own stop; minimizers[i] must belong to a plan of
+ * the batch, at most one per plan. */
+int deft_batch_minimize(deft_batch batch, deft_minimize *minimizers, int n);
+/* Rounds run, kernel launches of the rounds, evaluations served. */
+int deft_batch_stats(deft_batch batch, long long *rounds, long long *launches, long long *requests);
+/* Returns the plans to their own streams. */
+int deft_batch_destroy(deft_batch batch);
+
 /* --- profiling (the reference only has ad-hoc clock() timers,
  * src/new/Minimize.h:95-135) ------------------------------------------------
  * With profiling on, every kernel launch of the plan is bracketed by HIP
