@@ -105,6 +105,68 @@ def gpu_run(steps, warmup):
     return out, gold, precision
 
 
+def c3_run(count=64, seed=12345, reps=2):
+    """64 uneven grids (edges per axis from {32, 48, 64, 96}, dx = 0.1, 8 Gaussians and a
+    planar hard wall, eta ~ U(0.1, 0.45)): one batch against the same minimizations run
+    one after another; both from the bulk start, same ladder and precision rule as above."""
+    insts = W.c3_batch(count, seed=seed)
+    fluids = []
+    for shape, cell, V, eta in insts:
+        nb = eta / (4 * math.pi / 3 * R ** 3)
+        f = D.WhiteBearFluidVeff(shape, np.diag(cell), R=R, kT=KT, mu=D.find_chemical_potential(nb, R, KT),
+                                 fft_precision=D.FFT_FP32)
+        f.Vext = V
+        f.Veff = np.full(shape, -KT * math.log(nb))
+        fluids.append((f, f.Veff.copy(), abs(D.pressure(nb, R, KT) * np.prod(cell)) * PRECISION_FACTOR))
+
+    def minimizers():
+        ms = []
+        for f, v0, prec in fluids:
+            f.Veff = v0
+            m = D.Minimize(f)
+            m.precondition(True)
+            m.set_precision(prec)
+            m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
+            ms.append(m)
+        return ms
+
+    def outcome(ms):
+        r = [(m.stats()["iter"], m.energy(), m.stats()["stop_reason"]) for m in ms]
+        for m in ms:
+            m.close()
+        return r
+
+    best_seq = best_bat = None
+    for _ in range(reps):
+        ms = minimizers()
+        t0 = time.perf_counter()
+        for m in ms:
+            m.run()
+        dt = time.perf_counter() - t0
+        seq = outcome(ms)
+        best_seq = dt if best_seq is None else min(best_seq, dt)
+    batch = D.Batch([f for f, _, _ in fluids])
+    for _ in range(reps):
+        ms = minimizers()
+        t0 = time.perf_counter()
+        batch.minimize(ms)
+        dt = time.perf_counter() - t0
+        bat = outcome(ms)
+        best_bat = dt if best_bat is None else min(best_bat, dt)
+    st = batch.stats()
+    batch.close()
+    for f, _, _ in fluids:
+        f.close()
+    return {"workload": f"{count} uneven grids, edges per axis from {{32, 48, 64, 96}}, seed {seed}",
+            "points": int(sum(np.prod(s) for s, _, _, _ in insts)),
+            "batched_minimizations_per_s": count / best_bat, "sequential_minimizations_per_s": count / best_seq,
+            "batched_over_sequential": best_seq / best_bat,
+            "rounds_per_batch": st["rounds"] // reps, "launches_per_batch": st["launches"] // reps,
+            "iterations": [b[0] for b in bat],
+            "identical_to_sequential": all(a[:2] == b[:2] for a, b in zip(seq, bat)),
+            "all_converged": all(b[2] == "converged" for b in bat)}
+
+
 def rocprof_instance(family):
     """rocprof name of the kernel instance behind a profiler family (None when ambiguous)."""
     base, _, prec = family.rpartition("_")
@@ -157,6 +219,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=float, default=15.0)
+    ap.add_argument("--no-c3", action="store_true", help="skip the 64-grid batch measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -185,6 +248,7 @@ def main():
     achieved = kby / (kms * 1e-3) / 1e9
     total_ms = sum(v[1] for v in prof.values())
     total_by = sum(v[2] for v in prof.values())
+    c3 = None if args.no_c3 else c3_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
     r0, p0 = w["results"][-1], pl["results"][-1]
     line = {
@@ -210,6 +274,7 @@ def main():
                              "grad_calcs": p0["num_grad_calcs"], "dF_over_F_vs_fp64_oracle": pl["err"],
                              "max_dn_over_nmax_x_axis": pl["dn"]},
                    "warm_over_plain": value / plain_value,
+                   "c3_batch": c3,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname),

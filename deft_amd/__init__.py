@@ -132,6 +132,7 @@ _SIGNATURES = {
     "deft_net_loss_grad": ([_P, _I, _I, _P, _FP, _P, _P, _FP, _FP], _I),
     "deft_net_train_step": ([_P, _I, _I, _P, _FP, _P, _P, ctypes.c_float, _FP], _I),
     "deft_plan_warm_start": ([_P, _P, _D], _I),
+    "deft_plan_warm_start_scaled": ([_P, _P, _D, _I], _I),
     "deft_net_last_blend": ([_P, _DP], _I),
     "deft_buffer_create": ([ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_buffer_destroy": ([_P], _I),

@@ -501,6 +501,55 @@ __global__ __launch_bounds__(kNT) void k_boltz(const float *__restrict__ vext, f
   }
 }
 
+// coarse Boltzmann factor at point (X, Y, Z) of an (M/f)^3 grid: the fine
+// points within f/2 of fine point (fX, fY, fZ), weights 1/f per axis (1/(2f)
+// at distance exactly f/2), periodic, so the coarse value sits on its point
+__global__ __launch_bounds__(kNT) void k_boltz_coarse(const float *__restrict__ vext, float inv_kT,
+                                                      float *__restrict__ out, int M, int f) {
+  const int Mc = M / f;
+  const long long nc = (long long)Mc * Mc * Mc;
+  const int h = f / 2;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < nc; i += (long long)gridDim.x * kNT) {
+    const int X = (int)(i / ((long long)Mc * Mc)), Y = (int)((i / Mc) % Mc), Z = (int)(i % Mc);
+    float acc = 0.f;
+    for (int a = -h; a <= h; ++a) {
+      const float wa = (a == -h || a == h) ? 0.5f : 1.f;
+      const int x = (f * X + a + M) % M;
+      for (int b = -h; b <= h; ++b) {
+        const float wb = (b == -h || b == h) ? 0.5f : 1.f;
+        const int y = (f * Y + b + M) % M;
+        for (int c = -h; c <= h; ++c) {
+          const float wc = (c == -h || c == h) ? 0.5f : 1.f;
+          const int z = (f * Z + c + M) % M;
+          const float v = vext[((long long)x * M + y) * M + z];
+          acc += wa * wb * wc * ((v == INFINITY) ? 0.f : expf(-v * inv_kT));
+        }
+      }
+    }
+    out[i] = acc / (float)(f * f * f);
+  }
+}
+
+// Veff on the fine M^3 grid from the coarse prediction u ((M/f)^3), periodic
+// trilinear interpolation at coarse coordinates (x/f, y/f, z/f)
+__global__ __launch_bounds__(kNT) void k_veff_from_u_fine(const float *__restrict__ u, int M, int f, double kT,
+                                                          double log_nb, double lambda, double *__restrict__ xout) {
+  const int Mc = M / f;
+  const long long n = (long long)M * M * M;
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT) {
+    const int x = (int)(i / ((long long)M * M)), y = (int)((i / M) % M), z = (int)(i % M);
+    const int x0 = x / f, y0 = y / f, z0 = z / f;
+    const float tx = (float)(x - x0 * f) / f, ty = (float)(y - y0 * f) / f, tz = (float)(z - z0 * f) / f;
+    const int x1 = (x0 + 1) % Mc, y1 = (y0 + 1) % Mc, z1 = (z0 + 1) % Mc;
+    auto U = [&](int a, int b, int c) { return u[((long long)a * Mc + b) * Mc + c]; };
+    const float v = (1 - tx) * ((1 - ty) * ((1 - tz) * U(x0, y0, z0) + tz * U(x0, y0, z1)) +
+                                ty * ((1 - tz) * U(x0, y1, z0) + tz * U(x0, y1, z1))) +
+                    tx * ((1 - ty) * ((1 - tz) * U(x1, y0, z0) + tz * U(x1, y0, z1)) +
+                          ty * ((1 - tz) * U(x1, y1, z0) + tz * U(x1, y1, z1)));
+    xout[i] = -kT * (log_nb + lambda * (double)v);
+  }
+}
+
 __global__ __launch_bounds__(kNT) void k_veff_from_u(const float *__restrict__ u, double kT, double log_nb,
                                                      double lambda, double *__restrict__ x, long long n) {
   for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT)
@@ -1033,14 +1082,17 @@ int deft_net_make_batch(const float *data_dev, long long count, int M, int batch
   return DEFT_OK;
 }
 
-int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
+int deft_plan_warm_start_scaled(deft_plan p, deft_net n, double eta, int factor) {
   if (!p) return set_error(DEFT_ERR_ARG, "null plan");
   NN_TRY(check_net(n));
   if (!(eta > 0 && eta < 0.74)) return set_error(DEFT_ERR_ARG, "eta must lie in (0, 0.74)");
+  if (factor != 1 && factor != 2 && factor != 4) return set_error(DEFT_ERR_ARG, "factor must be 1, 2 or 4");
   if (p->Nx != p->Ny || p->Nx != p->Nz) return set_error(DEFT_ERR_UNSUPPORTED, "warm start needs a cubic grid");
-  const int M = p->Nx;
-  NN_TRY(check_shape(M, 1));
-  const size_t need = sizeof(float) * 2 * (size_t)p->N;
+  if (p->Nx % factor) return set_error(DEFT_ERR_ARG, "the grid edge must be a multiple of the factor");
+  const int M = p->Nx, Mc = M / factor;
+  NN_TRY(check_shape(Mc, 1));
+  const long long Nc = (long long)Mc * Mc * Mc;
+  const size_t need = sizeof(float) * 2 * (size_t)Nc;
   if (n->ws_bytes < need) {
     (void)hipFree(n->ws);
     n->ws = nullptr;
@@ -1048,13 +1100,17 @@ int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
     NN_HIP(hipMalloc(&n->ws, need));
     n->ws_bytes = need;
   }
-  float *boltz = n->ws, *u = n->ws + p->N;
-  hipLaunchKernelGGL(k_boltz, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT), boltz,
-                     p->N);
+  float *boltz = n->ws, *u = n->ws + Nc;
+  if (factor == 1)
+    hipLaunchKernelGGL(k_boltz, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT), boltz,
+                       p->N);
+  else
+    hipLaunchKernelGGL(k_boltz_coarse, dim3(blocks_for(Nc)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT),
+                       boltz, M, factor);
   NN_HIP(hipGetLastError());
   NN_HIP(hipStreamSynchronize(p->stream));
   const float e = (float)eta;
-  NN_TRY(deft_net_forward(n, M, 1, boltz, &e, u));
+  NN_TRY(deft_net_forward(n, Mc, 1, boltz, &e, u));
   const double nb = eta / (4 * M_PI / 3 * p->R * p->R * p->R);
   // Veff = -kT (ln n_b + lambda u): lambda = 1 unless the prediction puts the
   // weighted packing fraction at or past 1 somewhere (infinite or NaN energy);
@@ -1062,8 +1118,12 @@ int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
   double lambda = 1.0;
   for (int attempt = 0;; ++attempt) {
     p->npend = 0;
-    hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb), lambda,
-                       p->x, p->N);
+    if (factor == 1)
+      hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb),
+                         lambda, p->x, p->N);
+    else
+      hipLaunchKernelGGL(k_veff_from_u_fine, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, M, factor, p->kT,
+                         std::log(nb), lambda, p->x);
     NN_HIP(hipGetLastError());
     NN_HIP(hipStreamSynchronize(p->stream));
     if (lambda == 0.0) break;
@@ -1076,6 +1136,8 @@ int deft_plan_warm_start(deft_plan p, deft_net n, double eta) {
   n->last_lambda = lambda;
   return DEFT_OK;
 }
+
+int deft_plan_warm_start(deft_plan p, deft_net n, double eta) { return deft_plan_warm_start_scaled(p, n, eta, 1); }
 
 int deft_net_last_blend(deft_net n, double *lambda) {
   NN_TRY(check_net(n));

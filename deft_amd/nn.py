@@ -180,11 +180,12 @@ class WarmStartNet:
                                          mask.ptr if mask is not None else None, float(lr), ctypes.byref(loss)))
         return loss.value
 
-    def warm_start(self, fluid, eta):
+    def warm_start(self, fluid, eta, factor=1):
         """Sets fluid's Veff from the network's prediction for its Vext; returns the blend
         factor lambda (1: prediction used as is; < 1: pulled toward the bulk start
-        because the prediction's energy was not finite)."""
-        _check(lib().deft_plan_warm_start(fluid.handle, self._h, float(eta)))
+        because the prediction's energy was not finite).  factor > 1 runs the network on
+        a grid `factor` times coarser (for grids finer than the training spacing)."""
+        _check(lib().deft_plan_warm_start_scaled(fluid.handle, self._h, float(eta), int(factor)))
         lam = ctypes.c_double()
         _check(lib().deft_net_last_blend(self._h, ctypes.byref(lam)))
         return lam.value

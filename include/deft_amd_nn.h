@@ -88,6 +88,12 @@ int deft_net_make_batch(const float *data_dev, long long count, int M, int batch
  * packing fraction at or past 1), u is scaled by lambda = 1/2, 1/4, ... and
  * finally 0 (the bulk start) until the energy is finite. */
 int deft_plan_warm_start(deft_plan plan, deft_net net, double eta);
+/* The same for a grid finer than the network's training spacing: the network
+ * runs on the (edge/factor)^3 grid (the Boltzmann factor smoothed onto it with
+ * weights 1/factor per axis over the points within factor/2, half weight at
+ * exactly factor/2), and u is interpolated back trilinearly (periodic).
+ * factor 1, 2 or 4; factor 1 is deft_plan_warm_start. */
+int deft_plan_warm_start_scaled(deft_plan plan, deft_net net, double eta, int factor);
 /* lambda of the last warm start (1 = the network's prediction unchanged). */
 int deft_net_last_blend(deft_net net, double *lambda);
 

@@ -368,6 +368,41 @@ def test_warm_start_sets_veff_from_prediction(gpu):
 
 
 @pytest.mark.gpu
+def test_scaled_warm_start_matches_numpy(gpu):
+    """factor 2: the network runs on the 2x coarser grid (tent-smoothed Boltzmann factor)
+    and its u is interpolated back trilinearly."""
+    net = deft_amd.nn.WarmStartNet(SMALL, seed=4)
+    M, L, eta = 64, 3.2, 0.3
+    rng = np.random.default_rng(11)
+    V = deft_amd.nn.random_vext(M, L, rng)
+    f = deft_amd.WhiteBearFluidVeff((M, M, M), np.diag([L] * 3), R=1, kT=1.0,
+                                    mu=deft_amd.find_chemical_potential(eta / (4 * math.pi / 3)))
+    f.Vext = V
+    lam = net.warm_start(f, eta, factor=2)
+    ok = np.isfinite(V)
+    b = np.where(ok, np.exp(-np.where(ok, V, 0)), 0).astype(np.float64)
+    for ax in range(3):
+        b = 0.25 * np.roll(b, 1, axis=ax) + 0.5 * b + 0.25 * np.roll(b, -1, axis=ax)
+    bc = b[::2, ::2, ::2].astype(np.float32)
+    u = net.forward(_dev(bc[None]), [eta]).numpy()[0].astype(np.float64)
+    for ax in range(3):
+        t = (np.arange(M) % 2) / 2.0
+        shape = [1, 1, 1]
+        shape[ax] = M
+        t = t.reshape(shape)
+        lo = np.repeat(u, 2, axis=ax)
+        hi = np.repeat(np.roll(u, -1, axis=ax), 2, axis=ax)
+        u = (1 - t) * lo + t * hi
+    want = -(math.log(eta / (4 * math.pi / 3)) + lam * u)
+    got = f.Veff
+    assert np.sqrt(np.mean((got - want) ** 2)) < 1e-3 * np.sqrt(np.mean(want ** 2))
+    with pytest.raises(deft_amd.DeftError):
+        net.warm_start(f, eta, factor=3)
+    f.close()
+    net.close()
+
+
+@pytest.mark.gpu
 def test_warm_start_keeps_the_start_energy_finite(gpu):
     """A prediction that packs the fluid past n3 = 1 (here u = 10 everywhere) is pulled
     toward the bulk start until the energy is finite."""
