@@ -70,18 +70,23 @@ __global__ __launch_bounds__(kThreads) void kb_ypass(BatchGroup grp, const Batch
   ypass_body<T, L, INV, TB>(blockIdx.x - grp.off[i], blockIdx.y, a.g, which ? a.spec1 : a.spec0, a.twy);
 }
 
+// (x passes: the XCD pairing of the single-grid kernels is applied to the
+// batch-wide block index, so the tiles a block gets can belong to a
+// neighbouring grid; each tile is computed exactly as in the single launch)
 template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void kb_xconv_fwd(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
-  const int i = batch_find(grp, blockIdx.x);
+  const int b = xcd_pair_tile(blockIdx.x, gridDim.x);
+  const int i = batch_find(grp, b);
   const BatchArgs<T>& a = args[grp.idx[i]];
-  xconv_fwd_body<T, L, TB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.spec1, a.twx);
+  xconv_fwd_body<T, L, TB>(b - grp.off[i], a.g, a.spec0, a.spec1, a.twx);
 }
 
 template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void kb_xconv_bwd(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
-  const int i = batch_find(grp, blockIdx.x);
+  const int b = xcd_pair_tile(blockIdx.x, gridDim.x);
+  const int i = batch_find(grp, b);
   const BatchArgs<T>& a = args[grp.idx[i]];
-  xconv_bwd_body<T, L, TB>(blockIdx.x - grp.off[i], a.g, a.spec1, a.spec0, a.twx);
+  xconv_bwd_body<T, L, TB>(b - grp.off[i], a.g, a.spec1, a.spec0, a.twx);
 }
 
 template <typename T, int L, bool GRAD, int ZB>

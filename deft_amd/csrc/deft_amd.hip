@@ -632,7 +632,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
       hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
-      hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->Nk) != hipSuccess)
+      hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
   // reduced sums land directly in host-mapped pinned memory: no copy per evaluation
   if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)

@@ -45,7 +45,7 @@ enum Slot : int {
 };
 
 struct GridArgs {
-  const double* wtab;  // FP64 weight table: w3/N then w2/N per valid spectral element
+  const double* wtab;  // FP64 weight table: (w3/N, w2/N) per spectral element, rows padded to ldk
   int Nx, Ny, Nz, Nkz;
   int ldk;        // padded half-spectrum row stride (complex values)
   long long N;    // real points
@@ -96,6 +96,20 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restr
 
 static __device__ __forceinline__ int wrapf(int m, int n) { return (m > n / 2) ? m - n : m; }
 
+// Blocks are dealt to the 8 XCDs round robin (block b runs on XCD b % 8), and
+// each XCD has its own L2.  An x-pass tile of 8 KiB reads 64 of every 128
+// bytes of a row; the next tile reads the other half.  This mapping gives
+// tiles 2k and 2k+1 to blocks 16q + r and 16q + r + 8, which share an XCD, so
+// the second half of each line is an L2 hit instead of a second HBM fetch.
+// The last (nb mod 16) blocks keep their index.
+constexpr int kXcds = 8;
+static __device__ __forceinline__ int xcd_pair_tile(int b, int nb) {
+  const int full = nb / (2 * kXcds) * (2 * kXcds);
+  if (b >= full) return b;
+  const int q = b / (2 * kXcds), r = b - q * 2 * kXcds;
+  return q * 2 * kXcds + 2 * (r % kXcds) + r / kXcds;
+}
+
 // p[i] if ok, else 0; the load itself always goes to a valid address (safe)
 // (a conditionally assigned complex<double> otherwise lands in scratch)
 template <typename T>
@@ -135,11 +149,11 @@ __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y,
     const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
     const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
     const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
-    const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
-    const long long idx = ((long long)x * g.Ny + y) * g.Nkz + kz;
-    const double a3 = g.wtab[idx];
+    const long long idx = ((long long)x * g.Ny + y) * g.ldk + kz;
+    const double2 w = reinterpret_cast<const double2*>(g.wtab)[idx];
+    const double a3 = w.x;
     w3 = (T)a3;
-    w2 = (T)g.wtab[nk + idx];
+    w2 = (T)w.y;
     wk[0] = (T)(-a3 * kx);
     wk[1] = (T)(-a3 * ky);
     wk[2] = (T)(-a3 * kz_);
@@ -159,8 +173,9 @@ static __global__ __launch_bounds__(kThreads) void k_weight_table(GridArgs g, do
     const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
     double a3, a2, av;
     fm_weights(sqrt(kx * kx + ky * ky + kz_ * kz_), g.R, g.six_dr2, a3, a2, av);
-    tab[i] = a3 * g.invN;
-    tab[nk + i] = a2 * g.invN;
+    const long long j = r * g.ldk + kz;  // padded rows, (w3, w2) interleaved
+    tab[2 * j] = a3 * g.invN;
+    tab[2 * j + 1] = a2 * g.invN;
   }
 }
 
@@ -409,6 +424,32 @@ __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __r
   }
 }
 
+// Column tile of an x-pass block, aligned like the y pass: blocks below
+// Ny*(Nkz/C) take C consecutive kz columns of one y row (row-aligned, so a tile
+// reads whole 64- or 128-byte segments); the remaining blocks gather the
+// columns past the last full chunk of each row, C (y, kz) pairs per block.
+// Returns this thread's column (y, kz), whether it exists (`ok`), and the
+// number of columns of the block.  The block count equals ceil(Ny*Nkz/C).
+template <int C>
+__device__ __forceinline__ int x_tile_column(const GridArgs& g, int bx, int myc, int& y, int& kz, bool& ok) {
+  const int nfull = g.Nkz / C;
+  const int ntail = g.Nkz - nfull * C;
+  const int nmain = g.Ny * nfull;
+  if (bx < nmain) {
+    y = bx / nfull;
+    kz = (bx - y * nfull) * C + myc;
+    ok = true;
+    return C;
+  }
+  const int q0 = (bx - nmain) * C;
+  const int ncol = min(C, g.Ny * ntail - q0);
+  ok = myc < ncol;
+  const int q = q0 + (ok ? myc : 0);
+  y = q / ntail;
+  kz = nfull * C + (q - y * ntail);
+  return ncol;
+}
+
 // Weight products of field f (0: w3, 1: w2, 2..4: i w2v_j) for a thread's elements.
 template <typename T, int PT>
 __device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const T (&w3)[PT], const T (&w2)[PT],
@@ -439,12 +480,13 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
   __shared__ cx<T> S2[C * SS];
-  const long long ncols = (long long)g.Ny * g.Nkz;
   const long long plane = (long long)g.Ny * g.ldk;
-  const long long col0 = (long long)bx * C;
-  const int ncol = (int)min((long long)C, ncols - col0);
   const int myc = threadIdx.x % C;
-  const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
+  int cy, ckz;
+  bool cok;
+  const int ncol = x_tile_column<C>(g, bx, myc, cy, ckz, cok);  // cok == (myc < ncol)
+  (void)cok;
+  const long long off = (long long)cy * g.ldk + ckz;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
     A[myc * SS + skew<T>(xx)] = load_or_zero(in, (long long)xx * plane + off, 0, myc < ncol);
@@ -454,8 +496,7 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   cx<T>* W1 = (S == A) ? B : A;
   T w3[PT], w2[PT], wk[PT][3];
   {
-    const long long col = col0 + myc;
-    const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
+    const int y = cy, kz = ckz;
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
@@ -487,7 +528,7 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
 template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  xconv_fwd_body<T, L, TB>(blockIdx.x, g, in, out, tw);
+  xconv_fwd_body<T, L, TB>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -503,17 +544,17 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
-  const long long ncols = (long long)g.Ny * g.Nkz;
   const long long plane = (long long)g.Ny * g.ldk;
-  const long long col0 = (long long)bx * C;
-  const int ncol = (int)min((long long)C, ncols - col0);
   const int myc = threadIdx.x % C;
-  const long long off = myc < ncol ? col_offset(g, col0 + myc) : 0;
+  int cy, ckz;
+  bool cok;
+  const int ncol = x_tile_column<C>(g, bx, myc, cy, ckz, cok);  // cok == (myc < ncol)
+  (void)cok;
+  const long long off = (long long)cy * g.ldk + ckz;
   T w3[PT], w2[PT], wk[PT][3];
   cx<T> acc[PT];
   {
-    const long long col = col0 + myc;
-    const int y = (int)(col / g.Nkz), kz = (int)(col - (long long)y * g.Nkz);
+    const int y = cy, kz = ckz;
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
@@ -557,7 +598,7 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
 template <typename T, int L, int TB>
 __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  xconv_bwd_body<T, L, TB>(blockIdx.x, g, in, out, tw);
+  xconv_bwd_body<T, L, TB>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
 }
 
 // ---------------------------------------------------------------------------

@@ -89,7 +89,7 @@ struct deft_plan_s {
   void *twbuf = nullptr;
   deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
   deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
-  double *wtab = nullptr;  // FP64 weight table, 2 * Nx*Ny*Nkz doubles
+  double *wtab = nullptr;  // FP64 weight table, (w3, w2) per element of the padded half spectrum (2 ks doubles)
   double *tmp = nullptr;   // scratch for host-seam calls (grows on demand)
   size_t tmp_bytes = 0;
 
