@@ -52,6 +52,7 @@ def gpu_run(steps, warmup):
     f.Vext = W.c2_solute_vext(N, L, SOLUTE)
     tuning = f.autotune()
     print("autotuned pass variants:", tuning, file=sys.stderr, flush=True)
+    out_tuning = tuning
     net = NN.WarmStartNet.load()
     v0 = np.full((N, N, N), -KT * math.log(nb))
     gold = json.load(open(os.path.join(HERE, "tests", "golden", "oracle_c2.json")))
@@ -100,6 +101,7 @@ def gpu_run(steps, warmup):
     for _ in range(steps):
         one(True)
     out["profiled_elapsed"] = time.perf_counter() - t0
+    out["tuning"] = out_tuning
     out["prof"] = f.profile()
     f.set_profiling(False)
     return out, gold, precision
@@ -167,21 +169,29 @@ def c3_run(count=64, seed=12345, reps=2):
             "all_converged": all(b[2] == "converged" for b in bat)}
 
 
-def rocprof_instance(family):
-    """rocprof name of the kernel instance behind a profiler family (None when ambiguous)."""
+def rocprof_instance(family, tuning):
+    """rocprof name of the kernel instance behind a profiler family at the autotuned
+    variants (`tuning` as returned by autotune()); None when there is no single one."""
     base, _, prec = family.rpartition("_")
     T = {"f32": "float", "f64": "double"}.get(prec)
-    names = {"zfwd_density": f"deft::k_zfwd_density<{T}, {N}>", "xconv_fwd": f"deft::k_xconv_fwd<{T}, {N}>",
-             "xconv_bwd": f"deft::k_xconv_bwd<{T}, {N}>", "zphi_energy": f"deft::k_zphi<{T}, {N}, false>",
-             "zphi_grad": f"deft::k_zphi<{T}, {N}, true>", "zgrad": f"deft::k_zgrad<{T}, {N}>"}
-    return names.get(base) if T else None
+    if not T:
+        return None
+    t = tuning[0 if prec == "f32" else 1]
+    zb, ty, tx = t["zblock"], t["ytile"], t["xtile"]
+    names = {"zfwd_density": f"deft::k_zfwd_density<{T}, {N}, {zb}>",
+             "ypass1": f"deft::k_ypass<{T}, {N}, false, 1, {ty}>",
+             "ypass5": f"deft::k_ypass<{T}, {N}, true, 5, {ty}>",
+             "xconv_fwd": f"deft::k_xconv_fwd<{T}, {N}, {tx}>", "xconv_bwd": f"deft::k_xconv_bwd<{T}, {N}, {tx}>",
+             "zphi_energy": f"deft::k_zphi<{T}, {N}, false, {zb}>", "zphi_grad": f"deft::k_zphi<{T}, {N}, true, {zb}>",
+             "zgrad": f"deft::k_zgrad<{T}, {N}, {zb}>"}
+    return names.get(base)
 
 
-def committed_traffic(family):
+def committed_traffic(family, tuning):
     """HBM bytes per dispatch of that kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (profiles/c2_128_solute_pmc.json), or None."""
     path = os.path.join(HERE, "profiles", "c2_128_solute_pmc.json")
-    inst = rocprof_instance(family)
+    inst = rocprof_instance(family, tuning)
     if not inst or not os.path.exists(path):
         return None
     t = json.load(open(path))["per_dispatch_hbm_bytes"].get(inst)
@@ -277,7 +287,7 @@ def main():
                    "c3_batch": c3,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname, runs["tuning"]),
                      "traffic_source": "profiles/c2_128_solute_pmc.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
                                        "copy-calibrated, bytes per dispatch)",
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
