@@ -54,7 +54,7 @@ def gpu_run(steps, warmup):
     print("autotuned pass variants:", tuning, file=sys.stderr, flush=True)
     out_tuning = tuning
     net = NN.WarmStartNet.load()
-    v0 = np.full((N, N, N), -KT * math.log(nb))
+    v0 = -KT * math.log(nb)
     gold = json.load(open(os.path.join(HERE, "tests", "golden", "oracle_c2.json")))
     precision = abs(D.pressure(nb, R, KT) * L ** 3) * PRECISION_FACTOR
 
@@ -64,7 +64,7 @@ def gpu_run(steps, warmup):
         if warm:
             net.warm_start(f, ETA)
         else:
-            f.Veff = v0
+            f.fill_Veff(v0)
         t1 = time.perf_counter()
         m = D.Minimize(f)
         m.precondition(True)
@@ -118,13 +118,12 @@ def c3_run(count=64, seed=12345, reps=2):
         f = D.WhiteBearFluidVeff(shape, np.diag(cell), R=R, kT=KT, mu=D.find_chemical_potential(nb, R, KT),
                                  fft_precision=D.FFT_FP32)
         f.Vext = V
-        f.Veff = np.full(shape, -KT * math.log(nb))
-        fluids.append((f, f.Veff.copy(), abs(D.pressure(nb, R, KT) * np.prod(cell)) * PRECISION_FACTOR))
+        fluids.append((f, -KT * math.log(nb), abs(D.pressure(nb, R, KT) * np.prod(cell)) * PRECISION_FACTOR))
 
     def minimizers():
         ms = []
         for f, v0, prec in fluids:
-            f.Veff = v0
+            f.fill_Veff(v0)
             m = D.Minimize(f)
             m.precondition(True)
             m.set_precision(prec)

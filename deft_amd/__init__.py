@@ -82,6 +82,7 @@ _SIGNATURES = {
     "deft_plan_set_vext": ([_P, _DP], _I),
     "deft_plan_set_veff": ([_P, _DP], _I),
     "deft_plan_get_veff": ([_P, _DP], _I),
+    "deft_plan_fill_veff": ([_P, _D], _I),
     "deft_plan_get_density": ([_P, _DP], _I),
     "deft_plan_veff_device_ptr": ([_P, ctypes.POINTER(_DP)], _I),
     "deft_plan_sizes": ([_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)], _I),
@@ -302,6 +303,10 @@ class WhiteBearFluidVeff:
     def Veff(self, v):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), self.shape))
         _check(lib().deft_plan_set_veff(self._h, _dptr(a)))
+
+    def fill_Veff(self, value):
+        """Veff = value everywhere, written on the device."""
+        _check(lib().deft_plan_fill_veff(self._h, float(value)))
 
     def density(self):
         out = np.empty(self.shape)

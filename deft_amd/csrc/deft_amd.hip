@@ -812,6 +812,15 @@ int deft_plan_set_veff(deft_plan p, const double *veff) {
   return DEFT_OK;
 }
 
+int deft_plan_fill_veff(deft_plan p, double value) {
+  TRY(check_plan(p));
+  p->npend = 0;
+  hipLaunchKernelGGL(k_fill, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->x, value);
+  DEFT_HIP_CHECK(hipGetLastError());
+  TRY(plan_sync(p));
+  return DEFT_OK;
+}
+
 int deft_plan_get_veff(deft_plan p, double *veff) {
   TRY(check_plan(p));
   if (!veff) return set_error(DEFT_ERR_ARG, "null veff");

@@ -780,6 +780,11 @@ static __global__ __launch_bounds__(kThreads) void k_cg_reset(long long N, const
 }
 
 // x = x + s*d, rounded like the host expression x += s*d
+static __global__ __launch_bounds__(kThreads) void k_fill(long long N, double* __restrict__ x, double v) {
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)
+    x[i] = v;
+}
+
 static __global__ __launch_bounds__(kThreads) void k_axpy(long long N, double* __restrict__ x, const double* __restrict__ d,
                                                    double s) {
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)

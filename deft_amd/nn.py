@@ -291,7 +291,7 @@ def solve(fluid, vext, eta, R=1.0, kT=1.0, relative_precision=1e-10, warm=None):
     fluid.Vext = vext
     fluid.fft_precision = FFT_FP32
     if warm is None:
-        fluid.Veff = np.full(fluid.shape, -kT * math.log(nb))
+        fluid.fill_Veff(-kT * math.log(nb))
     else:
         warm.warm_start(fluid, eta)
     m = Minimize(fluid)

@@ -120,6 +120,10 @@ int deft_plan_set_mu(deft_plan plan, double mu);
 /* Vext (N doubles, host); stored as FP32 on the device (+inf kept). */
 int deft_plan_set_vext(deft_plan plan, const double *vext);
 int deft_plan_set_veff(deft_plan plan, const double *veff);
+/* Veff = value at every point, on the device (the uniform start -kT ln n_b of
+ * deft's drivers, e.g. papers/contact/figs/inner-sphere.cpp:197-198, without
+ * a host upload). */
+int deft_plan_fill_veff(deft_plan plan, double value);
 int deft_plan_get_veff(deft_plan plan, double *veff);
 /* n = exp(-Veff/kT), 0 on hard-wall points */
 int deft_plan_get_density(deft_plan plan, double *n);
