@@ -126,7 +126,7 @@ int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
-  TRY(plan_sync(p));
+  TRY(plan_wait(p));
   *out = p->hscal[slot];
   return DEFT_OK;
 }
@@ -150,6 +150,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->tmp);
   (void)hipFree(p->wtab);
   if (p->hscal) (void)hipHostFree(p->hscal);
+  if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -634,6 +635,11 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
       hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
+  if (hipHostMalloc((void **)&p->hflag, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&p->dflag, p->hflag, 0) != hipSuccess)
+    return fail(DEFT_ERR_ALLOC, "pinned allocation failed");
+  *p->hflag = 0;
   // reduced sums land directly in host-mapped pinned memory: no copy per evaluation
   if (hipHostMalloc(&p->hscal, sizeof(double) * NUM_SLOTS, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "pinned allocation failed");

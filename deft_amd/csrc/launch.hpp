@@ -211,7 +211,7 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
                        nzb, p->dscal);
   }
   DEFT_HIP_CHECK(hipGetLastError());
-  return plan_sync(p);
+  return plan_wait(p);
 }
 
 // --- batched evaluation ----------------------------------------------------

@@ -126,10 +126,19 @@ __global__ __launch_bounds__(kNT) void k_conv(const float *__restrict__ in, int 
 // fragment reads of 8 consecutive lanes fall in distinct banks); wave w computes
 // x = x0 + w, its four y rows as four 16-voxel MFMA row tiles that share each
 // weight fragment.
-template <int NT, int CIN>
+// Input modes of the tiled convolution: kPlain reads `in` on the output grid;
+// kUp reads `in` on the grid of half the edge at (x/2, y/2, z/2) (nearest
+// upsampling folded into the halo load); kSum reads in + in2 (a skip sum
+// folded into the halo load).  With HEAD, the epilogue applies the 1x1 head,
+// u[v] = hb + sum_c hw[c] relu(acc[v][c] + bias[c]), and writes u only.
+constexpr int kPlain = 0, kUp = 1, kSum = 2;
+
+template <int NT, int CIN, int MODE = kPlain, bool HEAD = false>
 __global__ __launch_bounds__(kNT) void k_conv_tile(const float *__restrict__ in, const __bf16 *__restrict__ W, int kp,
                                                    const float *__restrict__ bias, float *__restrict__ out, int M,
-                                                   int relu) {
+                                                   int relu, const float *__restrict__ in2 = nullptr,
+                                                   const float *__restrict__ hw = nullptr,
+                                                   const float *__restrict__ hb = nullptr) {
   constexpr int CS = CIN + 8;
   constexpr int HY = 6, HZ = 18, HV = 6 * HY * HZ;
   constexpr int CH8 = CIN / 8;
@@ -150,9 +159,23 @@ __global__ __launch_bounds__(kNT) void k_conv_tile(const float *__restrict__ in,
     gx += (gx < 0) ? M : (gx >= M ? -M : 0);
     gy += (gy < 0) ? M : (gy >= M ? -M : 0);
     gz += (gz < 0) ? M : (gz >= M ? -M : 0);
-    const float *src = in + (((s * M + gx) * M + gy) * M + gz) * CIN + c8 * 8;
-    const float4 p = *reinterpret_cast<const float4 *>(src);
-    const float4 q = *reinterpret_cast<const float4 *>(src + 4);
+    long long vin;
+    if constexpr (MODE == kUp) {
+      const int Mh = M >> 1;
+      vin = ((s * Mh + (gx >> 1)) * Mh + (gy >> 1)) * Mh + (gz >> 1);
+    } else {
+      vin = ((s * M + gx) * M + gy) * M + gz;
+    }
+    const float *src = in + vin * CIN + c8 * 8;
+    float4 p = *reinterpret_cast<const float4 *>(src);
+    float4 q = *reinterpret_cast<const float4 *>(src + 4);
+    if constexpr (MODE == kSum) {
+      const float *src2 = in2 + vin * CIN + c8 * 8;
+      const float4 p2 = *reinterpret_cast<const float4 *>(src2);
+      const float4 q2 = *reinterpret_cast<const float4 *>(src2 + 4);
+      p = float4{p.x + p2.x, p.y + p2.y, p.z + p2.z, p.w + p2.w};
+      q = float4{q.x + q2.x, q.y + q2.y, q.z + q2.z, q.w + q2.w};
+    }
     *reinterpret_cast<bf16x8 *>(&tile[hv * CS + c8 * 8]) =
         bf16x8{(__bf16)p.x, (__bf16)p.y, (__bf16)p.z, (__bf16)p.w, (__bf16)q.x, (__bf16)q.y, (__bf16)q.z, (__bf16)q.w};
   }
@@ -183,6 +206,27 @@ __global__ __launch_bounds__(kNT) void k_conv_tile(const float *__restrict__ in,
     }
   }
   constexpr int COUT = NT * 16;
+  if constexpr (HEAD) {
+    // u of the 4x4 voxels of each row tile: per lane the sum over its NT
+    // channels, then over the 16 lanes of a row group (xor shuffles)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const long long row = ((s * M + x0 + w) * M + y0 + r) * M + z0 + (lane >> 4) * 4;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float t = 0.f;
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+          const int co = n * 16 + (lane & 15);
+          t += hw[co] * fmaxf(acc[r][n][j] + bias[co], 0.f);
+        }
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((lane & 15) == 0) out[row + j] = t + hb[0];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
     const int co = n * 16 + (lane & 15);
@@ -584,7 +628,8 @@ struct deft_net_s {
   std::vector<float *> bufs;
   float *loss_acc = nullptr;   // host-mapped
   unsigned long long *count_dev = nullptr;
-  float *eta_dev = nullptr;
+  float *eta_host = nullptr;  // host-mapped pinned: the input kernel reads eta in place
+  float *eta_dev = nullptr;   // device address of eta_host
   float *ws = nullptr;  // warm-start staging: Boltzmann factor and prediction
   size_t ws_bytes = 0;
   double last_lambda = 1.0;  // blend factor of the last warm start
@@ -642,9 +687,38 @@ int ensure_buffers(deft_net_s *n, int M, int B) {
     const size_t bytes = sizeof(float) * (size_t)voxels(n, s.level) * s.ch;
     if (hipMalloc(&n->bufs[b], bytes) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "network buffer allocation failed");
   }
-  (void)hipFree(n->eta_dev);
-  if (hipMalloc(&n->eta_dev, sizeof(float) * B) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "allocation failed");
+  if (n->eta_host) (void)hipHostFree(n->eta_host);
+  n->eta_host = nullptr;
+  n->eta_dev = nullptr;
+  if (hipHostMalloc((void **)&n->eta_host, sizeof(float) * B, hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&n->eta_dev, n->eta_host, 0) != hipSuccess)
+    return set_error(DEFT_ERR_ALLOC, "pinned allocation failed");
   return DEFT_OK;
+}
+
+template <int CIN, int MODE, bool HEAD>
+void launch_conv_tile_fused(int nt, dim3 grid, hipStream_t st, const float *in, const float *in2, const __bf16 *W,
+                            int kp, const float *bias, float *out, int M, const float *hw, const float *hb) {
+  switch (nt) {
+    case 1: hipLaunchKernelGGL((k_conv_tile<1, CIN, MODE, HEAD>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, 1, in2, hw, hb); break;
+    case 2: hipLaunchKernelGGL((k_conv_tile<2, CIN, MODE, HEAD>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, 1, in2, hw, hb); break;
+    case 3: hipLaunchKernelGGL((k_conv_tile<3, CIN, MODE, HEAD>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, 1, in2, hw, hb); break;
+    default: hipLaunchKernelGGL((k_conv_tile<4, CIN, MODE, HEAD>), grid, dim3(kNT), 0, st, in, W, kp, bias, out, M, 1, in2, hw, hb); break;
+  }
+}
+
+// relu(conv(...)) on a level whose edge is a multiple of 16, input per MODE
+template <int MODE, bool HEAD>
+void launch_fused(hipStream_t st, int M, long long V, int cin, int cout, const float *in, const float *in2,
+                  const __bf16 *W, int kp, const float *bias, float *out, const float *hw, const float *hb) {
+  const dim3 grid((unsigned)(V / 256));
+  switch (cin) {
+    case 16: launch_conv_tile_fused<16, MODE, HEAD>(cout / 16, grid, st, in, in2, W, kp, bias, out, M, hw, hb); break;
+    case 32: launch_conv_tile_fused<32, MODE, HEAD>(cout / 16, grid, st, in, in2, W, kp, bias, out, M, hw, hb); break;
+    case 48: launch_conv_tile_fused<48, MODE, HEAD>(cout / 16, grid, st, in, in2, W, kp, bias, out, M, hw, hb); break;
+    default: launch_conv_tile_fused<64, MODE, HEAD>(cout / 16, grid, st, in, in2, W, kp, bias, out, M, hw, hb); break;
+  }
 }
 
 template <int CIN>
@@ -748,6 +822,46 @@ void forward(deft_net_s *n) {
                      n->params + n->head_w, n->params + n->head_b, b[OUT], voxels(n, 0));
 }
 
+// Inference forward with the upsampling, skip sums and head folded into the
+// convolutions (every level's edge a multiple of 16): the same values as
+// forward() up to the summation order of the head; u lands in `u`.
+bool fused_ok(const deft_net_s *n) { return !n->generic_conv && n->M % 128 == 0; }
+
+void forward_fused(deft_net_s *n, float *u) {
+  float **b = n->bufs.data();
+  const int *C = n->C;
+  auto blocks = [&](int buf) {
+    const BufSpec s = spec_of(n, buf);
+    return blocks_for(voxels(n, s.level) * s.ch);
+  };
+  auto cv = [&](int i) -> const Conv & { return n->conv[i]; };
+  const float *P = n->params;
+  conv_forward(n, 0, b[X0], b[A0]);
+  conv_forward(n, 1, b[A0], b[S0]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P1)), dim3(kNT), 0, n->stream, b[S0], b[P1], C[0], n->M >> 1, voxels(n, 1));
+  conv_forward(n, 2, b[P1], b[A1]);
+  conv_forward(n, 3, b[A1], b[S1]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P2)), dim3(kNT), 0, n->stream, b[S1], b[P2], C[1], n->M >> 2, voxels(n, 2));
+  conv_forward(n, 4, b[P2], b[A2]);
+  conv_forward(n, 5, b[A2], b[S2]);
+  hipLaunchKernelGGL(k_pool, dim3(blocks(P3)), dim3(kNT), 0, n->stream, b[S2], b[P3], C[2], n->M >> 3, voxels(n, 3));
+  conv_forward(n, 6, b[P3], b[A3]);
+  conv_forward(n, 7, b[A3], b[B3]);
+  // decoder: R = relu(conv(up(coarse))), D = relu(conv(R + S)), u = head(D0)
+  launch_fused<kUp, false>(n->stream, n->M >> 2, voxels(n, 2), C[3], C[2], b[B3], nullptr, cv(8).wb, cv(8).kp,
+                           P + cv(8).boff, b[R2], nullptr, nullptr);
+  launch_fused<kSum, false>(n->stream, n->M >> 2, voxels(n, 2), C[2], C[2], b[R2], b[S2], cv(9).wb, cv(9).kp,
+                            P + cv(9).boff, b[D2], nullptr, nullptr);
+  launch_fused<kUp, false>(n->stream, n->M >> 1, voxels(n, 1), C[2], C[1], b[D2], nullptr, cv(10).wb, cv(10).kp,
+                           P + cv(10).boff, b[R1], nullptr, nullptr);
+  launch_fused<kSum, false>(n->stream, n->M >> 1, voxels(n, 1), C[1], C[1], b[R1], b[S1], cv(11).wb, cv(11).kp,
+                            P + cv(11).boff, b[D1], nullptr, nullptr);
+  launch_fused<kUp, false>(n->stream, n->M, voxels(n, 0), C[1], C[0], b[D1], nullptr, cv(12).wb, cv(12).kp,
+                           P + cv(12).boff, b[R0], nullptr, nullptr);
+  launch_fused<kSum, true>(n->stream, n->M, voxels(n, 0), C[0], C[0], b[R0], b[S0], cv(13).wb, cv(13).kp,
+                           P + cv(13).boff, u, P + n->head_w, P + n->head_b);
+}
+
 // backward from G_OUT (gradient of the loss with respect to u) into n->grads
 void backward(deft_net_s *n) {
   float **b = n->bufs.data();
@@ -810,7 +924,9 @@ int check_shape(int M, int B) {
 
 int prepare(deft_net_s *n, int M, int B, const float *boltz_dev, const float *eta_host) {
   NN_TRY(ensure_buffers(n, M, B));
-  NN_HIP(hipMemcpyAsync(n->eta_dev, eta_host, sizeof(float) * B, hipMemcpyHostToDevice, n->stream));
+  // every entry point synchronises its stream before returning, so no kernel
+  // still reads the previous values
+  for (int b = 0; b < B; ++b) n->eta_host[b] = eta_host[b];
   const long long M3 = (long long)M * M * M;
   hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(M3 * B)), dim3(kNT), 0, n->stream, boltz_dev, n->eta_dev,
                      n->bufs[X0], M3, M3 * B);
@@ -917,7 +1033,7 @@ int deft_net_destroy(deft_net n) {
   (void)hipFree(n->adam_m);
   (void)hipFree(n->adam_v);
   (void)hipFree(n->count_dev);
-  (void)hipFree(n->eta_dev);
+  if (n->eta_host) (void)hipHostFree(n->eta_host);
   (void)hipFree(n->ws);
   if (n->loss_acc) (void)hipHostFree(n->loss_acc);
   if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -978,8 +1094,12 @@ int deft_net_forward(deft_net n, int M, int batch, const float *boltz_dev, const
   NN_TRY(check_shape(M, batch));
   if (!boltz_dev || !eta || !u_dev) return set_error(DEFT_ERR_ARG, "null buffer");
   NN_TRY(prepare(n, M, batch, boltz_dev, eta));
-  forward(n);
-  NN_HIP(hipMemcpyAsync(u_dev, n->bufs[OUT], sizeof(float) * voxels(n, 0), hipMemcpyDeviceToDevice, n->stream));
+  if (fused_ok(n)) {
+    forward_fused(n, u_dev);
+  } else {
+    forward(n);
+    NN_HIP(hipMemcpyAsync(u_dev, n->bufs[OUT], sizeof(float) * voxels(n, 0), hipMemcpyDeviceToDevice, n->stream));
+  }
   NN_HIP(hipGetLastError());
   NN_HIP(hipStreamSynchronize(n->stream));
   return DEFT_OK;

@@ -834,6 +834,16 @@ __device__ __forceinline__ void reduce_body(int slot, const double* __restrict__
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) t += red[w];
     out[slot] = t;
+    __threadfence_system();  // `out` may be host-mapped: visible before any later flag
+  }
+}
+
+// Raises the plan's completion flag (host-mapped) to seq once all earlier work
+// of the stream is done; the host waits on it instead of synchronising.
+static __global__ void k_signal(unsigned long long* __restrict__ flag, unsigned long long seq) {
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

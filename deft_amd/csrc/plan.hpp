@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -84,6 +85,9 @@ struct deft_plan_s {
   void *spec = nullptr;    // 6 half spectra of Nk complex (sized for FP64)
   double *partial = nullptr;
   int pstride = 0;         // partial-sum slot stride (>= blocks of any reducing kernel)
+  unsigned long long *hflag = nullptr;  // completion flag, host-mapped pinned
+  unsigned long long *dflag = nullptr;  // its device address
+  unsigned long long flag_seq = 0;
   double *hscal = nullptr;  // NUM_SLOTS reduced sums in host-mapped pinned memory
   double *dscal = nullptr;  // device address of hscal (written by k_reduce)
   void *twbuf = nullptr;
@@ -233,6 +237,30 @@ struct BatchStage {
   size_t cap = 0;
   size_t used = 0;
 };
+
+// Waits until the work queued so far on the plan's stream has finished: a
+// one-thread kernel raises a host-mapped flag and the host spins on it (a
+// stream synchronisation leaves the GPU idle for ~30 us per wait).  After
+// kSpinLimitUs without the flag the host falls back to a blocking
+// synchronisation, which also reports any error; with profiling on, or in a
+// batch, the blocking path is used throughout.
+constexpr long long kSpinLimitUs = 20000;
+inline int plan_wait(deft_plan_s *p) {
+  if (p->prof || p->batch || !p->hflag) return plan_sync(p);
+  const unsigned long long want = ++p->flag_seq;
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, p->stream, p->dflag, want);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_signal: ") + hipGetErrorString(e));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (long long n = 0;; ++n) {
+    if (__atomic_load_n(p->hflag, __ATOMIC_ACQUIRE) >= want) return DEFT_OK;
+    if ((n & 1023) == 1023 &&
+        std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count() >
+            kSpinLimitUs)
+      return plan_sync(p);
+    __builtin_ia32_pause();
+  }
+}
 
 // Kernel launch sequences, instantiated for T = float and T = double in
 // separate translation units.

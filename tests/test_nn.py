@@ -285,6 +285,25 @@ def test_tiled_and_untiled_convolutions_agree(gpu):
 
 
 @pytest.mark.gpu
+def test_fused_inference_matches_layer_by_layer_forward(gpu):
+    """At 128^3 (every level a multiple of 16) inference folds the upsampling, the skip
+    sums and the head into the convolutions; the untiled path runs layer by layer.
+    Only the head's summation order differs."""
+    net = deft_amd.nn.WarmStartNet.load() if os.path.exists(deft_amd.nn.WEIGHTS_PATH) else \
+        deft_amd.nn.WarmStartNet()
+    rng = np.random.default_rng(31)
+    M = 128
+    V = deft_amd.nn.random_vext(M, 12.8, rng)
+    ok = np.isfinite(V)
+    b = _dev(np.where(ok, np.exp(-np.where(ok, V, 0)), 0)[None])
+    u_fused = net.forward(b, [0.35]).numpy()
+    net.set_conv_kernel(False)
+    u_plain = net.forward(b, [0.35]).numpy()
+    assert np.max(np.abs(u_fused - u_plain)) <= 1e-5 * max(1.0, np.max(np.abs(u_plain)))
+    net.close()
+
+
+@pytest.mark.gpu
 def test_loss_without_mask_uses_boltz(gpu):
     net = deft_amd.nn.WarmStartNet(SMALL, seed=2)
     boltz, eta, target, mask = random_inputs(2, 16, 13)
