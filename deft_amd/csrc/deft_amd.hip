@@ -38,6 +38,7 @@ namespace {
 constexpr int kVersion = 1;
 constexpr int kConfirmIterations = 3;
 constexpr int kMaxPhaseHalvings = 12;
+constexpr int kMaxCollapseRestarts = 3;
 
 bool det3(const double *c, double &det) {
   det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
@@ -190,6 +191,7 @@ struct deft_minimize_s {
   int fuse_ls = -1;
   bool s2_accepted = true;  // the last quadratic step kept its trial point s2
   double good_step = 0.0;   // step of the last iteration that clearly lowered the energy
+  int collapse_restarts = 0;  // final-phase restarts after a collapsed last step
 };
 
 // ===========================================================================
@@ -457,6 +459,22 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
 // Phase 1 is skipped when the polishing gradient precision is FP64.
 int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   TRY(improve_energy(m, verbose, more));
+  // The reference's error estimate trusts one iteration's energy change, so a
+  // line search that collapsed to a vanishing step reads as convergence; in
+  // the final phase of the ladder such a verdict restarts the CG direction
+  // from the last good step instead (at most kMaxCollapseRestarts times).
+  if (!*more && m->switch_rp > 0 && m->phase >= 2 && m->stop == DEFT_STOP_CONVERGED && m->good_step > 0 &&
+      std::fabs(m->step) < 1e-3 * std::fabs(m->good_step) && m->collapse_restarts < kMaxCollapseRestarts) {
+    m->collapse_restarts++;
+    m->have_dir = false;
+    m->have_old = false;
+    m->oldgradsqr = 1.0;
+    m->step = m->good_step;
+    m->stop = DEFT_STOP_NONE;
+    if (verbose) printf("collapsed step: restarting the direction\n");
+    *more = 1;
+    return DEFT_OK;
+  }
   if (*more || m->switch_rp <= 0 || m->phase >= 2) return DEFT_OK;
   const int s = m->stop;
   if (s != DEFT_STOP_CONVERGED && s != DEFT_STOP_NEGATIVE_CURVATURE && s != DEFT_STOP_ENERGY_ROSE &&
