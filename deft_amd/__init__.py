@@ -34,6 +34,9 @@ HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "de
 
 FFT_FP32 = 0
 FFT_FP64 = 1
+WHITE_BEAR = 0
+WHITE_BEAR_MARK_II = 1
+WHITE_BEAR_MARK_II_SLOW = 2
 
 STOP_REASONS = {
     0: "none", 1: "maxiter", 2: "nan-energy", 3: "infinite-energy", 4: "nan-slope",
@@ -79,6 +82,8 @@ _SIGNATURES = {
     "deft_plan_set_tuning": ([_P, _I, _I, _I, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
     "deft_plan_set_mu": ([_P, _D], _I),
+    "deft_plan_set_functional": ([_P, _I], _I),
+    "deft_bulk_state": ([_I, _D, _D, _D, _DP, _DP, _DP], _I),
     "deft_plan_set_vext": ([_P, _DP], _I),
     "deft_plan_set_veff": ([_P, _DP], _I),
     "deft_plan_get_veff": ([_P, _DP], _I),
@@ -184,15 +189,26 @@ def supported_length(n):
 # bulk path
 # ---------------------------------------------------------------------------
 
-def find_chemical_potential(n, R=1.0, kT=1.0):
+def bulk_state(n, R=1.0, kT=1.0, functional=WHITE_BEAR):
+    """(f, mu, p) of ideal gas + the hard-sphere functional at bulk density n."""
+    f, mu, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib().deft_bulk_state(int(functional), n, R, kT, ctypes.byref(f), ctypes.byref(mu), ctypes.byref(p)))
+    return f.value, mu.value, p.value
+
+
+def find_chemical_potential(n, R=1.0, kT=1.0, functional=WHITE_BEAR):
     """mu of ideal gas + White-Bear at bulk density n (src/equation-of-state.cpp:189-192)."""
+    if functional != WHITE_BEAR:
+        return bulk_state(n, R, kT, functional)[1]
     out = ctypes.c_double()
     _check(lib().deft_bulk_chemical_potential(n, R, kT, ctypes.byref(out)))
     return out.value
 
 
-def pressure(n, R=1.0, kT=1.0):
+def pressure(n, R=1.0, kT=1.0, functional=WHITE_BEAR):
     """Bulk pressure (src/equation-of-state.cpp:164-170)."""
+    if functional != WHITE_BEAR:
+        return bulk_state(n, R, kT, functional)[2]
     out = ctypes.c_double()
     _check(lib().deft_bulk_pressure(n, R, kT, ctypes.byref(out)))
     return out.value
@@ -216,7 +232,7 @@ class WhiteBearFluidVeff:
     FP64 on the host, stored FP32 on the device; +inf marks hard walls.
     """
 
-    def __init__(self, shape, cell, R=1.0, kT=1.0, mu=0.0, fft_precision=FFT_FP32):
+    def __init__(self, shape, cell, R=1.0, kT=1.0, mu=0.0, fft_precision=FFT_FP32, functional=WHITE_BEAR):
         self.shape = tuple(int(s) for s in shape)
         cell = np.asarray(cell, dtype=np.float64)
         if cell.shape == (3,):
@@ -228,6 +244,9 @@ class WhiteBearFluidVeff:
         _check(lib().deft_plan_create(self.shape[0], self.shape[1], self.shape[2], _dptr(self.cell),
                                       self._R, self._kT, self._mu, self._prec, ctypes.byref(h)))
         self._h = h
+        if functional != WHITE_BEAR:
+            _check(lib().deft_plan_set_functional(self._h, int(functional)))
+        self._functional = int(functional)
         self.N = int(np.prod(self.shape))
         self.kshape = (self.shape[0], self.shape[1], self.shape[2] // 2 + 1)
         self.volume = abs(float(np.linalg.det(self.cell)))
@@ -303,6 +322,17 @@ class WhiteBearFluidVeff:
     def Veff(self, v):
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), self.shape))
         _check(lib().deft_plan_set_veff(self._h, _dptr(a)))
+
+    @property
+    def functional(self):
+        """WHITE_BEAR (HardSpheresWBnotensor), WHITE_BEAR_MARK_II (WhiteBearMarkII) or
+        WHITE_BEAR_MARK_II_SLOW (HardSpheresWBm2slow: Phi3 over 36 pi, not 24 pi)."""
+        return self._functional
+
+    @functional.setter
+    def functional(self, v):
+        _check(lib().deft_plan_set_functional(self._h, int(v)))
+        self._functional = int(v)
 
     def fill_Veff(self, value):
         """Veff = value everywhere, written on the device."""

@@ -439,6 +439,7 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   }
   const double rp = (m->switch_rp > 0 && m->phase == 0) ? m->switch_rp : m->relative_precision;
   if (!(m->error_estimate < m->precision) && !(m->error_estimate < rp * std::fabs(newE))) {
+    m->stop = DEFT_STOP_NONE;
     *more = 1;
     return DEFT_OK;
   }
@@ -458,7 +459,20 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
 // minimization, so the final answer is always an FP64-converged one.
 // Phase 1 is skipped when the polishing gradient precision is FP64.
 int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
+  // CONVERGED here means the last iteration met the precision and the final
+  // phase is confirming it
+  const bool met = m->switch_rp > 0 && m->phase >= 2 && m->stop == DEFT_STOP_CONVERGED;
   TRY(improve_energy(m, verbose, more));
+  // A confirming iteration whose line search runs into the round-off of the
+  // FP64 energies (no lower energy found, the point is left unchanged) keeps
+  // the verdict of the iteration before it, as the reference would have
+  // stopped there.
+  if (met && !*more &&
+      (m->stop == DEFT_STOP_ENERGY_ROSE || m->stop == DEFT_STOP_NO_CHANGE ||
+       m->stop == DEFT_STOP_NEGATIVE_CURVATURE)) {
+    m->stop = DEFT_STOP_CONVERGED;
+    return DEFT_OK;
+  }
   // The reference's error estimate trusts one iteration's energy change, so a
   // line search that collapsed to a vanishing step reads as convergence; in
   // the final phase of the ladder such a verdict restarts the CG direction
@@ -593,7 +607,7 @@ const char *deft_last_error(void) { return g_last_error.c_str(); }
 
 int deft_version(void) { return kVersion; }
 
-int deft_supported_length(int n) { return (n > 0 && n % 2 == 0 && length_ok(n)) ? 1 : 0; }
+int deft_supported_length(int n) { return (n > 0 && length_ok(n)) ? 1 : 0; }
 
 int deft_device_count(int *count) {
   if (!count) return set_error(DEFT_ERR_ARG, "null count");
@@ -608,7 +622,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   if (!out || !cell) return set_error(DEFT_ERR_ARG, "null argument");
   *out = nullptr;
   if (!deft_supported_length(nx) || !deft_supported_length(ny) || !deft_supported_length(nz))
-    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must be even and in {16,20,32,48,64,96,128,256}");
+    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must be in {16,20,21,32,48,64,96,128,201,256}");
   double det;
   if (!det3(cell, det)) return set_error(DEFT_ERR_ARG, "cell matrix is singular or not finite");
   if (!(R > 0) || !(kT > 0) || !std::isfinite(R) || !std::isfinite(kT) || !std::isfinite(mu))
@@ -810,6 +824,15 @@ int deft_plan_set_fft_precision(deft_plan p, int prec) {
   return DEFT_OK;
 }
 
+int deft_plan_set_functional(deft_plan p, int functional) {
+  TRY(check_plan(p));
+  if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW)
+    return set_error(DEFT_ERR_ARG, "unknown functional");
+  p->functional = functional;
+  return DEFT_OK;
+}
+
 int deft_plan_set_mu(deft_plan p, double mu) {
   TRY(check_plan(p));
   if (!std::isfinite(mu)) return set_error(DEFT_ERR_ARG, "mu must be finite");
@@ -987,8 +1010,8 @@ int deft_ifft(deft_plan p, const double *in, double *out) {
 }
 
 // --- bulk path (src/equation-of-state.cpp:164-192) ---------------------------
-static void bulk_terms(double n, double R, double kT, double *f_ex, double *df_ex) {
-  const WBConst c = make_wb_const(R);
+static void bulk_terms(double n, double R, double kT, double *f_ex, double *df_ex, int functional = 0) {
+  const WBConst c = make_wb_const(R, functional);
   const double n3 = 4 * M_PI / 3 * R * R * R * n, n2 = 4 * M_PI * R * R * n;
   double p1, p2, p3, d[5];
   wb_phi_partials(n3, n2, 0.0, 0.0, 0.0, c, p1, p2, p3, d);
@@ -1009,6 +1032,20 @@ int deft_bulk_free_energy_density(double n, double R, double kT, double *out) {
   double f, df;
   bulk_terms(n, R, kT, &f, &df);
   *out = kT * n * (std::log(n / water_nQ(kT)) - 1) + f;
+  return DEFT_OK;
+}
+
+int deft_bulk_state(int functional, double n, double R, double kT, double *f, double *mu, double *pr) {
+  if (!f || !mu || !pr || !(n > 0) || !(R > 0) || !(kT > 0)) return set_error(DEFT_ERR_ARG, "bad bulk arguments");
+  if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW)
+    return set_error(DEFT_ERR_ARG, "unknown functional");
+  double fe, dfe;
+  bulk_terms(n, R, kT, &fe, &dfe, functional);
+  const double lnn = std::log(n / water_nQ(kT));
+  *f = kT * n * (lnn - 1) + fe;
+  *mu = kT * lnn + dfe;
+  *pr = n * *mu - *f;
   return DEFT_OK;
 }
 

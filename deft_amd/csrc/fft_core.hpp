@@ -162,8 +162,24 @@ constexpr int lds_stride() {
 // Stockham stages
 // ---------------------------------------------------------------------------
 
+// Radices 2..8 have register butterflies; any other prime factor p of L is one
+// direct-DFT stage of radix p (O(p) per output, for odd grid edges such as
+// 201 = 3 * 67 that the reference's FFTW plans take in stride).
+constexpr int kMaxDirectRadix = 512;
+
+constexpr int smallest_factor(int n) {
+  for (int f = 2; f * f <= n; ++f)
+    if (n % f == 0) return f;
+  return n;
+}
+
 constexpr int pick_radix(int rem) {
-  return (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
+  return (rem % 8 == 0)   ? 8
+         : (rem % 4 == 0) ? 4
+         : (rem % 2 == 0) ? 2
+         : (rem % 3 == 0) ? 3
+         : (rem % 5 == 0) ? 5
+                          : smallest_factor(rem);
 }
 
 template <int L, int NS = 1>
@@ -176,11 +192,13 @@ constexpr int num_stages() {
 }
 
 constexpr bool supported_length(int L) {
-  int r = L;
-  while (r % 2 == 0) r /= 2;
-  while (r % 3 == 0) r /= 3;
-  while (r % 5 == 0) r /= 5;
-  return r == 1 && L >= 2;
+  if (L < 2) return false;
+  for (int r = L; r > 1;) {
+    const int f = pick_radix(r);
+    if (f > kMaxDirectRadix) return false;
+    r /= f;
+  }
+  return true;
 }
 
 // One radix-R stage: sub-transforms of length NS are combined into length NS*R.
@@ -213,11 +231,48 @@ __device__ __forceinline__ void stockham_stage(const cx<T>* __restrict__ src, cx
   }
 }
 
+// Radix-R stage for a prime R without a register butterfly: one output per
+// thread iteration, out[q] = sum_r w_R^(r q) * twiddled in[r] with w_R = tw[L/R]
+// (exponents reduced mod R), accumulated in T.  Same data movement as
+// stockham_stage.
+template <typename T, int L, int SS, int R, int NS, bool INV>
+__device__ __forceinline__ void stockham_stage_direct(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst,
+                                                      int nseq, const cx<T>* __restrict__ tw) {
+  constexpr int M = L / R;
+  constexpr int TWS = L / (NS * R);
+  const int total = nseq * L;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int s = e / L;
+    const int rem = e - s * L;
+    const int q = rem / M;
+    const int j = rem - q * M;
+    const int k = j % NS;
+    const cx<T>* in = src + s * SS;
+    cx<T> acc = in[skew<T>(j)];
+    int wq = q;  // (r q) mod R
+    for (int r = 1; r < R; ++r) {
+      cx<T> v = in[skew<T>(j + r * M)];
+      if constexpr (NS > 1) {
+        const cx<T> w = tw[k * r * TWS];
+        v = INV ? cmulc(v, w) : cmul(v, w);
+      }
+      const cx<T> w = tw[wq * M];
+      acc = acc + (INV ? cmulc(v, w) : cmul(v, w));
+      wq += q;
+      if (wq >= R) wq -= R;
+    }
+    dst[s * SS + skew<T>((j / NS) * NS * R + k + q * NS)] = acc;
+  }
+}
+
 template <typename T, int L, int SS, int NS, bool INV>
 __device__ __forceinline__ void stockham_run(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw) {
   if constexpr (NS < L) {
     constexpr int R = pick_radix(L / NS);
-    stockham_stage<T, L, SS, R, NS, INV>(a, b, nseq, tw);
+    if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
+      stockham_stage<T, L, SS, R, NS, INV>(a, b, nseq, tw);
+    else
+      stockham_stage_direct<T, L, SS, R, NS, INV>(a, b, nseq, tw);
     __syncthreads();
     stockham_run<T, L, SS, NS * R, INV>(b, a, nseq, tw);
   }
@@ -228,7 +283,7 @@ __device__ __forceinline__ void stockham_run(cx<T>* a, cx<T>* b, int nseq, const
 // must call it; the data in `a` must be visible (a __syncthreads() before).
 template <typename T, int L, int SS, bool INV>
 __device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw) {
-  static_assert(supported_length(L), "FFT length must be 2^a 3^b 5^c");
+  static_assert(supported_length(L), "FFT length has a prime factor above kMaxDirectRadix");
   stockham_run<T, L, SS, 1, INV>(a, b, nseq, tw);
   return (num_stages<L>() % 2 == 0) ? a : b;
 }

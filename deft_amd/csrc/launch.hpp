@@ -34,11 +34,13 @@ inline bool with_length(int L, Fn &&fn) {
     return true;
     DEFT_CASE(16)
     DEFT_CASE(20)
+    DEFT_CASE(21)
     DEFT_CASE(32)
     DEFT_CASE(48)
     DEFT_CASE(64)
     DEFT_CASE(96)
     DEFT_CASE(128)
+    DEFT_CASE(201)
     DEFT_CASE(256)
 #undef DEFT_CASE
     default:

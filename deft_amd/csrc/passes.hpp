@@ -199,7 +199,8 @@ __device__ __forceinline__ void store_r2c_halves(const cx<T>* Z, cx<T>* __restri
 
 // Load lines [line0, line0+2P) of half spectra and build the P packed complex
 // spectra Z = A + i B (Hermitian completion along z; the imaginary parts of the
-// kz = 0 and kz = Nz/2 elements are dropped, i.e. c2r returns the real part).
+// kz = 0 and, for even Nz, kz = Nz/2 elements are dropped, i.e. c2r returns the
+// real part).
 // Both buffers need zbuf_size<T, L, P>() elements.
 template <typename T, int L, int P>
 constexpr int zbuf_size() {
@@ -222,7 +223,7 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
     if (z <= L / 2) {
       a = raw[(2 * p) * NKZ + z];
       b = raw[(2 * p + 1) * NKZ + z];
-      if (z == 0 || z == L / 2) {
+      if (z == 0 || (L % 2 == 0 && z == L / 2)) {
         a.im = (T)0;
         b.im = (T)0;
       }

@@ -74,6 +74,7 @@ struct deft_plan_s {
   double recip[9] = {0};  // rows b1, b2, b3 with a_i . b_j = 2 pi delta_ij
   double volume = 0, dV = 0;
   double R = 1, kT = 1, mu = 0;
+  int functional = DEFT_WHITE_BEAR;  // DEFT_WHITE_BEAR or DEFT_WHITE_BEAR_MARK_II
   int prec = DEFT_FFT_FP32;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -146,7 +147,7 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.mu = p->mu;
   a.logNQ = std::log(water_nQ(p->kT));
   a.dV = p->dV;
-  a.wb = make_wb_const(p->R);
+  a.wb = make_wb_const(p->R, p->functional);
   return a;
 }
 

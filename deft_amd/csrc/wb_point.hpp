@@ -11,6 +11,21 @@
 // For |n3| < 0.05 f3 and f3' come from the Taylor series
 // f3 = g/(1-n)^2, g = 3/2 - sum_{k>=3} 2 n^(k-2) / (k (k-1) (k-2)),
 // since the closed form is 0/0 at n3 = 0 (inside hard walls).
+//
+// White-Bear Mark II (src/haskell/WhiteBear.hs:140-159): Phi2 and Phi3 take the
+// factors (1 + n3^2 phi2/9) and (1 - 4 n3 phi3/9) of Hansen-Goos and Roth,
+// phi2 = (6n - 3n^2 + 6(1-n) ln(1-n))/n^3, phi3 = (6n - 9n^2 + 6n^3 +
+// 6(1-n)^2 ln(1-n))/(4 n^3), with Phi3's prefactor 1/(24 pi (1-n)^2).  Both
+// are written as Phi2 = g2 (n2^2 - n2v^2)/(4 pi R), Phi3 = g3 n2 (n2^2 - 3 n2v^2)
+// /(36 pi): White-Bear has g2 = 1/(1-n), g3 = f3; Mark II g2 = (1 + S2/9)/(1-n),
+// g3 = 3/2 (1 - 4 S3/9)/(1-n)^2 with S2 = n^2 phi2, S3 = n phi3, whose series
+// S2 = sum_{m>=3} 6 n^(m-1)/(m(m-1)), S3 = n - sum_{m>=4} 3 n^(m-2)/(m(m-1)(m-2))
+// replace the closed forms below |n3| = 0.05.
+// The slower C++ version, HardSpheresWBm2slow (src/HardSpheres.cpp:161-185),
+// has 1/(36 pi) for Phi3's prefactor, i.e. g3 = (1 - 4 S3/9)/(1-n)^2: its bulk
+// is not Carnahan-Starling and it lacks Rosenfeld's low-density limit, but the
+// reference's published Mark II profiles (papers/contact/figs/
+// inner-sphereWBm2-*.dat) were computed with it; it is kept as a third choice.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,11 +37,58 @@ struct WBConst {
   double fpR2;      // 4 pi R^2
   double inv_fpR;   // 1/(4 pi R)
   double inv_fpR2;  // 1/(4 pi R^2)
+  int mark2;        // 0: White-Bear, 1: Mark II, 2: Mark II with the 1/(36 pi) Phi3
+  double g3_scale;  // Mark II: g3 = g3_scale (1 - 4 S3/9)/(1-n)^2 (3/2, or 1 for mark2 = 2)
 };
 
-inline WBConst make_wb_const(double R) {
+inline WBConst make_wb_const(double R, int mark2 = 0) {
   const double pi = 3.14159265358979323846;
-  return WBConst{4 * pi * R, 4 * pi * R * R, 1.0 / (4 * pi * R), 1.0 / (4 * pi * R * R)};
+  return WBConst{4 * pi * R, 4 * pi * R * R, 1.0 / (4 * pi * R), 1.0 / (4 * pi * R * R), mark2,
+                 mark2 == 2 ? 1.0 : 1.5};
+}
+
+constexpr double kWB2SeriesCutoff = 0.05;
+constexpr int kWB2SeriesTerms = 24;
+
+// Mark II factors g2, g2', g3, g3' (see above) given n, io = 1/(1-n), L = ln(1-n)
+// and g3's prefactor s3.
+__host__ __device__ inline void markii_of(double n, double io, double L, double s3, double& g2, double& g2p,
+                                          double& g3, double& g3p) {
+#pragma clang fp contract(off)
+  double S2, S2p, S3, S3p;
+  if (fabs(n) < kWB2SeriesCutoff) {
+    S2 = 0.0;
+    S2p = 0.0;
+    double p = n;  // n^(m-2)
+    for (int m = 3; m < kWB2SeriesTerms + 3; ++m) {
+      S2p += 6.0 * p / m;
+      S2 += 6.0 * (p * n) / (m * (m - 1));
+      p *= n;
+    }
+    S3 = n;
+    S3p = 1.0;
+    double q = n;  // n^(m-3)
+    for (int m = 4; m < kWB2SeriesTerms + 4; ++m) {
+      S3p -= 3.0 * q / (m * (m - 1));
+      S3 -= 3.0 * (q * n) / (m * (m - 1) * (m - 2));
+      q *= n;
+    }
+  } else {
+    const double om = 1.0 - n;
+    const double N2 = 6.0 * n - 3.0 * n * n + 6.0 * om * L;
+    S2 = N2 / n;
+    S2p = (-3.0 * n * n - 6.0 * n - 6.0 * L) / (n * n);
+    const double N3 = 6.0 * n - 9.0 * n * n + 6.0 * n * n * n + 6.0 * om * om * L;
+    const double N3p = -12.0 * n + 18.0 * n * n - 12.0 * om * L;
+    S3 = N3 / (4.0 * n * n);
+    S3p = (N3p * n - 2.0 * N3) / (4.0 * n * n * n);
+  }
+  const double a2 = 1.0 + S2 / 9.0;
+  g2 = io * a2;
+  g2p = io * io * a2 + io * S2p / 9.0;
+  const double a3 = 1.0 - 4.0 * S3 / 9.0;
+  g3 = s3 * a3 * io * io;
+  g3p = s3 * (-(4.0 / 9.0) * S3p * io * io + 2.0 * a3 * io * io * io);
 }
 
 constexpr double kInv36Pi = 1.0 / (36.0 * 3.14159265358979323846);
@@ -75,11 +137,16 @@ __host__ __device__ inline void wb_phi(double n3, double n2, double vx, double v
   const double io = 1.0 / om;
   const double L = log1p(-n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
-  double f3, df3;
-  f3_of(n3, om, io, L, f3, df3);
+  double f3, df3, g2 = io;
+  if (c.mark2) {
+    double g2p;
+    markii_of(n3, io, L, c.g3_scale, g2, g2p, f3, df3);
+  } else {
+    f3_of(n3, om, io, L, f3, df3);
+  }
   const double n2s = n2 * n2;
   p1 = -n2 * L * c.inv_fpR2;
-  p2 = (n2s - v2) * c.inv_fpR * io;
+  p2 = (n2s - v2) * c.inv_fpR * g2;
   p3 = f3 * n2 * (n2s - 3.0 * v2) * kInv36Pi;
 }
 
@@ -92,15 +159,24 @@ __host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx,
   const double L = log1p(-n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
-  f3_of(n3, om, io, L, f3, df3);
   const double n2s = n2 * n2;
   const double q = n2s - v2;
   const double t3 = n2 * (n2s - 3.0 * v2);
-  const double a = c.inv_fpR * io;  // 1/(4 pi R (1-n3))
+  double a, ap;  // Phi2 = q a, dPhi2/dn3 = q ap
+  if (c.mark2) {
+    double g2, g2p;
+    markii_of(n3, io, L, c.g3_scale, g2, g2p, f3, df3);
+    a = c.inv_fpR * g2;
+    ap = c.inv_fpR * g2p;
+  } else {
+    f3_of(n3, om, io, L, f3, df3);
+    a = c.inv_fpR * io;  // 1/(4 pi R (1-n3))
+    ap = a * io;
+  }
   p1 = -n2 * L * c.inv_fpR2;
   p2 = q * a;
   p3 = f3 * t3 * kInv36Pi;
-  d[0] = n2 * c.inv_fpR2 * io + q * a * io + df3 * t3 * kInv36Pi;
+  d[0] = n2 * c.inv_fpR2 * io + q * ap + df3 * t3 * kInv36Pi;
   d[1] = -L * c.inv_fpR2 + 2.0 * n2 * a + f3 * 3.0 * q * kInv36Pi;
   const double cv = -2.0 * a - f3 * 6.0 * n2 * kInv36Pi;
   d[2] = cv * vx;

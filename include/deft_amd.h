@@ -14,8 +14,10 @@
  * Conventions (the reference's, kept):
  *   - real fields are row-major, index (x*Ny + y)*Nz + z (src/Grid.h:37-42);
  *     half spectra (x*Ny + y)*(Nz/2+1) + kz, interleaved re/im doubles;
- *   - every grid edge is even (src/new/Vector.h:328-330) and a product of
- *     2, 3 and 5 from the supported list (deft_supported_length);
+ *   - grid edges come from the supported list (deft_supported_length): the
+ *     even ones the new API asserts (src/new/Vector.h:328-330) plus the odd
+ *     edges of the old-API drivers (201 = 1 + int(20/0.1 + e/100),
+ *     src/GridDescription.cpp:28, papers/contact/figs/inner-sphere.cpp:183-184);
  *   - gradients are ACCUMULATED into caller buffers (+=), and the pgrad output
  *     may be NULL (src/Functional.h:407-417, src/EffectivePotentialToDensity.cpp:62-79);
  *   - gradients are derivatives of the discrete energy sum, so they carry the dV
@@ -49,6 +51,20 @@ typedef enum {
   DEFT_FFT_FP32 = 0,
   DEFT_FFT_FP64 = 1
 } deft_fft_precision;
+
+/* Hard-sphere functional of a plan: White-Bear without tensor
+ * (HardSpheresWBnotensor, src/HardSpheres.cpp:137-159, the default),
+ * White-Bear Mark II (WhiteBearMarkII, src/haskell/WhiteBear.hs:140-159), or
+ * Mark II as the C++ HardSpheresWBm2slow writes it (src/HardSpheres.cpp:161-185:
+ * Phi3's prefactor 1/(36 pi) in place of 1/(24 pi)), the form the reference's
+ * published Mark II profiles (papers/contact/figs/inner-sphereWBm2-*.dat) match.
+ * The first two give the Carnahan-Starling bulk that deft_bulk_chemical_potential
+ * and friends compute; deft_bulk_state gives the bulk of any of the three. */
+typedef enum {
+  DEFT_WHITE_BEAR = 0,
+  DEFT_WHITE_BEAR_MARK_II = 1,
+  DEFT_WHITE_BEAR_MARK_II_SLOW = 2
+} deft_functional_kind;
 
 /* Why a minimizer stopped (mirrors the return-false branches of
  * src/new/Minimize.cpp:9-313). */
@@ -117,6 +133,8 @@ int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile,
 int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
+/* DEFT_WHITE_BEAR (default) or DEFT_WHITE_BEAR_MARK_II; drops nothing else. */
+int deft_plan_set_functional(deft_plan plan, int functional);
 /* Vext (N doubles, host); stored as FP32 on the device (+inf kept). */
 int deft_plan_set_vext(deft_plan plan, const double *vext);
 int deft_plan_set_veff(deft_plan plan, const double *veff);
@@ -162,6 +180,10 @@ int deft_ifft(deft_plan plan, const double *in, double *out);
 int deft_bulk_chemical_potential(double n, double R, double kT, double *mu);
 int deft_bulk_pressure(double n, double R, double kT, double *p);
 int deft_bulk_free_energy_density(double n, double R, double kT, double *f);
+/* f(n), mu(n) and p(n) of ideal gas + the given deft_functional_kind
+ * (find_chemical_potential of OfEffectivePotential(fhs + IdealGas()) in
+ * papers/contact/figs/inner-sphere.cpp:181-182, for every fhs it passes). */
+int deft_bulk_state(int functional, double n, double R, double kT, double *f, double *mu, double *p);
 
 /* --- minimizer seam ---------------------------------------------------------
  * Minimize (src/new/Minimize.h:13-199): Polak-Ribiere CG with the quadratic

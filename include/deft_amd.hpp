@@ -116,6 +116,9 @@ class WhiteBearFluid {
   const GridDescription &description() const { return gd_; }
   void set_mu(double mu) { check(deft_plan_set_mu(plan_, mu)); }
   void set_fft_precision(int p) { check(deft_plan_set_fft_precision(plan_, p)); }
+  // DEFT_WHITE_BEAR (HardSpheresNoTensor2), DEFT_WHITE_BEAR_MARK_II (WhiteBearMarkII)
+  // or DEFT_WHITE_BEAR_MARK_II_SLOW (HardSpheresWBm2slow)
+  void set_functional(int f) { check(deft_plan_set_functional(plan_, f)); }
   // Vext = +inf marks hard walls (n = 0 there)
   void set_Vext(const std::vector<double> &v) {
     size_check(v);

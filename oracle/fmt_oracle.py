@@ -85,8 +85,10 @@ class Grid:
     def __init__(self, shape, cell):
         self.shape = tuple(int(s) for s in shape)
         Nx, Ny, Nz = self.shape
-        if any(s % 2 for s in self.shape):
-            raise ValueError("every grid edge must be even (src/new/Vector.h:328-330)")
+        # The new API asserts even edges (src/new/Vector.h:328-330); the old-API Grid
+        # takes any edge (src/GridDescription.cpp:28, r2c of any size in src/Grid.cpp:470-478).
+        if min(self.shape) < 1:
+            raise ValueError("grid edges must be positive")
         self.cell = np.array(cell, dtype=np.float64).reshape(3, 3)
         self.volume = abs(float(np.linalg.det(self.cell)))
         self.N = Nx * Ny * Nz
@@ -204,7 +206,58 @@ def f3_and_derivative(n3):
     return np.where(small, f3_small, f3_big), np.where(small, df3_small, df3_big)
 
 
-def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True):
+WB2_SERIES_CUTOFF = 0.05
+WB2_SERIES_TERMS = 24
+
+
+def markii_factors(n3, g3_scale=1.5):
+    """White-Bear Mark II factors of src/haskell/WhiteBear.hs:140-159 (Hansen-Goos and
+    Roth 2006, eqs. 16-17):
+      Phi2 = (1 + n3^2 phi2(n3)/9) (n1 n2 - n1v.n2v)/(1-n3),
+      Phi3 = (1 - 4 n3 phi3(n3)/9) (n2^3 - 3 n2 n2v.n2v)/(24 pi (1-n3)^2),
+      phi2 = (6n - 3n^2 + 6(1-n) ln(1-n))/n^3, phi3 = (6n - 9n^2 + 6n^3 + 6(1-n)^2 ln(1-n))/(4 n^3).
+    Returned as g2 = (1 + S2/9)/(1-n) and g3 = 1.5 (1 - 4 S3/9)/(1-n)^2 with their n-derivatives,
+    so that Phi2 = g2 (n2^2 - n2v^2)/(4 pi R) and Phi3 = g3 (n2^3 - 3 n2 n2v^2)/(36 pi) as for
+    White-Bear (g2 = 1/(1-n), g3 = f3).  S2 = n^2 phi2 and S3 = n phi3 use their power series
+    below |n| = 0.05:  S2 = sum_{m>=3} 6 n^(m-1)/(m(m-1)),
+    S3 = n - sum_{m>=4} 3 n^(m-2)/(m(m-1)(m-2)).
+    g3_scale = 1 gives HardSpheresWBm2slow's Phi3 (src/HardSpheres.cpp:176-177), whose
+    prefactor is 1/(36 pi) instead of 1/(24 pi)."""
+    n = np.asarray(n3, dtype=np.float64)
+    small = np.abs(n) < WB2_SERIES_CUTOFF
+    nb = np.where(small, 0.5, n)
+    L = np.log1p(-nb)
+    N2 = 6 * nb - 3 * nb ** 2 + 6 * (1 - nb) * L
+    S2b = N2 / nb
+    S2pb = (-3 * nb ** 2 - 6 * nb - 6 * L) / nb ** 2
+    N3 = 6 * nb - 9 * nb ** 2 + 6 * nb ** 3 + 6 * (1 - nb) ** 2 * L
+    N3p = -12 * nb + 18 * nb ** 2 - 12 * (1 - nb) * L
+    S3b = N3 / (4 * nb ** 2)
+    S3pb = (N3p * nb - 2 * N3) / (4 * nb ** 3)
+    ns = np.where(small, n, 0.0)
+    S2s = np.zeros_like(ns)
+    S2ps = np.zeros_like(ns)
+    S3s = ns.copy()
+    S3ps = np.ones_like(ns)
+    for m in range(3, WB2_SERIES_TERMS + 3):
+        S2s += 6 * ns ** (m - 1) / (m * (m - 1))
+        S2ps += 6 * ns ** (m - 2) / m
+    for m in range(4, WB2_SERIES_TERMS + 4):
+        S3s -= 3 * ns ** (m - 2) / (m * (m - 1) * (m - 2))
+        S3ps -= 3 * ns ** (m - 3) / (m * (m - 1))
+    S2 = np.where(small, S2s, S2b)
+    S2p = np.where(small, S2ps, S2pb)
+    S3 = np.where(small, S3s, S3b)
+    S3p = np.where(small, S3ps, S3pb)
+    io = 1.0 / (1.0 - n)
+    g2 = io * (1 + S2 / 9)
+    g2p = io * io * (1 + S2 / 9) + io * S2p / 9
+    g3 = g3_scale * (1 - 4 * S3 / 9) * io * io
+    g3p = g3_scale * (-(4.0 / 9) * S3p * io * io + (1 - 4 * S3 / 9) * 2 * io ** 3)
+    return g2, g2p, g3, g3p
+
+
+def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True, functional="wb"):
     """Phi = Phi1 + Phi2 + Phi3 of the no-tensor White-Bear functional and its partials.
 
     Phi1 = -n0 ln(1-n3); Phi2 = (n1 n2 - n1v.n2v)/(1-n3);
@@ -218,16 +271,25 @@ def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True):
     omn = 1.0 - n3
     L = np.log1p(-n3)
     v2 = vx * vx + vy * vy + vz * vz
-    f3, df3 = f3_and_derivative(n3)
+    if functional == "wb":
+        f3, df3 = f3_and_derivative(n3)
+        g2, g2p = 1.0 / omn, 1.0 / (omn * omn)
+    elif functional == "wb2":
+        g2, g2p, f3, df3 = markii_factors(n3)
+    elif functional == "wb2slow":
+        # HardSpheresWBm2slow (src/HardSpheres.cpp:161-185): Phi3 over 36 pi, not 24 pi
+        g2, g2p, f3, df3 = markii_factors(n3, g3_scale=1.0)
+    else:
+        raise ValueError(functional)
     t3 = n2 * (n2 * n2 - 3.0 * v2)
     phi1 = -n2 * L / fpR2
-    phi2 = (n2 * n2 - v2) / (fpR * omn)
+    phi2 = (n2 * n2 - v2) * g2 / fpR
     phi3 = f3 * t3 / (36 * math.pi)
     if not want_partials:
         return phi1, phi2, phi3, None
-    d3 = n2 / (fpR2 * omn) + (n2 * n2 - v2) / (fpR * omn * omn) + df3 * t3 / (36 * math.pi)
-    d2 = -L / fpR2 + 2.0 * n2 / (fpR * omn) + f3 * (3.0 * n2 * n2 - 3.0 * v2) / (36 * math.pi)
-    cv = -2.0 / (fpR * omn) - f3 * 6.0 * n2 / (36 * math.pi)
+    d3 = n2 / (fpR2 * omn) + (n2 * n2 - v2) * g2p / fpR + df3 * t3 / (36 * math.pi)
+    d2 = -L / fpR2 + 2.0 * n2 * g2 / fpR + f3 * (3.0 * n2 * n2 - 3.0 * v2) / (36 * math.pi)
+    cv = -2.0 * g2 / fpR - f3 * 6.0 * n2 / (36 * math.pi)
     return phi1, phi2, phi3, (d3, d2, cv * vx, cv * vy, cv * vz)
 
 
@@ -235,33 +297,33 @@ def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True):
 # Homogeneous path: free energy per volume, chemical potential, pressure
 # ---------------------------------------------------------------------------
 
-def homogeneous_whitebear(n, R, kT=1.0):
+def homogeneous_whitebear(n, R, kT=1.0, functional="wb"):
     """Excess free energy per volume kT Phi(n) of a uniform fluid (k=0 weights, src/Functional.h:382-392)."""
     n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
     n2 = np.float64(4 * math.pi * R * R * n)
     z = np.float64(0.0)
-    p1, p2, p3, _ = whitebear_phi(n3, n2, z, z, z, R, want_partials=False)
+    p1, p2, p3, _ = whitebear_phi(n3, n2, z, z, z, R, want_partials=False, functional=functional)
     return float(kT * (p1 + p2 + p3))
 
 
-def homogeneous_whitebear_dn(n, R, kT=1.0):
+def homogeneous_whitebear_dn(n, R, kT=1.0, functional="wb"):
     """d(kT Phi)/dn of a uniform fluid, through the partials (chain rule over n3, n2)."""
     n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
     n2 = np.float64(4 * math.pi * R * R * n)
     z = np.float64(0.0)
-    _, _, _, d = whitebear_phi(n3, n2, z, z, z, R)
+    _, _, _, d = whitebear_phi(n3, n2, z, z, z, R, functional=functional)
     return float(kT * (d[0] * (4 * math.pi / 3 * R ** 3) + d[1] * (4 * math.pi * R * R)))
 
 
-def chemical_potential(n, R, kT=1.0):
+def chemical_potential(n, R, kT=1.0, functional="wb"):
     """mu = d f/dn of ideal gas + White-Bear at bulk density n (src/equation-of-state.cpp:189-192)."""
-    return float(kT * math.log(n / nQ(kT)) + homogeneous_whitebear_dn(n, R, kT))
+    return float(kT * math.log(n / nQ(kT)) + homogeneous_whitebear_dn(n, R, kT, functional))
 
 
-def pressure(n, R, kT=1.0):
+def pressure(n, R, kT=1.0, functional="wb"):
     """p = n df/dn - f (src/equation-of-state.cpp:164-170)."""
-    f = kT * n * (math.log(n / nQ(kT)) - 1) + homogeneous_whitebear(n, R, kT)
-    return float(n * chemical_potential(n, R, kT) - f)
+    f = kT * n * (math.log(n / nQ(kT)) - 1) + homogeneous_whitebear(n, R, kT, functional)
+    return float(n * chemical_potential(n, R, kT, functional) - f)
 
 
 def bulk_density(eta, R):
@@ -282,8 +344,9 @@ class WhiteBearFluidVeff:
     in place by the minimizer, like the reference's data vector).
     """
 
-    def __init__(self, grid, R, kT, mu, vext, x):
+    def __init__(self, grid, R, kT, mu, vext, x, functional="wb"):
         self.grid = grid
+        self.functional = functional
         self.R = float(R)
         self.kT = float(kT)
         self.mu = float(mu)
@@ -318,7 +381,7 @@ class WhiteBearFluidVeff:
         x = self.x if x is None else x
         n = self.density(x)
         nal = self.weighted_densities(n)
-        p1, p2, p3, _ = whitebear_phi(*nal, self.R, want_partials=False)
+        p1, p2, p3, _ = whitebear_phi(*nal, self.R, want_partials=False, functional=self.functional)
         ideal, ext, mun = self._local_terms(x, n)
         dV = self.grid.dV
         kT = self.kT
@@ -339,7 +402,7 @@ class WhiteBearFluidVeff:
         x = self.x if x is None else x
         n = self.density(x)
         nal = self.weighted_densities(n)
-        _, _, _, P = whitebear_phi(*nal, self.R)
+        _, _, _, P = whitebear_phi(*nal, self.R, functional=self.functional)
         acc = None
         for Pa, W, par in zip(P, self.W, self.parity):
             term = _rfftn(Pa) * (W * par)
@@ -620,14 +683,14 @@ def spherical_solute_vext(grid, radius, center=(0.0, 0.0, 0.0)):
     return np.where(r < radius, np.inf, 0.0)
 
 
-def make_fluid(grid, eta, R=1.0, kT=1.0, vext=None):
+def make_fluid(grid, eta, R=1.0, kT=1.0, vext=None, functional="wb"):
     """Fluid at bulk packing fraction eta, mu from the bulk equation of state, Veff = -kT ln n_b."""
     nb = bulk_density(eta, R)
-    mu = chemical_potential(nb, R, kT)
+    mu = chemical_potential(nb, R, kT, functional)
     if vext is None:
         vext = np.zeros(grid.shape)
     x = np.full(grid.shape, -kT * math.log(nb))
-    return WhiteBearFluidVeff(grid, R, kT, mu, vext, x)
+    return WhiteBearFluidVeff(grid, R, kT, mu, vext, x, functional)
 
 
 def minimize(f, relative_precision=1e-12, maxiter=100000, precondition=True, miniter=6, callback=None):

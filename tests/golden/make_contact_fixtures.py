@@ -1,0 +1,43 @@
+"""Extracts the reference's converged solute profiles into tests/golden/contact_inner_sphere.json.
+
+    python tests/golden/make_contact_fixtures.py
+
+Source: papers/contact/figs/inner-sphere{WB,WBm2}-04.0-{eta}.dat of the reference, the
+output of papers/contact/figs/inner-sphere.cpp:177-249 (radial_plot, :143-172): a hard
+solute of excluded-centre radius 2 (diameter argument 4, notinwall at :30-38) in a
+periodic 20^3 cell (padding 16), grid 201^3 (GridDescription(lat, 0.1),
+src/GridDescription.cpp:28), R = 1, kT = 1, mu from the bulk, Veff started at
+-ln(n_b c + 1e-4 n_b), old-API preconditioned CG to |E_bulk V| 1e-10 for at most 200
+iterations.  Column 0 is y of grid point (0, y, 0), column 1 the density there
+(the remaining columns are contact-value estimators outside this path).
+"WB" is HardSpheresNoTensor2 (White Bear), "WBm2" WhiteBearMarkII.
+Only those two columns, as printed (%g), are kept.
+"""
+
+import json
+import os
+
+REF = "/root/reference/papers/contact/figs"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def main():
+    out = {"diameter": 4.0, "cell": 20.0, "n": 201, "R": 1.0, "kT": 1.0, "profiles": {}}
+    for name in ("WB", "WBm2"):
+        for eta in ("0.10", "0.20", "0.30", "0.40", "0.50"):
+            path = os.path.join(REF, "inner-sphere%s-04.0-%s.dat" % (name, eta))
+            y, n = [], []
+            with open(path) as fh:
+                for line in fh:
+                    cols = line.split()
+                    if len(cols) < 2:
+                        continue
+                    y.append(float(cols[0]))
+                    n.append(float(cols[1]))
+            out["profiles"]["%s-%s" % (name, eta)] = {"eta": float(eta), "y": y, "density": n}
+    with open(os.path.join(HERE, "contact_inner_sphere.json"), "w") as fh:
+        json.dump(out, fh, indent=0)
+
+
+if __name__ == "__main__":
+    main()

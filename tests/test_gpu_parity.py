@@ -40,10 +40,13 @@ SHAPES = [((16, 16, 16), np.diag([3.2, 3.2, 3.2])),
           ((20, 16, 32), np.array([[2.0, 0.0, 0.1], [0.2, 1.9, 0.0], [0.0, 0.3, 3.4]])),
           ((64, 64, 64), np.diag([6.4, 6.4, 6.4])),
           ((96, 32, 128), np.diag([9.6, 3.2, 12.8]))]
+# odd edges (old-API grids): no Nyquist plane along z, direct-DFT radix-7 stage
+ODD_SHAPES = [((21, 16, 20), np.diag([3.1, 2.4, 3.0])),
+              ((16, 21, 21), np.array([[2.2, 0.0, 0.0], [0.3, 3.0, 0.0], [0.0, 0.2, 3.1]]))]
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 3e-6)])
-@pytest.mark.parametrize("shape,cell", SHAPES[:4])
+@pytest.mark.parametrize("shape,cell", SHAPES[:4] + ODD_SHAPES)
 def test_fft_seam_matches_numpy(gpu, shape, cell, prec, tol):
     print("fft", shape, prec, flush=True)
     g = O.Grid(shape, cell)
@@ -60,7 +63,7 @@ def test_fft_seam_matches_numpy(gpu, shape, cell, prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 3e-6)])
-@pytest.mark.parametrize("shape,cell", SHAPES)
+@pytest.mark.parametrize("shape,cell", SHAPES + ODD_SHAPES)
 def test_weighted_densities(gpu, shape, cell, prec, tol):
     print("weighted", shape, prec, flush=True)
     g, of, f = _fluid(gpu, shape, cell, prec=prec, seed=3)
@@ -71,7 +74,7 @@ def test_weighted_densities(gpu, shape, cell, prec, tol):
 
 
 @pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
-@pytest.mark.parametrize("shape,cell", SHAPES)
+@pytest.mark.parametrize("shape,cell", SHAPES + ODD_SHAPES)
 def test_energy_and_gradient(gpu, shape, cell, prec, etol, gtol):
     print("energy/grad", shape, prec, flush=True)
     vext = None

@@ -21,7 +21,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_supported_lengths():
-    for n in (16, 20, 32, 48, 64, 96, 128, 256):
+    for n in (16, 20, 21, 32, 48, 64, 96, 128, 201, 256):
         assert deft_amd.supported_length(n)
     for n in (0, 15, 18, 100, 512):
         assert not deft_amd.supported_length(n)
@@ -34,6 +34,20 @@ def test_bulk_path_matches_oracle(eta):
     assert abs(deft_amd.pressure(n) / O.pressure(n, 1.0) - 1) < 1e-12
     f = 1.0 * n * (math.log(n / O.nQ(1.0)) - 1) + O.homogeneous_whitebear(n, 1.0)
     assert abs(deft_amd.bulk_free_energy_density(n) / f - 1) < 1e-13
+
+
+@pytest.mark.parametrize("functional,name", [(0, "wb"), (1, "wb2"), (2, "wb2slow")])
+@pytest.mark.parametrize("eta", [0.1, 0.4])
+def test_bulk_state_of_every_functional(eta, functional, name):
+    n = O.bulk_density(eta, 1.0)
+    f, mu, p = deft_amd.bulk_state(n, 1.0, 1.0, functional)
+    assert abs(mu / O.chemical_potential(n, 1.0, functional=name) - 1) < 1e-13
+    assert abs(p / O.pressure(n, 1.0, functional=name) - 1) < 1e-12
+    fo = n * (math.log(n / O.nQ(1.0)) - 1) + O.homogeneous_whitebear(n, 1.0, functional=name)
+    assert abs(f / fo - 1) < 1e-13
+    assert deft_amd.find_chemical_potential(n, functional=functional) == mu
+    with pytest.raises(deft_amd.DeftError):
+        deft_amd.bulk_state(n, 1.0, 1.0, 3)
 
 
 def test_bad_arguments_raise_without_device():

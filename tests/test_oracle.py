@@ -48,17 +48,20 @@ def test_golden_numbers_of_the_survey():
     assert abs(1 / O.nQ(1.0) / 2.646476976618268e-6 - 1) < 1e-14
 
 
-def test_fft_convention_against_brute_force_dft():
-    """fft = dV * sum_r f(r) exp(-i k.r) on the half spectrum; ifft = (1/V) sum_k (Hermitian)."""
-    g = O.Grid((4, 6, 8), np.diag([1.3, 2.1, 0.7]))
+@pytest.mark.parametrize("shape", [(4, 6, 8), (5, 3, 7)])
+def test_fft_convention_against_brute_force_dft(shape):
+    """fft = dV * sum_r f(r) exp(-i k.r) on the half spectrum; ifft = (1/V) sum_k (Hermitian).
+    Odd edges (old-API grids) keep every kz < Nz/2 + 1 and have no Nyquist plane."""
+    g = O.Grid(shape, np.diag([1.3, 2.1, 0.7]))
     rng = np.random.default_rng(0)
     f = rng.standard_normal(g.shape)
     F = O.fft(g, f)
     Nx, Ny, Nz = g.shape
+    assert F.shape == (Nx, Ny, Nz // 2 + 1)
     x = np.arange(Nx)[:, None, None]
     y = np.arange(Ny)[None, :, None]
     z = np.arange(Nz)[None, None, :]
-    for (a, b, c) in [(0, 0, 0), (1, 2, 3), (3, 5, 4), (2, 1, 0)]:
+    for (a, b, c) in [(0, 0, 0), (1, 2, 3), (3, 2, 2), (2, 1, 0), (Nx - 1, Ny - 1, Nz // 2)]:
         ph = np.exp(-2j * np.pi * (a * x / Nx + b * y / Ny + c * z / Nz))
         assert abs(F[a, b, c] - g.dV * np.sum(f * ph)) < 1e-13
     back = O.ifft(g, F)
@@ -177,9 +180,10 @@ def test_phi_partials_by_finite_differences():
             assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0]))
 
 
-def test_gradient_by_finite_differences():
+@pytest.mark.parametrize("shape", [(16, 16, 20), (15, 21, 17)])
+def test_gradient_by_finite_differences(shape):
     """Central differences along a random direction (src/Functional.cpp:159-212)."""
-    g = O.Grid((16, 16, 20), np.diag([3.2, 3.0, 3.4]))
+    g = O.Grid(shape, np.diag([3.2, 3.0, 3.4]))
     rng = np.random.default_rng(1)
     vext = O.planar_wall_vext(g, 1.0)
     vext = np.where(np.isfinite(vext), 0.3 * np.cos(g.positions()[..., 2]), vext)
@@ -275,3 +279,133 @@ def test_vector_weight_is_minus_step_weight():
     g = O.Grid((32, 48, 20), np.diag([3.1, 4.9, 2.3]))
     w3, w2, wv = O.weights(g, 1.0)
     assert np.max(np.abs(wv + w3)) <= 1e-13 * np.max(np.abs(w3))
+
+
+# ---------------------------------------------------------------------------
+# White-Bear Mark II (src/haskell/WhiteBear.hs:140-159)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("eta", [0.1, 0.3, 0.4, 0.45])
+def test_mark_ii_bulk_is_carnahan_starling(eta):
+    """Mark II keeps the Carnahan-Starling bulk of Mark I (Hansen-Goos and Roth 2006)."""
+    n = O.bulk_density(eta, 1.0)
+    f_cs = n * (4 * eta - 3 * eta ** 2) / (1 - eta) ** 2
+    mu_cs = (8 * eta - 9 * eta ** 2 + 3 * eta ** 3) / (1 - eta) ** 3
+    p_cs = n * (1 + eta + eta ** 2 - eta ** 3) / (1 - eta) ** 3
+    assert abs(O.homogeneous_whitebear(n, 1.0, functional="wb2") / f_cs - 1) < 1e-13
+    assert abs((O.chemical_potential(n, 1.0, functional="wb2") - math.log(n / O.nQ(1.0))) / mu_cs - 1) < 1e-13
+    assert abs(O.pressure(n, 1.0, functional="wb2") / p_cs - 1) < 1e-12
+
+
+@pytest.mark.parametrize("eta", [0.02, 0.1, 0.3, 0.45])
+def test_mark_ii_scaled_particle_relation(eta):
+    """Mark II is built so that dPhi/dn3 at bulk weighted densities is the Carnahan-Starling
+    pressure (the scaled-particle relation for an infinitely large particle); Mark I misses it.
+    A slip in either phi2 or phi3 breaks this."""
+    for R in (1.0, 0.7):
+        n = O.bulk_density(eta, R)
+        p_cs = n * (1 + eta + eta ** 2 - eta ** 3) / (1 - eta) ** 3
+        n3, n2, z = np.array([eta]), np.array([4 * math.pi * R * R * n]), np.zeros(1)
+        d2 = O.whitebear_phi(n3, n2, z, z, z, R, functional="wb2")[3][0][0]
+        d1 = O.whitebear_phi(n3, n2, z, z, z, R, functional="wb")[3][0][0]
+        assert abs(d2 / p_cs - 1) < 1e-12
+        assert abs(d1 / p_cs - 1) > 1e-4
+
+
+def test_mark_ii_low_density_limit_and_series():
+    """phi2, phi3 -> 1 as n3 -> 0 (Rosenfeld's low-density limit: g2 = 1, g3 = 3/2, g2' = 1 + 2/9 * 0,
+    dS2/dn = 2n, dS3/dn = 1); the series branch below |n3| = 0.05 meets the closed form."""
+    g2, g2p, g3, g3p = O.markii_factors(np.array([0.0]))
+    assert g2[0] == 1.0 and abs(g3[0] - 1.5) < 1e-15
+    # g2' = d/dn[(1 + S2/9)/(1-n)] = 1 at 0;  g3' = 1.5 (2 - 4/9) = 7/3 at 0
+    assert abs(g2p[0] - 1.0) < 1e-15 and abs(g3p[0] - 7.0 / 3.0) < 1e-14
+    for c in (0.05, -0.05):
+        lo, hi = O.markii_factors(np.array([c * (1 - 1e-12)])), O.markii_factors(np.array([c * (1 + 1e-12)]))
+        for a, b in zip(lo, hi):
+            assert abs(a[0] - b[0]) < 1e-12 * max(1.0, abs(a[0]))
+    # closed forms of phi2 and phi3 at moderate n, written from the Haskell expressions
+    for n in (0.1, 0.3, 0.49):
+        L = math.log(1 - n)
+        phi2 = (6 * n - 3 * n * n + 6 * (1 - n) * L) / n ** 3
+        phi3 = (6 * n - 9 * n * n + 6 * n ** 3 + 6 * (1 - n) ** 2 * L) / (4 * n ** 3)
+        g2, _, g3, _ = O.markii_factors(np.array([n]))
+        assert abs(g2[0] - (1 + n * n * phi2 / 9) / (1 - n)) < 1e-13
+        assert abs(g3[0] - 1.5 * (1 - 4 * n * phi3 / 9) / (1 - n) ** 2) < 1e-13
+    h = 1e-6
+    for n in (-0.04, 0.01, 0.0499, 0.0501, 0.2, 0.45):
+        f = O.markii_factors(np.array([n]))
+        fp, fm = O.markii_factors(np.array([n + h])), O.markii_factors(np.array([n - h]))
+        for val, deriv in ((0, 1), (2, 3)):
+            fd = (fp[val][0] - fm[val][0]) / (2 * h)
+            assert abs(fd - f[deriv][0]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_mark_ii_phi_partials_by_finite_differences():
+    rng = np.random.default_rng(5)
+    R = 1.0
+    for _ in range(20):
+        n3 = rng.uniform(-0.03, 0.5)
+        n2 = rng.uniform(0.1, 3.0)
+        v = rng.uniform(-0.5, 0.5, 3)
+        args = [n3, n2, *v]
+        _, _, _, P = O.whitebear_phi(*[np.array([a]) for a in args], R, functional="wb2")
+        for i in range(5):
+            h = 1e-6
+            ap, am = list(args), list(args)
+            ap[i] += h
+            am[i] -= h
+            fp = sum(O.whitebear_phi(*[np.array([a]) for a in ap], R, want_partials=False, functional="wb2")[:3])
+            fm = sum(O.whitebear_phi(*[np.array([a]) for a in am], R, want_partials=False, functional="wb2")[:3])
+            assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0]))
+
+
+def test_mark_ii_gradient_and_uniform_fluid():
+    g = O.Grid((16, 16, 20), np.diag([3.2, 3.0, 3.4]))
+    rng = np.random.default_rng(2)
+    vext = O.planar_wall_vext(g, 1.0)
+    f = O.make_fluid(g, 0.35, vext=vext, functional="wb2")
+    f.x += 0.3 * rng.standard_normal(g.shape)
+    E, gr, pg = f.energy_grad_and_precond()
+    d = rng.standard_normal(g.shape)
+    eps = 1e-4
+    fd = (f.energy(f.x + eps * d) - f.energy(f.x - eps * d)) / (2 * eps)
+    assert abs(fd - np.vdot(gr, d)) < 1e-7 * abs(fd)
+    # the wall makes the two functionals differ (vector terms and phi2, phi3)
+    f1 = O.make_fluid(g, 0.35, vext=vext)
+    f1.x[...] = f.x
+    assert abs(f1.energy() / E - 1) > 1e-5
+    u = O.make_fluid(O.Grid.cubic(16, 3.2), 0.4, functional="wb2")
+    Eu, _, pgu = u.energy_grad_and_precond()
+    p = O.pressure(O.bulk_density(0.4, 1.0), 1.0, functional="wb2")
+    assert abs(Eu + p * u.grid.volume) < 1e-12 * p * u.grid.volume
+    assert np.max(np.abs(pgu)) < 1e-12
+
+
+def test_mark_ii_slow_matches_the_cpp_expressions():
+    """HardSpheresWBm2slow (src/HardSpheres.cpp:161-185) written out as the C++ does,
+    phi2_n3 = (1 + (6 - 3n + 6(1-n) ln(1-n)/n)/9)/(1-n),
+    phi3_n3 = (1 - (6 - 9n + 6n^2 + 6(1-n)^2 ln(1-n)/n)/(9n))/(1-n)^2, Phi3 = phi3_n3 VTT/(36 pi),
+    against the oracle's S2/S3 parametrisation; its bulk is not Carnahan-Starling."""
+    for n in (0.07, 0.2, 0.35, 0.5):
+        L = math.log(1 - n)
+        phi2_n3 = (1 + (6 - 3 * n + 6 * (1 - n) * L / n) / 9) / (1 - n)
+        phi3_n3 = (1 - (6 - 9 * n + 6 * n * n + 6 * (1 - n) ** 2 * L / n) / 9 / n) / (1 - n) ** 2
+        g2, _, g3, _ = O.markii_factors(np.array([n]), g3_scale=1.0)
+        assert abs(g2[0] / phi2_n3 - 1) < 1e-13
+        assert abs(g3[0] / phi3_n3 - 1) < 1e-12
+    for eta in (0.1, 0.4):
+        nb = O.bulk_density(eta, 1.0)
+        f_cs = nb * (4 * eta - 3 * eta ** 2) / (1 - eta) ** 2
+        assert abs(O.homogeneous_whitebear(nb, 1.0, functional="wb2slow") / f_cs - 1) > 1e-3
+    rng = np.random.default_rng(6)
+    for _ in range(10):
+        args = [rng.uniform(0.01, 0.5), rng.uniform(0.1, 3.0), *rng.uniform(-0.5, 0.5, 3)]
+        _, _, _, P = O.whitebear_phi(*[np.array([a]) for a in args], 1.0, functional="wb2slow")
+        for i in range(5):
+            h = 1e-6
+            ap, am = list(args), list(args)
+            ap[i] += h
+            am[i] -= h
+            fp = sum(O.whitebear_phi(*[np.array([a]) for a in ap], 1.0, want_partials=False, functional="wb2slow")[:3])
+            fm = sum(O.whitebear_phi(*[np.array([a]) for a in am], 1.0, want_partials=False, functional="wb2slow")[:3])
+            assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0]))

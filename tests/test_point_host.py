@@ -1,0 +1,50 @@
+"""The HIP path's pointwise White-Bear and Mark II (both forms) code (energy density and the five
+partials, deft_amd/csrc/wb_point.hpp) built for the host and checked against the
+oracle on both sides of the series cut-offs (|n3| = 0.05), with no GPU."""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import fmt_oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path_factory.mktemp("point") / "point_harness")
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-x", "hip", "--offload-arch=gfx950", "--cuda-host-only",
+                    os.path.join(HERE, "host", "point_harness.cpp"), "-o", exe], check=True,
+                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return exe
+
+
+def _inputs():
+    rng = np.random.default_rng(9)
+    n3 = np.concatenate([rng.uniform(-0.049, 0.049, 40), rng.uniform(0.051, 0.6, 40),
+                         [0.0, 1e-9, -1e-9, 0.0499999, 0.0500001, -0.0499999, -0.0500001]])
+    m = len(n3)
+    n2 = rng.uniform(0.0, 3.0, m)
+    v = rng.uniform(-0.7, 0.7, (m, 3))
+    return n3, n2, v
+
+
+@pytest.mark.parametrize("functional,code", [("wb", 0), ("wb2", 1), ("wb2slow", 2)])
+def test_point_code_matches_oracle(harness, functional, code):
+    n3, n2, v = _inputs()
+    text = "".join("%d %.17g %.17g %.17g %.17g %.17g\n" % (code, a, b, *c) for a, b, c in zip(n3, n2, v))
+    out = subprocess.run([harness], input=text.encode(), stdout=subprocess.PIPE, check=True).stdout.decode()
+    got = np.array([[float(x) for x in line.split()] for line in out.splitlines()])
+    p1, p2, p3, P = O.whitebear_phi(n3, n2, v[:, 0], v[:, 1], v[:, 2], 1.0, functional=functional)
+    ref = np.stack([p1, p2, p3, *P], axis=1)
+    scale = np.maximum(np.abs(ref), 1e-3)
+    assert np.max(np.abs(got[:, :8] - ref) / scale) < 1e-13
+    # the energy-only function (what every kernel sums) agrees as well
+    assert np.max(np.abs(got[:, 8:11] - ref[:, :3]) / scale[:, :3]) < 1e-13
