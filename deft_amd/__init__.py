@@ -37,6 +37,7 @@ FFT_FP64 = 1
 WHITE_BEAR = 0
 WHITE_BEAR_MARK_II = 1
 WHITE_BEAR_MARK_II_SLOW = 2
+WHITE_BEAR_TENSOR = 3
 
 STOP_REASONS = {
     0: "none", 1: "maxiter", 2: "nan-energy", 3: "infinite-energy", 4: "nan-slope",
@@ -326,7 +327,8 @@ class WhiteBearFluidVeff:
     @property
     def functional(self):
         """WHITE_BEAR (HardSpheresWBnotensor), WHITE_BEAR_MARK_II (WhiteBearMarkII) or
-        WHITE_BEAR_MARK_II_SLOW (HardSpheresWBm2slow: Phi3 over 36 pi, not 24 pi)."""
+        WHITE_BEAR_MARK_II_SLOW (HardSpheresWBm2slow: Phi3 over 36 pi, not 24 pi) or
+        WHITE_BEAR_TENSOR (TensorWhiteBear)."""
         return self._functional
 
     @functional.setter
@@ -398,7 +400,7 @@ class WhiteBearFluidVeff:
         return out
 
     def weighted_densities(self):
-        out = np.empty((5,) + self.shape)
+        out = np.empty((11 if self._functional == WHITE_BEAR_TENSOR else 5,) + self.shape)
         _check(lib().deft_weighted_densities(self._h, _dptr(out)))
         return out
 

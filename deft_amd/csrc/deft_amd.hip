@@ -731,6 +731,8 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
   for (int i = 0; i < n; ++i) {
     if (!plans[i]) return set_error(DEFT_ERR_ARG, "null plan");
     if (plans[i]->batch) return set_error(DEFT_ERR_STATE, "a plan is already in a batch");
+    if (plans[i]->functional == DEFT_WHITE_BEAR_TENSOR)
+      return set_error(DEFT_ERR_UNSUPPORTED, "tensor White-Bear plans are not batched");
     for (int j = 0; j < i; ++j)
       if (plans[j] == plans[i]) return set_error(DEFT_ERR_ARG, "a plan appears twice");
   }
@@ -827,8 +829,27 @@ int deft_plan_set_fft_precision(deft_plan p, int prec) {
 int deft_plan_set_functional(deft_plan p, int functional) {
   TRY(check_plan(p));
   if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
-      functional != DEFT_WHITE_BEAR_MARK_II_SLOW)
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR)
     return set_error(DEFT_ERR_ARG, "unknown functional");
+  if (functional == DEFT_WHITE_BEAR_TENSOR && p->batch)
+    return set_error(DEFT_ERR_STATE, "plans in a batch cannot take the tensor functional");
+  if (functional == DEFT_WHITE_BEAR_TENSOR && p->spec_fields < 12) {
+    // eleven weighted-density spectra plus the density spectrum
+    void *spec = nullptr;
+    const size_t bytes = 12 * sizeof(cx<double>) * (size_t)p->ks;
+    if (hipStreamSynchronize(p->stream) != hipSuccess) return set_error(DEFT_ERR_HIP, "stream synchronisation failed");
+    if (hipMalloc(&spec, bytes) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "hipMalloc of the tensor spectra failed");
+    if (hipMemset(spec, 0, bytes) != hipSuccess) {
+      (void)hipFree(spec);
+      return set_error(DEFT_ERR_HIP, "initialisation failed");
+    }
+    (void)hipFree(p->spec);
+    p->spec = spec;
+    p->spec_fields = 12;
+  }
+  // eleven fields per z line: the smaller z blocks keep the tensor Phi pass at
+  // two waves per SIMD
+  if (functional == DEFT_WHITE_BEAR_TENSOR) p->tz[0] = p->tz[1] = kZBlocks[0];
   p->functional = functional;
   return DEFT_OK;
 }
@@ -956,12 +977,13 @@ int deft_functional_grad(deft_plan p, const double *veff, double *outgrad, doubl
 int deft_weighted_densities(deft_plan p, double *out5) {
   TRY(check_plan(p));
   if (!out5) return set_error(DEFT_ERR_ARG, "null output");
-  TRY(ensure_tmp(p, 5 * sizeof(double) * (size_t)p->N));
+  const int nw = p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5;
+  TRY(ensure_tmp(p, nw * sizeof(double) * (size_t)p->N));
   if (p->prec == DEFT_FFT_FP64)
     TRY(Launch<double>::weighted_densities(p, p->tmp));
   else
     TRY(Launch<float>::weighted_densities(p, p->tmp));
-  DEFT_HIP_CHECK(hipMemcpyAsync(out5, p->tmp, 5 * sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  DEFT_HIP_CHECK(hipMemcpyAsync(out5, p->tmp, nw * sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
   TRY(plan_sync(p));
   return DEFT_OK;
 }
@@ -1037,8 +1059,9 @@ int deft_bulk_free_energy_density(double n, double R, double kT, double *out) {
 
 int deft_bulk_state(int functional, double n, double R, double kT, double *f, double *mu, double *pr) {
   if (!f || !mu || !pr || !(n > 0) || !(R > 0) || !(kT > 0)) return set_error(DEFT_ERR_ARG, "bad bulk arguments");
+  // (the tensor functional's bulk is White-Bear's: T = 0 and n2v = 0 there)
   if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
-      functional != DEFT_WHITE_BEAR_MARK_II_SLOW)
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR)
     return set_error(DEFT_ERR_ARG, "unknown functional");
   double fe, dfe;
   bulk_terms(n, R, kT, &fe, &dfe, functional);

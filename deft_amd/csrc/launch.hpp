@@ -114,9 +114,25 @@ static void launch_ypass(deft_plan_s *p, const GridArgs &g, cx<T> *data, bool in
   });
 }
 
+// Tensor White-Bear plans run eleven weighted fields; their x passes use the
+// first tile size only (fewer instantiations).  The tensor launches are
+// compiled in their own translation units (tensor_f32.hip, tensor_f64.hip).
+inline int plan_fields(const deft_plan_s *p) { return p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5; }
+
+template <typename T>
+struct TensorLaunch {
+  static void xconv_fwd(deft_plan_s *p, const GridArgs &g);
+  static void xconv_bwd(deft_plan_s *p, const GridArgs &g);
+  static void zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb);
+};
+
 template <typename T>
 static void launch_xconv_fwd(deft_plan_s *p, const GridArgs &g, int tb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
+  if (plan_fields(p) == 11) {
+    TensorLaunch<T>::xconv_fwd(p, g);
+    return;
+  }
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
     with_tb(tb, [&](auto TBc) {
@@ -131,6 +147,10 @@ static void launch_xconv_fwd(deft_plan_s *p, const GridArgs &g, int tb) {
 template <typename T>
 static void launch_xconv_bwd(deft_plan_s *p, const GridArgs &g, int tb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
+  if (plan_fields(p) == 11) {
+    TensorLaunch<T>::xconv_bwd(p, g);
+    return;
+  }
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
     with_tb(tb, [&](auto TBc) {
@@ -145,6 +165,10 @@ static void launch_xconv_bwd(deft_plan_s *p, const GridArgs &g, int tb) {
 template <typename T>
 static void launch_zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
+  if (plan_fields(p) == 11) {
+    TensorLaunch<T>::zphi(p, g, grad, zb);
+    return;
+  }
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
     with_zb(zb, [&](auto ZBc) {
@@ -180,7 +204,7 @@ static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
   });
   launch_ypass<T>(p, g, spec_ptr<T>(p, 0), false, 1, p->ty[ti]);
   launch_xconv_fwd<T>(p, g, p->tx[ti]);
-  launch_ypass<T>(p, g, spec_ptr<T>(p, 1), true, 5, p->ty[ti]);
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 1), true, plan_fields(p), p->ty[ti]);
 }
 
 template <typename T>
@@ -192,7 +216,7 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
   launch_density_to_weighted<T>(p, g);
   launch_zphi<T>(p, g, grad, zb);
   if (grad) {
-    launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, 5, p->ty[ti]);
+    launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, plan_fields(p), p->ty[ti]);
     launch_xconv_bwd<T>(p, g, p->tx[ti]);
     launch_ypass<T>(p, g, spec_ptr<T>(p, 0), true, 1, p->ty[ti]);
     with_length(p->Nz, [&](auto LZ) {
@@ -454,7 +478,7 @@ int Launch<T>::weighted_densities(deft_plan_s *p, double *out_dev) {
     constexpr int L = decltype(LZ)::value;
     with_zb(p->tz[ti], [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
-      hipLaunchKernelGGL((k_zinv_plain<T, L, ZB>), dim3(zblocks_for(p, ZB), 5), dim3(kThreads), 0, p->stream, g,
+      hipLaunchKernelGGL((k_zinv_plain<T, L, ZB>), dim3(zblocks_for(p, ZB), plan_fields(p)), dim3(kThreads), 0, p->stream, g,
                          spec_ptr<T>(p, 1), out_dev, p->N, 1.0, tw_ptr<T>(p, 2));
     });
   });

@@ -454,7 +454,7 @@ __device__ __forceinline__ int x_tile_column(const GridArgs& g, int bx, int myc,
 // Weight products of field f (0: w3, 1: w2, 2..4: i w2v_j) for a thread's elements.
 template <typename T, int PT>
 __device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const T (&w3)[PT], const T (&w2)[PT],
-                                            const T (&wk)[PT][3]) {
+                                            const T (&wk)[PT][3], const T (&wt)[PT][6]) {
   switch (f) {
     case 0:
       return scale(s, w3[i]);
@@ -464,15 +464,44 @@ __device__ __forceinline__ cx<T> weight_mul(int f, int i, const cx<T>& s, const 
       return mul_i(scale(s, wk[i][0]));
     case 3:
       return mul_i(scale(s, wk[i][1]));
-    default:
+    case 4:
       return mul_i(scale(s, wk[i][2]));
+    default:
+      return scale(s, wt[i][f - 5]);
   }
+}
+
+// The six shell-tensor weights (xx, yy, zz, xy, yz, zx; scaled by 1/N) of
+// spectral element (x, y, kz) given its scaled w3 and w2.
+template <typename T>
+__device__ __forceinline__ void element_tensor_weights(const GridArgs& g, int x, int y, int kz, T w3, T w2,
+                                                       T (&wt)[6]) {
+  const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+  const T kx = (T)(mx * g.b[0] + my * g.b[3] + mz * g.b[6]);
+  const T ky = (T)(mx * g.b[1] + my * g.b[4] + mz * g.b[7]);
+  const T kz_ = (T)(mx * g.b[2] + my * g.b[5] + mz * g.b[8]);
+  const T k = sqrt(kx * kx + ky * ky + kz_ * kz_);
+  if (k == (T)0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) wt[j] = (T)0;
+    return;
+  }
+  const T a = tensor_amplitude<T>(k, (T)g.R, (T)g.six_dr2, w3, w2, (T)g.invN);
+  const T ik = (T)1 / k;
+  const T hx = kx * ik, hy = ky * ik, hz = kz_ * ik;
+  const T third = (T)(1.0 / 3.0);
+  wt[0] = a * (hx * hx - third);
+  wt[1] = a * (hy * hy - third);
+  wt[2] = a * (hz * hz - third);
+  wt[3] = a * (hx * hy);
+  wt[4] = a * (hy * hz);
+  wt[5] = a * (hz * hx);
 }
 
 // ---------------------------------------------------------------------------
 // x pass with the weight multiply: n~ -> n3, n2, n2v_x, n2v_y, n2v_z spectra.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int TB>
+template <typename T, int L, int TB, int NW = 5>
 __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const cx<T>* __restrict__ in,
                                                cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols<T>(L, TB);
@@ -495,24 +524,29 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   __syncthreads();
   cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
   cx<T>* W1 = (S == A) ? B : A;
-  T w3[PT], w2[PT], wk[PT][3];
+  T w3[PT], w2[PT], wk[PT][3], wt[PT][6];
   {
     const int y = cy, kz = ckz;
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
-      if (e < C * L && myc < ncol) element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) wt[i][j] = (T)0;
+      if (e < C * L && myc < ncol) {
+        element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
+        if constexpr (NW == 11) element_tensor_weights<T>(g, e / C, y, kz, w3[i], w2[i], wt[i]);
+      }
     }
   }
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < NW; ++f) {
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       if (e < C * L) {
         const int xx = e / C;
-        W1[myc * SS + skew<T>(xx)] = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(xx)], w3, w2, wk);
+        W1[myc * SS + skew<T>(xx)] = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(xx)], w3, w2, wk, wt);
       }
     }
     __syncthreads();
@@ -526,10 +560,10 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   }
 }
 
-template <typename T, int L, int TB>
+template <typename T, int L, int TB, int NW = 5>
 __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  xconv_fwd_body<T, L, TB>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
+  xconv_fwd_body<T, L, TB, NW>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void k_xconv_fwd(GridArgs g, const cx<T>*
 // Even weights keep their sign, odd (vector) weights flip it (deft
 // src/Functional.h:407-417).
 // ---------------------------------------------------------------------------
-template <typename T, int L, int TB>
+template <typename T, int L, int TB, int NW = 5>
 __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const cx<T>* __restrict__ in,
                                                cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
   constexpr int C = ccols<T>(L, TB);
@@ -552,7 +586,7 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
   const int ncol = x_tile_column<C>(g, bx, myc, cy, ckz, cok);  // cok == (myc < ncol)
   (void)cok;
   const long long off = (long long)cy * g.ldk + ckz;
-  T w3[PT], w2[PT], wk[PT][3];
+  T w3[PT], w2[PT], wk[PT][3], wt[PT][6];
   cx<T> acc[PT];
   {
     const int y = cy, kz = ckz;
@@ -560,12 +594,17 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) wt[i][j] = (T)0;
       acc[i] = {(T)0, (T)0};
-      if (e < C * L && myc < ncol) element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
+      if (e < C * L && myc < ncol) {
+        element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
+        if constexpr (NW == 11) element_tensor_weights<T>(g, e / C, y, kz, w3[i], w2[i], wt[i]);
+      }
     }
   }
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < NW; ++f) {
     const cx<T>* src = in + f * g.ks;
     for (int e = threadIdx.x; e < C * L; e += kThreads) {
       const int xx = e / C;
@@ -577,8 +616,9 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
       if (e < C * L) {
-        const cx<T> p = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(e / C)], w3, w2, wk);
-        acc[i] = (f < 2) ? acc[i] + p : acc[i] - p;
+        const cx<T> p = weight_mul<T, PT>(f, i, S[myc * SS + skew<T>(e / C)], w3, w2, wk, wt);
+        // even kernels (w3, w2, tensor) are their own adjoints, odd ones (vector) flip sign
+        acc[i] = (f < 2 || f >= 5) ? acc[i] + p : acc[i] - p;
       }
     }
     __syncthreads();
@@ -596,17 +636,17 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
   }
 }
 
-template <typename T, int L, int TB>
+template <typename T, int L, int TB, int NW = 5>
 __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>* __restrict__ in,
                                                         cx<T>* __restrict__ out, const cx<T>* __restrict__ tw) {
-  xconv_bwd_body<T, L, TB>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
+  xconv_bwd_body<T, L, TB, NW>(xcd_pair_tile(blockIdx.x, gridDim.x), g, in, out, tw);
 }
 
 // ---------------------------------------------------------------------------
 // z pass 2: c2r of the five weighted densities -> FP64 Phi (energy partial
 // sums) and, with GRAD, the five partials -> r2c back into the same buffers.
 // ---------------------------------------------------------------------------
-template <typename T, int L, bool GRAD, int ZB>
+template <typename T, int L, bool GRAD, int ZB, int NW = 5>
 __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __restrict__ fields,
                                           double* __restrict__ partial, int pstride, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
@@ -617,9 +657,9 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
-  T val[5][PT];
+  T val[NW][PT];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < NW; ++f) {
     load_c2r_packed<T, L, P>(fields + f * g.ks, line0, lines, g.ldk, A, B);
     const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
 #pragma unroll
@@ -640,7 +680,25 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
     const int e = threadIdx.x + i * kThreads;
     const bool valid = (e < NPT) && (line0 + e / L < lines);
     double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-    if constexpr (GRAD) {
+    if constexpr (NW == 11) {
+      double v[11];
+#pragma unroll
+      for (int f = 0; f < 11; ++f) v[f] = (double)val[f][i];
+      if constexpr (GRAD) {
+        double dd[11];
+#pragma unroll
+        for (int f = 0; f < 11; ++f) dd[f] = 0.0;
+        if (valid) {
+          double q1, q2, q3;
+          wbt_phi(v, g.wb, p1, p2, p3);
+          wbt_phi_partials(v, g.wb, q1, q2, q3, dd);
+        }
+#pragma unroll
+        for (int f = 0; f < 11; ++f) val[f][i] = (T)dd[f];
+      } else {
+        if (valid) wbt_phi(v, g.wb, p1, p2, p3);
+      }
+    } else if constexpr (GRAD) {
       double dd[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (valid) {
         // the energy terms come from wb_phi, as in the energy-only variant
@@ -663,7 +721,7 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   }
   if constexpr (GRAD) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
+    for (int f = 0; f < NW; ++f) {
 #pragma unroll
       for (int i = 0; i < PT; ++i) {
         const int e = threadIdx.x + i * kThreads;
@@ -685,11 +743,11 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, bx);
 }
 
-template <typename T, int L, bool GRAD, int ZB>
+template <typename T, int L, bool GRAD, int ZB, int NW = 5>
 __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
                                                    double* __restrict__ partial, int pstride,
                                                    const cx<T>* __restrict__ tw) {
-  zphi_body<T, L, GRAD, ZB>(blockIdx.x, g, fields, partial, pstride, tw);
+  zphi_body<T, L, GRAD, ZB, NW>(blockIdx.x, g, fields, partial, pstride, tw);
 }
 
 // ---------------------------------------------------------------------------

@@ -83,7 +83,8 @@ struct deft_plan_s {
   // gradient, preconditioned gradient, previous gradient
   double *x = nullptr, *d = nullptr, *g = nullptr, *pg = nullptr, *og = nullptr;
   float *vext = nullptr;   // FP32, +inf marks hard walls
-  void *spec = nullptr;    // 6 half spectra of Nk complex (sized for FP64)
+  void *spec = nullptr;    // spec_fields half spectra of ks complex values (sized for FP64)
+  int spec_fields = 6;     // 6, or 12 once the tensor functional was chosen
   double *partial = nullptr;
   int pstride = 0;         // partial-sum slot stride (>= blocks of any reducing kernel)
   unsigned long long *hflag = nullptr;  // completion flag, host-mapped pinned

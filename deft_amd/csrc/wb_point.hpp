@@ -26,6 +26,13 @@
 // is not Carnahan-Starling and it lacks Rosenfeld's low-density limit, but the
 // reference's published Mark II profiles (papers/contact/figs/
 // inner-sphereWBm2-*.dat) were computed with it; it is kept as a third choice.
+//
+// Tensor White-Bear (TensorWhiteBear, src/haskell/WhiteBear.hs:44-51,
+// src/HardSpheres.cpp:30-67): Phi1, Phi2 as White-Bear and
+// Phi3 = f3/(36 pi) X, X = n2^3 - 3 n2 n2v.n2v + 9 (n2v.T.n2v - tr(T^3)/2),
+// T the traceless shell tensor density; eleven fields (n3, n2, n2v_xyz,
+// T_xx, T_yy, T_zz, T_xy, T_yz, T_zx), off-diagonal partials counting both
+// symmetric entries.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,14 +44,17 @@ struct WBConst {
   double fpR2;      // 4 pi R^2
   double inv_fpR;   // 1/(4 pi R)
   double inv_fpR2;  // 1/(4 pi R^2)
-  int mark2;        // 0: White-Bear, 1: Mark II, 2: Mark II with the 1/(36 pi) Phi3
+  int mark2;        // 0: White-Bear (or tensor), 1: Mark II, 2: Mark II with the 1/(36 pi) Phi3
   double g3_scale;  // Mark II: g3 = g3_scale (1 - 4 S3/9)/(1-n)^2 (3/2, or 1 for mark2 = 2)
+  double R;
 };
 
-inline WBConst make_wb_const(double R, int mark2 = 0) {
+// functional: 0 White-Bear, 1 Mark II, 2 Mark II (36 pi form), 3 tensor White-Bear
+inline WBConst make_wb_const(double R, int functional = 0) {
   const double pi = 3.14159265358979323846;
+  const int mark2 = (functional == 1 || functional == 2) ? functional : 0;
   return WBConst{4 * pi * R, 4 * pi * R * R, 1.0 / (4 * pi * R), 1.0 / (4 * pi * R * R), mark2,
-                 mark2 == 2 ? 1.0 : 1.5};
+                 mark2 == 2 ? 1.0 : 1.5, R};
 }
 
 constexpr double kWB2SeriesCutoff = 0.05;
@@ -182,6 +192,97 @@ __host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx,
   d[2] = cv * vx;
   d[3] = cv * vy;
   d[4] = cv * vz;
+}
+
+// Tensor White-Bear energy density terms; v[11] = n3, n2, n2v_x/y/z, T_xx, T_yy, T_zz, T_xy, T_yz, T_zx.
+__host__ __device__ inline void wbt_phi(const double* v, const WBConst& c, double& p1, double& p2, double& p3) {
+#pragma clang fp contract(off)
+  const double n3 = v[0], n2 = v[1], vx = v[2], vy = v[3], vz = v[4];
+  const double txx = v[5], tyy = v[6], tzz = v[7], txy = v[8], tyz = v[9], tzx = v[10];
+  const double om = 1.0 - n3;
+  const double io = 1.0 / om;
+  const double L = log1p(-n3);
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  double f3, df3;
+  f3_of(n3, om, io, L, f3, df3);
+  const double tvx = txx * vx + txy * vy + tzx * vz, tvy = txy * vx + tyy * vy + tyz * vz,
+               tvz = tzx * vx + tyz * vy + tzz * vz;
+  const double vtv = vx * tvx + vy * tvy + vz * tvz;
+  const double tr3 = 6.0 * txy * tyz * tzx + txx * (txx * txx + 3.0 * txy * txy + 3.0 * tzx * tzx) +
+                     tyy * (3.0 * txy * txy + tyy * tyy + 3.0 * tyz * tyz) +
+                     tzz * (3.0 * tzx * tzx + 3.0 * tyz * tyz + tzz * tzz);
+  const double X = n2 * n2 * n2 - 3.0 * n2 * v2 + 9.0 * (vtv - 0.5 * tr3);
+  p1 = -n2 * L * c.inv_fpR2;
+  p2 = (n2 * n2 - v2) * c.inv_fpR * io;
+  p3 = f3 * X * kInv36Pi;
+}
+
+// Tensor White-Bear terms and the eleven partials.
+__host__ __device__ inline void wbt_phi_partials(const double* v, const WBConst& c, double& p1, double& p2,
+                                                 double& p3, double* d) {
+  const double n3 = v[0], n2 = v[1], vx = v[2], vy = v[3], vz = v[4];
+  const double txx = v[5], tyy = v[6], tzz = v[7], txy = v[8], tyz = v[9], tzx = v[10];
+  const double om = 1.0 - n3;
+  const double io = 1.0 / om;
+  const double L = log1p(-n3);
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  double f3, df3;
+  f3_of(n3, om, io, L, f3, df3);
+  const double tvx = txx * vx + txy * vy + tzx * vz, tvy = txy * vx + tyy * vy + tyz * vz,
+               tvz = tzx * vx + tyz * vy + tzz * vz;
+  const double vtv = vx * tvx + vy * tvy + vz * tvz;
+  const double tr3 = 6.0 * txy * tyz * tzx + txx * (txx * txx + 3.0 * txy * txy + 3.0 * tzx * tzx) +
+                     tyy * (3.0 * txy * txy + tyy * tyy + 3.0 * tyz * tyz) +
+                     tzz * (3.0 * tzx * tzx + 3.0 * tyz * tyz + tzz * tzz);
+  const double X = n2 * n2 * n2 - 3.0 * n2 * v2 + 9.0 * (vtv - 0.5 * tr3);
+  const double q = n2 * n2 - v2;
+  const double a = c.inv_fpR * io;
+  p1 = -n2 * L * c.inv_fpR2;
+  p2 = q * a;
+  p3 = f3 * X * kInv36Pi;
+  const double cf = f3 * kInv36Pi;
+  d[0] = n2 * c.inv_fpR2 * io + q * a * io + df3 * X * kInv36Pi;
+  d[1] = -L * c.inv_fpR2 + 2.0 * n2 * a + cf * 3.0 * q;
+  d[2] = -2.0 * a * vx + cf * (18.0 * tvx - 6.0 * n2 * vx);
+  d[3] = -2.0 * a * vy + cf * (18.0 * tvy - 6.0 * n2 * vy);
+  d[4] = -2.0 * a * vz + cf * (18.0 * tvz - 6.0 * n2 * vz);
+  // (T^2) components
+  const double sxx = txx * txx + txy * txy + tzx * tzx, syy = txy * txy + tyy * tyy + tyz * tyz,
+               szz = tzx * tzx + tyz * tyz + tzz * tzz;
+  const double sxy = txx * txy + txy * tyy + tzx * tyz, syz = txy * tzx + tyy * tyz + tyz * tzz,
+               szx = txx * tzx + txy * tyz + tzx * tzz;
+  d[5] = cf * (9.0 * vx * vx - 13.5 * sxx);
+  d[6] = cf * (9.0 * vy * vy - 13.5 * syy);
+  d[7] = cf * (9.0 * vz * vz - 13.5 * szz);
+  d[8] = cf * (18.0 * vx * vy - 27.0 * sxy);
+  d[9] = cf * (18.0 * vy * vz - 27.0 * syz);
+  d[10] = cf * (18.0 * vz * vx - 27.0 * szx);
+}
+
+// Amplitude a of the traceless shell tensor weight W_ij = a (k^_i k^_j - delta_ij/3)
+// (src/haskell/FMT.hs:82-88, src/ReciprocalOperators.h:292-404):
+// a = s 4 pi R^2 A(kR), A(x) = ((x^2 - 3) sin x/x + 3 cos x)/x^2, which equals
+// w2 - 3 w3/R from the scalar weights; below kR = 1 (where that difference
+// cancels) the series A = sum_{m>=2} (-1)^(m+1) 4 m (m-1) x^(2m-2)/(2m+1)!.
+// w3, w2 and the result carry the same 1/N factor `scale`.
+constexpr double kTensorSeriesCutoff = 1.0;
+template <typename T>
+__device__ __forceinline__ T tensor_amplitude(T k, T R, T six_dr2, T w3, T w2, T scale) {
+  const T x = k * R;
+  if (x >= (T)kTensorSeriesCutoff) return w2 - (T)3 * w3 / R;
+  const T x2 = x * x;
+  // terms m = 2..13, Horner in x^2: coefficients c_m = (-1)^(m+1) 4 m (m-1)/(2m+1)!
+  T A = (T)0;
+#pragma unroll
+  for (int m = 13; m >= 2; --m) {
+    double f = 1.0;
+    for (int i = 2; i <= 2 * m + 1; ++i) f *= i;
+    const double cm = ((m & 1) ? 1.0 : -1.0) * 4.0 * m * (m - 1) / f;
+    A = A * x2 + (T)cm;
+  }
+  A *= x2;  // the m = 2 term is -x^2/15
+  const T s = exp(-six_dr2 * k * k);
+  return s * (T)(4.0 * 3.14159265358979323846) * R * R * A * scale;
 }
 
 // Fourier transforms of the fundamental-measure weights at wave vector k

@@ -63,7 +63,11 @@ typedef enum {
 typedef enum {
   DEFT_WHITE_BEAR = 0,
   DEFT_WHITE_BEAR_MARK_II = 1,
-  DEFT_WHITE_BEAR_MARK_II_SLOW = 2
+  DEFT_WHITE_BEAR_MARK_II_SLOW = 2,
+  /* TensorWhiteBear (src/haskell/WhiteBear.hs:44-51, src/HardSpheres.cpp:30-67):
+   * Phi3 with the shell tensor density (six more weighted fields); same bulk as
+   * White-Bear; not available in batches */
+  DEFT_WHITE_BEAR_TENSOR = 3
 } deft_functional_kind;
 
 /* Why a minimizer stopped (mirrors the return-false branches of
@@ -164,7 +168,8 @@ int deft_functional_grad(deft_plan plan, const double *veff, double *outgrad, do
 int deft_functional_grad_device(deft_plan plan, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
                                 double *energy);
 /* The five weighted densities n3, n2, n2v_x, n2v_y, n2v_z of the resident Veff
- * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R). */
+ * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R).
+ * Tensor White-Bear plans add the shell tensor T_xx, T_yy, T_zz, T_xy, T_yz, T_zx (11*N). */
 int deft_weighted_densities(deft_plan plan, double *out5);
 
 /* --- FFT seam --------------------------------------------------------------

@@ -20,6 +20,10 @@ It restates, in numpy/scipy FP64, the reference algorithm of deft
 * the White-Bear excess free energy density without tensor
   (Phi1 + Phi2 + Phi3)                                          (src/HardSpheres.cpp:137-159, src/haskell/WhiteBear.hs:47-60)
   and its closed-form partial derivatives                       (src/haskell/WhiteBear.hs:114-121)
+* White-Bear Mark II in both of the reference's forms           (src/haskell/WhiteBear.hs:140-159, src/HardSpheres.cpp:161-185)
+* tensor White-Bear: the traceless shell tensor weight and the
+  tensor third term                                             (src/haskell/FMT.hs:82-88, src/ReciprocalOperators.h:292-404,
+                                                                 src/haskell/WhiteBear.hs:44-51, src/HardSpheres.cpp:30-67)
 * the ideal gas with the water-mass thermal density nQ           (src/IdealGas.cpp:54-72, src/haskell/IdealGas.hs:7-20)
 * the external potential and chemical potential terms           (src/ExternalPotential.cpp:19-21, src/ChemicalPotential.cpp:19-22)
 * the effective-potential variable n = exp(-Veff/kT), its chain
@@ -153,6 +157,42 @@ def weights(grid, R):
                   4 * math.pi * R ** 3 * (kR2 ** 2 * (1.0 / 5040 - 1.0 / 720) + kR2 * (1.0 / 24 - 1.0 / 120) - 1.0 / 3),
                   s * 4 * math.pi * (R * np.cos(kRs) - np.sin(kRs) / ks) / ks ** 2)
     return w3, w2, wv
+
+
+TENSOR_SERIES_CUTOFF = 1.0
+TENSOR_SERIES_TERMS = 12
+TENSOR_COMPONENTS = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0))  # xx, yy, zz, xy, yz, zx
+
+
+def tensor_weights(grid, R):
+    """The shell tensor weight of src/haskell/FMT.hs:82-88 (tshell), equal to the C++
+    xx/yy/zz/xy/yz/zxshell_op of src/ReciprocalOperators.h:292-404:
+      W_ij(k) = s 4 pi [((kR^2 - 3) sin kR/kR + 3 cos kR)/k^4 k_i k_j
+                        + (sin kR/kR - kR sin kR/3 - cos kR)/k^2 delta_ij].
+    The two brackets are A(kR) R^2 k^_i k^_j and -A(kR) R^2 delta_ij / 3 with
+    A(x) = ((x^2 - 3) sin x/x + 3 cos x)/x^2, so W is traceless:
+      W_ij = s 4 pi R^2 A(kR) (k^_i k^_j - delta_ij/3),
+    and s 4 pi R^2 A(kR) = w2(k) - 3 w3(k)/R.  For kR < 1 A uses its series
+    A(x) = sum_{m>=2} (-1)^(m+1) 4 m (m-1) x^(2m-2)/(2m+1)!  (closed form cancels);
+    W = 0 at k = 0.  Returns the six components in TENSOR_COMPONENTS order."""
+    k = grid.k
+    dr = grid.dV ** (1.0 / 3.0)
+    x = k * R
+    small = x < TENSOR_SERIES_CUTOFF
+    xb = np.where(small, 1.5, x)
+    A_closed = ((xb * xb - 3) * np.sin(xb) / xb + 3 * np.cos(xb)) / (xb * xb)
+    xs = np.where(small, x, 0.0)
+    A_ser = np.zeros_like(xs)
+    for m in range(2, TENSOR_SERIES_TERMS + 2):
+        A_ser += (-1) ** (m + 1) * 4 * m * (m - 1) * xs ** (2 * m - 2) / math.factorial(2 * m + 1)
+    A = np.where(small, A_ser, A_closed)
+    amp = np.exp(-SPREADING * k * k * dr * dr) * 4 * math.pi * R * R * A
+    ks = np.where(k > 0, k, 1.0)
+    kh = (grid.kx / ks, grid.ky / ks, grid.kz / ks)
+    out = []
+    for i, j in TENSOR_COMPONENTS:
+        out.append(np.where(k > 0, amp * (kh[i] * kh[j] - (1.0 / 3.0 if i == j else 0.0)), 0.0))
+    return out
 
 
 def fft(grid, f):
@@ -297,12 +337,48 @@ def whitebear_phi(n3, n2, vx, vy, vz, R, want_partials=True, functional="wb"):
 # Homogeneous path: free energy per volume, chemical potential, pressure
 # ---------------------------------------------------------------------------
 
+def whitebear_tensor_phi(n3, n2, vx, vy, vz, txx, tyy, tzz, txy, tyz, tzx, R, want_partials=True):
+    """Tensor White-Bear (TensorWhiteBear, src/haskell/WhiteBear.hs:44-51,
+    src/HardSpheres.cpp:30-67): Phi1 and Phi2 as White-Bear,
+    Phi3t = f3(n3)/(36 pi) X,  X = n2^3 - 3 n2 n2v.n2v + 9 (n2v.T.n2v - tr(T^3)/2)
+    with T the (traceless) shell tensor density.  Partials with respect to the 11 fields
+    (n3, n2, n2v_x, n2v_y, n2v_z, T_xx, T_yy, T_zz, T_xy, T_yz, T_zx), each off-diagonal
+    component standing for both of its symmetric entries."""
+    fpR = 4 * math.pi * R
+    fpR2 = 4 * math.pi * R * R
+    omn = 1.0 - n3
+    L = np.log1p(-n3)
+    v2 = vx * vx + vy * vy + vz * vz
+    f3, df3 = f3_and_derivative(n3)
+    Tv = (txx * vx + txy * vy + tzx * vz, txy * vx + tyy * vy + tyz * vz, tzx * vx + tyz * vy + tzz * vz)
+    vTv = vx * Tv[0] + vy * Tv[1] + vz * Tv[2]
+    tr3 = (6 * txy * tyz * tzx + txx * (txx * txx + 3 * txy * txy + 3 * tzx * tzx)
+           + tyy * (3 * txy * txy + tyy * tyy + 3 * tyz * tyz) + tzz * (3 * tzx * tzx + 3 * tyz * tyz + tzz * tzz))
+    X = n2 * n2 * n2 - 3 * n2 * v2 + 9 * (vTv - tr3 / 2)
+    phi1 = -n2 * L / fpR2
+    phi2 = (n2 * n2 - v2) / (fpR * omn)
+    phi3 = f3 * X / (36 * math.pi)
+    if not want_partials:
+        return phi1, phi2, phi3, None
+    c = f3 / (36 * math.pi)
+    d3 = n2 / (fpR2 * omn) + (n2 * n2 - v2) / (fpR * omn * omn) + df3 * X / (36 * math.pi)
+    d2 = -L / fpR2 + 2.0 * n2 / (fpR * omn) + c * (3 * n2 * n2 - 3 * v2)
+    dv = [-2.0 * v / (fpR * omn) + c * (-6 * n2 * v + 18 * t) for v, t in zip((vx, vy, vz), Tv)]
+    sq = (txx * txx + txy * txy + tzx * tzx, txy * txy + tyy * tyy + tyz * tyz, tzx * tzx + tyz * tyz + tzz * tzz,
+          txx * txy + txy * tyy + tzx * tyz, txy * tzx + tyy * tyz + tyz * tzz, txx * tzx + txy * tyz + tzx * tzz)
+    dt = [c * (9 * vx * vx - 13.5 * sq[0]), c * (9 * vy * vy - 13.5 * sq[1]), c * (9 * vz * vz - 13.5 * sq[2]),
+          c * (18 * vx * vy - 27 * sq[3]), c * (18 * vy * vz - 27 * sq[4]), c * (18 * vz * vx - 27 * sq[5])]
+    return phi1, phi2, phi3, (d3, d2, *dv, *dt)
+
+
 def homogeneous_whitebear(n, R, kT=1.0, functional="wb"):
     """Excess free energy per volume kT Phi(n) of a uniform fluid (k=0 weights, src/Functional.h:382-392)."""
     n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
     n2 = np.float64(4 * math.pi * R * R * n)
     z = np.float64(0.0)
-    p1, p2, p3, _ = whitebear_phi(n3, n2, z, z, z, R, want_partials=False, functional=functional)
+    # tensor White-Bear reduces to White-Bear in bulk (T = 0, n2v = 0)
+    fn = "wb" if functional == "wbt" else functional
+    p1, p2, p3, _ = whitebear_phi(n3, n2, z, z, z, R, want_partials=False, functional=fn)
     return float(kT * (p1 + p2 + p3))
 
 
@@ -311,7 +387,8 @@ def homogeneous_whitebear_dn(n, R, kT=1.0, functional="wb"):
     n3 = np.float64(4 * math.pi / 3 * R ** 3 * n)
     n2 = np.float64(4 * math.pi * R * R * n)
     z = np.float64(0.0)
-    _, _, _, d = whitebear_phi(n3, n2, z, z, z, R, functional=functional)
+    fn = "wb" if functional == "wbt" else functional
+    _, _, _, d = whitebear_phi(n3, n2, z, z, z, R, functional=fn)
     return float(kT * (d[0] * (4 * math.pi / 3 * R ** 3) + d[1] * (4 * math.pi * R * R)))
 
 
@@ -357,6 +434,9 @@ class WhiteBearFluidVeff:
         w3, w2, wv = weights(grid, self.R)
         self.W = (w3, w2, 1j * wv * grid.kx, 1j * wv * grid.ky, 1j * wv * grid.kz)
         self.parity = (1.0, 1.0, -1.0, -1.0, -1.0)  # w(-k) = +/- w(k) (src/Functional.h:407-417)
+        if functional == "wbt":
+            self.W = self.W + tuple(tensor_weights(grid, self.R))
+            self.parity = self.parity + (1.0,) * 6
         self.last_terms = {}
         self.n_energy = 0
         self.n_grad = 0
@@ -364,6 +444,11 @@ class WhiteBearFluidVeff:
     def density(self, x=None):
         x = self.x if x is None else x
         return np.where(self.finite, np.exp(-x / self.kT), 0.0)
+
+    def _phi(self, nal, partials):
+        if self.functional == "wbt":
+            return whitebear_tensor_phi(*nal, self.R, want_partials=partials)
+        return whitebear_phi(*nal, self.R, want_partials=partials, functional=self.functional)
 
     def weighted_densities(self, n):
         nk = _rfftn(n)
@@ -381,7 +466,7 @@ class WhiteBearFluidVeff:
         x = self.x if x is None else x
         n = self.density(x)
         nal = self.weighted_densities(n)
-        p1, p2, p3, _ = whitebear_phi(*nal, self.R, want_partials=False, functional=self.functional)
+        p1, p2, p3, _ = self._phi(nal, False)
         ideal, ext, mun = self._local_terms(x, n)
         dV = self.grid.dV
         kT = self.kT
@@ -402,7 +487,7 @@ class WhiteBearFluidVeff:
         x = self.x if x is None else x
         n = self.density(x)
         nal = self.weighted_densities(n)
-        _, _, _, P = whitebear_phi(*nal, self.R, functional=self.functional)
+        _, _, _, P = self._phi(nal, True)
         acc = None
         for Pa, W, par in zip(P, self.W, self.parity):
             term = _rfftn(Pa) * (W * par)

@@ -10,7 +10,8 @@ src/GridDescription.cpp:28), R = 1, kT = 1, mu from the bulk, Veff started at
 -ln(n_b c + 1e-4 n_b), old-API preconditioned CG to |E_bulk V| 1e-10 for at most 200
 iterations.  Column 0 is y of grid point (0, y, 0), column 1 the density there
 (the remaining columns are contact-value estimators outside this path).
-"WB" is HardSpheresNoTensor2 (White Bear), "WBm2" WhiteBearMarkII.
+"WB" is HardSpheresNoTensor2 (White Bear), "WBT" TensorWhiteBear (files at eta 0.30
+and 0.40 only), "WBm2" WhiteBearMarkII.
 Only those two columns, as printed (%g), are kept.
 """
 
@@ -23,9 +24,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def main():
     out = {"diameter": 4.0, "cell": 20.0, "n": 201, "R": 1.0, "kT": 1.0, "profiles": {}}
-    for name in ("WB", "WBm2"):
+    for name in ("WB", "WBT", "WBm2"):
         for eta in ("0.10", "0.20", "0.30", "0.40", "0.50"):
             path = os.path.join(REF, "inner-sphere%s-04.0-%s.dat" % (name, eta))
+            if not os.path.exists(path):
+                continue
             y, n = [], []
             with open(path) as fh:
                 for line in fh:

@@ -409,3 +409,107 @@ def test_mark_ii_slow_matches_the_cpp_expressions():
             fp = sum(O.whitebear_phi(*[np.array([a]) for a in ap], 1.0, want_partials=False, functional="wb2slow")[:3])
             fm = sum(O.whitebear_phi(*[np.array([a]) for a in am], 1.0, want_partials=False, functional="wb2slow")[:3])
             assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0]))
+
+
+# ---------------------------------------------------------------------------
+# Tensor White-Bear (src/haskell/WhiteBear.hs:44-51, src/HardSpheres.cpp:30-67)
+# ---------------------------------------------------------------------------
+
+def _cpp_tensor_op(kvec, R, dr, i, j):
+    """xxshell_op / xyshell_op ... of src/ReciprocalOperators.h:292-404, written out as the C++ does."""
+    k = math.sqrt(sum(c * c for c in kvec))
+    kR = k * R
+    s = math.exp(-6.0 * k * k * dr * dr)
+    if i == j:
+        cos2 = kvec[i] * kvec[i] / (k * k)
+        return s * (4 * math.pi * R * R) * (math.sin(kR) / kR * (-1.0 / 3 + cos2 - 3 * cos2 / (kR * kR) + 1 / (kR * kR))
+                                             + math.cos(kR) / (kR * kR) * (-1 + 3 * cos2))
+    return -s * (4 * math.pi) * kvec[i] * kvec[j] / k ** 4 * ((3 - kR * kR) * math.sin(kR) / kR - 3 * math.cos(kR))
+
+
+def test_tensor_weights_match_the_reference_operators():
+    g = O.Grid((12, 10, 14), np.array([[3.0, 0.0, 0.0], [0.4, 2.6, 0.0], [0.0, 0.3, 3.3]]))
+    R = 1.1
+    T = O.tensor_weights(g, R)
+    dr = g.dV ** (1.0 / 3.0)
+    rng = np.random.default_rng(3)
+    picks = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, 12, 40), rng.integers(0, 10, 40),
+                                                           rng.integers(0, 8, 40))]
+    for (a, b, c) in picks:
+        kv = (g.kx[a, b, c], g.ky[a, b, c], g.kz[a, b, c])
+        if g.k[a, b, c] == 0:
+            assert all(t[a, b, c] == 0 for t in T)
+            continue
+        # the closed form cancels at small kR: compare at the accuracy it keeps
+        tol = 1e-12 if g.k[a, b, c] * R > 1 else 1e-12 / (g.k[a, b, c] * R) ** 4
+        for t, (i, j) in zip(T, O.TENSOR_COMPONENTS):
+            ref = _cpp_tensor_op(kv, R, dr, i, j)
+            assert abs(t[a, b, c] - ref) < tol * 4 * math.pi * R * R, (a, b, c, i, j)
+    # traceless, and the amplitude identity s 4 pi R^2 A = w2 - 3 w3 / R
+    assert np.max(np.abs(T[0] + T[1] + T[2])) < 1e-14
+    w3, w2, _ = O.weights(g, R)
+    kh2 = np.where(g.k > 0, (g.kx / np.where(g.k > 0, g.k, 1)) ** 2 - 1.0 / 3.0, 0.0)
+    big = g.k * R > 1
+    assert np.allclose(T[0][big], ((w2 - 3 * w3 / R) * kh2)[big], rtol=1e-12, atol=1e-14)
+
+
+def test_tensor_weight_is_the_shell_tensor_in_real_space():
+    """Along x the planar projection of delta(R - r)(x^2/r^2 - 1/3) is 2 pi R (x^2/R^2 - 1/3) on |x| < R;
+    its cosine transform (quadrature, no formula shared with the oracle) is W_xx / smear."""
+    from scipy.integrate import quad
+    R = 0.9
+    g = O.Grid((40, 8, 8), np.diag([12.0, 3.0, 3.0]))
+    T = O.tensor_weights(g, R)
+    dr = g.dV ** (1.0 / 3.0)
+    for m in (1, 2, 3, 7, 13):
+        k = g.kx[m, 0, 0]
+        F = quad(lambda x: 2 * math.pi * R * (x * x / (R * R) - 1.0 / 3.0) * math.cos(k * x), -R, R,
+                 epsabs=1e-12, epsrel=1e-12, limit=200)[0]
+        s = math.exp(-6.0 * k * k * dr * dr)
+        assert abs(T[0][m, 0, 0] / s - F) < 1e-10 * 4 * math.pi * R * R, m
+        assert abs(T[1][m, 0, 0] + 0.5 * T[0][m, 0, 0]) < 1e-15 and T[3][m, 0, 0] == 0
+
+
+def test_tensor_phi_partials_and_limits():
+    rng = np.random.default_rng(8)
+    R = 1.0
+    for _ in range(15):
+        args = [rng.uniform(-0.03, 0.5), rng.uniform(0.1, 3.0), *rng.uniform(-0.5, 0.5, 3), *rng.uniform(-0.4, 0.4, 6)]
+        _, _, _, P = O.whitebear_tensor_phi(*[np.array([a]) for a in args], R)
+        for i in range(11):
+            h = 1e-6
+            ap, am = list(args), list(args)
+            ap[i] += h
+            am[i] -= h
+            fp = sum(O.whitebear_tensor_phi(*[np.array([a]) for a in ap], R, want_partials=False)[:3])
+            fm = sum(O.whitebear_tensor_phi(*[np.array([a]) for a in am], R, want_partials=False)[:3])
+            assert abs((fp - fm)[0] / (2 * h) - P[i][0]) < 1e-6 * max(1, abs(P[i][0])), i
+        # with T = 0 the tensor functional is White-Bear
+        z = [np.array([a]) for a in args[:5]] + [np.zeros(1)] * 6
+        assert np.allclose(O.whitebear_tensor_phi(*z, R)[:3], O.whitebear_phi(*z[:5], R)[:3], rtol=1e-15, atol=0)
+    # a uniaxial tensor aligned with n2v (planar symmetry): X = n2^3 - 3 n2 v^2 + 9 (v^2 t - 3 t^3/4) for
+    # T = t diag(1, -1/2, -1/2), v = (v, 0, 0): tr T^3 = t^3 (1 - 1/8 - 1/8) = 3 t^3 / 4
+    n2, v, t = 1.3, 0.4, 0.2
+    X = n2 ** 3 - 3 * n2 * v * v + 9 * (v * v * t - 3 * t ** 3 / 8)
+    n3 = 0.3
+    f3 = O.f3_and_derivative(np.array([n3]))[0][0]
+    a = [np.array([q]) for q in (n3, n2, v, 0.0, 0.0, t, -t / 2, -t / 2, 0.0, 0.0, 0.0)]
+    assert abs(O.whitebear_tensor_phi(*a, 1.0)[2][0] - f3 * X / (36 * math.pi)) < 1e-15
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 20), (15, 21, 17)])
+def test_tensor_gradient_and_uniform_fluid(shape):
+    g = O.Grid(shape, np.diag([3.2, 3.0, 3.4]))
+    rng = np.random.default_rng(12)
+    f = O.make_fluid(g, 0.35, vext=O.planar_wall_vext(g, 1.0), functional="wbt")
+    f.x += 0.3 * rng.standard_normal(g.shape)
+    E, gr, pg = f.energy_grad_and_precond()
+    d = rng.standard_normal(g.shape)
+    eps = 1e-4
+    fd = (f.energy(f.x + eps * d) - f.energy(f.x - eps * d)) / (2 * eps)
+    assert abs(fd - np.vdot(gr, d)) < 1e-7 * abs(fd)
+    u = O.make_fluid(O.Grid.cubic(16, 3.2), 0.4, functional="wbt")
+    Eu, _, pgu = u.energy_grad_and_precond()
+    p = O.pressure(O.bulk_density(0.4, 1.0), 1.0)
+    assert abs(Eu + p * u.grid.volume) < 1e-12 * p * u.grid.volume
+    assert np.max(np.abs(pgu)) < 1e-12

@@ -1,4 +1,4 @@
-"""The HIP path's pointwise White-Bear and Mark II (both forms) code (energy density and the five
+"""The HIP path's pointwise White-Bear, Mark II (both forms) and tensor White-Bear code (energy density and the five
 partials, deft_amd/csrc/wb_point.hpp) built for the host and checked against the
 oracle on both sides of the series cut-offs (|n3| = 0.05), with no GPU."""
 
@@ -43,8 +43,22 @@ def test_point_code_matches_oracle(harness, functional, code):
     out = subprocess.run([harness], input=text.encode(), stdout=subprocess.PIPE, check=True).stdout.decode()
     got = np.array([[float(x) for x in line.split()] for line in out.splitlines()])
     p1, p2, p3, P = O.whitebear_phi(n3, n2, v[:, 0], v[:, 1], v[:, 2], 1.0, functional=functional)
-    ref = np.stack([p1, p2, p3, *P], axis=1)
+    _check(got, np.stack([p1, p2, p3, *P], axis=1))
+
+
+def _check(got, ref):
     scale = np.maximum(np.abs(ref), 1e-3)
-    assert np.max(np.abs(got[:, :8] - ref) / scale) < 1e-13
+    nref = ref.shape[1]
+    assert np.max(np.abs(got[:, :nref] - ref) / scale) < 1e-13
     # the energy-only function (what every kernel sums) agrees as well
-    assert np.max(np.abs(got[:, 8:11] - ref[:, :3]) / scale[:, :3]) < 1e-13
+    assert np.max(np.abs(got[:, nref:nref + 3] - ref[:, :3]) / scale[:, :3]) < 1e-13
+
+
+def test_tensor_point_code_matches_oracle(harness):
+    n3, n2, v = _inputs()
+    t = np.random.default_rng(10).uniform(-0.4, 0.4, (len(n3), 6))
+    text = "".join("3 " + " ".join("%.17g" % q for q in (a, b, *c, *d)) + "\n" for a, b, c, d in zip(n3, n2, v, t))
+    out = subprocess.run([harness], input=text.encode(), stdout=subprocess.PIPE, check=True).stdout.decode()
+    got = np.array([[float(x) for x in line.split()] for line in out.splitlines()])
+    p1, p2, p3, P = O.whitebear_tensor_phi(n3, n2, v[:, 0], v[:, 1], v[:, 2], *t.T, 1.0)
+    _check(got, np.stack([p1, p2, p3, *P], axis=1))
