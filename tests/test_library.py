@@ -36,7 +36,7 @@ def test_bulk_path_matches_oracle(eta):
     assert abs(deft_amd.bulk_free_energy_density(n) / f - 1) < 1e-13
 
 
-@pytest.mark.parametrize("functional,name", [(0, "wb"), (1, "wb2"), (2, "wb2slow")])
+@pytest.mark.parametrize("functional,name", [(0, "wb"), (1, "wb2"), (2, "wb2slow"), (3, "wbt")])
 @pytest.mark.parametrize("eta", [0.1, 0.4])
 def test_bulk_state_of_every_functional(eta, functional, name):
     n = O.bulk_density(eta, 1.0)
@@ -47,7 +47,7 @@ def test_bulk_state_of_every_functional(eta, functional, name):
     assert abs(f / fo - 1) < 1e-13
     assert deft_amd.find_chemical_potential(n, functional=functional) == mu
     with pytest.raises(deft_amd.DeftError):
-        deft_amd.bulk_state(n, 1.0, 1.0, 3)
+        deft_amd.bulk_state(n, 1.0, 1.0, 4)
 
 
 def test_bad_arguments_raise_without_device():
