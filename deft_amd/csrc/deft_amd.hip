@@ -731,8 +731,8 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
   for (int i = 0; i < n; ++i) {
     if (!plans[i]) return set_error(DEFT_ERR_ARG, "null plan");
     if (plans[i]->batch) return set_error(DEFT_ERR_STATE, "a plan is already in a batch");
-    if (plans[i]->functional == DEFT_WHITE_BEAR_TENSOR)
-      return set_error(DEFT_ERR_UNSUPPORTED, "tensor White-Bear plans are not batched");
+    if (plans[i]->functional != DEFT_WHITE_BEAR)
+      return set_error(DEFT_ERR_UNSUPPORTED, "batches take White-Bear plans only");
     for (int j = 0; j < i; ++j)
       if (plans[j] == plans[i]) return set_error(DEFT_ERR_ARG, "a plan appears twice");
   }
@@ -831,8 +831,8 @@ int deft_plan_set_functional(deft_plan p, int functional) {
   if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
       functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR)
     return set_error(DEFT_ERR_ARG, "unknown functional");
-  if (functional == DEFT_WHITE_BEAR_TENSOR && p->batch)
-    return set_error(DEFT_ERR_STATE, "plans in a batch cannot take the tensor functional");
+  if (functional != DEFT_WHITE_BEAR && p->batch)
+    return set_error(DEFT_ERR_STATE, "plans in a batch keep the White-Bear functional");
   if (functional == DEFT_WHITE_BEAR_TENSOR && p->spec_fields < 12) {
     // eleven weighted-density spectra plus the density spectrum
     void *spec = nullptr;

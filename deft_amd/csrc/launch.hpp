@@ -114,9 +114,10 @@ static void launch_ypass(deft_plan_s *p, const GridArgs &g, cx<T> *data, bool in
   });
 }
 
-// Tensor White-Bear plans run eleven weighted fields; their x passes use the
-// first tile size only (fewer instantiations).  The tensor launches are
-// compiled in their own translation units (tensor_f32.hip, tensor_f64.hip).
+// Plans with another functional than White-Bear run their own Phi pass
+// (Mark II: compile-time factors; tensor White-Bear: eleven weighted fields,
+// whose x passes use the first tile size only).  Those launches are compiled
+// in their own translation units (tensor_f32.hip, tensor_f64.hip).
 inline int plan_fields(const deft_plan_s *p) { return p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5; }
 
 template <typename T>
@@ -165,7 +166,7 @@ static void launch_xconv_bwd(deft_plan_s *p, const GridArgs &g, int tb) {
 template <typename T>
 static void launch_zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
-  if (plan_fields(p) == 11) {
+  if (p->functional != DEFT_WHITE_BEAR) {
     TensorLaunch<T>::zphi(p, g, grad, zb);
     return;
   }

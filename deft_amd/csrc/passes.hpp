@@ -646,7 +646,8 @@ __global__ __launch_bounds__(kThreads) void k_xconv_bwd(GridArgs g, const cx<T>*
 // z pass 2: c2r of the five weighted densities -> FP64 Phi (energy partial
 // sums) and, with GRAD, the five partials -> r2c back into the same buffers.
 // ---------------------------------------------------------------------------
-template <typename T, int L, bool GRAD, int ZB, int NW = 5>
+// FN: 0 White-Bear, 1 Mark II (either form), 3 tensor White-Bear (11 fields)
+template <typename T, int L, bool GRAD, int ZB, int FN = 0>
 __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __restrict__ fields,
                                           double* __restrict__ partial, int pstride, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
@@ -657,6 +658,7 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
+  constexpr int NW = FN == 3 ? 11 : 5;
   T val[NW][PT];
 #pragma unroll
   for (int f = 0; f < NW; ++f) {
@@ -703,17 +705,17 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
       if (valid) {
         // the energy terms come from wb_phi, as in the energy-only variant
         double q1, q2, q3;
-        wb_phi((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i], (double)val[4][i], g.wb,
-               p1, p2, p3);
-        wb_phi_partials((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i],
-                        (double)val[4][i], g.wb, q1, q2, q3, dd);
+        wb_phi_k<FN == 1>((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i],
+                          (double)val[4][i], g.wb, p1, p2, p3);
+        wb_phi_partials_k<FN == 1>((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i],
+                                   (double)val[4][i], g.wb, q1, q2, q3, dd);
       }
 #pragma unroll
       for (int f = 0; f < 5; ++f) val[f][i] = (T)dd[f];
     } else {
       if (valid)
-        wb_phi((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i], (double)val[4][i], g.wb,
-               p1, p2, p3);
+        wb_phi_k<FN == 1>((double)val[0][i], (double)val[1][i], (double)val[2][i], (double)val[3][i],
+                          (double)val[4][i], g.wb, p1, p2, p3);
     }
     acc[0] += p1;
     acc[1] += p2;
@@ -743,11 +745,11 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, bx);
 }
 
-template <typename T, int L, bool GRAD, int ZB, int NW = 5>
+template <typename T, int L, bool GRAD, int ZB, int FN = 0>
 __global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
                                                    double* __restrict__ partial, int pstride,
                                                    const cx<T>* __restrict__ tw) {
-  zphi_body<T, L, GRAD, ZB, NW>(blockIdx.x, g, fields, partial, pstride, tw);
+  zphi_body<T, L, GRAD, ZB, FN>(blockIdx.x, g, fields, partial, pstride, tw);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,5 +1,6 @@
-// Launches of the tensor White-Bear passes (eleven weighted fields), kept out of
-// the main launch translation units so the two compile in parallel.
+// Launches of the passes of the other functionals (tensor White-Bear's eleven
+// weighted fields, Mark II's Phi pass), kept out of the main launch
+// translation units so they compile in parallel.
 #pragma once
 
 #include "launch.hpp"
@@ -31,17 +32,26 @@ void TensorLaunch<T>::xconv_bwd(deft_plan_s *p, const GridArgs &g) {
 template <typename T>
 void TensorLaunch<T>::zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
+  const bool tensor = p->functional == DEFT_WHITE_BEAR_TENSOR;
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
     with_zb(zb, [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
-      ProfScope ps(p, kid<T>(grad ? K_ZPHI_EG : K_ZPHI_E), (grad ? 22 : 11) * cb);
-      if (grad)
-        hipLaunchKernelGGL((k_zphi<T, L, true, ZB, 11>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g,
-                           spec_ptr<T>(p, 1), p->partial, p->pstride, tw_ptr<T>(p, 2));
+      const int nw = tensor ? 11 : 5;
+      ProfScope ps(p, kid<T>(grad ? K_ZPHI_EG : K_ZPHI_E), (grad ? 2 : 1) * nw * cb);
+      const dim3 grid(zblocks_for(p, ZB));
+      if (tensor && grad)
+        hipLaunchKernelGGL((k_zphi<T, L, true, ZB, 3>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+                           p->partial, p->pstride, tw_ptr<T>(p, 2));
+      else if (tensor)
+        hipLaunchKernelGGL((k_zphi<T, L, false, ZB, 3>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+                           p->partial, p->pstride, tw_ptr<T>(p, 2));
+      else if (grad)
+        hipLaunchKernelGGL((k_zphi<T, L, true, ZB, 1>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+                           p->partial, p->pstride, tw_ptr<T>(p, 2));
       else
-        hipLaunchKernelGGL((k_zphi<T, L, false, ZB, 11>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g,
-                           spec_ptr<T>(p, 1), p->partial, p->pstride, tw_ptr<T>(p, 2));
+        hipLaunchKernelGGL((k_zphi<T, L, false, ZB, 1>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+                           p->partial, p->pstride, tw_ptr<T>(p, 2));
     });
   });
 }

@@ -137,9 +137,11 @@ __host__ __device__ inline void f3_and_derivative(double n, double& f3, double& 
   f3_of(n, om, 1.0 / om, log1p(-n), f3, df3);
 }
 
-// Energy density terms only.
-__host__ __device__ inline void wb_phi(double n3, double n2, double vx, double vy, double vz, const WBConst& c,
-                                       double& p1, double& p2, double& p3) {
+// Energy density terms only; M2 selects the Mark II factors at compile time
+// (their series would otherwise cost the White-Bear kernels registers).
+template <bool M2>
+__host__ __device__ inline void wb_phi_k(double n3, double n2, double vx, double vy, double vz, const WBConst& c,
+                                         double& p1, double& p2, double& p3) {
   // no FMA contraction: every kernel that sums the energy density calls this
   // function, so energies of the same Veff agree to the last bit
 #pragma clang fp contract(off)
@@ -148,7 +150,7 @@ __host__ __device__ inline void wb_phi(double n3, double n2, double vx, double v
   const double L = log1p(-n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3, g2 = io;
-  if (c.mark2) {
+  if constexpr (M2) {
     double g2p;
     markii_of(n3, io, L, c.g3_scale, g2, g2p, f3, df3);
   } else {
@@ -161,9 +163,10 @@ __host__ __device__ inline void wb_phi(double n3, double n2, double vx, double v
 }
 
 // Energy density terms and the five partials (d3, d2, dvx, dvy, dvz).
-__host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx, double vy, double vz,
-                                                const WBConst& c, double& p1, double& p2, double& p3,
-                                                double* d) {
+template <bool M2>
+__host__ __device__ inline void wb_phi_partials_k(double n3, double n2, double vx, double vy, double vz,
+                                                  const WBConst& c, double& p1, double& p2, double& p3,
+                                                  double* d) {
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
   const double L = log1p(-n3);
@@ -173,7 +176,7 @@ __host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx,
   const double q = n2s - v2;
   const double t3 = n2 * (n2s - 3.0 * v2);
   double a, ap;  // Phi2 = q a, dPhi2/dn3 = q ap
-  if (c.mark2) {
+  if constexpr (M2) {
     double g2, g2p;
     markii_of(n3, io, L, c.g3_scale, g2, g2p, f3, df3);
     a = c.inv_fpR * g2;
@@ -192,6 +195,22 @@ __host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx,
   d[2] = cv * vx;
   d[3] = cv * vy;
   d[4] = cv * vz;
+}
+
+// Runtime dispatch on the plan's functional (host code and tests).
+__host__ __device__ inline void wb_phi(double n3, double n2, double vx, double vy, double vz, const WBConst& c,
+                                       double& p1, double& p2, double& p3) {
+  if (c.mark2)
+    wb_phi_k<true>(n3, n2, vx, vy, vz, c, p1, p2, p3);
+  else
+    wb_phi_k<false>(n3, n2, vx, vy, vz, c, p1, p2, p3);
+}
+__host__ __device__ inline void wb_phi_partials(double n3, double n2, double vx, double vy, double vz,
+                                                const WBConst& c, double& p1, double& p2, double& p3, double* d) {
+  if (c.mark2)
+    wb_phi_partials_k<true>(n3, n2, vx, vy, vz, c, p1, p2, p3, d);
+  else
+    wb_phi_partials_k<false>(n3, n2, vx, vy, vz, c, p1, p2, p3, d);
 }
 
 // Tensor White-Bear energy density terms; v[11] = n3, n2, n2v_x/y/z, T_xx, T_yy, T_zz, T_xy, T_yz, T_zx.

@@ -59,14 +59,15 @@ typedef enum {
  * Phi3's prefactor 1/(36 pi) in place of 1/(24 pi)), the form the reference's
  * published Mark II profiles (papers/contact/figs/inner-sphereWBm2-*.dat) match.
  * The first two give the Carnahan-Starling bulk that deft_bulk_chemical_potential
- * and friends compute; deft_bulk_state gives the bulk of any of the three. */
+ * and friends compute; deft_bulk_state gives the bulk of any of them.
+ * Batches (deft_batch_*) take White-Bear plans only. */
 typedef enum {
   DEFT_WHITE_BEAR = 0,
   DEFT_WHITE_BEAR_MARK_II = 1,
   DEFT_WHITE_BEAR_MARK_II_SLOW = 2,
   /* TensorWhiteBear (src/haskell/WhiteBear.hs:44-51, src/HardSpheres.cpp:30-67):
    * Phi3 with the shell tensor density (six more weighted fields); same bulk as
-   * White-Bear; not available in batches */
+   * White-Bear */
   DEFT_WHITE_BEAR_TENSOR = 3
 } deft_functional_kind;
 

@@ -105,6 +105,8 @@ def test_switching_functional_on_a_plan(gpu):
     assert f.energy() == e2
     with pytest.raises(gpu.DeftError):
         f.functional = 7
+    with pytest.raises(gpu.DeftError):
+        gpu.Batch([f])  # batches take White-Bear plans only
 
 
 def test_mark_ii_cg_trajectory_fp64(gpu):
