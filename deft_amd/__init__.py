@@ -123,6 +123,7 @@ _SIGNATURES = {
     "deft_plan_profile_get": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_longlong), _DP, _DP], _I),
     "deft_batch_create": ([ctypes.POINTER(_P), _I, ctypes.POINTER(_P)], _I),
     "deft_batch_minimize": ([_P, ctypes.POINTER(_P), _I], _I),
+    "deft_batch_set_graphs": ([_P, _I], _I),
     "deft_batch_stats": ([_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong)], _I),
     "deft_batch_destroy": ([_P], _I),
@@ -509,12 +510,14 @@ class Batch:
     """Several fluids minimized together (deft_batch_*): every evaluation the
     minimizers ask for runs in a lockstep round of batched launches."""
 
-    def __init__(self, fluids):
+    def __init__(self, fluids, graphs=False):
         self.fluids = list(fluids)
         arr = (_P * len(self.fluids))(*[f.handle for f in self.fluids])
         h = ctypes.c_void_p()
         _check(lib().deft_batch_create(arr, len(self.fluids), ctypes.byref(h)))
         self._h = h
+        if graphs:
+            _check(lib().deft_batch_set_graphs(self._h, 1))
 
     def minimize(self, minimizers):
         """Runs each minimizer (one per fluid of the batch) to its own stop."""

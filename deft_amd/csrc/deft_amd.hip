@@ -12,6 +12,8 @@
 #include <thread>
 #include <vector>
 
+#include <ucontext.h>
+
 #include "launch.hpp"
 
 using namespace deft;
@@ -66,6 +68,7 @@ bool length_ok(int n) {
 }
 
 int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond);
+int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir);
 
 int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
   if (p->batch) return batch_evaluate(p, prec, grad, have_old, precond);
@@ -208,12 +211,32 @@ struct deft_batch_s {
   std::condition_variable cv;
   int active = 0;                  // minimizer threads still running
   std::vector<BatchReq> posted[2]; // this round's requests: FP32, FP64
+  std::vector<BatchReq> posted_cg; // this round's CG updates
   int nposted = 0;
   long long round = 0;
   int round_rc = DEFT_OK;
   std::string round_err;
   BatchStage stage;
   long long rounds_run = 0, launches = 0, requests = 0;
+  bool graphs = false;  // rounds replay one captured graph per precision
+  BatchGraph graph[2];  // FP32, FP64
+  // The minimizers run as fibers (user-level contexts) of the thread that
+  // called deft_batch_minimize: a fiber posts its request and switches back to
+  // the scheduler, which runs the round once every live fiber has posted.
+  ucontext_t sched;
+  struct Fiber *current = nullptr;
+  // host-mapped completion flag of the batch stream (see flag_wait)
+  unsigned long long *hflag = nullptr, *dflag = nullptr, flag_seq = 0;
+};
+
+struct Fiber {
+  ucontext_t ctx;
+  std::vector<char> stack;
+  deft_minimize_s *m = nullptr;
+  deft_batch_s *b = nullptr;
+  int rc = DEFT_OK;
+  std::string err;
+  bool done = false;
 };
 
 namespace {
@@ -291,22 +314,33 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
   double beta = m->have_old ? (m->pgg - m->pgo) / m->oldgradsqr : m->pgg / m->oldgradsqr;
   if (beta < 0 || beta != beta || m->oldgradsqr == 0) beta = 0;
   m->oldgradsqr = m->pgg;
-  {
-    ProfScope ps(p, 2 * K_CG_UPDATE, 40.0 * p->N);
-    hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
-                     beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
+  double gdotd;
+  if (p->batch) {
+    // batched plans: the update joins a round with the other grids' updates
+    TRY(batch_cg(p, 0, beta, m->have_dir ? 1 : 0));
+    gdotd = p->hscal[SLOT_GD];
+  } else {
+    {
+      ProfScope ps(p, 2 * K_CG_UPDATE, 40.0 * p->N);
+      hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d,
+                         p->og, beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
+    }
+    TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
   }
   m->have_dir = true;
   m->have_old = true;
-  double gdotd;
-  TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
   if (gdotd > 0) {
-    {
-      ProfScope ps(p, 2 * K_CG_RESET, 16.0 * p->N);
-      hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
-                         p->pstride);
+    if (p->batch) {
+      TRY(batch_cg(p, 1, 0.0, 0));
+      gdotd = p->hscal[SLOT_GD];
+    } else {
+      {
+        ProfScope ps(p, 2 * K_CG_RESET, 16.0 * p->N);
+        hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d,
+                           p->partial, p->pstride);
+      }
+      TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
     }
-    TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
     if (verbose) printf("\t\treset to gradient\n");
   }
 
@@ -528,49 +562,125 @@ int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
 
 // Launches the posted requests as one round and wakes the waiting minimizers;
 // the caller holds b->mu.
+// The round's CG updates: one launch over all of them and one reduction
+// launch (tables at `at` in the staging buffer).
+int launch_cg_batch(deft_batch_s *b, size_t at) {
+  const std::vector<BatchReq> &q = b->posted_cg;
+  const int n = (int)q.size();
+  std::vector<CgArgs> args(n);
+  std::vector<int> off(n + 1, 0), idx(n);
+  for (int r = 0; r < n; ++r) {
+    deft_plan_s *p = q[r].p;
+    args[r] = CgArgs{p->N, p->pg, p->g, p->d, p->og, q[r].beta, q[r].have_dir, q[r].cg == 2 ? 1 : 0,
+                     p->partial, p->pstride, p->eblocks, p->dscal};
+    off[r + 1] = off[r] + p->eblocks;
+    idx[r] = r;
+  }
+  at = (at + 255) / 256 * 256;
+  const size_t a_at = at, o_at = a_at + sizeof(CgArgs) * n, i_at = o_at + sizeof(int) * (n + 1);
+  const size_t end = i_at + sizeof(int) * n;
+  if (end > b->stage.cap) return set_error(DEFT_ERR_ALLOC, "batch staging buffer too small");
+  std::memcpy(b->stage.host + a_at, args.data(), sizeof(CgArgs) * n);
+  std::memcpy(b->stage.host + o_at, off.data(), sizeof(int) * (n + 1));
+  std::memcpy(b->stage.host + i_at, idx.data(), sizeof(int) * n);
+  DEFT_HIP_CHECK(hipMemcpyAsync(b->stage.dev + a_at, b->stage.host + a_at, end - a_at, hipMemcpyHostToDevice,
+                                b->stream));
+  const CgArgs *dargs = reinterpret_cast<const CgArgs *>(b->stage.dev + a_at);
+  const BatchGroup grp{reinterpret_cast<const int *>(b->stage.dev + o_at),
+                       reinterpret_cast<const int *>(b->stage.dev + i_at), n};
+  hipLaunchKernelGGL(kb_cg, dim3(off[n]), dim3(kThreads), 0, b->stream, grp, dargs);
+  hipLaunchKernelGGL(kb_cg_reduce, dim3(n), dim3(kThreads), 0, b->stream, dargs);
+  DEFT_HIP_CHECK(hipGetLastError());
+  b->launches += 2;
+  if (!b->graphs) b->stage.used = end;
+  return DEFT_OK;
+}
+
 int run_round(deft_batch_s *b) {
   b->stage.used = 0;
   int rc = DEFT_OK;
-  if (!b->posted[0].empty())
-    rc = Launch<float>::evaluate_batch(b->stream, b->stage, b->posted[0].data(), (int)b->posted[0].size(),
-                                       &b->launches);
-  if (rc == DEFT_OK && !b->posted[1].empty())
-    rc = Launch<double>::evaluate_batch(b->stream, b->stage, b->posted[1].data(), (int)b->posted[1].size(),
-                                        &b->launches);
+  const int np = (int)b->plans.size();
+  // staging: graph tables for FP32 and FP64 in the first two thirds, the CG
+  // tables in the last (grouped rounds append everything from the start)
+  if (!b->posted_cg.empty()) rc = launch_cg_batch(b, b->graphs ? 2 * (b->stage.cap / 3) : 0);
+  if (rc != DEFT_OK) {
+  } else if (b->graphs) {
+    if (!b->posted[0].empty())
+      rc = Launch<float>::evaluate_batch_graph(b->stream, b->stage, 0, b->plans.data(), np, b->posted[0].data(),
+                                               (int)b->posted[0].size(), b->graph[0], &b->launches);
+    if (rc == DEFT_OK && !b->posted[1].empty())
+      rc = Launch<double>::evaluate_batch_graph(b->stream, b->stage, (b->stage.cap / 3 + 255) / 256 * 256,
+                                                b->plans.data(), np, b->posted[1].data(), (int)b->posted[1].size(),
+                                                b->graph[1], &b->launches);
+  } else {
+    if (!b->posted[0].empty())
+      rc = Launch<float>::evaluate_batch(b->stream, b->stage, b->posted[0].data(), (int)b->posted[0].size(),
+                                         &b->launches);
+    if (rc == DEFT_OK && !b->posted[1].empty())
+      rc = Launch<double>::evaluate_batch(b->stream, b->stage, b->posted[1].data(), (int)b->posted[1].size(),
+                                          &b->launches);
+  }
   if (rc == DEFT_OK) {
-    const hipError_t e = hipStreamSynchronize(b->stream);
-    if (e != hipSuccess) rc = set_error(DEFT_ERR_HIP, std::string("batch round: ") + hipGetErrorString(e));
+    const int w = b->hflag ? flag_wait(b->stream, b->hflag, b->dflag, b->flag_seq) : 0;
+    if (w < 0) {
+      rc = DEFT_ERR_HIP;
+    } else if (w == 0) {
+      const hipError_t e = hipStreamSynchronize(b->stream);
+      if (e != hipSuccess) rc = set_error(DEFT_ERR_HIP, std::string("batch round: ") + hipGetErrorString(e));
+    }
   }
   b->round_rc = rc;
   b->round_err = rc == DEFT_OK ? std::string() : g_last_error;
   b->requests += b->nposted;
   b->posted[0].clear();
   b->posted[1].clear();
+  b->posted_cg.clear();
   b->nposted = 0;
   b->round++;
   b->rounds_run++;
-  b->cv.notify_all();
   return rc;
 }
 
-int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+// Posts a request of a minimizer thread and returns once its round has run.
+int batch_post(deft_plan_s *p, const BatchReq &req, int prec) {
   deft_batch_s *b = p->batch;
-  std::unique_lock<std::mutex> lk(b->mu);
-  b->posted[prec == DEFT_FFT_FP64 ? 1 : 0].push_back(BatchReq{p, grad, have_old, precond});
-  b->nposted++;
-  const long long mine = b->round;
-  if (b->nposted >= std::max(b->active, 1))
-    run_round(b);
+  if (!b->current) return set_error(DEFT_ERR_STATE, "batched plans evaluate inside deft_batch_minimize only");
+  if (req.cg)
+    b->posted_cg.push_back(req);
   else
-    b->cv.wait(lk, [&] { return b->round != mine; });
+    b->posted[prec == DEFT_FFT_FP64 ? 1 : 0].push_back(req);
+  b->nposted++;
+  Fiber *f = b->current;
+  if (swapcontext(&f->ctx, &b->sched) != 0) return set_error(DEFT_ERR_STATE, "fiber switch failed");
+  // resumed by the scheduler after the round ran
   if (b->round_rc != DEFT_OK) return set_error(b->round_rc, b->round_err);
   return DEFT_OK;
 }
 
-void batch_thread_done(deft_batch_s *b) {
-  std::unique_lock<std::mutex> lk(b->mu);
-  b->active--;
-  if (b->nposted > 0 && b->nposted >= b->active) run_round(b);
+int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+  return batch_post(p, BatchReq{p, grad, have_old, precond}, prec);
+}
+
+// CG update (or reset) of a batched plan in the next round; grad.direction lands in hscal
+int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir) {
+  BatchReq r{p, false, false, false};
+  r.cg = reset ? 2 : 1;
+  r.beta = beta;
+  r.have_dir = have_dir;
+  return batch_post(p, r, DEFT_FFT_FP64);
+}
+
+thread_local Fiber *g_fiber_start = nullptr;
+
+// Body of a minimizer fiber; returning resumes the scheduler (uc_link).
+void fiber_main() {
+  Fiber *f = g_fiber_start;
+  int more = 1, rc = DEFT_OK;
+  while (more && rc == DEFT_OK) rc = improve_energy_mixed(f->m, 0, &more);
+  f->rc = rc;
+  if (rc != DEFT_OK) f->err = g_last_error;
+  f->done = true;
+  f->b->active--;
 }
 
 // Copies between the compact half-spectrum layout of the seam and the padded
@@ -741,11 +851,21 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
     delete b;
     return set_error(DEFT_ERR_HIP, "stream creation failed");
   }
-  b->stage.cap = 65536 + (size_t)n * 4096;
+  b->stage.cap = 131072 + (size_t)n * 8192;
+  if (hipHostMalloc((void **)&b->hflag, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&b->dflag, b->hflag, 0) != hipSuccess) {
+    if (b->hflag) (void)hipHostFree(b->hflag);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return set_error(DEFT_ERR_ALLOC, "batch flag allocation failed");
+  }
+  *b->hflag = 0;
   if (hipHostMalloc((void **)&b->stage.host, b->stage.cap, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&b->stage.dev, b->stage.cap) != hipSuccess) {
     if (b->stage.host) (void)hipHostFree(b->stage.host);
     if (b->stage.dev) (void)hipFree(b->stage.dev);
+    (void)hipHostFree(b->hflag);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return set_error(DEFT_ERR_ALLOC, "batch staging allocation failed");
@@ -757,6 +877,7 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
     p->own_stream = p->stream;
     p->stream = b->stream;
     p->batch = b;
+    p->batch_index = (int)b->plans.size();
     b->plans.push_back(p);
   }
   *out = b;
@@ -771,11 +892,24 @@ int deft_batch_destroy(deft_batch b) {
     p->stream = p->own_stream;
     p->own_stream = nullptr;
     p->batch = nullptr;
+    p->batch_index = -1;
+  }
+  for (BatchGraph &g : b->graph) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
   }
   (void)hipHostFree(b->stage.host);
   (void)hipFree(b->stage.dev);
+  (void)hipHostFree(b->hflag);
   (void)hipStreamDestroy(b->stream);
   delete b;
+  return DEFT_OK;
+}
+
+int deft_batch_set_graphs(deft_batch b, int on) {
+  if (!b) return set_error(DEFT_ERR_ARG, "null batch");
+  if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is minimizing");
+  b->graphs = on != 0;
   return DEFT_OK;
 }
 
@@ -1139,26 +1273,42 @@ int deft_batch_minimize(deft_batch b, deft_minimize *mins, int n) {
     for (int j = 0; j < i; ++j)
       if (mins[j]->p == mins[i]->p) return set_error(DEFT_ERR_ARG, "two minimizers share a plan");
   }
-  {
-    std::lock_guard<std::mutex> lk(b->mu);
-    if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is already minimizing");
-    b->active = n;
+  if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is already minimizing");
+  constexpr size_t kFiberStack = 512 * 1024;
+  std::vector<Fiber> fibers(n);
+  for (int i = 0; i < n; ++i) {
+    Fiber &f = fibers[i];
+    f.m = mins[i];
+    f.b = b;
+    f.stack.resize(kFiberStack);
+    if (getcontext(&f.ctx) != 0) return set_error(DEFT_ERR_STATE, "getcontext failed");
+    f.ctx.uc_stack.ss_sp = f.stack.data();
+    f.ctx.uc_stack.ss_size = f.stack.size();
+    f.ctx.uc_link = &b->sched;
+    makecontext(&f.ctx, fiber_main, 0);
   }
-  std::vector<int> rcs(n, DEFT_OK);
-  std::vector<std::string> errs(n);
-  std::vector<std::thread> threads;
-  threads.reserve(n);
+  b->active = n;
+  std::vector<bool> begun(n, false);
+  while (b->active > 0) {
+    // every live fiber runs until it posts its next request or finishes
+    for (int i = 0; i < n; ++i) {
+      Fiber &f = fibers[i];
+      if (f.done) continue;
+      b->current = &f;
+      if (!begun[i]) {
+        begun[i] = true;
+        g_fiber_start = &f;
+      }
+      if (swapcontext(&b->sched, &f.ctx) != 0) {
+        b->current = nullptr;
+        return set_error(DEFT_ERR_STATE, "fiber switch failed");
+      }
+    }
+    b->current = nullptr;
+    if (b->nposted > 0) run_round(b);
+  }
   for (int i = 0; i < n; ++i)
-    threads.emplace_back([&, i] {
-      int more = 1, rc = DEFT_OK;
-      while (more && rc == DEFT_OK) rc = improve_energy_mixed(mins[i], 0, &more);
-      rcs[i] = rc;
-      if (rc != DEFT_OK) errs[i] = g_last_error;
-      batch_thread_done(b);
-    });
-  for (std::thread &t : threads) t.join();
-  for (int i = 0; i < n; ++i)
-    if (rcs[i] != DEFT_OK) return set_error(rcs[i], errs[i]);
+    if (fibers[i].rc != DEFT_OK) return set_error(fibers[i].rc, fibers[i].err);
   return DEFT_OK;
 }
 

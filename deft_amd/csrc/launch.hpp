@@ -305,6 +305,7 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
     a.grad = p->g;
     a.pgrad = p->pg;
     a.dscal = p->dscal;
+    a.kind = reqs[i].grad ? 2 : 1;
   }
   auto P = [&](int i) { return reqs[i].p; };
   auto all = [&](int) { return true; };
@@ -374,48 +375,228 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
   const dim3 blk(kThreads);
   zpass(g_dens, [&](auto LZ, auto ZBc, const Group &g) {
     constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
-    hipLaunchKernelGGL((kb_zfwd_density<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+    hipLaunchKernelGGL((kb_zfwd_density<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
   tpass(g_y1, [&](auto LY, auto TBc, const Group &g) {
     constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0);
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 0);
   });
   tpass(g_xf, [&](auto LX, auto TBc, const Group &g) {
     constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_xconv_fwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+    hipLaunchKernelGGL((kb_xconv_fwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
   tpass(g_y1, [&](auto LY, auto TBc, const Group &g) {  // same grouping: inverse y of the 5 fields
     constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1);
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1, 0);
   });
   zpass(g_phi, [&](auto LZ, auto ZBc, const Group &g) {
     constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
     if (g.flag)
-      hipLaunchKernelGGL((kb_zphi<T, L, true, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+      hipLaunchKernelGGL((kb_zphi<T, L, true, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
     else
-      hipLaunchKernelGGL((kb_zphi<T, L, false, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+      hipLaunchKernelGGL((kb_zphi<T, L, false, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
   tpass(g_yg, [&](auto LY, auto TBc, const Group &g) {
     constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1);
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1, 0);
   });
   tpass(g_xb, [&](auto LX, auto TBc, const Group &g) {
     constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_xconv_bwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+    hipLaunchKernelGGL((kb_xconv_bwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
   tpass(g_yg, [&](auto LY, auto TBc, const Group &g) {
     constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
-    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0);
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 0);
   });
   zpass(g_zg, [&](auto LZ, auto ZBc, const Group &g) {
     constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
-    hipLaunchKernelGGL((kb_zgrad<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+    hipLaunchKernelGGL((kb_zgrad<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
   for (const Group &g : g_red) {
-    hipLaunchKernelGGL(kb_reduce<T>, dim3(g.total()), blk, 0, st, grp(g), dargs, g.flag);
+    hipLaunchKernelGGL(kb_reduce<T>, dim3(g.total()), blk, 0, st, grp(g), dargs, g.flag, 0);
     ++nl;
   }
   DEFT_HIP_CHECK(hipGetLastError());
+  if (launches) *launches += nl;
+  return DEFT_OK;
+}
+
+template <typename T>
+static void fill_batch_args(BatchArgs<T> &a, deft_plan_s *p, int ti) {
+  a.g = grid_args(p);
+  a.x = p->x;
+  a.d = p->d;
+  a.nzb = zblocks_for(p, p->tz[ti]);
+  a.vext = p->vext;
+  a.spec0 = spec_ptr<T>(p, 0);
+  a.spec1 = spec_ptr<T>(p, 1);
+  a.partial = p->partial;
+  a.pstride = p->pstride;
+  a.twx = tw_ptr<T>(p, 0);
+  a.twy = tw_ptr<T>(p, 1);
+  a.twz = tw_ptr<T>(p, 2);
+  a.og = p->og;
+  a.grad = p->g;
+  a.pgrad = p->pg;
+  a.dscal = p->dscal;
+}
+
+template <typename T>
+int Launch<T>::evaluate_batch_graph(hipStream_t st, BatchStage &S, size_t region, deft_plan_s *const *plans,
+                                    int nplans, const BatchReq *reqs, int n, BatchGraph &gr, long long *launches) {
+  using namespace batch_detail;
+  if (n <= 0) return DEFT_OK;
+  const int ti = prec_index<T>();
+  std::vector<BatchArgs<T>> args(nplans);
+  for (int i = 0; i < nplans; ++i) {
+    std::memset(&args[i], 0, sizeof(BatchArgs<T>));
+    fill_batch_args<T>(args[i], plans[i], ti);
+  }
+  for (int r = 0; r < n; ++r) {
+    deft_plan_s *p = reqs[r].p;
+    BatchArgs<T> &a = args[p->batch_index];
+    take_moves(p, a.m1, a.m2, a.nmoves);
+    a.have_old = reqs[r].have_old ? 1 : 0;
+    a.precond = reqs[r].precond ? 1 : 0;
+    a.kind = reqs[r].grad ? 2 : 1;
+  }
+  std::vector<int> sig;
+  for (int i = 0; i < nplans; ++i) {
+    const deft_plan_s *p = plans[i];
+    sig.insert(sig.end(), {p->Nx, p->Ny, p->Nz, p->tz[ti], p->ty[ti], p->tx[ti]});
+  }
+  const size_t args_bytes = sizeof(BatchArgs<T>) * nplans;
+  if (gr.exec && (sig != gr.sig || gr.at != region)) {
+    (void)hipGraphExecDestroy(gr.exec);
+    (void)hipGraphDestroy(gr.graph);
+    gr.exec = nullptr;
+    gr.graph = nullptr;
+  }
+  if (gr.exec) {
+    std::memcpy(S.host + region, args.data(), args_bytes);
+    DEFT_HIP_CHECK(hipMemcpyAsync(S.dev + region, S.host + region, args_bytes, hipMemcpyHostToDevice, st));
+    DEFT_HIP_CHECK(hipGraphLaunch(gr.exec, st));
+    if (launches) *launches += gr.kernels;
+    return DEFT_OK;
+  }
+  // first round (or changed shapes): lay out the tables and capture the passes
+  auto P = [&](int i) { return plans[i]; };
+  auto all = [&](int) { return true; };
+  auto zkey = [&](int i) { return std::make_tuple(P(i)->Nz, P(i)->tz[ti], 0); };
+  auto ykey = [&](int i) { return std::make_tuple(P(i)->Ny, P(i)->ty[ti], 0); };
+  auto xkey = [&](int i) { return std::make_tuple(P(i)->Nx, P(i)->tx[ti], 0); };
+  auto zbl = [&](int i) { return zblocks_for(P(i), P(i)->tz[ti]); };
+  auto ybl = [&](int i) { return y_blocks<T>(P(i), P(i)->ty[ti]); };
+  auto xbl = [&](int i) { return x_blocks<T>(P(i), P(i)->tx[ti]); };
+  std::vector<Group> g_z = make_groups(nplans, zkey, zbl, all);
+  std::vector<Group> g_y = make_groups(nplans, ykey, ybl, all);
+  std::vector<Group> g_x = make_groups(nplans, xkey, xbl, all);
+  std::vector<Group> g_red = make_groups(nplans, [](int) { return std::make_tuple(0, 0, 8); },
+                                         [](int) { return 8; }, all);
+  std::vector<Group> *stages[] = {&g_z, &g_y, &g_x, &g_red};
+  size_t at = region + args_bytes;
+  for (auto *gv : stages)
+    for (Group &g : *gv) {
+      at = align_up(at, 16);
+      g.off_at = at;
+      at += sizeof(int) * g.off.size();
+      g.idx_at = at;
+      at += sizeof(int) * g.members.size();
+    }
+  if (at > S.cap) return set_error(DEFT_ERR_ALLOC, "batch staging buffer too small");
+  std::memcpy(S.host + region, args.data(), args_bytes);
+  for (auto *gv : stages)
+    for (Group &g : *gv) {
+      std::memcpy(S.host + g.off_at, g.off.data(), sizeof(int) * g.off.size());
+      std::memcpy(S.host + g.idx_at, g.members.data(), sizeof(int) * g.members.size());
+    }
+  DEFT_HIP_CHECK(hipMemcpyAsync(S.dev + region, S.host + region, at - region, hipMemcpyHostToDevice, st));
+  const BatchArgs<T> *dargs = reinterpret_cast<const BatchArgs<T> *>(S.dev + region);
+  auto grp = [&](const Group &g) {
+    return BatchGroup{reinterpret_cast<const int *>(S.dev + g.off_at), reinterpret_cast<const int *>(S.dev + g.idx_at),
+                      (int)g.members.size()};
+  };
+  long long nl = 0;
+  auto zpass = [&](auto launch) {
+    for (const Group &g : g_z)
+      with_length(g.L, [&](auto LZ) {
+        with_zb(g.tune, [&](auto ZBc) {
+          launch(LZ, ZBc, g);
+          ++nl;
+        });
+      });
+  };
+  auto tpass = [&](const std::vector<Group> &gs, auto launch) {
+    for (const Group &g : gs)
+      with_length(g.L, [&](auto LA) {
+        with_tb(g.tune, [&](auto TBc) {
+          launch(LA, TBc, g);
+          ++nl;
+        });
+      });
+  };
+  const dim3 blk(kThreads);
+  DEFT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  // sel: 1 every request, 2 gradient requests, 3 energy-only requests
+  zpass([&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zfwd_density<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 1);
+  });
+  tpass(g_y, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 1);
+  });
+  tpass(g_x, [&](auto LX, auto TBc, const Group &g) {
+    constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_xconv_fwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 1);
+  });
+  tpass(g_y, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1, 1);
+  });
+  zpass([&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zphi<T, L, false, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 3);
+  });
+  zpass([&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zphi<T, L, true, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 2);
+  });
+  tpass(g_y, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 5), blk, 0, st, grp(g), dargs, 1, 2);
+  });
+  tpass(g_x, [&](auto LX, auto TBc, const Group &g) {
+    constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_xconv_bwd<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 2);
+  });
+  tpass(g_y, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 2);
+  });
+  zpass([&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zgrad<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 2);
+  });
+  for (const Group &g : g_red) {
+    hipLaunchKernelGGL(kb_reduce<T>, dim3(g.total()), blk, 0, st, grp(g), dargs, 8, 1);
+    ++nl;
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (ec != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(ec));
+  hipGraphExec_t exec = nullptr;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return set_error(DEFT_ERR_HIP, "batch graph instantiation failed");
+  }
+  gr.graph = graph;
+  gr.exec = exec;
+  gr.at = region;
+  gr.args_bytes = args_bytes;
+  gr.sig = sig;
+  gr.kernels = nl;
+  DEFT_HIP_CHECK(hipGraphLaunch(exec, st));
   if (launches) *launches += nl;
   return DEFT_OK;
 }

@@ -85,6 +85,7 @@ struct deft_plan_s {
   float *vext = nullptr;   // FP32, +inf marks hard walls
   void *spec = nullptr;    // spec_fields half spectra of ks complex values (sized for FP64)
   int spec_fields = 6;     // 6, or 12 once the tensor functional was chosen
+  int batch_index = -1;    // position in its batch
   double *partial = nullptr;
   int pstride = 0;         // partial-sum slot stride (>= blocks of any reducing kernel)
   unsigned long long *hflag = nullptr;  // completion flag, host-mapped pinned
@@ -230,6 +231,9 @@ inline int plan_sync(deft_plan_s *p) {
 struct BatchReq {
   deft_plan_s *p;
   bool grad, have_old, precond;
+  int cg = 0;  // 0: an evaluation; 1: CG update, 2: CG reset (then grad.direction)
+  double beta = 0.0;
+  int have_dir = 0;
 };
 
 // Pinned host staging and its device copy for one round's argument tables.
@@ -240,6 +244,20 @@ struct BatchStage {
   size_t used = 0;
 };
 
+// One precision's graph of a batch round: every pass launched over every grid
+// of the batch (blocks of grids without a matching request return at once),
+// so one captured graph serves every round.  The staging region at `at` holds
+// the grids' BatchArgs (rewritten every round) followed by the group tables
+// (written once); `sig` is the per-plan shape and tuning the graph was
+// captured for.
+struct BatchGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t at = 0, args_bytes = 0;
+  std::vector<int> sig;
+  long long kernels = 0;
+};
+
 // Waits until the work queued so far on the plan's stream has finished: a
 // one-thread kernel raises a host-mapped flag and the host spins on it (a
 // stream synchronisation leaves the GPU idle for ~30 us per wait).  After
@@ -247,21 +265,30 @@ struct BatchStage {
 // synchronisation, which also reports any error; with profiling on, or in a
 // batch, the blocking path is used throughout.
 constexpr long long kSpinLimitUs = 20000;
-inline int plan_wait(deft_plan_s *p) {
-  if (p->prof || p->batch || !p->hflag) return plan_sync(p);
-  const unsigned long long want = ++p->flag_seq;
-  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, p->stream, p->dflag, want);
+// The wait on any stream with its own flag pair; 1 = flag seen, 0 = spin limit
+// reached (the caller then synchronises the stream), negative = launch error.
+inline int flag_wait(hipStream_t st, unsigned long long *hflag, unsigned long long *dflag,
+                     unsigned long long &seq) {
+  const unsigned long long want = ++seq;
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, st, dflag, want);
   const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_signal: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_signal: ") + hipGetErrorString(e)), -1;
   const auto t0 = std::chrono::steady_clock::now();
   for (long long n = 0;; ++n) {
-    if (__atomic_load_n(p->hflag, __ATOMIC_ACQUIRE) >= want) return DEFT_OK;
+    if (__atomic_load_n(hflag, __ATOMIC_ACQUIRE) >= want) return 1;
     if ((n & 1023) == 1023 &&
         std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count() >
             kSpinLimitUs)
-      return plan_sync(p);
+      return 0;
     __builtin_ia32_pause();
   }
+}
+
+inline int plan_wait(deft_plan_s *p) {
+  if (p->prof || p->batch || !p->hflag) return plan_sync(p);
+  const int r = flag_wait(p->stream, p->hflag, p->dflag, p->flag_seq);
+  if (r < 0) return DEFT_ERR_HIP;
+  return r == 1 ? DEFT_OK : plan_sync(p);
 }
 
 // Kernel launch sequences, instantiated for T = float and T = double in
@@ -272,6 +299,10 @@ struct Launch {
   // passes are launched once per group of grids with equal axis length and
   // tuning; enqueued on `st` without synchronising.  launches += kernels run.
   static int evaluate_batch(hipStream_t st, BatchStage &stage, const BatchReq *reqs, int n, long long *launches);
+  // The same through a graph over all `nplans` plans of the batch (plans[i]
+  // has batch_index i), captured on first use into `gr` and replayed after.
+  static int evaluate_batch_graph(hipStream_t st, BatchStage &stage, size_t region, deft_plan_s *const *plans,
+                                  int nplans, const BatchReq *reqs, int n, BatchGraph &gr, long long *launches);
   // Energy (grad=false) or energy + grad + pgrad (grad=true) of the resident
   // Veff after applying pending moves; reduced sums land in p->hscal.
   static int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond);

@@ -247,16 +247,25 @@ int deft_minimize_phase_iterations(deft_minimize m, int *fp32_iterations, int *m
  * A batch joins plans of any grid shapes (uneven edges such as {32, 48, 64,
  * 96} per axis).  deft_batch_minimize runs one minimizer per plan, each with
  * the unchanged algorithm above (and its mixed-precision ladder if set), in a
- * host thread of its own; every evaluation they ask for joins a lockstep
- * round, launched as one kernel per pass and group of grids with equal axis
- * length, with one device synchronisation per round.  Each grid gets bit for
- * bit the energies and iterates of its own unbatched run.  While in a batch a
- * plan's work runs on the batch's stream, and the plan cannot be destroyed. */
+ * fiber (user-level context) of the calling thread; every evaluation and
+ * every CG direction update they ask for joins a lockstep round, launched as
+ * one kernel per pass and group of grids with equal axis length, with one
+ * device wait per round.  deft_batch_set_graphs(batch, 1) makes a round one
+ * hipGraph per precision instead, captured on the first round: every pass
+ * over every grid of the batch, blocks of grids without a matching request
+ * returning at once; later rounds copy the argument table and replay it.
+ * Each grid gets bit for bit the energies and iterates of its own unbatched
+ * run.  While in a batch a plan's work runs on the batch's stream, and the
+ * plan cannot be destroyed. */
 typedef struct deft_batch_s *deft_batch;
 int deft_batch_create(deft_plan *plans, int n, deft_batch *out);
 /* Runs every minimizer to its own stop; minimizers[i] must belong to a plan of
  * the batch, at most one per plan. */
 int deft_batch_minimize(deft_batch batch, deft_minimize *minimizers, int n);
+/* 1: rounds replay captured graphs; 0 (default): each round launches only the
+ * requested passes, grouped afresh (faster on the 64-grid batch: the graphs'
+ * idle blocks cost more than the launches they save).  Not while minimizing. */
+int deft_batch_set_graphs(deft_batch batch, int on);
 /* Rounds run, kernel launches of the rounds, evaluations served. */
 int deft_batch_stats(deft_batch batch, long long *rounds, long long *launches, long long *requests);
 /* Returns the plans to their own streams. */

@@ -39,8 +39,12 @@ def make_min(f, ladder):
     return m
 
 
+@pytest.mark.parametrize("graphs", [True, False])
 @pytest.mark.parametrize("ladder", [True, False])
-def test_batch_follows_unbatched_trajectories(gpu, ladder):
+def test_batch_follows_unbatched_trajectories(gpu, ladder, graphs):
+    """With graphs the same captured graph is replayed every round, with each
+    grid's own eager work (CG updates, reductions) between the replays; every
+    replay's energies must equal the unbatched ones bit for bit."""
     insts = small_c3(5, seed=7)
     prec = deft_amd.FFT_FP32 if ladder else deft_amd.FFT_FP64
     alone = []
@@ -53,7 +57,7 @@ def test_batch_follows_unbatched_trajectories(gpu, ladder):
         f.close()
     fluids = [make_fluid(inst, prec) for inst in insts]
     mins = [make_min(f, ladder) for f in fluids]
-    b = deft_amd.Batch(fluids)
+    b = deft_amd.Batch(fluids, graphs=graphs)
     b.minimize(mins)
     st = b.stats()
     for i, (m, f) in enumerate(zip(mins, fluids)):
@@ -67,6 +71,15 @@ def test_batch_follows_unbatched_trajectories(gpu, ladder):
     assert st["requests"] >= max(a[0]["num_energy_calcs"] for a in alone)
     assert st["rounds"] < st["requests"]
     assert st["rounds"] <= total_evals
+    if graphs:
+        # a second minimization on the same batch replays the graphs captured by the first
+        for f, inst in zip(fluids, insts):
+            f.Veff = np.full(inst[0], -math.log(inst[3] / (4 * math.pi / 3)))
+        mins2 = [make_min(f, ladder) for f in fluids]
+        b.minimize(mins2)
+        for i, m in enumerate(mins2):
+            assert m.energy() == alone[i][1], i
+            m.close()
     b.close()
     for i, f in enumerate(fluids):
         assert np.array_equal(f.Veff, alone[i][2]), i  # the plan is usable again after the batch
