@@ -84,6 +84,7 @@ _SIGNATURES = {
     "deft_plan_set_fft_precision": ([_P, _I], _I),
     "deft_plan_set_mu": ([_P, _D], _I),
     "deft_plan_set_functional": ([_P, _I], _I),
+    "deft_contact_values": ([_P, _DP], _I),
     "deft_bulk_state": ([_I, _D, _D, _D, _DP, _DP, _DP], _I),
     "deft_plan_set_vext": ([_P, _DP], _I),
     "deft_plan_set_veff": ([_P, _DP], _I),
@@ -399,6 +400,13 @@ class WhiteBearFluidVeff:
                                                ctypes.byref(by)))
             out[name.value.decode()] = (n.value, ms.value, by.value)
         return out
+
+    def contact_values(self):
+        """n0, gSigmaS, YuWu_S, nA, gSigmaA, Gross of the resident density (the fields the
+        contact-paper drivers write beside it, papers/contact/figs/inner-sphere.cpp:220-233)."""
+        out = np.empty((7,) + self.shape)
+        _check(lib().deft_contact_values(self._h, _dptr(out)))
+        return {k: out[i] for i, k in enumerate(("n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross", "gSigmaS_cpp"))}
 
     def weighted_densities(self):
         out = np.empty((11 if self._functional == WHITE_BEAR_TENSOR else 5,) + self.shape)

@@ -14,6 +14,7 @@
 
 #include <ucontext.h>
 
+#include "contact.hpp"
 #include "launch.hpp"
 
 using namespace deft;
@@ -1118,6 +1119,51 @@ int deft_weighted_densities(deft_plan p, double *out5) {
   else
     TRY(Launch<float>::weighted_densities(p, p->tmp));
   DEFT_HIP_CHECK(hipMemcpyAsync(out5, p->tmp, nw * sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost, p->stream));
+  TRY(plan_sync(p));
+  return DEFT_OK;
+}
+
+int deft_contact_values(deft_plan p, double *out7) {
+  TRY(check_plan(p));
+  if (!out7) return set_error(DEFT_ERR_ARG, "null output");
+  if (p->functional == DEFT_WHITE_BEAR_TENSOR)
+    return set_error(DEFT_ERR_UNSUPPORTED, "contact values are defined for White-Bear and Mark II");
+  TRY(flush_moves(p));
+  const size_t N = (size_t)p->N;
+  TRY(ensure_tmp(p, 23 * sizeof(double) * N));
+  double *w5 = p->tmp, *shell2 = w5 + 5 * N, *step2 = w5 + 6 * N, *abcd = w5 + 7 * N, *num = w5 + 15 * N,
+         *out = w5 + 16 * N;
+  const GridArgs g = grid_args(p);
+  const double R = p->R;
+  const dim3 eb(p->eblocks), tb(kThreads);
+  const int kblocks = (int)std::min<long long>(4096, (p->Nk + kThreads - 1) / kThreads);
+  cx<double> *f0 = spec_ptr<double>(p, 0), *f2 = spec_ptr<double>(p, 2), *f3 = spec_ptr<double>(p, 3);
+  TRY(Launch<double>::weighted_densities(p, w5));
+  hipLaunchKernelGGL(k_density, eb, tb, 0, p->stream, p->N, p->x, p->vext, p->kT, num);
+  TRY(Launch<double>::fft_forward(p, num, 1.0));
+  hipLaunchKernelGGL(k_spec_apply, dim3(kblocks), tb, 0, p->stream, g, f0, f2, (int)SK_COPY, 0, R, 1.0, 0);
+  const double invN = 1.0 / (double)p->N;
+  hipLaunchKernelGGL(k_spec_apply, dim3(kblocks), tb, 0, p->stream, g, f2, f0, (int)SK_SHELL, 0, 2 * R, 1.0, 0);
+  TRY(Launch<double>::fft_inverse(p, shell2, invN));
+  hipLaunchKernelGGL(k_spec_apply, dim3(kblocks), tb, 0, p->stream, g, f2, f0, (int)SK_STEP, 0, 2 * R, 1.0, 0);
+  TRY(Launch<double>::fft_inverse(p, step2, invN));
+  const double s3 = p->functional == DEFT_WHITE_BEAR ? 0.0 : (p->functional == DEFT_WHITE_BEAR_MARK_II_SLOW ? 1.0 : 1.5);
+  hipLaunchKernelGGL(k_contact_point, eb, tb, 0, p->stream, p->N, w5, shell2, step2, R, s3, out, abcd);
+  // gSigmaA numerator: shell(A) - shell'(B) + vshell'.(C) + vshell.(D)
+  const int kinds[8] = {SK_SHELL, SK_SHELL_PRIME, SK_VSHELL_PRIME, SK_VSHELL_PRIME, SK_VSHELL_PRIME,
+                        SK_VSHELL, SK_VSHELL, SK_VSHELL};
+  const int comps[8] = {0, 0, 0, 1, 2, 0, 1, 2};
+  const double coefs[8] = {1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+  for (int f = 0; f < 8; ++f) {
+    TRY(Launch<double>::fft_forward(p, abcd + f * N, 1.0));
+    hipLaunchKernelGGL(k_spec_apply, dim3(kblocks), tb, 0, p->stream, g, f0, f3, kinds[f], comps[f], R, coefs[f],
+                       f > 0 ? 1 : 0);
+  }
+  hipLaunchKernelGGL(k_spec_apply, dim3(kblocks), tb, 0, p->stream, g, f3, f0, (int)SK_COPY, 0, R, 1.0, 0);
+  TRY(Launch<double>::fft_inverse(p, num, invN));
+  hipLaunchKernelGGL(k_divide, eb, tb, 0, p->stream, p->N, num, shell2, out + 4 * N);
+  DEFT_HIP_CHECK(hipGetLastError());
+  DEFT_HIP_CHECK(hipMemcpyAsync(out7, out, 7 * sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
   TRY(plan_sync(p));
   return DEFT_OK;
 }

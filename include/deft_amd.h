@@ -172,6 +172,15 @@ int deft_functional_grad_device(deft_plan plan, const double *veff_dev, double *
  * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R).
  * Tensor White-Bear plans add the shell tensor T_xx, T_yy, T_zz, T_xy, T_yz, T_zx (11*N). */
 int deft_weighted_densities(deft_plan plan, double *out5);
+/* Contact-density estimators of the resident density, the six fields the
+ * contact-paper drivers write beside it (papers/contact/figs/inner-sphere.cpp:
+ * 220-233, radial_plot columns 3-8): n0, gSigmaS, YuWu_S, nA, gSigmaA, Gross
+ * (src/haskell/WhiteBear.hs:68-142 gSigmaS/gSigmaA/nA/correlation_gross with the
+ * plan's White-Bear or Mark II Phi, src/ContactDensity.cpp:362-400 YuWu), then
+ * gSigmaS as the C++ by-hand gSigmaScpp / gSigmaSm2cpp writes it
+ * (src/ContactDensity.cpp:231-355).  FP64; 7*N doubles, field-major; NaN where
+ * n2 = 0 (deep inside hard walls).  Not for the tensor functional. */
+int deft_contact_values(deft_plan plan, double *out7);
 
 /* --- FFT seam --------------------------------------------------------------
  * fft(gd, f) = r2c * dV (src/Grid.cpp:470-478): in N doubles, out 2*Nk doubles.

@@ -788,3 +788,119 @@ def minimize(f, relative_precision=1e-12, maxiter=100000, precondition=True, min
         if callback is not None:
             callback(m)
     return m
+
+
+# ---------------------------------------------------------------------------
+# Contact-density estimators of the contact paper (post-processing of a
+# converged density; src/haskell/WhiteBear.hs:68-138, src/ContactDensity.cpp:255-400)
+# ---------------------------------------------------------------------------
+
+def _independent_partials(n3, n2, vx, vy, vz, R, functional):
+    """dPhi/dn0, dn1, dn2, dn3 and the vector partials dPhi/dn1v, dPhi/dn2v with
+    n0, n1, n2, n3, n1v, n2v independent (the Haskell `derive a 1 phit` of
+    src/haskell/WhiteBear.hs:86-87), for Phi2 = g2 (n1 n2 - n1v.n2v) and
+    Phi3 = g3 n2 (n2^2 - 3 n2v.n2v)/(36 pi): White-Bear g2 = 1/(1-n3), g3 = f3;
+    Mark II its factors."""
+    omn = 1.0 - n3
+    L = np.log1p(-n3)
+    if functional == "wb":
+        g3, g3p = f3_and_derivative(n3)
+        g2, g2p = 1.0 / omn, 1.0 / (omn * omn)
+    elif functional in ("wb2", "wb2slow"):
+        g2, g2p, g3, g3p = markii_factors(n3, g3_scale=1.0 if functional == "wb2slow" else 1.5)
+    else:
+        raise ValueError(functional)
+    n0 = n2 / (4 * math.pi * R * R)
+    n1 = n2 / (4 * math.pi * R)
+    v = (vx, vy, vz)
+    v1 = [c / (4 * math.pi * R) for c in v]
+    v2 = vx * vx + vy * vy + vz * vz
+    vtt = n2 * (n2 * n2 - 3 * v2)
+    c36 = 1.0 / (36 * math.pi)
+    dn0 = -L
+    dn1 = g2 * n2
+    dn2 = g2 * n1 + g3 * (3 * n2 * n2 - 3 * v2) * c36
+    dn3 = n0 / omn + g2p * (n1 * n2 - (v1[0] * vx + v1[1] * vy + v1[2] * vz)) + g3p * vtt * c36
+    dn1v = [-g2 * c for c in v]
+    dn2v = [-g2 * a - g3 * 6 * n2 * c * c36 for a, c in zip(v1, v)]
+    return dn0, dn1, dn2, dn3, dn1v, dn2v
+
+
+def contact_values(fluid, functional=None):
+    """The six fields the contact-paper drivers write next to the density
+    (papers/contact/figs/inner-sphere.cpp:220-233): n0, gSigmaS, YuWu_S, nA,
+    gSigmaA and the Gross estimate, in that order, then gSigmaS in the C++
+    by-hand form (gSigmaScpp / gSigmaSm2cpp, src/ContactDensity.cpp:231-255,
+    269-355): dA/dR = n2 (dn3 - dn1/(4 pi R^2) - dn0/(2 pi R^3))
+    - n2' (dn2 + dn1/(4 pi R) + dn0/(4 pi R^2)) - sum_j n2v_j (c2 n2' + c1 n2'/(4 pi R)
+    + c1 n2v_j/(4 pi R^2)) with n2' = -2 n2/R for every component and c2, c1 the
+    vector partials over n2v.
+
+    gSigmaS = dA/dR / (kT n0^2 4 pi (2R)^2) with the local
+      dA/dR/kT = dn3 n2 + 2 n2 dn2/R + n0 dn1 + (2/R) dn2v.n2v + 3 dn1v.n2v/(4 pi R^2)
+      (src/haskell/WhiteBear.hs:76-87 with n2' = -2 n2/R, n2v' = -2 n2v/R; the dn0 term cancels);
+    gSigmaA = [shell(A) - shell'(B) + vshell'.(C) + vshell.(D)] / shell_2R(n)
+      with A = dn3 - dn1/(4 pi R^2) - dn0/(2 pi R^3), B = dn2 + dn1/(4 pi R) + dn0/(4 pi R^2),
+      C = dn2v + dn1v/(4 pi R), D = dn1v/(4 pi R^2)
+      (src/haskell/WhiteBear.hs:119-138; shell' = s 4 pi (-sin kR/k - R cos kR),
+      vshell'_j = i s 4 pi R sin(kR)/k k_j, src/haskell/FMT.hs:66-81);
+    YuWu_S = gHS zeta^2 with gHS of Fu and Wu 2005 eq. 13 (src/ContactDensity.cpp:362-400);
+    Gross = (1 - eta/2)/(1 - eta)^3, eta = step_2R(n)/8 (src/haskell/WhiteBear.hs:140-142);
+    nA = shell_2R(n)/(4 pi (2R)^2).
+    """
+    functional = fluid.functional if functional is None else functional
+    g = fluid.grid
+    R = fluid.R
+    n = fluid.density()
+    n3, n2, vx, vy, vz = fluid.weighted_densities(n)[:5]
+    nk = _rfftn(n)
+    w3d, w2d, _ = weights(g, 2 * R)
+    shell2 = _irfftn(nk * w2d, g.shape)
+    step2 = _irfftn(nk * w3d, g.shape)
+    dn0, dn1, dn2, dn3, dn1v, dn2v = _independent_partials(n3, n2, vx, vy, vz, R, functional)
+    n0 = n2 / (4 * math.pi * R * R)
+    fpR, fpR2 = 4 * math.pi * R, 4 * math.pi * R * R
+    v = (vx, vy, vz)
+    dAdR = (dn3 * n2 + 2 * n2 * dn2 / R + n0 * dn1
+            + (2 / R) * sum(a * c for a, c in zip(dn2v, v)) + 3 * sum(a * c for a, c in zip(dn1v, v)) / (fpR * R))
+    gS = dAdR / (n0 * n0 * 4 * math.pi * (2 * R) ** 2)
+    # kernels of the A-estimator
+    k = g.k
+    dr = g.dV ** (1.0 / 3.0)
+    s = np.exp(-SPREADING * k * k * dr * dr)
+    kR = k * R
+    ks = np.where(k > 0, k, 1.0)
+    w3, w2, wv = weights(g, R)
+    shellp = np.where(k > 0, s * 4 * math.pi * (-np.sin(kR) / ks - R * np.cos(kR)), -8 * math.pi * R)
+    vsp = np.where(k > 0, s * 4 * math.pi * R * np.sin(kR) / ks, 0.0)
+    kv = (g.kx, g.ky, g.kz)
+    A = dn3 - dn1 / fpR2 - dn0 / (2 * math.pi * R ** 3)
+    B = dn2 + dn1 / fpR + dn0 / fpR2
+    acc = _rfftn(A) * w2 - _rfftn(B) * shellp
+    for j in range(3):
+        C = dn2v[j] + dn1v[j] / fpR
+        D = dn1v[j] / fpR2
+        acc = acc + _rfftn(C) * (1j * vsp * kv[j]) + _rfftn(D) * (1j * wv * kv[j])
+    gA = _irfftn(acc, g.shape) / shell2
+    invd = 1.0 / (1.0 - n3)
+    zeta2 = (R / 3.0) * n2
+    zeta = 1.0 - (vx * vx + vy * vy + vz * vz) / (n2 * n2)
+    ghs = invd * (1.0 + 0.5 * (invd * zeta2) * zeta * (3.0 + invd * zeta2))
+    yuwu = ghs * zeta * zeta
+    eta = step2 / 8.0
+    gross = (1 - 0.5 * eta) / (1 - eta) ** 3
+    nA = shell2 / (4 * math.pi * (2 * R) ** 2)
+    n2p = -2 * n2 / R
+    # dn1v = c1 n2v, dn2v = c2 n2v with the scalar factors
+    if functional == "wb":
+        g2 = 1.0 / (1.0 - n3)
+        g3 = f3_and_derivative(n3)[0]
+    else:
+        g2, _, g3, _ = markii_factors(n3, g3_scale=1.0 if functional == "wb2slow" else 1.5)
+    c1 = -g2
+    c2 = -g2 / fpR - g3 * 6 * n2 / (36 * math.pi)
+    cpp = n2 * (dn3 - dn1 / fpR2 - dn0 / (2 * math.pi * R ** 3)) - n2p * (dn2 + dn1 / fpR + dn0 / fpR2)
+    for c in v:
+        cpp = cpp - c * (c2 * n2p + c1 * n2p / fpR + c1 * c / fpR2)
+    gS_cpp = cpp / (4 * math.pi * (2 * R) ** 2) / (n0 * n0)
+    return n0, gS, yuwu, nA, gA, gross, gS_cpp

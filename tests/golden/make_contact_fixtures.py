@@ -8,11 +8,12 @@ solute of excluded-centre radius 2 (diameter argument 4, notinwall at :30-38) in
 periodic 20^3 cell (padding 16), grid 201^3 (GridDescription(lat, 0.1),
 src/GridDescription.cpp:28), R = 1, kT = 1, mu from the bulk, Veff started at
 -ln(n_b c + 1e-4 n_b), old-API preconditioned CG to |E_bulk V| 1e-10 for at most 200
-iterations.  Column 0 is y of grid point (0, y, 0), column 1 the density there
-(the remaining columns are contact-value estimators outside this path).
+iterations.  Column 0 is y of grid point (0, y, 0), column 1 the density there,
+columns 2-7 the contact-density estimators written beside it (:220-233): n0,
+gSigmaS, YuWu_S, nA, gSigmaA, Gross.
 "WB" is HardSpheresNoTensor2 (White Bear), "WBT" TensorWhiteBear (files at eta 0.30
 and 0.40 only), "WBm2" WhiteBearMarkII.
-Only those two columns, as printed (%g), are kept.
+The columns are kept as printed (%g).
 """
 
 import json
@@ -29,15 +30,17 @@ def main():
             path = os.path.join(REF, "inner-sphere%s-04.0-%s.dat" % (name, eta))
             if not os.path.exists(path):
                 continue
-            y, n = [], []
+            rows = []
             with open(path) as fh:
                 for line in fh:
                     cols = line.split()
-                    if len(cols) < 2:
+                    if len(cols) < 8:
                         continue
-                    y.append(float(cols[0]))
-                    n.append(float(cols[1]))
-            out["profiles"]["%s-%s" % (name, eta)] = {"eta": float(eta), "y": y, "density": n}
+                    rows.append([float(c) for c in cols[:8]])
+            prof = {"eta": float(eta), "y": [r[0] for r in rows], "density": [r[1] for r in rows]}
+            for c, key in enumerate(("n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross")):
+                prof[key] = [r[2 + c] for r in rows]
+            out["profiles"]["%s-%s" % (name, eta)] = prof
     with open(os.path.join(HERE, "contact_inner_sphere.json"), "w") as fh:
         json.dump(out, fh, indent=0)
 

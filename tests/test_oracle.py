@@ -513,3 +513,19 @@ def test_tensor_gradient_and_uniform_fluid(shape):
     p = O.pressure(O.bulk_density(0.4, 1.0), 1.0)
     assert abs(Eu + p * u.grid.volume) < 1e-12 * p * u.grid.volume
     assert np.max(np.abs(pgu)) < 1e-12
+
+
+@pytest.mark.parametrize("fn", ["wb", "wb2"])
+@pytest.mark.parametrize("eta", [0.1, 0.3, 0.45])
+def test_contact_values_bulk_identities(fn, eta):
+    """In a uniform fluid dA/dR is the R-derivative of the bulk free energy, so both
+    gSigma estimators (and Yu-Wu, Gross) give the Carnahan-Starling contact value
+    (1 - eta/2)/(1 - eta)^3 through the virial route; n0 = nA = n."""
+    g = O.Grid((16, 12, 20), np.diag([3.2, 2.4, 4.0]))
+    f = O.make_fluid(g, eta, functional=fn)
+    n0, gS, yuwu, nA, gA, gross, gS_cpp = O.contact_values(f)
+    cs = (1 - eta / 2) / (1 - eta) ** 3
+    nb = O.bulk_density(eta, 1.0)
+    for v in (gS, yuwu, gA, gross, gS_cpp):
+        assert np.max(np.abs(v / cs - 1)) < 1e-12
+    assert np.max(np.abs(n0 / nb - 1)) < 1e-12 and np.max(np.abs(nA / nb - 1)) < 1e-12
