@@ -56,6 +56,7 @@ def test_radial_and_z_plots_read_back(gpu, tmp_path):
     drivers.z_plot(str(q), f)
     rz = np.loadtxt(str(q))
     assert rz.shape == (16, 9) and np.allclose(rz[:, 1], rz[:, 2])
+    f.energy()
     text = drivers.printme(f, "  ")
     assert "total energy =" in text and len(text.splitlines()) == 7
     m.close()
