@@ -196,6 +196,7 @@ struct deft_minimize_s {
   bool s2_accepted = true;  // the last quadratic step kept its trial point s2
   double good_step = 0.0;   // step of the last iteration that clearly lowered the energy
   int collapse_restarts = 0;  // final-phase restarts after a collapsed last step
+  bool restart_met = false;   // the iteration before a collapse restart met the precision
 };
 
 // ===========================================================================
@@ -496,12 +497,17 @@ int improve_energy(deft_minimize_s *m, int verbose, int *more) {
 int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   // CONVERGED here means the last iteration met the precision and the final
   // phase is confirming it
-  const bool met = m->switch_rp > 0 && m->phase >= 2 && m->stop == DEFT_STOP_CONVERGED;
+  // (a no-change verdict still inside the confirmation window counts as met:
+  // two line searches in a row returned to their start point)
+  const bool met = m->switch_rp > 0 && m->phase >= 2 &&
+                   (m->stop == DEFT_STOP_CONVERGED || m->stop == DEFT_STOP_NO_CHANGE || m->restart_met);
+  m->restart_met = false;
   TRY(improve_energy(m, verbose, more));
   // A confirming iteration whose line search runs into the round-off of the
   // FP64 energies (no lower energy found, the point is left unchanged) keeps
   // the verdict of the iteration before it, as the reference would have
-  // stopped there.
+  // stopped there; so does the first iteration after a collapse restart (the
+  // fresh steepest-descent direction finds no lower energy either).
   if (met && !*more &&
       (m->stop == DEFT_STOP_ENERGY_ROSE || m->stop == DEFT_STOP_NO_CHANGE ||
        m->stop == DEFT_STOP_NEGATIVE_CURVATURE)) {
@@ -515,6 +521,7 @@ int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
   if (!*more && m->switch_rp > 0 && m->phase >= 2 && m->stop == DEFT_STOP_CONVERGED && m->good_step > 0 &&
       std::fabs(m->step) < 1e-3 * std::fabs(m->good_step) && m->collapse_restarts < kMaxCollapseRestarts) {
     m->collapse_restarts++;
+    m->restart_met = true;
     m->have_dir = false;
     m->have_old = false;
     m->oldgradsqr = 1.0;

@@ -745,11 +745,21 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, bx);
 }
 
+// At most 128 VGPRs (four waves per SIMD): the FP64 energy+gradient pass at
+// 512-value blocks needs 132 otherwise and drops to three waves.
 template <typename T, int L, bool GRAD, int ZB, int FN = 0>
-__global__ __launch_bounds__(kThreads) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
+__global__ __launch_bounds__(kThreads, 4) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
                                                    double* __restrict__ partial, int pstride,
                                                    const cx<T>* __restrict__ tw) {
   zphi_body<T, L, GRAD, ZB, FN>(blockIdx.x, g, fields, partial, pstride, tw);
+}
+
+// the eleven-field tensor variant keeps the default register budget
+template <typename T, int L, bool GRAD, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zphi11(GridArgs g, cx<T>* __restrict__ fields,
+                                                     double* __restrict__ partial, int pstride,
+                                                     const cx<T>* __restrict__ tw) {
+  zphi_body<T, L, GRAD, ZB, 3>(blockIdx.x, g, fields, partial, pstride, tw);
 }
 
 // ---------------------------------------------------------------------------

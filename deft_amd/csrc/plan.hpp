@@ -117,7 +117,7 @@ struct deft_plan_s {
   hipStream_t own_stream = nullptr;  // the plan's stream while it is in a batch
   // tuning per FFT precision (index 0 FP32, 1 FP64): z-pass block size (complex
   // values), y-pass and x-pass tile bytes; set by deft_plan_autotune
-  int tz[2] = {1024, 1024};
+  int tz[2] = {1024, 512};  // (FP64 at 1024 exceeds the Phi pass register budget)
   int ty[2] = {16384, 16384};
   int tx[2] = {8192, 8192};
 };

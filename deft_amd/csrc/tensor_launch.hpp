@@ -41,10 +41,10 @@ void TensorLaunch<T>::zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb)
       ProfScope ps(p, kid<T>(grad ? K_ZPHI_EG : K_ZPHI_E), (grad ? 2 : 1) * nw * cb);
       const dim3 grid(zblocks_for(p, ZB));
       if (tensor && grad)
-        hipLaunchKernelGGL((k_zphi<T, L, true, ZB, 3>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+        hipLaunchKernelGGL((k_zphi11<T, L, true, ZB>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
                            p->partial, p->pstride, tw_ptr<T>(p, 2));
       else if (tensor)
-        hipLaunchKernelGGL((k_zphi<T, L, false, ZB, 3>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
+        hipLaunchKernelGGL((k_zphi11<T, L, false, ZB>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
                            p->partial, p->pstride, tw_ptr<T>(p, 2));
       else if (grad)
         hipLaunchKernelGGL((k_zphi<T, L, true, ZB, 1>), grid, dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 1),
