@@ -248,6 +248,9 @@ def main():
               f"and shard by instance); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     runs, gold, precision = gpu_run(args.steps, args.warmup)
+    # attainable HBM bandwidth on this device (2 GiB streaming copy, larger than the
+    # 256 MB last-level cache), next to the spec peak
+    copy_gbps = D.copy_bandwidth(1 << 31, 10)
     w, pl = runs["warm"], runs["plain"]
     value = args.steps / w["elapsed"]
     plain_value = args.steps / pl["elapsed"]
@@ -292,6 +295,7 @@ def main():
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
                      "kernel_time_fraction_of_wall": total_ms * 1e-3 / runs["profiled_elapsed"],
+                     "streaming_copy_GBps": copy_gbps, "frac_of_streaming_copy": achieved / copy_gbps,
                      "measured": f"HIP events around every launch over {args.steps} further warm-start steps "
                                  f"({runs['profiled_elapsed'] / args.steps * 1e3:.2f} ms per step with events)"},
         "cpu_baseline": cpu,

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "deft_plan_set_mu": ([_P, _D], _I),
     "deft_plan_set_functional": ([_P, _I], _I),
     "deft_contact_values": ([_P, _DP], _I),
+    "deft_copy_bandwidth": ([ctypes.c_longlong, _I, _DP], _I),
     "deft_bulk_state": ([_I, _D, _D, _D, _DP, _DP, _DP], _I),
     "deft_plan_set_vext": ([_P, _DP], _I),
     "deft_plan_set_veff": ([_P, _DP], _I),
@@ -191,6 +192,13 @@ def supported_length(n):
 # ---------------------------------------------------------------------------
 # bulk path
 # ---------------------------------------------------------------------------
+
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Attainable HBM bandwidth (GB/s) of a streaming device copy."""
+    out = ctypes.c_double()
+    _check(lib().deft_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(out)))
+    return out.value
+
 
 def bulk_state(n, R=1.0, kT=1.0, functional=WHITE_BEAR):
     """(f, mu, p) of ideal gas + the hard-sphere functional at bulk density n."""

@@ -1130,6 +1130,60 @@ int deft_weighted_densities(deft_plan p, double *out5) {
   return DEFT_OK;
 }
 
+// four independent 16-byte loads in flight per thread and iteration
+static __global__ __launch_bounds__(kThreads) void k_stream_copy(long long n, const double2 *__restrict__ in,
+                                                                   double2 *__restrict__ out) {
+  const long long stride = (long long)gridDim.x * kThreads;
+  long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double2 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+    out[i] = a;
+    out[i + stride] = b;
+    out[i + 2 * stride] = c;
+    out[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) out[i] = in[i];
+}
+
+int deft_copy_bandwidth(long long bytes, int reps, double *gbps) {
+  if (!gbps || bytes < (1 << 20) || reps < 1) return set_error(DEFT_ERR_ARG, "need >= 1 MiB, reps >= 1 and an output");
+  const long long n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DEFT_OK;
+  if (hipMalloc(&a, 16 * n) != hipSuccess || hipMalloc(&b, 16 * n) != hipSuccess) {
+    rc = set_error(DEFT_ERR_ALLOC, "copy buffers");
+  } else if (hipMemset(a, 0, 16 * n) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = set_error(DEFT_ERR_HIP, "copy set-up");
+  } else {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // the best of several launch widths (blocks per CU)
+    double best = 0.0;
+    for (int per_cu : {2, 4, 8, 16, 32}) {
+      const dim3 grid((unsigned)std::min<long long>((long long)cus * per_cu, (n + kThreads - 1) / kThreads));
+      hipLaunchKernelGGL(k_stream_copy, grid, dim3(kThreads), 0, nullptr, n, a, b);
+      (void)hipEventRecord(e0, nullptr);
+      for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, grid, dim3(kThreads), 0, nullptr, n, a, b);
+      (void)hipEventRecord(e1, nullptr);
+      float ms = 0.f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.f)) {
+        rc = set_error(DEFT_ERR_HIP, "copy timing");
+        break;
+      }
+      best = std::max(best, 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) / 1e9);
+    }
+    if (rc == DEFT_OK) *gbps = best;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 int deft_contact_values(deft_plan p, double *out7) {
   TRY(check_plan(p));
   if (!out7) return set_error(DEFT_ERR_ARG, "null output");

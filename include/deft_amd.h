@@ -291,6 +291,12 @@ int deft_plan_profile_reset(deft_plan plan);
 int deft_kernel_count(void);
 int deft_plan_profile_get(deft_plan plan, int kernel, const char **name, long long *launches, double *total_ms,
                           double *total_bytes);
+/* Attainable HBM bandwidth on the current device: a streaming copy of `bytes`
+ * (16-byte vector loads and stores, four in flight per thread) timed with HIP
+ * events over `reps` launches after one warm-up, best of 2-32 blocks per CU;
+ * GB/s = 2 bytes / mean time.  A yardstick for the per-kernel figures above
+ * (no reference counterpart). */
+int deft_copy_bandwidth(long long bytes, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -299,3 +299,11 @@ def test_every_tuning_variant_matches_oracle(gpu, prec, etol, gtol):
     E, gr, pg = f.energy_grad_and_precond()
     assert abs(E - Eo) <= etol * abs(Eo)
     assert np.array_equal(f.Veff, of.x)  # tuning never touches the state
+
+
+def test_copy_bandwidth_is_a_sane_yardstick(gpu):
+    """The streaming-copy figure the bench reports next to the spec peak."""
+    g = gpu.copy_bandwidth(1 << 28, 5)
+    assert 500.0 < g < 8000.0, g
+    with pytest.raises(gpu.DeftError):
+        gpu.copy_bandwidth(1000, 5)
