@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kNT) void k_boltz(const float *__restrict__ vext, f
                                                float *__restrict__ boltz, long long n) {
   for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long long)gridDim.x * kNT) {
     const float v = vext[i];
-    boltz[i] = (v == INFINITY) ? 0.f : expf(-v * inv_kT);
+    boltz[i] = isfinite(v) ? expf(-v * inv_kT) : 0.f;  // walls and constrained points
   }
 }
 
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(kNT) void k_boltz_coarse(const float *__restrict__ 
           const float wc = (c == -h || c == h) ? 0.5f : 1.f;
           const int z = (f * Z + c + M) % M;
           const float v = vext[((long long)x * M + y) * M + z];
-          acc += wa * wb * wc * ((v == INFINITY) ? 0.f : expf(-v * inv_kT));
+          acc += wa * wb * wc * (isfinite(v) ? expf(-v * inv_kT) : 0.f);
         }
       }
     }

@@ -266,10 +266,13 @@ __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, dou
         x[idx] = xv;
       }
       const float ve = vext[idx];
-      if (isfinite(ve)) {
+      // +inf: hard wall, n = 0; NaN: constrained point (the old-API
+      // constrain(mask, f), src/Functional.cpp:759-838): its density counts in
+      // the energy, without external potential, and its gradient is zero
+      if (!isinf(ve)) {
         nv = exp(-xv / g.kT);
         acc[0] += -nv * (xv + g.kT * g.logNQ + g.kT);
-        acc[1] += (double)ve * nv;
+        if (!isnan(ve)) acc[1] += (double)ve * nv;
         acc[2] += -g.mu * nv;
       }
     }
@@ -877,7 +880,7 @@ static __global__ __launch_bounds__(kThreads) void k_density(long long N, const 
                                                              const float* __restrict__ vext, double kT,
                                                              double* __restrict__ n) {
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)
-    n[i] = isfinite(vext[i]) ? exp(-x[i] / kT) : 0.0;
+    n[i] = isinf(vext[i]) ? 0.0 : exp(-x[i] / kT);
 }
 
 // out += a (accumulating into caller buffers, as deft's grad does)

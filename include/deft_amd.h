@@ -138,9 +138,13 @@ int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile,
 int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
-/* DEFT_WHITE_BEAR (default) or DEFT_WHITE_BEAR_MARK_II; drops nothing else. */
+/* A deft_functional_kind (default DEFT_WHITE_BEAR); drops nothing else. */
 int deft_plan_set_functional(deft_plan plan, int functional);
-/* Vext (N doubles, host); stored as FP32 on the device (+inf kept). */
+/* Vext (N doubles, host); stored as FP32 on the device.  +inf marks a hard wall
+ * (density exactly 0); NaN marks a constrained point as the old API's
+ * constrain(mask, f) (src/Functional.cpp:759-838) treats mask = 0: its density
+ * exp(-Veff/kT) counts in the energy (no external potential), its gradient and
+ * pgrad are zero, so its Veff stays where the caller put it. */
 int deft_plan_set_vext(deft_plan plan, const double *vext);
 int deft_plan_set_veff(deft_plan plan, const double *veff);
 /* Veff = value at every point, on the device (the uniform start -kT ln n_b of

@@ -428,7 +428,11 @@ class WhiteBearFluidVeff:
         self.kT = float(kT)
         self.mu = float(mu)
         self.vext = np.asarray(vext, dtype=np.float64).reshape(grid.shape)
+        # +inf: hard wall (n = 0); NaN: constrained point of the old API's
+        # constrain(mask, f) (src/Functional.cpp:759-838): density and energy as
+        # usual without external potential, gradient and pgrad zero
         self.finite = np.isfinite(self.vext)
+        self.present = ~np.isinf(self.vext)
         self.vext_f = np.where(self.finite, self.vext, 0.0)
         self.x = np.array(x, dtype=np.float64).reshape(grid.shape)
         w3, w2, wv = weights(grid, self.R)
@@ -443,7 +447,7 @@ class WhiteBearFluidVeff:
 
     def density(self, x=None):
         x = self.x if x is None else x
-        return np.where(self.finite, np.exp(-x / self.kT), 0.0)
+        return np.where(self.present, np.exp(-x / self.kT), 0.0)
 
     def _phi(self, nal, partials):
         if self.functional == "wbt":
@@ -456,7 +460,7 @@ class WhiteBearFluidVeff:
 
     def _local_terms(self, x, n):
         kT = self.kT
-        ideal = np.where(self.finite, -n * (x + kT * math.log(nQ(kT)) + kT), 0.0)
+        ideal = np.where(self.present, -n * (x + kT * math.log(nQ(kT)) + kT), 0.0)
         ext = self.vext_f * n
         mun = -self.mu * n
         return ideal, ext, mun

@@ -117,6 +117,39 @@ def test_inner_sphere_profile_matches_reference(gpu, solute, key):
             assert rel_c.max() < 3e-5, c
 
 
+@pytest.mark.parametrize("key", ["WB-0.40", "WBm2-0.10"])
+def test_reference_constraint_setup_reproduces_the_whole_column(gpu, key):
+    """The driver's own set-up (inner-sphere.cpp:186-198): constrain(mask, f) with the
+    solute points masked (NaN in Vext) and Veff started at -ln(n_b mask + 1e-4 n_b);
+    the frozen interior then shows the 1e-4 n_b density the files print there."""
+    fx = FIX["profiles"][key]
+    n, L = FIX["n"], FIX["cell"]
+    eta = fx["eta"]
+    nb = eta / (4 * math.pi / 3)
+    functional = deft_amd.WHITE_BEAR_MARK_II_SLOW if key.startswith("WBm2") else deft_amd.WHITE_BEAR
+    wall = W.sphere_vext((n, n, n), (L, L, L), (0.0, 0.0, 0.0), FIX["diameter"] / 2)
+    inside = np.isinf(wall)
+    f = deft_amd.WhiteBearFluidVeff((n, n, n), np.diag([L] * 3), R=FIX["R"], kT=FIX["kT"],
+                                    mu=deft_amd.find_chemical_potential(nb, functional=functional),
+                                    fft_precision=deft_amd.FFT_FP32, functional=functional)
+    f.Vext = np.where(inside, np.nan, 0.0)
+    f.Veff = -np.log(np.where(inside, 0.0, nb) + 1e-4 * nb)
+    m = deft_amd.Minimize(f)
+    m.precondition(True)
+    m.set_relative_precision(1e-12)
+    m.set_mixed_precision(1e-5, deft_amd.FFT_FP64)
+    m.set_maxiter(3000)
+    m.run()
+    assert m.stats()["stop_reason"] == "converged"
+    prof = np.asarray(f.density())[0, :n // 2, 0]
+    ref = np.array(fx["density"])
+    rel = np.abs(prof - ref) / ref
+    print(key, "whole column max rel", rel.max(), "mean", rel.mean(), flush=True)
+    assert rel.max() < 4e-4 and rel.mean() < 5e-5
+    m.close()
+    f.close()
+
+
 def test_haskell_mark_ii_is_not_what_the_published_profiles_used(gpu, solute):
     fx = FIX["profiles"]["WBm2-0.40"]
     _, prof = _profile(solute, fx["eta"], deft_amd.WHITE_BEAR_MARK_II)

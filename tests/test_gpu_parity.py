@@ -307,3 +307,22 @@ def test_copy_bandwidth_is_a_sane_yardstick(gpu):
     assert 500.0 < g < 8000.0, g
     with pytest.raises(gpu.DeftError):
         gpu.copy_bandwidth(1000, 5)
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+def test_constrained_points_match_oracle(gpu, prec, etol, gtol):
+    """NaN in Vext: the old API's constrain(mask, f) (density counts, gradient zero)."""
+    shape, cell = (32, 32, 32), np.diag([4.0, 4.0, 4.0])
+    g = O.Grid(shape, cell)
+    wall = O.spherical_solute_vext(g, 1.3, center=(2.0, 2.0, 2.0))
+    vext = np.where(np.isinf(wall), np.nan, 0.3 * np.cos(g.positions()[..., 0]))
+    vext = vext.astype(np.float32).astype(np.float64)
+    g, of, f = _fluid(gpu, shape, cell, vext=vext, prec=prec, seed=8)
+    E, gr, pg = f.energy_grad_and_precond()
+    Eo, gro, pgo = of.energy_grad_and_precond()
+    assert abs(E - Eo) <= etol * abs(Eo)
+    assert _rel(gr, gro) < gtol and _rel(pg, pgo) < gtol
+    inside = np.isnan(vext)
+    assert np.all(gr[inside] == 0) and np.all(pg[inside] == 0)
+    n = f.density()
+    assert np.all(n[inside] > 0)

@@ -529,3 +529,24 @@ def test_contact_values_bulk_identities(fn, eta):
     for v in (gS, yuwu, gA, gross, gS_cpp):
         assert np.max(np.abs(v / cs - 1)) < 1e-12
     assert np.max(np.abs(n0 / nb - 1)) < 1e-12 and np.max(np.abs(nA / nb - 1)) < 1e-12
+
+
+def test_constrained_points_follow_the_reference_constraint():
+    """NaN in Vext is constrain(mask, f) with mask 0 (src/Functional.cpp:759-838): the energy
+    is that of the unconstrained functional (no external potential there), the gradient
+    is the unconstrained one times the mask."""
+    g = O.Grid((16, 16, 20), np.diag([3.2, 3.0, 3.4]))
+    rng = np.random.default_rng(21)
+    mask = O.spherical_solute_vext(g, 1.1, center=(1.6, 1.5, 1.7))  # inf inside
+    vc = np.where(np.isinf(mask), np.nan, 0.0)
+    fc = O.make_fluid(g, 0.3, vext=vc)
+    fu = O.make_fluid(g, 0.3, vext=np.zeros(g.shape))
+    x = fc.x + 0.3 * rng.standard_normal(g.shape)
+    fc.x[...] = x
+    fu.x[...] = x
+    Ec, gc, pc = fc.energy_grad_and_precond()
+    Eu, gu, pu = fu.energy_grad_and_precond()
+    assert Ec == Eu
+    inside = np.isnan(vc)
+    assert inside.any() and np.all(gc[inside] == 0) and np.all(pc[inside] == 0)
+    assert np.array_equal(gc[~inside], gu[~inside]) and np.array_equal(pc[~inside], pu[~inside])
