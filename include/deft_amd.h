@@ -138,7 +138,7 @@ int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile,
 int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
-/* A deft_functional_kind (default DEFT_WHITE_BEAR); drops nothing else. */
+/* One of the deft_functional_kind values, default DEFT_WHITE_BEAR; drops nothing else. */
 int deft_plan_set_functional(deft_plan plan, int functional);
 /* Vext (N doubles, host); stored as FP32 on the device.  +inf marks a hard wall
  * (density exactly 0); NaN marks a constrained point as the old API's
