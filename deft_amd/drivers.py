@@ -74,15 +74,22 @@ def print_double(prefix, x, width=26, digits=14, min_digits=11):
 
 def printme(fluid, prefix=""):
     """The per-term energy report of the generated printme for the last evaluation
-    (names padded to the longest one, then print_double of the value)."""
+    (names padded to one more than the longest, "total energy" included in that
+    width, then print_double of the value), one line per term."""
     terms = dict(fluid.last_terms)
     if not terms:
         raise ValueError("no evaluation yet")
     terms.pop("total", None)
-    names = list(terms) + ["total energy"]
-    vals = list(terms.values()) + [sum(terms.values())]
-    width = 1 + max(len(s) for s in names)
-    lines = []
-    for name, v in zip(names, vals):
-        lines.append("%s%s =%s" % (prefix, name.rjust(width), print_double("", v)))
-    return "\n".join(lines)
+    width = 1 + max(len(s) for s in list(terms) + ["total energy"])
+    return "\n".join("%s%s =%s" % (prefix, name.rjust(width), print_double("", v)) for name, v in terms.items())
+
+
+def print_info(minimizer, fluid, prefix="", with_iteration=True):
+    """Minimize::print_info (src/new/Minimize.cpp:315-321): the iteration header, printme
+    and the total energy."""
+    out = ""
+    if with_iteration:
+        out += "\n%s==== Iteration %d ====\n" % (prefix, minimizer.stats()["iter"])
+    out += printme(fluid, prefix) + "\n"
+    out += "%s total energy =%s\n\n" % (prefix, print_double("", minimizer.energy()))
+    return out

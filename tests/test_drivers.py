@@ -58,6 +58,17 @@ def test_radial_and_z_plots_read_back(gpu, tmp_path):
     assert rz.shape == (16, 9) and np.allclose(rz[:, 1], rz[:, 2])
     f.energy()
     text = drivers.printme(f, "  ")
-    assert "total energy =" in text and len(text.splitlines()) == 7
+    assert len(text.splitlines()) == 6
+    info = drivers.print_info(m, f, "  ")
+    assert "==== Iteration" in info and "total energy =" in info
     m.close()
     f.close()
+
+
+def test_printme_layout():
+    class F:
+        last_terms = {"kTphi1": 1.5, "ideal": -2.25e-7, "total": 0.0}
+    lines = drivers.printme(F(), "> ").splitlines()
+    w = 1 + len("total energy")
+    assert lines[0] == "> " + "kTphi1".rjust(w) + " =" + drivers.print_double("", 1.5)
+    assert lines[1].startswith("> " + "ideal".rjust(w) + " = ") and "e-07" in lines[1]
