@@ -40,8 +40,7 @@ enum Slot : int {
   SLOT_PGG = 6,    // pgrad . grad
   SLOT_PGO = 7,    // pgrad . oldgrad
   SLOT_GD = 8,     // grad . direction
-  SLOT_NORM = 9,   // generic dot for tests
-  NUM_SLOTS = 10
+  NUM_SLOTS = 9
 };
 
 struct GridArgs {
@@ -863,16 +862,6 @@ static __global__ __launch_bounds__(kThreads) void k_axpy(long long N, double* _
                                                    double s) {
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)
     x[i] = __dadd_rn(x[i], __dmul_rn(s, d[i]));
-}
-
-// partial a.b into SLOT_NORM
-static __global__ __launch_bounds__(kThreads) void k_dot(long long N, const double* __restrict__ a,
-                                                  const double* __restrict__ b, double* __restrict__ partial,
-                                                  int pstride) {
-  double acc[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads)
-    acc[0] += a[i] * b[i];
-  block_sum_store<1>(acc, partial, SLOT_NORM, pstride, blockIdx.x);
 }
 
 // n = exp(-Veff/kT), 0 on hard walls
