@@ -207,6 +207,9 @@ constexpr int zbuf_size() {
 }
 
 template <typename T, int L, int P>
+__device__ __forceinline__ void pack_c2r(const cx<T>* raw, cx<T>* Z);
+
+template <typename T, int L, int P>
 __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, long long line0, long long lines,
                                                 int ldk, cx<T>* raw, cx<T>* Z) {
   constexpr int NKZ = L / 2 + 1;
@@ -215,6 +218,14 @@ __device__ __forceinline__ void load_c2r_packed(const cx<T>* __restrict__ src, l
     raw[e] = load_or_zero(src, (line0 + l) * ldk + k, 0, line0 + l < lines);
   }
   __syncthreads();
+  pack_c2r<T, L, P>(raw, Z);
+}
+
+// The second half of load_c2r_packed: the raw half-spectrum rows in `raw`
+// (visible to the block) -> packed complex lines in Z, then a barrier.
+template <typename T, int L, int P>
+__device__ __forceinline__ void pack_c2r(const cx<T>* raw, cx<T>* Z) {
+  constexpr int NKZ = L / 2 + 1;
   constexpr int SS = lds_stride<T, L>();
   for (int e = threadIdx.x; e < P * L; e += kThreads) {
     const int p = e / L, z = e - p * L;
@@ -662,9 +673,30 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   const long long line0 = (long long)bx * 2 * P;
   constexpr int NW = FN == 3 ? 11 : 5;
   T val[NW][PT];
+  // the next field's half-spectrum rows are loaded into registers while the
+  // current field is packed and transformed (two fields' loads in flight)
+  constexpr int NRAW = 2 * P * (L / 2 + 1);
+  constexpr int PF = (NRAW + kThreads - 1) / kThreads;
+  cx<T> pre[PF];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = threadIdx.x + i * kThreads;
+      const int l = e / (L / 2 + 1), k = e - l * (L / 2 + 1);
+      pre[i] = load_or_zero(fields + f * g.ks, (line0 + l) * g.ldk + k, 0, e < NRAW && line0 + l < lines);
+    }
+  };
+  fetch(0);
 #pragma unroll
   for (int f = 0; f < NW; ++f) {
-    load_c2r_packed<T, L, P>(fields + f * g.ks, line0, lines, g.ldk, A, B);
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = threadIdx.x + i * kThreads;
+      if (e < NRAW) A[e] = pre[i];
+    }
+    __syncthreads();
+    if (f + 1 < NW) fetch(f + 1);
+    pack_c2r<T, L, P>(A, B);
     const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
@@ -747,12 +779,20 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, bx);
 }
 
-// At most 128 VGPRs (four waves per SIMD): the FP64 energy+gradient pass at
-// 512-value blocks needs 132 otherwise and drops to three waves.
+// At most 128 VGPRs (four waves per SIMD): the White-Bear FP64
+// energy+gradient pass at 512-value blocks needs 132 otherwise and drops to
+// three waves.  The FP64 Mark II variants keep the default budget (held to
+// 128 they spill to scratch; at 128^3 the default budget with the field
+// prefetch ran 5-12% faster than four waves without it); the FP32 Mark II
+// ones measured faster at four waves, spills included.
+template <typename T, int ZB, int FN>
+constexpr int zphi_min_waves() {
+  return (FN == 1 && sizeof(T) == 8) ? 1 : 4;
+}
 template <typename T, int L, bool GRAD, int ZB, int FN = 0>
-__global__ __launch_bounds__(kThreads, 4) void k_zphi(GridArgs g, cx<T>* __restrict__ fields,
-                                                   double* __restrict__ partial, int pstride,
-                                                   const cx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(kThreads, (zphi_min_waves<T, ZB, FN>())) void k_zphi(
+    GridArgs g, cx<T>* __restrict__ fields, double* __restrict__ partial, int pstride,
+    const cx<T>* __restrict__ tw) {
   zphi_body<T, L, GRAD, ZB, FN>(blockIdx.x, g, fields, partial, pstride, tw);
 }
 
