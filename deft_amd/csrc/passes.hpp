@@ -616,14 +616,39 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
       }
     }
   }
+  // FP32: the next field's columns are loaded into registers while the
+  // current field is transformed and accumulated (two fields' loads in
+  // flight; 10% faster at 128^3).  FP64 loads each field in turn: there the
+  // extra registers halve the occupancy and the pass ran 10% slower.
+  constexpr bool kPrefetch = sizeof(T) == 4;
+  cx<T> pre[PT];
+  auto fetch = [&](int f) {
+    const cx<T>* src = in + f * g.ks;
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int e = threadIdx.x + i * kThreads;  // e % C == myc
+      pre[i] = load_or_zero(src, (long long)(e / C) * plane + off, 0, e < C * L && myc < ncol);
+    }
+  };
+  if constexpr (kPrefetch) fetch(0);
 #pragma unroll
   for (int f = 0; f < NW; ++f) {
-    const cx<T>* src = in + f * g.ks;
-    for (int e = threadIdx.x; e < C * L; e += kThreads) {
-      const int xx = e / C;
-      A[myc * SS + skew<T>(xx)] = load_or_zero(src, (long long)xx * plane + off, 0, myc < ncol);
+    if constexpr (kPrefetch) {
+#pragma unroll
+      for (int i = 0; i < PT; ++i) {
+        const int e = threadIdx.x + i * kThreads;
+        if (e < C * L) A[myc * SS + skew<T>(e / C)] = pre[i];
+      }
+      __syncthreads();
+      if (f + 1 < NW) fetch(f + 1);
+    } else {
+      const cx<T>* src = in + f * g.ks;
+      for (int e = threadIdx.x; e < C * L; e += kThreads) {
+        const int xx = e / C;
+        A[myc * SS + skew<T>(xx)] = load_or_zero(src, (long long)xx * plane + off, 0, myc < ncol);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
