@@ -2,7 +2,7 @@
 // equilibrium density from exp(-Vext/kT) and the bulk packing fraction, with
 // hand-written BF16-MFMA forward and backward kernels (FP32 accumulation and
 // activations) and Adam training.  No counterpart in deft; the conjugate
-// gradient minimizer stays the judge of convergence.
+// gradient minimizer alone decides convergence.
 //
 // Layout: activations are FP32 channels-last, act[v * C + c], voxel
 // v = ((s*M + x)*M + y)*M + z for sample s of a batch; every level is periodic.
