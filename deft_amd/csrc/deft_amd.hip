@@ -126,12 +126,7 @@ void queue_move(deft_plan_s *p, double s) {
 }
 
 int reduce_slot(deft_plan_s *p, int slot, int nb, double *out) {
-  {
-    ProfScope ps(p, 2 * K_REDUCE, 8.0 * nb);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot, nb, p->dscal);
-  }
-  DEFT_HIP_CHECK(hipGetLastError());
-  TRY(plan_wait(p));
+  TRY(plan_reduce_wait(p, slot, 1, nb));
   *out = p->hscal[slot];
   return DEFT_OK;
 }
@@ -151,6 +146,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->vext);
   (void)hipFree(p->spec);
   (void)hipFree(p->partial);
+  (void)hipFree(p->dcount);
   (void)hipFree(p->twbuf);
   (void)hipFree(p->tmp);
   (void)hipFree(p->wtab);
@@ -783,8 +779,11 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
       hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
-      hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess)
+      hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess ||
+      hipMalloc(&p->dcount, sizeof(unsigned)) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
+  if (hipMemsetAsync(p->dcount, 0, sizeof(unsigned), p->stream) != hipSuccess)
+    return fail(DEFT_ERR_HIP, "hipMemsetAsync failed");
   if (hipHostMalloc((void **)&p->hflag, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
           hipSuccess ||
       hipHostGetDevicePointer((void **)&p->dflag, p->hflag, 0) != hipSuccess)

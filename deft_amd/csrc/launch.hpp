@@ -231,14 +231,9 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
       });
     });
   }
-  {
-    const int nslots = grad ? 8 : 6;
-    ProfScope ps(p, 2 * K_REDUCE, 8.0 * nslots * nzb);
-    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3(kThreads), 0, p->stream, p->partial, p->pstride, (int)SLOT_IDEAL,
-                       nzb, p->dscal);
-  }
   DEFT_HIP_CHECK(hipGetLastError());
-  return plan_wait(p);
+  // every pass wrote its partial sums from nzb blocks
+  return plan_reduce_wait(p, SLOT_IDEAL, grad ? 8 : 6, nzb);
 }
 
 // --- batched evaluation ----------------------------------------------------

@@ -981,4 +981,23 @@ static __global__ __launch_bounds__(kThreads) void k_reduce(const double* __rest
   reduce_body(slot0 + blockIdx.x, partial, pstride, nb, out);
 }
 
+// k_reduce that also raises the completion flag: each block, after its sum
+// has reached the host (reduce_body's system fence), counts itself in `cnt`;
+// the last one re-arms the counter and stores seq to the host-mapped flag,
+// so the host needs no separate flag kernel.
+static __global__ __launch_bounds__(kThreads) void k_reduce_flag(const double* __restrict__ partial, int pstride,
+                                                          int slot0, int nb, double* __restrict__ out,
+                                                          unsigned* __restrict__ cnt,
+                                                          unsigned long long* __restrict__ flag,
+                                                          unsigned long long seq) {
+  reduce_body(slot0 + blockIdx.x, partial, pstride, nb, out);
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == gridDim.x - 1) {
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
 }  // namespace deft

@@ -93,6 +93,7 @@ struct deft_plan_s {
   unsigned long long flag_seq = 0;
   double *hscal = nullptr;  // NUM_SLOTS reduced sums in host-mapped pinned memory
   double *dscal = nullptr;  // device address of hscal (written by k_reduce)
+  unsigned *dcount = nullptr;  // arrival counter of k_reduce_flag, zero between launches
   void *twbuf = nullptr;
   deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
   deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
@@ -265,14 +266,9 @@ struct BatchGraph {
 // synchronisation, which also reports any error; with profiling on, or in a
 // batch, the blocking path is used throughout.
 constexpr long long kSpinLimitUs = 20000;
-// The wait on any stream with its own flag pair; 1 = flag seen, 0 = spin limit
-// reached (the caller then synchronises the stream), negative = launch error.
-inline int flag_wait(hipStream_t st, unsigned long long *hflag, unsigned long long *dflag,
-                     unsigned long long &seq) {
-  const unsigned long long want = ++seq;
-  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, st, dflag, want);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_signal: ") + hipGetErrorString(e)), -1;
+// Spins until *hflag >= want: 1 = flag seen, 0 = spin limit reached (the
+// caller then synchronises the stream).
+inline int flag_spin_wait(const unsigned long long *hflag, unsigned long long want) {
   const auto t0 = std::chrono::steady_clock::now();
   for (long long n = 0;; ++n) {
     if (__atomic_load_n(hflag, __ATOMIC_ACQUIRE) >= want) return 1;
@@ -284,11 +280,36 @@ inline int flag_wait(hipStream_t st, unsigned long long *hflag, unsigned long lo
   }
 }
 
-inline int plan_wait(deft_plan_s *p) {
-  if (p->prof || p->batch || !p->hflag) return plan_sync(p);
-  const int r = flag_wait(p->stream, p->hflag, p->dflag, p->flag_seq);
-  if (r < 0) return DEFT_ERR_HIP;
-  return r == 1 ? DEFT_OK : plan_sync(p);
+// The wait on any stream with its own flag pair: a one-thread kernel raises
+// the flag; as flag_spin_wait, or negative on a launch error.
+inline int flag_wait(hipStream_t st, unsigned long long *hflag, unsigned long long *dflag,
+                     unsigned long long &seq) {
+  const unsigned long long want = ++seq;
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, st, dflag, want);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_signal: ") + hipGetErrorString(e)), -1;
+  return flag_spin_wait(hflag, want);
+}
+
+// Reduces slots [slot0, slot0 + nslots) of nb blocks' partial sums into
+// p->hscal and waits for them; when the wait spins, the reduction's last block
+// raises the flag itself (one launch instead of two).
+inline int plan_reduce_wait(deft_plan_s *p, int slot0, int nslots, int nb) {
+  const bool spin = !(p->prof || p->batch || !p->hflag || !p->dcount);
+  const unsigned long long want = spin ? ++p->flag_seq : 0;
+  {
+    ProfScope ps(p, 2 * K_REDUCE, 8.0 * nslots * nb);
+    if (spin)
+      hipLaunchKernelGGL(k_reduce_flag, dim3(nslots), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot0, nb,
+                         p->dscal, p->dcount, p->dflag, want);
+    else
+      hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3(kThreads), 0, p->stream, p->partial, p->pstride, slot0, nb,
+                         p->dscal);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(DEFT_ERR_HIP, std::string("k_reduce: ") + hipGetErrorString(e));
+  if (!spin) return plan_sync(p);
+  return flag_spin_wait(p->hflag, want) == 1 ? DEFT_OK : plan_sync(p);
 }
 
 // Kernel launch sequences, instantiated for T = float and T = double in
