@@ -39,9 +39,13 @@ WHITE_BEAR_MARK_II = 1
 WHITE_BEAR_MARK_II_SLOW = 2
 WHITE_BEAR_TENSOR = 3
 
+PGRAD_NEW = 1
+PGRAD_EFFECTIVE_POTENTIAL = 2
+
 STOP_REASONS = {
     0: "none", 1: "maxiter", 2: "nan-energy", 3: "infinite-energy", 4: "nan-slope",
     5: "zero-slope", 6: "negative-curvature", 7: "energy-rose", 8: "no-change", 9: "converged",
+    10: "not-improved", 11: "gradient-unconfirmed",
 }
 
 
@@ -97,6 +101,22 @@ _SIGNATURES = {
     "deft_functional_energy": ([_P, _DP, _DP, ctypes.POINTER(EnergyTerms)], _I),
     "deft_functional_grad": ([_P, _DP, _DP, _DP, _DP], _I),
     "deft_functional_grad_device": ([_P, _DP, _DP, _DP, _DP], _I),
+    "deft_functional_grad_kind": ([_P, _DP, _DP, _DP, _DP, _I], _I),
+    "deft_functional_grad_device_kind": ([_P, _DP, _DP, _DP, _DP, _I], _I),
+    "deft_minimize_set_gradient_check": ([_P, _D], _I),
+    "deft_minimize_gradient_check_get": ([_P, _DP, _IP], _I),
+    "deft_cg_create": ([_P, _I, _D, ctypes.POINTER(_P)], _I),
+    "deft_cg_destroy": ([_P], _I),
+    "deft_cg_set_precision": ([_P, _D], _I),
+    "deft_cg_set_maxiter": ([_P, _I], _I),
+    "deft_cg_set_fft_precision": ([_P, _I], _I),
+    "deft_cg_improve_energy": ([_P, _I, _IP], _I),
+    "deft_cg_energy": ([_P, _DP], _I),
+    "deft_cg_grad": ([_P, _DP], _I),
+    "deft_cg_pgrad": ([_P, _DP], _I),
+    "deft_cg_minimize": ([_P], _I),
+    "deft_cg_invalidate": ([_P], _I),
+    "deft_cg_stats_get": ([_P, ctypes.POINTER(MinimizeStats)], _I),
     "deft_weighted_densities": ([_P, _DP], _I),
     "deft_fft": ([_P, _DP, _DP], _I),
     "deft_ifft": ([_P, _DP, _DP], _I),
@@ -368,6 +388,18 @@ class WhiteBearFluidVeff:
     def grad(self, Veff=None):
         return self.energy_grad_and_precond(Veff)[1]
 
+    def integralgrad(self, Veff=None):
+        """(energy, grad, pgrad) with the old API's pgrad (dF/dn)/(Veff - kT)
+        (Functional::integralgrad, src/Functional.h:144-149; src/EffectivePotentialToDensity.cpp:62-71)."""
+        if Veff is not None:
+            self.Veff = Veff
+        g = np.zeros(self.shape)
+        pg = np.zeros(self.shape)
+        e = ctypes.c_double()
+        _check(lib().deft_functional_grad_kind(self._h, None, _dptr(g), _dptr(pg), ctypes.byref(e),
+                                               PGRAD_EFFECTIVE_POTENTIAL))
+        return e.value, g, pg
+
     def energy_grad_and_precond(self, Veff=None):
         if Veff is not None:
             self.Veff = Veff
@@ -513,6 +545,16 @@ class Minimize:
         _check(lib().deft_minimize_phase_iterations(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def set_gradient_check(self, factor):
+        """Gradient confirmation of the ladder's final verdict (0 disables)."""
+        _check(lib().deft_minimize_set_gradient_check(self._h, float(factor)))
+
+    def gradient_check(self):
+        """(Newton error estimate of the last confirmation, restarts it caused)."""
+        e, r = ctypes.c_double(), ctypes.c_int()
+        _check(lib().deft_minimize_gradient_check_get(self._h, ctypes.byref(e), ctypes.byref(r)))
+        return e.value, r.value
+
     def set_fused_line_search(self, mode):
         """1: energy+gradient at the quadratic step's trial point, 0: energy only,
         -1: on with the mixed-precision ladder (default)."""
@@ -520,6 +562,91 @@ class Minimize:
 
     def invalidate_cache(self):
         _check(lib().deft_minimize_invalidate(self._h))
+
+
+class ConjugateGradient:
+    """deft's old-API minimizer stack (MinimizerInterface, src/Minimizer.h:16-81) on a
+    :class:`WhiteBearFluidVeff`: ``ConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer,
+    stepsize)`` (src/ConjugateGradient.cpp:59-108), or with ``preconditioned``
+    ``PreconditionedConjugateGradient`` (:115-160), optionally wrapped in ``Precision`` and
+    ``MaxIter`` (see :func:`Precision`, :func:`MaxIter`).  x is the fluid's resident Veff."""
+
+    def __init__(self, f, preconditioned=False, stepsize=10.0):
+        self.f = f
+        h = ctypes.c_void_p()
+        _check(lib().deft_cg_create(f.handle, 1 if preconditioned else 0, float(stepsize), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().deft_cg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, p):
+        _check(lib().deft_cg_set_precision(self._h, float(p)))
+
+    def set_maxiter(self, n):
+        _check(lib().deft_cg_set_maxiter(self._h, int(n)))
+
+    def set_fft_precision(self, p):
+        _check(lib().deft_cg_set_fft_precision(self._h, int(p)))
+
+    def improve_energy(self, verbose=False):
+        more = ctypes.c_int()
+        _check(lib().deft_cg_improve_energy(self._h, 1 if verbose else 0, ctypes.byref(more)))
+        return bool(more.value)
+
+    def energy(self):
+        e = ctypes.c_double()
+        _check(lib().deft_cg_energy(self._h, ctypes.byref(e)))
+        return e.value
+
+    def grad(self):
+        out = np.empty(self.f.shape)
+        _check(lib().deft_cg_grad(self._h, _dptr(out)))
+        return out
+
+    def pgrad(self):
+        out = np.empty(self.f.shape)
+        _check(lib().deft_cg_pgrad(self._h, _dptr(out)))
+        return out
+
+    def minimize(self):
+        """minimize(f, gd, x): iteration count, step and direction start over."""
+        _check(lib().deft_cg_minimize(self._h))
+
+    def invalidate_cache(self):
+        _check(lib().deft_cg_invalidate(self._h))
+
+    def stats(self):
+        s = MinimizeStats()
+        _check(lib().deft_cg_stats_get(self._h, ctypes.byref(s)))
+        d = {n: getattr(s, n) for n, _ in s._fields_}
+        d["stop_reason"] = STOP_REASONS.get(d["stop_reason"], str(d["stop_reason"]))
+        return d
+
+
+def PreconditionedConjugateGradient(f, stepsize=10.0):
+    """PreconditionedConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer, stepsize)."""
+    return ConjugateGradient(f, preconditioned=True, stepsize=stepsize)
+
+
+def Precision(precision, m):
+    """Precision(precision, m) (src/Precision.cpp:23-156) around an old-API minimizer."""
+    m.set_precision(precision)
+    return m
+
+
+def MaxIter(maxiter, m):
+    """MaxIter(maxiter, m) (src/Minimizer.cpp) around an old-API minimizer."""
+    m.set_maxiter(maxiter)
+    return m
 
 
 class Batch:

@@ -25,7 +25,7 @@ struct BatchArgs {
   int nmoves;
   int have_old, precond;
   int nzb;  // z-pass blocks (reduction length)
-  const float* vext;
+  const T* vext;  // FP32 for FP32 passes, FP64 for FP64 passes
   cx<T>* spec0;  // the density spectrum / back-convolution
   cx<T>* spec1;  // the five weighted-density spectra
   double* partial;

@@ -16,6 +16,7 @@
 
 #include "contact.hpp"
 #include "launch.hpp"
+#include "minimizer.hpp"
 
 using namespace deft;
 
@@ -38,46 +39,22 @@ int set_error(int code, const std::string &msg) {
 
 namespace {
 
-constexpr int kVersion = 1;
-constexpr int kConfirmIterations = 3;
-constexpr int kMaxPhaseHalvings = 12;
-constexpr int kMaxCollapseRestarts = 3;
-
-bool det3(const double *c, double &det) {
-  det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
-  return std::isfinite(det) && det != 0.0;
-}
-
-// rows b_j of 2 pi inv(cell)^T, so that a_i . b_j = 2 pi delta_ij
-void reciprocal(const double *c, double det, double *b) {
-  double inv[9];
-  inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
-  inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
-  inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
-  inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
-  inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
-  inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
-  inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
-  inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
-  inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
-  for (int j = 0; j < 3; ++j)
-    for (int k = 0; k < 3; ++k) b[j * 3 + k] = 2 * M_PI * inv[k * 3 + j];
-}
+constexpr int kVersion = 2;
 
 bool length_ok(int n) {
   return with_length(n, [](auto) {});
 }
 
-int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond);
+int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, int precond);
 int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir);
 
-int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, int precond) {
   if (p->batch) return batch_evaluate(p, prec, grad, have_old, precond);
   if (prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
   return Launch<float>::evaluate(p, grad, have_old, precond);
 }
 
-int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
+int evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
   return evaluate_at(p, p->prec, grad, have_old, precond);
 }
 
@@ -144,6 +121,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->pg);
   (void)hipFree(p->og);
   (void)hipFree(p->vext);
+  (void)hipFree(p->vext64);
   (void)hipFree(p->spec);
   (void)hipFree(p->partial);
   (void)hipFree(p->dcount);
@@ -159,40 +137,18 @@ void free_plan(deft_plan_s *p) {
 }  // namespace
 
 // ===========================================================================
-// minimizer state (src/new/Minimize.h:13-199)
+// minimizers: the control logic is minimizer.hpp's, on the plan's device
+// vectors through DeviceOps
 // ===========================================================================
-struct deft_minimize_s {
+// new API Minimize (src/new/Minimize.h:13-199) with the precision ladder
+struct deft_minimize_s : deft::MinState {
   deft_plan_s *p = nullptr;
-  int iter = 0, maxiter = 10000000, miniter = 6;
-  double precision = 0.0, relative_precision = 1e-15;
-  bool use_precond = false;
-  double step = 0.1, orig_stepsize = 0.1, oldgradsqr = 1.0;
-  double deltaE = 0.0, dEdn = 0.0, log_avg = 0.0, error_estimate = 0.0;
-  bool have_dir = false, have_old = false;
-  bool E_valid = false;
-  double E = 0.0;
-  bool G_valid = false;
-  double pgg = 0.0, pgo = 0.0;
-  int num_energy = 0, num_grad = 0;
-  int stop = DEFT_STOP_NONE;
-  // mixed precision: run FP32 FFTs until this relative precision (or until the
-  // FP32 line search stalls), then continue with FP64 FFTs; 0 disables
-  double switch_rp = 0.0;
-  int polish_grad_prec = DEFT_FFT_FP32;  // gradient precision after the switch
-  int e_prec = -1, g_prec = -1;          // evaluation precisions (-1: the plan's)
-  int fp32_iters = -1;      // iterations done in phase 0 (-1: no switch yet)
-  int iter_at_phase2 = -1;  // iteration count when phase 2 began (-1: not reached)
-  int phase = 0;            // precision phase of the mixed ladder
-  int confirm_until = 0;    // the final phase runs at least until this iteration
-  // fused line search: the trial point s2 of the quadratic step gets energy and
-  // gradient from one evaluation (when both use one FFT precision), so an
-  // accepted s2 needs no second evaluation at the start of the next iteration;
-  // -1: on exactly when the mixed-precision ladder is on
-  int fuse_ls = -1;
-  bool s2_accepted = true;  // the last quadratic step kept its trial point s2
-  double good_step = 0.0;   // step of the last iteration that clearly lowered the energy
-  int collapse_restarts = 0;  // final-phase restarts after a collapsed last step
-  bool restart_met = false;   // the iteration before a collapse restart met the precision
+};
+
+// old API ConjugateGradient / PreconditionedConjugateGradient with
+// QuadraticLineMinimizer, Precision and MaxIter (src/ConjugateGradient.cpp:59-171)
+struct deft_cg_s : deft::OldCgState {
+  deft_plan_s *p = nullptr;
 };
 
 // ===========================================================================
@@ -239,329 +195,65 @@ struct Fiber {
 
 namespace {
 
-void invalidate(deft_minimize_s *m) {
-  m->E_valid = false;
-  m->G_valid = false;
-}
-
-int energy_prec(const deft_minimize_s *m) { return m->e_prec >= 0 ? m->e_prec : m->p->prec; }
-int grad_prec(const deft_minimize_s *m) { return m->g_prec >= 0 ? m->g_prec : m->p->prec; }
-
-int m_energy(deft_minimize_s *m, double *E) {
-  if (!m->E_valid) {
-    TRY(evaluate_at(m->p, energy_prec(m), false, false, false));
-    m->p->last_terms = terms_from_sums(m->p);
-    m->E = m->p->last_terms.total;
-    m->E_valid = true;
-    m->num_energy++;
+// The plan's device vectors as the minimizers' backend (minimizer.hpp).
+struct DeviceOps {
+  deft_plan_s *p;
+  int plan_prec() const { return p->prec; }
+  int energy(int prec, double *E) {
+    TRY(evaluate_at(p, prec, false, false, PGRAD_NONE));
+    p->last_terms = terms_from_sums(p);
+    *E = p->last_terms.total;
+    return DEFT_OK;
   }
-  *E = m->E;
-  return DEFT_OK;
-}
-
-// Minimize::pgrad (src/new/Minimize.h:136-165): with preconditioning the
-// energy, grad and pgrad come from one evaluation; without, pgrad = grad.
-// When gradients run at a lower FFT precision than energies, the evaluation's
-// energy is not used: the cached energy of the same Veff is kept (or computed
-// at the energy precision), so the line search only ever compares energies of
-// one precision.
-int m_pgrad(deft_minimize_s *m) {
-  if (m->G_valid) return DEFT_OK;
-  const bool same = energy_prec(m) == grad_prec(m);
-  double Ekeep = m->E;
-  bool keep = !same && m->E_valid;
-  if (m->use_precond) invalidate(m);
-  TRY(evaluate_at(m->p, grad_prec(m), true, m->have_old, m->use_precond));
-  m->p->last_terms = terms_from_sums(m->p);
-  if (same && (m->use_precond || !m->E_valid)) {
-    m->E = m->p->last_terms.total;
-    m->E_valid = true;
-  } else if (keep) {
-    m->E = Ekeep;
-    m->E_valid = true;
+  int gradient(int prec, bool have_old, int kind, double *E, double *pgg, double *pgo) {
+    TRY(evaluate_at(p, prec, true, have_old, kind));
+    p->last_terms = terms_from_sums(p);
+    *E = p->last_terms.total;
+    *pgg = p->hscal[SLOT_PGG];
+    *pgo = p->hscal[SLOT_PGO];
+    return DEFT_OK;
   }
-  if (m->use_precond && same) m->num_energy++;
-  m->num_grad++;
-  m->pgg = m->p->hscal[SLOT_PGG];
-  m->pgo = m->p->hscal[SLOT_PGO];
-  m->G_valid = true;
-  return DEFT_OK;
-}
-
-inline bool better(double a, double b) { return a < b || b != b; }
-
-int stop_with(deft_minimize_s *m, int reason, int *more) {
-  m->stop = reason;
-  *more = 0;
-  return DEFT_OK;
-}
-
-// One Minimize::improve_energy (src/new/Minimize.cpp:9-313).
-int improve_energy(deft_minimize_s *m, int verbose, int *more) {
-  deft_plan_s *p = m->p;
-  m->iter++;
-  if (m->iter >= m->maxiter) return stop_with(m, DEFT_STOP_MAXITER, more);
-  double E0;
-  TRY(m_energy(m, &E0));
-  const double old_deltaE = m->deltaE;
-  if (E0 != E0) return stop_with(m, DEFT_STOP_NAN_ENERGY, more);
-
-  // Polak-Ribiere direction with automatic reset
-  TRY(m_pgrad(m));
-  invalidate(m);
-  double beta = m->have_old ? (m->pgg - m->pgo) / m->oldgradsqr : m->pgg / m->oldgradsqr;
-  if (beta < 0 || beta != beta || m->oldgradsqr == 0) beta = 0;
-  m->oldgradsqr = m->pgg;
-  double gdotd;
-  if (p->batch) {
-    // batched plans: the update joins a round with the other grids' updates
-    TRY(batch_cg(p, 0, beta, m->have_dir ? 1 : 0));
-    gdotd = p->hscal[SLOT_GD];
-  } else {
+  // direction = -pg + beta direction (or -pg), oldgrad = grad; grad.direction
+  int cg_update(double beta, bool have_dir, double *gd) {
+    if (p->batch) {
+      // batched plans: the update joins a round with the other grids' updates
+      TRY(batch_cg(p, 0, beta, have_dir ? 1 : 0));
+      *gd = p->hscal[SLOT_GD];
+      return DEFT_OK;
+    }
     {
       ProfScope ps(p, 2 * K_CG_UPDATE, 40.0 * p->N);
-      hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d,
-                         p->og, beta, m->have_dir ? 1 : 0, p->partial, p->pstride);
+      hipLaunchKernelGGL(k_cg_update, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->pg, p->g, p->d, p->og,
+                         beta, have_dir ? 1 : 0, p->partial, p->pstride);
     }
-    TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
+    return reduce_slot(p, SLOT_GD, p->eblocks, gd);
   }
-  m->have_dir = true;
-  m->have_old = true;
-  if (gdotd > 0) {
+  // direction = -grad; grad.direction
+  int cg_reset(double *gd) {
     if (p->batch) {
       TRY(batch_cg(p, 1, 0.0, 0));
-      gdotd = p->hscal[SLOT_GD];
-    } else {
-      {
-        ProfScope ps(p, 2 * K_CG_RESET, 16.0 * p->N);
-        hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d,
-                           p->partial, p->pstride);
-      }
-      TRY(reduce_slot(p, SLOT_GD, p->eblocks, &gdotd));
+      *gd = p->hscal[SLOT_GD];
+      return DEFT_OK;
     }
-    if (verbose) printf("\t\treset to gradient\n");
+    {
+      ProfScope ps(p, 2 * K_CG_RESET, 16.0 * p->N);
+      hipLaunchKernelGGL(k_cg_reset, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, p->g, p->d, p->partial,
+                         p->pstride);
+    }
+    return reduce_slot(p, SLOT_GD, p->eblocks, gd);
   }
+  void move(double s) { queue_move(p, s); }
+  double newton_scale() const { return p->kT / p->dV; }
+};
 
-  // quadratic line minimization
-  const double slope = -gdotd;
-  if (E0 >= std::numeric_limits<double>::max() || E0 <= -std::numeric_limits<double>::max())
-    return stop_with(m, DEFT_STOP_INFINITE_ENERGY, more);
-  if (slope != slope) return stop_with(m, DEFT_STOP_NAN_SLOPE, more);
-  if (slope == 0) return stop_with(m, DEFT_STOP_ZERO_SLOPE, more);
-  if (m->step == m->orig_stepsize && std::fabs(E0) > 1e-200 && std::fabs(slope) > 1e-100)
-    m->step = 0.001 * std::fabs(E0) / slope;
-  if (slope * m->step < 0) m->step *= -1;
-  double step1 = m->step;
-  queue_move(p, step1);
-  double Etried = -137, E;
-  // Below the last precision phase the halving is capped: a line search that
-  // needs more than kMaxPhaseHalvings halvings has reached the round-off of the
-  // phase's energies, and the phase ends where the iteration began.
-  const bool capped = m->switch_rp > 0 && m->phase < 2;
-  int halvings = 0;
-  while (true) {
-    step1 *= 0.5;
-    queue_move(p, -step1);
-    invalidate(m);
-    TRY(m_energy(m, &E));
-    if (E == Etried) break;
-    Etried = E;
-    if (!better(E0, E)) break;
-    if (capped && ++halvings > kMaxPhaseHalvings) {
-      queue_move(p, -step1);
-      invalidate(m);
-      m->deltaE = 0;
-      return stop_with(m, DEFT_STOP_NO_CHANGE, more);
-    }
-  }
-  double E1;
-  TRY(m_energy(m, &E1));
-  const double curvature = 2.0 * (E1 - E0 + step1 * slope) / (step1 * step1);
-  if (curvature <= 0.0) {
-    if (better(E0, E1)) {
-      invalidate(m);
-      queue_move(p, -step1);
-      return stop_with(m, DEFT_STOP_NEGATIVE_CURVATURE, more);
-    }
-    double Ebest = E0;
-    while (true) {
-      TRY(m_energy(m, &E));
-      if (!(E <= Ebest)) break;
-      Ebest = E;
-      queue_move(p, step1);
-      invalidate(m);
-      step1 *= 2;
-    }
-    step1 *= 0.5;
-    queue_move(p, -step1);
-    m->step = step1;
-    invalidate(m);
-  } else if (E1 == E0) {
-    do {
-      invalidate(m);
-      queue_move(p, -step1);
-      step1 *= 2;
-      queue_move(p, step1);
-      TRY(m_energy(m, &E));
-    } while (E == E0);
-    if (E > E0) {
-      queue_move(p, -step1);
-      invalidate(m);
-      step1 = 0;
-    } else if (E != E) {
-      queue_move(p, -step1);
-      invalidate(m);
-      step1 = 0;
-    }
-  } else {
-    const double step2 = slope / curvature;
-    m->step = step2;
-    invalidate(m);
-    queue_move(p, step2 - step1);
-    double E2;
-    // fused only while trial points are being kept: a rejected s2 wastes its gradient
-    const bool fuse = (m->fuse_ls > 0 || (m->fuse_ls < 0 && m->switch_rp > 0)) && m->s2_accepted &&
-                      energy_prec(m) == grad_prec(m);
-    if (fuse) {
-      // E(s2) from the evaluation that also yields the gradient the next
-      // iteration starts from; the energies and the trajectory are unchanged
-      const int ne = m->num_energy;
-      TRY(m_pgrad(m));
-      if (m->num_energy == ne) m->num_energy++;
-      E2 = m->E;
-    } else {
-      TRY(m_energy(m, &E2));
-    }
-    m->s2_accepted = !(better(E1, E2) && better(E1, E0));
-    if (!m->s2_accepted) {
-      invalidate(m);
-      queue_move(p, -(step2 - step1));
-      m->step = step1;
-    } else if (E2 > E0) {
-      invalidate(m);
-      queue_move(p, -step2);
-      return stop_with(m, DEFT_STOP_ENERGY_ROSE, more);
-    }
-  }
-
-  // convergence estimate (src/new/Minimize.cpp:263-312)
-  double newE;
-  TRY(m_energy(m, &newE));
-  m->deltaE = newE - E0;
-  const double w = 0.01;
-  m->dEdn = (1 - w) * m->dEdn + w * m->deltaE;
-  m->dEdn = std::max(m->deltaE, old_deltaE);
-  const double new_log = (m->deltaE != 0 && old_deltaE != 0) ? std::log(std::fabs(m->deltaE / old_deltaE)) : 0.0;
-  m->log_avg = (1 - w) * m->log_avg + w * new_log;
-  const double ratio_avg = std::exp(m->log_avg);
-  double error_guess = std::fabs(std::max(m->dEdn, m->deltaE) / m->log_avg);
-  if (ratio_avg >= 1) error_guess = m->precision + std::fabs(newE);
-  m->error_estimate = 2 * error_guess;
-  // the step of the last iteration whose decrease stands clear of the round-off
-  // of its energies: the next precision phase starts from it
-  if (m->deltaE < -(energy_prec(m) == DEFT_FFT_FP32 ? 1e-6 : 1e-13) * std::fabs(newE)) m->good_step = m->step;
-  if (verbose)
-    printf("iter %4d  E = %.15g  dE = %.3e  err = %.3e  step = %.3e  (#E %d, #G %d)\n", m->iter, newE, m->deltaE,
-           m->error_estimate, m->step, m->num_energy, m->num_grad);
-  const int miniter = std::max(m->miniter, m->confirm_until);
-  if (m->deltaE == 0 && old_deltaE == 0) {
-    m->stop = DEFT_STOP_NO_CHANGE;
-    *more = m->iter < miniter;
-    return DEFT_OK;
-  }
-  const double rp = (m->switch_rp > 0 && m->phase == 0) ? m->switch_rp : m->relative_precision;
-  if (!(m->error_estimate < m->precision) && !(m->error_estimate < rp * std::fabs(newE))) {
-    m->stop = DEFT_STOP_NONE;
-    *more = 1;
-    return DEFT_OK;
-  }
-  m->stop = DEFT_STOP_CONVERGED;
-  *more = m->iter < miniter;
-  return DEFT_OK;
+int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
+  DeviceOps ops{m->p};
+  return deft::improve_energy_mixed(*m, ops, verbose, more);
 }
 
-// improve_energy plus the FP32 -> FP64 precision ladder:
-//   phase 0: energies and gradients with FP32 FFTs;
-//   phase 1: line-search energies with FP64 FFTs, gradients still FP32 (the
-//            CG direction and convergence history carry over);
-//   phase 2: everything FP64 (the CG direction restarts).
-// A phase ends whenever the reference algorithm would stop (converged at the
-// phase's precision, or a stalled line search); NaN, infinity and the
-// iteration cap stop the minimizer at any phase.  Only phase 2 may end the
-// minimization, so the final answer is always an FP64-converged one.
-// Phase 1 is skipped when the polishing gradient precision is FP64.
-int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {
-  // CONVERGED here means the last iteration met the precision and the final
-  // phase is confirming it
-  // (a no-change verdict still inside the confirmation window counts as met:
-  // two line searches in a row returned to their start point)
-  const bool met = m->switch_rp > 0 && m->phase >= 2 &&
-                   (m->stop == DEFT_STOP_CONVERGED || m->stop == DEFT_STOP_NO_CHANGE || m->restart_met);
-  m->restart_met = false;
-  TRY(improve_energy(m, verbose, more));
-  // A confirming iteration whose line search runs into the round-off of the
-  // FP64 energies (no lower energy found, the point is left unchanged) keeps
-  // the verdict of the iteration before it, as the reference would have
-  // stopped there; so does the first iteration after a collapse restart (the
-  // fresh steepest-descent direction finds no lower energy either).
-  if (met && !*more &&
-      (m->stop == DEFT_STOP_ENERGY_ROSE || m->stop == DEFT_STOP_NO_CHANGE ||
-       m->stop == DEFT_STOP_NEGATIVE_CURVATURE)) {
-    m->stop = DEFT_STOP_CONVERGED;
-    return DEFT_OK;
-  }
-  // The reference's error estimate trusts one iteration's energy change, so a
-  // line search that collapsed to a vanishing step reads as convergence; in
-  // the final phase of the ladder such a verdict restarts the CG direction
-  // from the last good step instead (at most kMaxCollapseRestarts times).
-  if (!*more && m->switch_rp > 0 && m->phase >= 2 && m->stop == DEFT_STOP_CONVERGED && m->good_step > 0 &&
-      std::fabs(m->step) < 1e-3 * std::fabs(m->good_step) && m->collapse_restarts < kMaxCollapseRestarts) {
-    m->collapse_restarts++;
-    m->restart_met = true;
-    m->have_dir = false;
-    m->have_old = false;
-    m->oldgradsqr = 1.0;
-    m->step = m->good_step;
-    m->stop = DEFT_STOP_NONE;
-    if (verbose) printf("collapsed step: restarting the direction\n");
-    *more = 1;
-    return DEFT_OK;
-  }
-  if (*more || m->switch_rp <= 0 || m->phase >= 2) return DEFT_OK;
-  const int s = m->stop;
-  if (s != DEFT_STOP_CONVERGED && s != DEFT_STOP_NEGATIVE_CURVATURE && s != DEFT_STOP_ENERGY_ROSE &&
-      s != DEFT_STOP_NO_CHANGE)
-    return DEFT_OK;
-  const int old_g = grad_prec(m);
-  if (m->phase == 0) {
-    m->fp32_iters = m->iter;
-    m->phase = (m->polish_grad_prec == DEFT_FFT_FP32) ? 1 : 2;
-  } else {
-    m->phase = 2;
-  }
-  if (m->phase == 2) {
-    m->iter_at_phase2 = m->iter;
-    // a phase may hand over on one spuriously small energy change (the error
-    // estimate trusts a single iteration); the all-FP64 phase has to confirm
-    // convergence over a few iterations of its own
-    m->confirm_until = m->iter + kConfirmIterations;
-  }
-  m->e_prec = DEFT_FFT_FP64;
-  m->g_prec = (m->phase == 1) ? DEFT_FFT_FP32 : DEFT_FFT_FP64;
-  invalidate(m);
-  // A phase usually ends on a line search that ran into the round-off of its
-  // energies and shrank the step to nothing; the next phase starts from the
-  // step of the last iteration that clearly lowered the energy.
-  if (m->good_step > 0 && std::fabs(m->step) < 1e-3 * std::fabs(m->good_step)) m->step = m->good_step;
-  if (m->g_prec != old_g) {
-    m->have_dir = false;
-    m->have_old = false;
-    m->oldgradsqr = 1.0;
-  }
-  m->stop = DEFT_STOP_NONE;
-  if (verbose) printf("precision phase %d after %d iterations\n", m->phase, m->iter);
-  *more = 1;
-  return DEFT_OK;
+int m_energy(deft_minimize_s *m, double *E) {
+  DeviceOps ops{m->p};
+  return min_energy(*m, ops, E);
 }
 
 // Launches the posted requests as one round and wakes the waiting minimizers;
@@ -661,13 +353,13 @@ int batch_post(deft_plan_s *p, const BatchReq &req, int prec) {
   return DEFT_OK;
 }
 
-int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, bool precond) {
+int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, int precond) {
   return batch_post(p, BatchReq{p, grad, have_old, precond}, prec);
 }
 
 // CG update (or reset) of a batched plan in the next round; grad.direction lands in hscal
 int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir) {
-  BatchReq r{p, false, false, false};
+  BatchReq r{p, false, false, PGRAD_NONE};
   r.cg = reset ? 2 : 1;
   r.beta = beta;
   r.have_dir = have_dir;
@@ -777,6 +469,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   if (hipMalloc(&p->x, nb) != hipSuccess || hipMalloc(&p->d, nb) != hipSuccess || hipMalloc(&p->g, nb) != hipSuccess ||
       hipMalloc(&p->pg, nb) != hipSuccess || hipMalloc(&p->og, nb) != hipSuccess ||
       hipMalloc(&p->vext, sizeof(float) * (size_t)p->N) != hipSuccess ||
+      hipMalloc(&p->vext64, sizeof(double) * (size_t)p->N) != hipSuccess ||
       hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
       hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess ||
@@ -825,7 +518,8 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
     return fail(DEFT_ERR_HIP, "twiddle upload failed");
   if (hipMemset(p->x, 0, nb) != hipSuccess || hipMemset(p->d, 0, nb) != hipSuccess ||
       hipMemset(p->og, 0, nb) != hipSuccess || hipMemset(p->g, 0, nb) != hipSuccess ||
-      hipMemset(p->pg, 0, nb) != hipSuccess || hipMemset(p->vext, 0, sizeof(float) * (size_t)p->N) != hipSuccess)
+      hipMemset(p->pg, 0, nb) != hipSuccess || hipMemset(p->vext, 0, sizeof(float) * (size_t)p->N) != hipSuccess ||
+      hipMemset(p->vext64, 0, nb) != hipSuccess)
     return fail(DEFT_ERR_HIP, "initialisation failed");
   if (hipMemset(p->spec, 0, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess)
     return fail(DEFT_ERR_HIP, "initialisation failed");
@@ -1008,6 +702,7 @@ int deft_plan_set_vext(deft_plan p, const double *vext) {
   std::vector<float> f((size_t)p->N);
   for (long long i = 0; i < p->N; ++i) f[i] = (float)vext[i];
   DEFT_HIP_CHECK(hipMemcpyAsync(p->vext, f.data(), sizeof(float) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
+  DEFT_HIP_CHECK(hipMemcpyAsync(p->vext64, vext, sizeof(double) * (size_t)p->N, hipMemcpyHostToDevice, p->stream));
   TRY(plan_sync(p));
   return DEFT_OK;
 }
@@ -1070,7 +765,7 @@ int deft_plan_sizes(deft_plan p, long long *n_real, long long *n_spectral) {
 int deft_functional_energy(deft_plan p, const double *veff, double *energy, deft_energy_terms *terms) {
   TRY(check_plan(p));
   if (veff) TRY(deft_plan_set_veff(p, veff));
-  TRY(evaluate(p, false, false, false));
+  TRY(evaluate(p, false, false, PGRAD_NONE));
   p->last_terms = terms_from_sums(p);
   if (energy) *energy = p->last_terms.total;
   if (terms) *terms = p->last_terms;
@@ -1079,13 +774,20 @@ int deft_functional_energy(deft_plan p, const double *veff, double *energy, deft
 
 int deft_functional_grad_device(deft_plan p, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
                                 double *energy) {
+  return deft_functional_grad_device_kind(p, veff_dev, outgrad_dev, outpgrad_dev, energy, DEFT_PGRAD_NEW);
+}
+
+int deft_functional_grad_device_kind(deft_plan p, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
+                                     double *energy, int pgrad_kind) {
   TRY(check_plan(p));
   if (!outgrad_dev) return set_error(DEFT_ERR_ARG, "null outgrad");
+  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL)
+    return set_error(DEFT_ERR_ARG, "unknown pgrad kind");
   if (veff_dev) {
     p->npend = 0;
     DEFT_HIP_CHECK(hipMemcpyAsync(p->x, veff_dev, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToDevice, p->stream));
   }
-  TRY(evaluate(p, true, false, true));
+  TRY(evaluate(p, true, false, pgrad_kind));
   p->last_terms = terms_from_sums(p);
   if (energy) *energy = p->last_terms.total;
   hipLaunchKernelGGL(k_accumulate, dim3(p->eblocks), dim3(kThreads), 0, p->stream, p->N, outgrad_dev, p->g);
@@ -1097,10 +799,17 @@ int deft_functional_grad_device(deft_plan p, const double *veff_dev, double *out
 }
 
 int deft_functional_grad(deft_plan p, const double *veff, double *outgrad, double *outpgrad, double *energy) {
+  return deft_functional_grad_kind(p, veff, outgrad, outpgrad, energy, DEFT_PGRAD_NEW);
+}
+
+int deft_functional_grad_kind(deft_plan p, const double *veff, double *outgrad, double *outpgrad, double *energy,
+                              int pgrad_kind) {
   TRY(check_plan(p));
   if (!outgrad) return set_error(DEFT_ERR_ARG, "null outgrad");
+  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL)
+    return set_error(DEFT_ERR_ARG, "unknown pgrad kind");
   if (veff) TRY(deft_plan_set_veff(p, veff));
-  TRY(evaluate(p, true, false, true));
+  TRY(evaluate(p, true, false, pgrad_kind));
   p->last_terms = terms_from_sums(p);
   if (energy) *energy = p->last_terms.total;
   std::vector<double> h((size_t)p->N);
@@ -1363,7 +1072,7 @@ int deft_minimize_set_miniter(deft_minimize m, int v) {
 int deft_minimize_precondition(deft_minimize m, int v) {
   if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
   m->use_precond = v != 0;
-  invalidate(m);
+  min_invalidate(*m);
   return DEFT_OK;
 }
 
@@ -1465,7 +1174,7 @@ int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_pr
     m->e_prec = DEFT_FFT_FP32;
     m->g_prec = DEFT_FFT_FP32;
   }
-  invalidate(m);
+  min_invalidate(*m);
   return DEFT_OK;
 }
 
@@ -1513,7 +1222,128 @@ int deft_plan_profile_get(deft_plan p, int kernel, const char **name, long long 
 
 int deft_minimize_invalidate(deft_minimize m) {
   if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
-  invalidate(m);
+  min_invalidate(*m);
+  return DEFT_OK;
+}
+
+int deft_minimize_set_gradient_check(deft_minimize m, double factor) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (!(factor >= 0)) return set_error(DEFT_ERR_ARG, "factor must be >= 0");
+  m->grad_check = factor;
+  return DEFT_OK;
+}
+
+int deft_minimize_gradient_check_get(deft_minimize m, double *last_grad_error, int *restarts) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (last_grad_error) *last_grad_error = m->last_grad_error;
+  if (restarts) *restarts = m->grad_restarts;
+  return DEFT_OK;
+}
+
+// --- old-API minimizer stack -------------------------------------------------
+int deft_cg_create(deft_plan p, int preconditioned, double stepsize, deft_cg *out) {
+  TRY(check_plan(p));
+  if (!out) return set_error(DEFT_ERR_ARG, "null output");
+  if (p->batch) return set_error(DEFT_ERR_STATE, "the old-API minimizer runs on plans outside batches");
+  if (!std::isfinite(stepsize) || stepsize == 0) return set_error(DEFT_ERR_ARG, "stepsize must be finite and nonzero");
+  deft_cg_s *m = new deft_cg_s();
+  m->p = p;
+  m->preconditioned = preconditioned != 0;
+  m->step = m->orig_step = stepsize;
+  *out = m;
+  return DEFT_OK;
+}
+
+int deft_cg_destroy(deft_cg m) {
+  delete m;
+  return DEFT_OK;
+}
+
+int deft_cg_set_precision(deft_cg m, double precision) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (!(precision >= 0)) return set_error(DEFT_ERR_ARG, "precision must be >= 0");
+  m->precision = precision;
+  // PrecisionMinimizer's constructor (src/Precision.cpp:39-46)
+  m->error_estimate = std::max(1.0, 2 * precision);
+  m->dEdn = 0.0;
+  m->log_avg = 0.0;
+  m->deltaE = 0.0;
+  return DEFT_OK;
+}
+
+int deft_cg_set_maxiter(deft_cg m, int maxiter) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->maxiter = maxiter;
+  return DEFT_OK;
+}
+
+int deft_cg_set_fft_precision(deft_cg m, int fft_precision) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (fft_precision != -1 && fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  m->prec = fft_precision;
+  m->E_valid = m->G_valid = false;
+  return DEFT_OK;
+}
+
+int deft_cg_improve_energy(deft_cg m, int verbose, int *more) {
+  if (!m || !more) return set_error(DEFT_ERR_ARG, "null argument");
+  DeviceOps ops{m->p};
+  return old_improve_energy(*m, ops, verbose, more);
+}
+
+int deft_cg_energy(deft_cg m, double *energy) {
+  if (!m || !energy) return set_error(DEFT_ERR_ARG, "null argument");
+  DeviceOps ops{m->p};
+  return old_energy(*m, ops, energy);
+}
+
+// grad() / pgrad() of the current point into host buffers (N doubles, written)
+static int cg_vector(deft_cg m, double *out, bool pgrad) {
+  if (!m || !out) return set_error(DEFT_ERR_ARG, "null argument");
+  deft_plan_s *p = m->p;
+  const bool want_pg = pgrad && m->preconditioned;
+  // the device holds the gradient of the last evaluation; evaluate afresh
+  // unless the cached one is of the current point and of the right kind
+  DeviceOps ops{p};
+  double E, pgg, pgo;
+  TRY(ops.gradient(old_prec(*m, p->prec), false, want_pg ? MIN_PGRAD_OLD : MIN_PGRAD_NONE, &E, &pgg, &pgo));
+  if (!m->E_valid) {
+    m->E = E;
+    m->E_valid = true;
+  }
+  DEFT_HIP_CHECK(hipMemcpyAsync(out, pgrad ? p->pg : p->g, sizeof(double) * (size_t)p->N, hipMemcpyDeviceToHost,
+                                p->stream));
+  TRY(plan_sync(p));
+  return DEFT_OK;
+}
+
+int deft_cg_grad(deft_cg m, double *grad) { return cg_vector(m, grad, false); }
+
+int deft_cg_pgrad(deft_cg m, double *pgrad) { return cg_vector(m, pgrad, true); }
+
+int deft_cg_minimize(deft_cg m) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  old_restart(*m);
+  return DEFT_OK;
+}
+
+int deft_cg_invalidate(deft_cg m) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  m->E_valid = m->G_valid = false;
+  return DEFT_OK;
+}
+
+int deft_cg_stats_get(deft_cg m, deft_minimize_stats *s) {
+  if (!m || !s) return set_error(DEFT_ERR_ARG, "null argument");
+  s->iter = m->iter;
+  s->num_energy_calcs = m->num_energy;
+  s->num_grad_calcs = m->num_grad;
+  s->stop_reason = m->stop;
+  s->energy = m->E;
+  s->delta_energy = m->deltaE;
+  s->error_estimate = m->error_estimate;
+  s->step = m->step;
   return DEFT_OK;
 }
 

@@ -21,6 +21,12 @@ namespace deft {
     if (e_ != hipSuccess) return set_error(DEFT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+#define TRY_LAUNCH(expr)             \
+  do {                               \
+    const int rc_ = (expr);          \
+    if (rc_ != DEFT_OK) return rc_;  \
+  } while (0)
+
 template <int V>
 using ic = std::integral_constant<int, V>;
 
@@ -198,9 +204,10 @@ static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
     constexpr int L = decltype(LZ)::value;
     with_zb(p->tz[ti], [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
-      ProfScope ps(p, kid<T>(K_ZFWD_DENSITY), (nm > 0 ? 28.0 : 12.0) * p->N + cb);
+      ProfScope ps(p, kid<T>(K_ZFWD_DENSITY), (nm > 0 ? 24.0 : 8.0) * p->N + (double)sizeof(T) * p->N + cb);
       hipLaunchKernelGGL((k_zfwd_density<T, L, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g, p->x,
-                         p->d, m1, m2, nm, p->vext, spec_ptr<T>(p, 0), p->partial, p->pstride, tw_ptr<T>(p, 2));
+                         p->d, m1, m2, nm, vext_ptr<T>(p), spec_ptr<T>(p, 0), p->partial, p->pstride,
+                         tw_ptr<T>(p, 2));
     });
   });
   launch_ypass<T>(p, g, spec_ptr<T>(p, 0), false, 1, p->ty[ti]);
@@ -209,11 +216,20 @@ static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
 }
 
 template <typename T>
-int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) {
+int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
+  int nzb = 0;
+  TRY_LAUNCH(enqueue_evaluation(p, grad, have_old, precond, &nzb));
+  // every pass wrote its partial sums from nzb blocks
+  return plan_reduce_wait(p, SLOT_IDEAL, grad ? 8 : 6, nzb);
+}
+
+template <typename T>
+int Launch<T>::enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nzb_out) {
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
   const int zb = p->tz[ti];
   const int nzb = zblocks_for(p, zb);
+  *nzb_out = nzb;
   launch_density_to_weighted<T>(p, g);
   launch_zphi<T>(p, g, grad, zb);
   if (grad) {
@@ -224,16 +240,16 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond) 
       constexpr int L = decltype(LZ)::value;
       with_zb(zb, [&](auto ZBc) {
         constexpr int ZB = decltype(ZBc)::value;
-        ProfScope ps(p, kid<T>(K_ZGRAD), (double)sizeof(cx<T>) * p->Nk + (have_old ? 36.0 : 28.0) * p->N);
+        ProfScope ps(p, kid<T>(K_ZGRAD),
+                     (double)sizeof(cx<T>) * p->Nk + (have_old ? 32.0 : 24.0) * p->N + (double)sizeof(T) * p->N);
         hipLaunchKernelGGL((k_zgrad<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0), p->x,
-                           p->vext, p->og, have_old ? 1 : 0, precond ? 1 : 0, p->g, p->pg, p->partial, p->pstride,
+                           vext_ptr<T>(p), p->og, have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride,
                            tw_ptr<T>(p, 2));
       });
     });
   }
   DEFT_HIP_CHECK(hipGetLastError());
-  // every pass wrote its partial sums from nzb blocks
-  return plan_reduce_wait(p, SLOT_IDEAL, grad ? 8 : 6, nzb);
+  return DEFT_OK;
 }
 
 // --- batched evaluation ----------------------------------------------------
@@ -286,9 +302,9 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
     a.x = p->x;
     a.d = p->d;
     a.have_old = reqs[i].have_old ? 1 : 0;
-    a.precond = reqs[i].precond ? 1 : 0;
+    a.precond = reqs[i].precond;
     a.nzb = zblocks_for(p, p->tz[ti]);
-    a.vext = p->vext;
+    a.vext = vext_ptr<T>(p);
     a.spec0 = spec_ptr<T>(p, 0);
     a.spec1 = spec_ptr<T>(p, 1);
     a.partial = p->partial;
@@ -422,7 +438,7 @@ static void fill_batch_args(BatchArgs<T> &a, deft_plan_s *p, int ti) {
   a.x = p->x;
   a.d = p->d;
   a.nzb = zblocks_for(p, p->tz[ti]);
-  a.vext = p->vext;
+  a.vext = vext_ptr<T>(p);
   a.spec0 = spec_ptr<T>(p, 0);
   a.spec1 = spec_ptr<T>(p, 1);
   a.partial = p->partial;
@@ -452,7 +468,7 @@ int Launch<T>::evaluate_batch_graph(hipStream_t st, BatchStage &S, size_t region
     BatchArgs<T> &a = args[p->batch_index];
     take_moves(p, a.m1, a.m2, a.nmoves);
     a.have_old = reqs[r].have_old ? 1 : 0;
-    a.precond = reqs[r].precond ? 1 : 0;
+    a.precond = reqs[r].precond;
     a.kind = reqs[r].grad ? 2 : 1;
   }
   std::vector<int> sig;

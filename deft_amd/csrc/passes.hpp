@@ -253,7 +253,7 @@ __device__ __forceinline__ void pack_c2r(const cx<T>* raw, cx<T>* Z) {
 template <typename T, int L, int ZB>
 __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, double* __restrict__ x,
                                                   const double* __restrict__ d, double m1, double m2, int nmoves,
-                                                  const float* __restrict__ vext, cx<T>* __restrict__ out,
+                                                  const T* __restrict__ vext, cx<T>* __restrict__ out,
                                                   double* __restrict__ partial, int pstride,
                                                   const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
@@ -275,7 +275,7 @@ __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, dou
         if (nmoves > 1) xv = __dadd_rn(xv, __dmul_rn(m2, dv));
         x[idx] = xv;
       }
-      const float ve = vext[idx];
+      const T ve = vext[idx];
       // +inf: hard wall, n = 0; NaN: constrained point (the old-API
       // constrain(mask, f), src/Functional.cpp:759-838): its density counts in
       // the energy, without external potential, and its gradient is zero
@@ -301,7 +301,7 @@ __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, dou
 template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zfwd_density(GridArgs g, double* __restrict__ x,
                                                            const double* __restrict__ d, double m1, double m2,
-                                                           int nmoves, const float* __restrict__ vext,
+                                                           int nmoves, const T* __restrict__ vext,
                                                            cx<T>* __restrict__ out, double* __restrict__ partial,
                                                            int pstride, const cx<T>* __restrict__ tw) {
   zfwd_density_body<T, L, ZB>(blockIdx.x, g, x, d, m1, m2, nmoves, vext, out, partial, pstride, tw);
@@ -831,11 +831,18 @@ __global__ __launch_bounds__(kThreads) void k_zphi11(GridArgs g, cx<T>* __restri
 
 // ---------------------------------------------------------------------------
 // z pass 3: c2r of the back-convolution -> dF/dn -> grad and pgrad (FP64),
-// with pgrad.grad and pgrad.oldgrad partial sums.
+// with pgrad.grad and pgrad.oldgrad partial sums.  The preconditioned
+// gradient is one of (`precond`):
+//   PGRAD_NONE 0   pgrad = grad
+//   PGRAD_NEW  1   grad/n = -(dF/dn)/kT, the new API's (src/haskell/NewCode.hs:229-242)
+//   PGRAD_OLD  2   (dF/dn)/(Veff - kT), the old API's OfEffectivePotential
+//                  (src/EffectivePotentialToDensity.cpp:62-71)
 // ---------------------------------------------------------------------------
+enum PgradKind : int { PGRAD_NONE = 0, PGRAD_NEW = 1, PGRAD_OLD = 2 };
+
 template <typename T, int L, int ZB>
 __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T>* __restrict__ accs,
-                                           const double* __restrict__ x, const float* __restrict__ vext,
+                                           const double* __restrict__ x, const T* __restrict__ vext,
                                            const double* __restrict__ oldgrad, int have_old, int precond,
                                            double* __restrict__ grad, double* __restrict__ pgrad,
                                            double* __restrict__ partial, int pstride,
@@ -856,14 +863,14 @@ __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T
       const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
       const double conv = (double)((l & 1) ? v.im : v.re);
       const double xv = x[idx];
-      const float ve = vext[idx];
+      const T ve = vext[idx];
       double gr = 0.0, pg = 0.0;
       if (isfinite(ve)) {
         const double n = exp(-xv / g.kT);
         const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu;
         const double dfdn = (g.kT * conv + local) * g.dV;
         gr = -dfdn * n / g.kT;
-        pg = precond ? -dfdn / g.kT : gr;
+        pg = precond == PGRAD_NEW ? -dfdn / g.kT : (precond == PGRAD_OLD ? dfdn / (xv - g.kT) : gr);
       }
       grad[idx] = gr;
       pgrad[idx] = pg;
@@ -875,7 +882,7 @@ __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T
 }
 template <typename T, int L, int ZB>
 __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __restrict__ accs,
-                                                    const double* __restrict__ x, const float* __restrict__ vext,
+                                                    const double* __restrict__ x, const T* __restrict__ vext,
                                                     const double* __restrict__ oldgrad, int have_old, int precond,
                                                     double* __restrict__ grad, double* __restrict__ pgrad,
                                                     double* __restrict__ partial, int pstride,

@@ -82,7 +82,10 @@ struct deft_plan_s {
   // FP64 vectors (N each): Veff (the minimization variable), CG direction,
   // gradient, preconditioned gradient, previous gradient
   double *x = nullptr, *d = nullptr, *g = nullptr, *pg = nullptr, *og = nullptr;
-  float *vext = nullptr;   // FP32, +inf marks hard walls
+  // Vext at both FFT precisions: the FP32 passes read the rounded copy, the
+  // FP64 passes the caller's values (+inf marks hard walls, NaN constrained points)
+  float *vext = nullptr;
+  double *vext64 = nullptr;
   void *spec = nullptr;    // spec_fields half spectra of ks complex values (sized for FP64)
   int spec_fields = 6;     // 6, or 12 once the tensor functional was chosen
   int batch_index = -1;    // position in its batch
@@ -131,6 +134,27 @@ inline double water_nQ(double kT) {
   return std::pow(mass * kT / (2 * M_PI), 1.5);
 }
 
+inline bool det3(const double *c, double &det) {
+  det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) + c[2] * (c[3] * c[7] - c[4] * c[6]);
+  return std::isfinite(det) && det != 0.0;
+}
+
+// rows b_j of 2 pi inv(cell)^T, so that a_i . b_j = 2 pi delta_ij
+inline void reciprocal(const double *c, double det, double *b) {
+  double inv[9];
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) b[j * 3 + k] = 2 * M_PI * inv[k * 3 + j];
+}
+
 inline GridArgs grid_args(const deft_plan_s *p) {
   GridArgs a{};
   a.wtab = p->wtab;
@@ -157,6 +181,17 @@ inline GridArgs grid_args(const deft_plan_s *p) {
 template <typename T>
 inline cx<T> *spec_ptr(deft_plan_s *p, int field) {
   return reinterpret_cast<cx<T> *>(p->spec) + (long long)field * p->ks;
+}
+
+template <typename T>
+inline const T *vext_ptr(const deft_plan_s *p);
+template <>
+inline const float *vext_ptr<float>(const deft_plan_s *p) {
+  return p->vext;
+}
+template <>
+inline const double *vext_ptr<double>(const deft_plan_s *p) {
+  return p->vext64;
 }
 
 template <typename T>
@@ -231,7 +266,8 @@ inline int plan_sync(deft_plan_s *p) {
 // One grid's evaluation request in a batched round.
 struct BatchReq {
   deft_plan_s *p;
-  bool grad, have_old, precond;
+  bool grad, have_old;
+  int precond;  // PgradKind
   int cg = 0;  // 0: an evaluation; 1: CG update, 2: CG reset (then grad.direction)
   double beta = 0.0;
   int have_dir = 0;
@@ -326,7 +362,10 @@ struct Launch {
                                   int nplans, const BatchReq *reqs, int n, BatchGraph &gr, long long *launches);
   // Energy (grad=false) or energy + grad + pgrad (grad=true) of the resident
   // Veff after applying pending moves; reduced sums land in p->hscal.
-  static int evaluate(deft_plan_s *p, bool grad, bool have_old, bool precond);
+  static int evaluate(deft_plan_s *p, bool grad, bool have_old, int precond);
+  // The same passes enqueued on p->stream without the reduction and the wait;
+  // *nzb = blocks whose partial sums the passes wrote (slots SLOT_IDEAL..).
+  static int enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nzb);
   // times the tunable kernel variants on this grid and keeps the fastest
   static int autotune(deft_plan_s *p);
   // fft seam: real FP64 (device, N) -> spectrum in field 0 (scaled by scl)

@@ -83,11 +83,29 @@ typedef enum {
   DEFT_STOP_NEGATIVE_CURVATURE = 6,
   DEFT_STOP_ENERGY_ROSE = 7,
   DEFT_STOP_NO_CHANGE = 8,
-  DEFT_STOP_CONVERGED = 9
+  DEFT_STOP_CONVERGED = 9,
+  /* the old-API conjugate gradient did not lower the energy (its
+   * improve_energy returned false, src/ConjugateGradient.cpp:107,159) */
+  DEFT_STOP_NOT_IMPROVED = 10,
+  /* the precision ladder's final verdict was not confirmed by the gradient
+   * at the final point (no counterpart in the reference) */
+  DEFT_STOP_GRADIENT = 11
 } deft_stop_reason;
+
+/* Preconditioned gradient of deft_functional_grad_kind:
+ *   DEFT_PGRAD_NEW                 grad/n = -(dF/dn)/kT, the new API's
+ *                                  energy_grad_and_precond (src/haskell/NewCode.hs:229-242)
+ *   DEFT_PGRAD_EFFECTIVE_POTENTIAL (dF/dn)/(Veff - kT), what the old API's
+ *                                  OfEffectivePotential::grad accumulates into outpgrad
+ *                                  (src/EffectivePotentialToDensity.cpp:62-71) */
+typedef enum {
+  DEFT_PGRAD_NEW = 1,
+  DEFT_PGRAD_EFFECTIVE_POTENTIAL = 2
+} deft_pgrad_kind;
 
 typedef struct deft_plan_s *deft_plan;
 typedef struct deft_minimize_s *deft_minimize;
+typedef struct deft_cg_s *deft_cg;
 
 /* Per-term energies of the last evaluation, like the reference's printme /
  * print_summary (src/Functional.cpp:281-316): kT*int Phi1, kT*int Phi2,
@@ -140,7 +158,8 @@ int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
 /* One of the deft_functional_kind values, default DEFT_WHITE_BEAR; drops nothing else. */
 int deft_plan_set_functional(deft_plan plan, int functional);
-/* Vext (N doubles, host); stored as FP32 on the device.  +inf marks a hard wall
+/* Vext (N doubles, host); kept on the device in FP64 for the FP64 passes and
+ * rounded to FP32 for the FP32 passes.  +inf marks a hard wall
  * (density exactly 0); NaN marks a constrained point as the old API's
  * constrain(mask, f) (src/Functional.cpp:759-838) treats mask = 0: its density
  * exp(-Veff/kT) counts in the energy (no external potential), its gradient and
@@ -163,15 +182,27 @@ int deft_plan_sizes(deft_plan plan, long long *n_real, long long *n_spectral);
  * NewFunctional::energy.  veff may be NULL to use the resident Veff; otherwise
  * it is uploaded and becomes the resident Veff. */
 int deft_functional_energy(deft_plan plan, const double *veff, double *energy, deft_energy_terms *terms);
-/* FunctionalInterface::grad (src/Functional.h:147-149,184-188) with the
- * OfEffectivePotential chain rule (src/EffectivePotentialToDensity.cpp:62-79):
+/* The generated new-API class's energy_grad_and_precond (src/haskell/NewCode.hs:229-242)
+ * in the accumulate form of FunctionalInterface::grad (src/Functional.h:147-149,184-188),
+ * with the OfEffectivePotential chain rule (src/EffectivePotentialToDensity.cpp:62-79):
  *   outgrad  += dF/dVeff = -(n/kT) dF/dn
- *   outpgrad += grad/n = -dF/dn / kT  (new-API preconditioner, src/haskell/NewCode.hs:229-242)
- * outpgrad and energy may be NULL.  Host buffers of N doubles. */
+ *   outpgrad += grad/n = -dF/dn / kT  (new-API preconditioner)
+ * outpgrad and energy may be NULL.  Host buffers of N doubles.  Equal to
+ * deft_functional_grad_kind with DEFT_PGRAD_NEW. */
 int deft_functional_grad(deft_plan plan, const double *veff, double *outgrad, double *outpgrad, double *energy);
 /* Same with device buffers (veff may be NULL). */
 int deft_functional_grad_device(deft_plan plan, const double *veff_dev, double *outgrad_dev, double *outpgrad_dev,
                                 double *energy);
+/* Functional::integralgrad(kT, gd, x, g, pg) of the old API (src/Functional.h:144-149)
+ * for pgrad_kind = DEFT_PGRAD_EFFECTIVE_POTENTIAL:
+ *   outgrad  += -(n/kT) dF/dn,   outpgrad += (dF/dn)/(Veff - kT)
+ * (src/EffectivePotentialToDensity.cpp:62-71); constrained (Vext = NaN) and
+ * hard-wall points get 0 in both, as constrain(mask, f) masks them
+ * (src/Functional.cpp:809-824).  DEFT_PGRAD_NEW gives deft_functional_grad. */
+int deft_functional_grad_kind(deft_plan plan, const double *veff, double *outgrad, double *outpgrad, double *energy,
+                              int pgrad_kind);
+int deft_functional_grad_device_kind(deft_plan plan, const double *veff_dev, double *outgrad_dev,
+                                     double *outpgrad_dev, double *energy, int pgrad_kind);
 /* The five weighted densities n3, n2, n2v_x, n2v_y, n2v_z of the resident Veff
  * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R).
  * Tensor White-Bear plans add the shell tensor T_xx, T_yy, T_zz, T_xy, T_yz, T_zx (11*N). */
@@ -226,6 +257,47 @@ int deft_minimize_stats_get(deft_minimize m, deft_minimize_stats *stats);
 /* Drops cached energy/gradient (after the caller changed Veff or the plan). */
 int deft_minimize_invalidate(deft_minimize m);
 
+/* --- old-API minimizer seam -------------------------------------------------
+ * MinimizerInterface (src/Minimizer.h:16-81) as built by the contact-paper
+ * drivers, Precision(p, PreconditionedConjugateGradient(f, gd, kT, &x,
+ * QuadraticLineMinimizer)) (papers/contact/figs/inner-sphere.cpp:205-210):
+ *   deft_cg_create       ConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer, stepsize)
+ *                        (preconditioned = 0, src/ConjugateGradient.cpp:59-108) or
+ *                        PreconditionedConjugateGradient(...) (= 1, :115-160, with the
+ *                        old-API pgrad (dF/dn)/(Veff - kT)); x is the plan's resident
+ *                        Veff, which the minimizer does not own (src/Minimizer.h:19).
+ *                        The reference's factories default stepsize to 10
+ *                        (src/LineMinimizer.h:18-21).
+ *   deft_cg_set_precision  wraps it in Precision(precision, m) (src/Precision.cpp:23-156);
+ *                        0 removes the wrapper
+ *   deft_cg_set_maxiter  wraps it in MaxIter(maxiter, m) (src/Minimizer.cpp); < 0 removes it
+ *   deft_cg_improve_energy  MinimizerInterface::improve_energy: *more = 1 while the
+ *                        outermost wrapper wants to go on
+ *   deft_cg_energy, deft_cg_grad, deft_cg_pgrad  energy(), grad(), pgrad() of the
+ *                        current point (grad/pgrad: N doubles into host buffers, written)
+ *   deft_cg_minimize     minimize(f, gd, x): iteration count, step, direction and old
+ *                        gradient start over (src/ConjugateGradient.cpp:34-44)
+ * The reference recomputes every energy() call; here a point's energy is
+ * computed once and reused until the point moves (same values, same
+ * iterates; num_energy_calcs counts the evaluations actually run).  The
+ * evaluations use the plan's FFT precision unless deft_cg_set_fft_precision
+ * chooses one (-1: the plan's). */
+int deft_cg_create(deft_plan plan, int preconditioned, double stepsize, deft_cg *out);
+int deft_cg_destroy(deft_cg cg);
+int deft_cg_set_precision(deft_cg cg, double precision);
+int deft_cg_set_maxiter(deft_cg cg, int maxiter);
+int deft_cg_set_fft_precision(deft_cg cg, int fft_precision);
+int deft_cg_improve_energy(deft_cg cg, int verbose, int *more);
+int deft_cg_energy(deft_cg cg, double *energy);
+int deft_cg_grad(deft_cg cg, double *grad);
+int deft_cg_pgrad(deft_cg cg, double *pgrad);
+int deft_cg_minimize(deft_cg cg);
+/* Drops the cached energy and gradient (after the caller changed Veff). */
+int deft_cg_invalidate(deft_cg cg);
+/* iter, evaluations, stop reason, energy, Precision's last delta E (old minus
+ * new) and error estimate, remembered step */
+int deft_cg_stats_get(deft_cg cg, deft_minimize_stats *stats);
+
 /* --- mixed precision (no counterpart in the reference, which is FP64 only) ---
  * With switch_relative_precision > 0 the minimizer climbs a precision ladder:
  *   phase 0: energies and gradients with FP32 FFTs, until the error estimate
@@ -250,6 +322,19 @@ int deft_minimize_set_mixed_precision(deft_minimize m, double switch_relative_pr
  * algorithm; only the evaluation counts change.  mode 1 on, 0 off, -1 (default) on exactly when
  * the mixed-precision ladder is on. */
 int deft_minimize_set_fused_line_search(deft_minimize m, int mode);
+/* Gradient confirmation of the ladder's final verdict (no counterpart in the
+ * reference, whose estimate trusts the energy changes alone): with
+ * preconditioning on, a stop that phase 2 would report as converged is
+ * accepted only if (kT/dV) pg.g / 2 -- the Newton estimate, from the gradient
+ * at the final point, of the energy still to be gained -- is at most factor x
+ * max(precision, relative_precision |E|); otherwise the CG direction restarts
+ * from steepest descent (at most 4 times, then the stop is
+ * DEFT_STOP_GRADIENT).  factor 0 disables it; the default is set when the
+ * ladder is switched on.  Plain (non-ladder) minimizations never use it. */
+int deft_minimize_set_gradient_check(deft_minimize m, double factor);
+/* Newton estimate of the last confirmation (-1 if none ran) and the number of
+ * restarts it caused. */
+int deft_minimize_gradient_check_get(deft_minimize m, double *last_grad_error, int *restarts);
 /* Iterations run in phase 0, -1 if the ladder was not used or not left. */
 int deft_minimize_fp32_iterations(deft_minimize m, int *iterations);
 /* Iterations of phase 0 and phase 1 and the current phase (0, 1 or 2). */

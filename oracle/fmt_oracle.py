@@ -35,6 +35,12 @@ It restates, in numpy/scipy FP64, the reference algorithm of deft
 * the new-API Polak-Ribiere conjugate gradient with quadratic
   line search, cached energies and the geometric error
   estimate                                                      (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
+* the old-API minimizer stack: ConjugateGradient and
+  PreconditionedConjugateGradient (pgrad (dF/dn)/(Veff - kT))
+  with QuadraticLineMinimizer, Precision and MaxIter            (src/ConjugateGradient.cpp:59-171,
+                                                                 src/QuadraticLineMinimizer.cpp:27-215,
+                                                                 src/Precision.cpp:23-156, src/Minimizer.cpp,
+                                                                 src/EffectivePotentialToDensity.cpp:62-71)
 
 One deliberate numerical difference from the reference: the Phi3 prefactor
 f3(n3) = (n3 + (1-n3)^2 ln(1-n3)) / (n3^2 (1-n3)^2) is evaluated with its Taylor
@@ -518,6 +524,17 @@ class WhiteBearFluidVeff:
     def have_preconditioner(self):
         return True
 
+    def grad_and_pgrad(self, x=None):
+        """Old-API integralgrad with pgrad (src/EffectivePotentialToDensity.cpp:62-71):
+        grad = -(n/kT) dF/dn, pgrad = (dF/dn)/(Veff - kT); zero on hard-wall and
+        constrained points (the constraint masks both, src/Functional.cpp:809-824)."""
+        x = self.x if x is None else x
+        g, n = self.dF_dn(x)
+        self.n_grad += 1
+        with np.errstate(divide="ignore", invalid="ignore"):
+            pg = np.where(self.finite, g / (x - self.kT), 0.0)
+        return -g * n / self.kT, pg
+
 
 # ---------------------------------------------------------------------------
 # New-API minimizer (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
@@ -745,6 +762,190 @@ class Minimize:
             return True
         self.stop_reason = "converged"
         return self.iter < self.miniter
+
+
+# ---------------------------------------------------------------------------
+# Old-API minimizers (src/ConjugateGradient.cpp, src/QuadraticLineMinimizer.cpp,
+# src/Precision.cpp, src/Minimizer.cpp).  ``f`` provides ``x`` (modified in
+# place), ``energy()``, ``grad()`` and ``grad_and_pgrad()``.  As in the
+# reference, energy() is evaluated afresh on every call.
+# ---------------------------------------------------------------------------
+
+class OldConjugateGradient:
+    """ConjugateGradientType (src/ConjugateGradient.cpp:59-108) or, with
+    ``preconditioned``, PreconditionedConjugateGradientType (:115-160), each
+    with QuadraticLineMinimizer and the remembered step (default 10,
+    src/LineMinimizer.h:18-21)."""
+
+    def __init__(self, f, preconditioned=False, stepsize=10.0):
+        self.f = f
+        self.preconditioned = preconditioned
+        self.step = float(stepsize)
+        self.orig_step = float(stepsize)
+        self.iter = 0
+        self.direction = np.zeros_like(f.x)
+        self.oldgrad = np.zeros_like(f.x)
+        self.oldgradsqr = 0.0
+        self.linmin_reason = None
+        self.log = []
+
+    def energy(self):
+        return self.f.energy()
+
+    def minimize(self):
+        """minimize(f, gd, x): src/ConjugateGradient.cpp:34-44."""
+        self.step = self.orig_step
+        self.iter = 0
+        self.direction = np.zeros_like(self.f.x)
+        self.oldgrad = np.zeros_like(self.f.x)
+        self.oldgradsqr = 0.0
+
+    def improve_energy(self):
+        self.iter += 1
+        E0 = self.energy()
+        if E0 != E0:
+            return False
+        if not self.preconditioned:
+            g = -self.f.grad()
+            beta = float(np.vdot(g, g - self.oldgrad)) / self.oldgradsqr if self.oldgradsqr else math.nan
+            self.oldgrad = g
+            if beta < 0 or beta != beta or self.oldgradsqr == 0:
+                beta = 0.0
+            self.oldgradsqr = float(np.vdot(self.oldgrad, self.oldgrad))
+            self.direction = g + beta * self.direction
+            gdotd = float(np.vdot(self.oldgrad, self.direction))
+            if gdotd < 0:
+                self.direction = self.oldgrad.copy()
+                gdotd = float(np.vdot(self.oldgrad, self.direction))
+        else:
+            grad, pgrad = self.f.grad_and_pgrad()
+            beta = (float(np.vdot(-pgrad, -grad - self.oldgrad)) / self.oldgradsqr if self.oldgradsqr
+                    else math.nan)
+            self.oldgrad = -grad
+            if beta < 0 or beta != beta or self.oldgradsqr == 0:
+                beta = 0.0
+            self.oldgradsqr = float(np.vdot(-pgrad, self.oldgrad))
+            self.direction = -pgrad + beta * self.direction
+            gdotd = float(np.vdot(self.oldgrad, self.direction))
+        self.linmin_reason = quadratic_line_minimize(self, -gdotd)
+        E = self.energy()
+        self.log.append((self.iter, E, self.step))
+        if self.preconditioned:
+            return E < E0 or beta != 0
+        return E < E0
+
+
+def quadratic_line_minimize(cg, slope):
+    """QuadraticLineMinimizerType::improve_energy (src/QuadraticLineMinimizer.cpp:27-198)
+    along cg.direction with slope = grad.direction, updating cg.step; returns why
+    it returned early (None on the ordinary paths)."""
+    f, d = cg.f, cg.direction
+    E0 = f.energy()
+    if E0 != E0:
+        return "nan-energy"
+    if E0 > 1e307 or E0 < -1e307:
+        return "infinite-energy"
+    if slope != slope:
+        return "nan-slope"
+    if slope == 0:
+        return "zero-slope"
+    if slope * cg.step > 0:
+        cg.step *= -1
+    step1 = cg.step
+    f.x += step1 * d
+    Etried = -137.0
+    while True:
+        step1 *= 0.5
+        f.x -= step1 * d
+        if f.energy() == Etried:
+            break
+        Etried = f.energy()
+        if not _better(E0, f.energy()):
+            break
+    E1 = f.energy()
+    curvature = 2.0 * (E1 - E0 - step1 * slope) / (step1 * step1)
+    step2 = -slope / curvature if curvature != 0 else math.inf
+    if curvature <= 0.0:
+        if _better(E0, E1):
+            f.x -= step1 * d
+            return "negative-curvature"
+        Ebest = E0
+        while f.energy() <= Ebest:
+            Ebest = f.energy()
+            f.x += step1 * d
+            step1 *= 2
+        step1 *= 0.5
+        f.x -= step1 * d
+        cg.step = step1
+    elif E1 == E0:
+        while True:
+            f.x -= step1 * d
+            step1 *= 2
+            f.x += step1 * d
+            if f.energy() != E0:
+                break
+        if f.energy() > E0:
+            f.x -= step1 * d
+        elif f.energy() != f.energy():
+            f.x -= step1 * d
+    else:
+        cg.step = step2
+        f.x += (step2 - step1) * d
+        E2 = f.energy()
+        if _better(E1, E2) and _better(E1, E0):
+            f.x -= (step2 - step1) * d
+        elif E2 > E0:
+            f.x -= step2 * d
+            return "energy-rose"
+    return None
+
+
+class OldPrecision:
+    """PrecisionMinimizer (src/Precision.cpp:23-156) around a minimizer, with
+    MaxIterType (src/Minimizer.cpp) outside it when ``maxiter`` is given."""
+
+    def __init__(self, precision, m, maxiter=None):
+        self.m = m
+        self.criterion = float(precision)
+        self.error_estimate = max(1.0, 2 * precision)
+        self.dEdn = 0.0
+        self.log_avg = 0.0
+        self.deltaE = 0.0
+        self.w = 0.1
+        self.maxiter = maxiter
+        self.iter = 0
+        self.stop_reason = None
+
+    def energy(self):
+        return self.m.energy()
+
+    def improve_energy(self):
+        if self.maxiter is not None and self.iter > self.maxiter:
+            self.stop_reason = "maxiter"
+            return False
+        self.iter += 1
+        old_deltaE = self.deltaE
+        old_energy = self.m.energy()
+        not_done = self.m.improve_energy()
+        new_energy = self.m.energy()
+        self.deltaE = old_energy - new_energy
+        w = self.w
+        self.dEdn = (1 - w) * self.dEdn + w * self.deltaE
+        self.dEdn = max(self.deltaE, old_deltaE)
+        new_log = math.log(abs(self.deltaE / old_deltaE)) if (self.deltaE and old_deltaE) else 0.0
+        self.log_avg = (1 - w) * self.log_avg + w * new_log
+        ratio_avg = math.exp(self.log_avg)
+        error_guess = abs(max(self.dEdn, self.deltaE) / self.log_avg) if self.log_avg else math.inf
+        if ratio_avg >= 1:
+            error_guess = self.criterion + abs(new_energy)
+        self.error_estimate = 2 * error_guess
+        if self.deltaE == 0 and old_deltaE == 0:
+            self.stop_reason = "no-change"
+            return False
+        if not_done and not (self.error_estimate < self.criterion):
+            return True
+        self.stop_reason = "converged" if not_done else "not-improved"
+        return False
 
 
 # ---------------------------------------------------------------------------

@@ -98,6 +98,35 @@ def test_energy_and_gradient(gpu, shape, cell, prec, etol, gtol):
         assert np.all(gr[wall] == 0) and np.all(pg[wall] == 0)
 
 
+@pytest.mark.parametrize("shape,cell", [SHAPES[1], SHAPES[3]])
+def test_fp64_vext_is_not_rounded(gpu, shape, cell):
+    """A smooth external potential that FP32 cannot represent (eight Gaussians, as the
+    batched workload draws them): the FP64 passes use it unrounded, so energy and
+    gradient meet the FP64 oracle on the exact potential at round-off; rounding it to
+    FP32 first would move the energy by far more than that."""
+    print("fp64 vext", shape, flush=True)
+    g = O.Grid(shape, cell)
+    rng = np.random.default_rng(21)
+    pos = g.positions()
+    L = np.diag(np.asarray(cell, dtype=np.float64)) if np.allclose(cell, np.diag(np.diag(cell))) else None
+    vext = np.zeros(g.shape)
+    for _ in range(8):
+        c = rng.uniform(0, 1, 3) * np.diag(cell)
+        d = pos - c
+        if L is not None:
+            d -= L * np.round(d / L)
+        vext += rng.uniform(-2, 2) * np.exp(-np.sum(d * d, axis=-1) / (2 * rng.uniform(0.5, 2) ** 2))
+    assert np.max(np.abs(vext - vext.astype(np.float32))) > 1e-9
+    g, of, f = _fluid(gpu, shape, cell, vext=vext, prec=1, seed=4, amp=0.1)
+    E, gr, pg = f.energy_grad_and_precond()
+    Eo, gro, pgo = of.energy_grad_and_precond()
+    assert abs(E - Eo) <= 1e-12 * abs(Eo)
+    assert _rel(gr, gro) < 1e-10 and _rel(pg, pgo) < 1e-10
+    of32 = O.make_fluid(g, 0.3, vext=vext.astype(np.float32).astype(np.float64))
+    of32.x = of.x
+    assert abs(of32.energy() - Eo) > 10 * abs(E - Eo)
+
+
 def test_white_bear_known_answer(gpu):
     """tests/new-hard-spheres.cpp:52-63: uniform n = 0.1, R = 1, cell 2^3, Nx = 20 -> 2.72225468848892."""
     for prec, tol in ((1, 2e-11), (0, 1e-6)):
