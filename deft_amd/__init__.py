@@ -103,6 +103,7 @@ _SIGNATURES = {
     "deft_functional_grad_device": ([_P, _DP, _DP, _DP, _DP], _I),
     "deft_functional_grad_kind": ([_P, _DP, _DP, _DP, _DP, _I], _I),
     "deft_functional_grad_device_kind": ([_P, _DP, _DP, _DP, _DP, _I], _I),
+    "deft_functional_grad_noalloc": ([_I, _I, _I, _DP, _D, _D, _D, _I, _I, _P, _P, _P, _P, _P, _P], _I),
     "deft_minimize_set_gradient_check": ([_P, _D], _I),
     "deft_minimize_gradient_check_get": ([_P, _DP, _IP], _I),
     "deft_cg_create": ([_P, _I, _D, ctypes.POINTER(_P)], _I),
@@ -647,6 +648,16 @@ def MaxIter(maxiter, m):
     """MaxIter(maxiter, m) (src/Minimizer.cpp) around an old-API minimizer."""
     m.set_maxiter(maxiter)
     return m
+
+
+def functional_grad_noalloc(shape, cell, R, kT, mu, fft_precision, pgrad_kind, veff_dev, vext_dev, outgrad_dev,
+                            outpgrad_dev, energy_dev, stream=None):
+    """deft_functional_grad_noalloc on device buffers given as integers (addresses); no
+    allocation, enqueued on `stream` (an integer hipStream_t or None) without waiting."""
+    cell = np.ascontiguousarray(np.asarray(cell, dtype=np.float64).reshape(3, 3))
+    _check(lib().deft_functional_grad_noalloc(int(shape[0]), int(shape[1]), int(shape[2]), _dptr(cell), float(R),
+                                              float(kT), float(mu), int(fft_precision), int(pgrad_kind),
+                                              veff_dev, vext_dev, outgrad_dev, outpgrad_dev, energy_dev, stream))
 
 
 class Batch:

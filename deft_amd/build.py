@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(HERE, "libdeft_amd.so")
 
-SOURCES = ["deft_amd.hip", "launch_f32.hip", "launch_f64.hip", "tensor_f32.hip", "tensor_f64.hip", "nn.hip"]
+SOURCES = ["deft_amd.hip", "launch_f32.hip", "launch_f64.hip", "tensor_f32.hip", "tensor_f64.hip", "nn.hip",
+           "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops keeps the compiler from forming v_pk_{add,mul,fma}_f32.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

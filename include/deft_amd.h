@@ -203,6 +203,24 @@ int deft_functional_grad_kind(deft_plan plan, const double *veff, double *outgra
                               int pgrad_kind);
 int deft_functional_grad_device_kind(deft_plan plan, const double *veff_dev, double *outgrad_dev,
                                      double *outpgrad_dev, double *energy, int pgrad_kind);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): deft_functional_grad_device_kind of a White-Bear fluid
+ * given by its parameters (as deft_plan_create takes them, default tuning)
+ * instead of a plan.  It calls no hipMalloc or hipFree: every intermediate
+ * buffer comes from one library-owned device array of 36 MiB per GPU, held
+ * from the library's load until the process ends, so grids are limited to
+ * nx*ny*nz <= 64^3 = 262144 points (larger ones: DEFT_ERR_ARG).  Inputs and
+ * outputs are device buffers of N doubles: veff_dev (required), vext_dev
+ * (NULL: no external potential; +inf hard wall, NaN constrained point),
+ * outgrad_dev (+=, required), outpgrad_dev (+=, may be NULL) and energy_dev
+ * (one double, written, may be NULL).  The work is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) and the call returns without waiting.
+ * All calls on one GPU share its array: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+int deft_functional_grad_noalloc(int nx, int ny, int nz, const double cell[9], double R, double kT, double mu,
+                                 int fft_precision, int pgrad_kind, const double *veff_dev, const double *vext_dev,
+                                 double *outgrad_dev, double *outpgrad_dev, double *energy_dev, void *stream);
 /* The five weighted densities n3, n2, n2v_x, n2v_y, n2v_z of the resident Veff
  * (5*N doubles, field-major); n0 = n2/(4 pi R^2), n1 = n2/(4 pi R), n1v = n2v/(4 pi R).
  * Tensor White-Bear plans add the shell tensor T_xx, T_yy, T_zz, T_xy, T_yz, T_zx (11*N). */
