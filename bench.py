@@ -23,6 +23,8 @@ import argparse
 import json
 import math
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -178,10 +180,13 @@ def rocprof_instance(family, tuning):
     t = tuning[0 if prec == "f32" else 1]
     zb, ty, tx = t["zblock"], t["ytile"], t["xtile"]
     names = {"zfwd_density": f"deft::k_zfwd_density<{T}, {N}, {zb}>",
-             "ypass1": f"deft::k_ypass<{T}, {N}, false, 1, {ty}>",
-             "ypass5": f"deft::k_ypass<{T}, {N}, true, 5, {ty}>",
-             "xconv_fwd": f"deft::k_xconv_fwd<{T}, {N}, {tx}>", "xconv_bwd": f"deft::k_xconv_bwd<{T}, {N}, {tx}>",
-             "zphi_energy": f"deft::k_zphi<{T}, {N}, false, {zb}>", "zphi_grad": f"deft::k_zphi<{T}, {N}, true, {zb}>",
+             "ypass1_fwd": f"deft::k_ypass<{T}, {N}, false, 1, {ty}>",
+             "ypass1_inv": f"deft::k_ypass<{T}, {N}, true, 1, {ty}>",
+             "ypass5_inv": f"deft::k_ypass<{T}, {N}, true, 5, {ty}>",
+             "ypass5_fwd": f"deft::k_ypass<{T}, {N}, false, 5, {ty}>",
+             "xconv_fwd": f"deft::k_xconv_fwd<{T}, {N}, {tx}, 5>", "xconv_bwd": f"deft::k_xconv_bwd<{T}, {N}, {tx}, 5>",
+             "zphi_energy": f"deft::k_zphi<{T}, {N}, false, {zb}, 0>",
+             "zphi_grad": f"deft::k_zphi<{T}, {N}, true, {zb}, 0>",
              "zgrad": f"deft::k_zgrad<{T}, {N}, {zb}>"}
     return names.get(base)
 
@@ -197,29 +202,86 @@ def committed_traffic(family, tuning):
     return None if t is None else t["fetch_bytes"] + t["write_bytes"]
 
 
+def whole_path_bytes(prof, steps):
+    """Algorithmic HBM bytes of one minimization by the whole-path rule: every 3-D FFT
+    reads its input and writes its output once (B_fft = 4N + 8Nk FP32, 8N + 16Nk FP64),
+    one energy B_E = 6 B_fft + 12N (+ 24N for the line-search move it follows), one
+    energy + gradient B_EG = 12 B_fft + 20N, and 40N of CG vectors per iteration; the
+    evaluations are counted from the profiled steps (energy-only = Phi energy passes,
+    energy + gradient = Phi gradient passes, iterations = CG updates)."""
+    Nr = float(N) ** 3
+    Nk = float(N) * N * (N // 2 + 1)
+    total = 0.0
+    for prec, bfft in (("f32", 4 * Nr + 8 * Nk), ("f64", 8 * Nr + 16 * Nk)):
+        nE = prof.get("zphi_energy_" + prec, (0,))[0]
+        nEG = prof.get("zphi_grad_" + prec, (0,))[0]
+        total += nE * (6 * bfft + 12 * Nr + 24 * Nr) + nEG * (12 * bfft + 20 * Nr)
+    total += prof.get("cg_update", (0,))[0] * 40 * Nr
+    return total / steps
+
+
+def _cpu_info():
+    model = platform.processor()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        fftw = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True).stdout
+        fftw = "libfftw3" in fftw
+    except OSError:
+        fftw = False
+    return model, os.cpu_count(), fftw
+
+
 def cpu_baseline(sample_seconds):
-    """The FP64 oracle (numpy/scipy, one thread) on the same workload, timed over a
-    bounded number of CG iterations and scaled by the oracle's own iteration count."""
-    from oracle import fmt_oracle as O
-    import threadpoolctl
-    gold = json.load(open(os.path.join(HERE, "tests", "golden", "oracle_c2.json")))
-    with threadpoolctl.threadpool_limits(1):
-        g = O.Grid.cubic(N, L)
-        f = O.make_fluid(g, ETA, R=R, kT=KT, vext=O.spherical_solute_vext(g, SOLUTE))
-        m = O.Minimize(f)
-        m.set_relative_precision(1e-12)
-        m.precondition(True)
-        t0 = time.perf_counter()
-        iters = 0
-        while time.perf_counter() - t0 < sample_seconds and m.improve_energy():
-            iters += 1
-        dt = time.perf_counter() - t0
-    per_iter = dt / max(iters, 1)
-    per_min = per_iter * gold["iterations"]
-    return {"value": 1.0 / per_min, "unit": "minimizations/s", "cores": 1, "kind": "port",
-            "sample": f"{iters} CG iterations of the same 128^3 minimization in {dt:.1f} s on one host core "
-                      f"(numpy/scipy FP64 oracle), scaled by the oracle's {gold['iterations']} iterations "
-                      f"to convergence ({per_min:.1f} s per minimization)"}
+    """deft's FP64 CPU path restated in C (oracle/fmt_cpu.c: the new API's cached-energy
+    dataflow, the project's own FFT), one single-threaded minimization per process as in the
+    reference, one process per host core of this GPU's CPU share, same workload, start and
+    convergence rule as the GPU's plain run.  Each process runs the minimization for a bounded
+    sample and reports its per-evaluation times; a minimization costs the full run's own
+    evaluation counts (tests/golden/cpu_c2_rule.json: the same program run to convergence under
+    the same rule) at those times.  Throughput = sum over processes of 1 / (time per minimization)."""
+    exe = os.path.join(HERE, "oracle", "build", "fmt_cpu")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(HERE, "oracle"), "build/fmt_cpu"], check=True,
+                       capture_output=True)
+    full = json.load(open(os.path.join(HERE, "tests", "golden", "cpu_c2_rule.json")))
+    cmd = [exe, "solute", str(N), str(L), str(ETA), str(SOLUTE), repr(full["precision"]), "0"]
+
+    def per_min(r):
+        te = r["seconds_energy"] / max(r["energy_evals"], 1)
+        tg = r["seconds_grad"] / max(r["grad_evals"], 1)
+        evals = r["seconds_energy"] + r["seconds_grad"]
+        overhead = r["seconds"] / evals if evals > 0 else 1.0  # CG vector work between evaluations
+        return (te * full["energy_evals"] + tg * full["grad_evals"]) * overhead
+
+    alone = json.loads(subprocess.run(cmd + [str(sample_seconds / 3)], capture_output=True, text=True,
+                                      check=True).stdout)
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    procs = [subprocess.Popen(cmd + [str(sample_seconds)], stdout=subprocess.PIPE, text=True) for _ in range(cores)]
+    runs = [json.loads(p.communicate()[0]) for p in procs]
+    if any(p.returncode for p in procs):
+        raise RuntimeError("CPU baseline process failed")
+    rate = sum(1.0 / per_min(r) for r in runs)
+    model, nproc, fftw = _cpu_info()
+    t1 = per_min(alone)
+    return {"value": rate, "unit": "minimizations/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent single-threaded processes, each running the 128^3 minimization for "
+                      f"{sample_seconds:.0f} s ({sum(r['energy_evals'] + r['grad_evals'] for r in runs)} evaluations "
+                      f"in all), scaled by the full run's {full['energy_evals']} energy and {full['grad_evals']} "
+                      f"energy+gradient evaluations ({full['iterations']} iterations) under the same rule",
+            "cpu_model": model, "nproc": nproc, "processes": cores,
+            "fft": "the project's own FP64 Stockham FFT" + ("" if not fftw else " (libfftw3 present but not used)")
+                   + ("; FFTW is not installed" if not fftw else ""),
+            "program": "oracle/fmt_cpu.c (new-API Minimize, FP64, one thread per minimization)",
+            "convergence_rule": "|p_bulk V| * 1e-10 absolute, plain start",
+            "iterations": full["iterations"], "dF_over_F_vs_fp64_oracle": full["dF_over_F_vs_oracle_c2"],
+            "single_core_seconds_per_minimization": t1,
+            "all_cores_seconds_per_minimization": cores / rate}
 
 
 def main():
@@ -255,11 +317,14 @@ def main():
     value = args.steps / w["elapsed"]
     plain_value = args.steps / pl["elapsed"]
     prof = runs["prof"]
-    # dominant kernel by event time
+    # dominant kernel: the kernel instance (one profiler entry: pass, direction and FFT
+    # precision at the autotuned variant) with the most event time over the profiled steps
     kname, (kl, kms, kby) = max(((k, v) for k, v in prof.items() if v[0] > 0), key=lambda kv: kv[1][1])
     achieved = kby / (kms * 1e-3) / 1e9
     total_ms = sum(v[1] for v in prof.values())
     total_by = sum(v[2] for v in prof.values())
+    path_bytes = whole_path_bytes(prof, args.steps)
+    path_gbps = path_bytes / (w["elapsed"] / args.steps) / 1e9
     c3 = None if args.no_c3 else c3_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
     r0, p0 = w["results"][-1], pl["results"][-1]
@@ -296,9 +361,16 @@ def main():
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
                      "kernel_time_fraction_of_wall": total_ms * 1e-3 / runs["profiled_elapsed"],
                      "streaming_copy_GBps": copy_gbps, "frac_of_streaming_copy": achieved / copy_gbps,
+                     "whole_path": {"bytes_per_minimization": path_bytes, "achieved_GBps": path_gbps,
+                                    "frac": path_gbps / HBM_PEAK_GBS,
+                                    "rule": "sum over the counted evaluations of B_E = 6 B_fft + 36N (energy + "
+                                            "line-search move) and B_EG = 12 B_fft + 20N, plus 40N per CG "
+                                            "iteration; B_fft = 4N + 8Nk (FP32) / 8N + 16Nk (FP64); over the "
+                                            "warm-start step's wall time (network inference included)"},
                      "measured": f"HIP events around every launch over {args.steps} further warm-start steps "
                                  f"({runs['profiled_elapsed'] / args.steps * 1e3:.2f} ms per step with events)"},
         "cpu_baseline": cpu,
+        "gpu_over_cpu_baseline": None if not cpu else {"warm": value / cpu["value"], "plain": plain_value / cpu["value"]},
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1)}
                     for k, v in prof.items() if v[0] > 0},
     }

@@ -55,7 +55,7 @@ def build(verbose=False):
     return LIB
 
 
-EXAMPLES = ["solute"]
+EXAMPLES = ["solute", "inner_sphere"]
 
 
 def build_examples(verbose=False):

@@ -571,10 +571,26 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
     delete b;
     return set_error(DEFT_ERR_ALLOC, "batch staging allocation failed");
   }
+  // flush and drain every plan before binding any of them, so a failure leaves
+  // no plan pointing at the batch
+  int rc = DEFT_OK;
+  for (int i = 0; i < n && rc == DEFT_OK; ++i) {
+    rc = flush_moves(plans[i]);
+    if (rc == DEFT_OK && hipStreamSynchronize(plans[i]->stream) != hipSuccess)
+      rc = set_error(DEFT_ERR_HIP, "hipStreamSynchronize of a plan failed");
+  }
+  if (rc != DEFT_OK) {
+    const std::string err = g_last_error;
+    (void)hipHostFree(b->stage.host);
+    (void)hipFree(b->stage.dev);
+    (void)hipHostFree(b->hflag);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return set_error(rc, err);
+  }
+  b->plans.reserve(n);
   for (int i = 0; i < n; ++i) {
     deft_plan_s *p = plans[i];
-    TRY(flush_moves(p));
-    DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
     p->own_stream = p->stream;
     p->stream = b->stream;
     p->batch = b;
@@ -1115,7 +1131,14 @@ int deft_batch_minimize(deft_batch b, deft_minimize *mins, int n) {
         g_fiber_start = &f;
       }
       if (swapcontext(&b->sched, &f.ctx) != 0) {
+        // the fibers cannot be resumed: the batch stops minimizing (their
+        // minimizers keep the state of their last completed round)
         b->current = nullptr;
+        b->active = 0;
+        b->nposted = 0;
+        b->posted[0].clear();
+        b->posted[1].clear();
+        b->posted_cg.clear();
         return set_error(DEFT_ERR_STATE, "fiber switch failed");
       }
     }

@@ -101,7 +101,7 @@ inline void take_moves(deft_plan_s *p, double &m1, double &m2, int &nm) {
 template <typename T>
 static void launch_ypass(deft_plan_s *p, const GridArgs &g, cx<T> *data, bool inv, int nf, int tb) {
   const double cb = (double)sizeof(cx<T>) * p->Nk;
-  const int fam = nf == 1 ? K_YPASS1 : K_YPASS5;
+  const int fam = nf == 1 ? (inv ? K_YPASS1_INV : K_YPASS1) : (inv ? K_YPASS5 : K_YPASS5_FWD);
   with_length(p->Ny, [&](auto LY) {
     constexpr int L = decltype(LY)::value;
     with_tb(tb, [&](auto TBc) {

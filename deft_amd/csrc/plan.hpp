@@ -15,12 +15,13 @@
 namespace deft {
 
 // Kernel families timed by the per-plan profiler (HIP events around each
-// launch); each family is counted separately for FP32 and FP64 FFTs, so one
-// entry is one kernel instance for a given grid.
+// launch); each family is counted separately for FP32 and FP64 FFTs and for
+// each transform direction, so one entry is one kernel instance for a given
+// grid.
 enum KernelId : int {
   K_ZFWD_DENSITY = 0,
-  K_YPASS1,  // y pass over one spectrum
-  K_YPASS5,  // y pass over five spectra
+  K_YPASS1,      // forward y pass of the density spectrum
+  K_YPASS5,      // inverse y pass of the five weighted-density spectra
   K_XCONV_FWD,
   K_ZPHI_E,
   K_ZPHI_EG,
@@ -31,6 +32,8 @@ enum KernelId : int {
   K_AXPY,
   K_REDUCE,
   K_FFT_SEAM,
+  K_YPASS5_FWD,  // forward y pass of the five partial-derivative spectra
+  K_YPASS1_INV,  // inverse y pass of the back-convolution
   NUM_FAMILIES
 };
 constexpr int NUM_KERNELS = 2 * NUM_FAMILIES;  // entry 2*family + (FP64 ? 1 : 0)
@@ -42,10 +45,11 @@ constexpr int kid(int family) {
 
 inline const char *kernel_name(int id) {
   static const char *names[NUM_KERNELS] = {
-      "zfwd_density_f32", "zfwd_density_f64", "ypass1_f32", "ypass1_f64", "ypass5_f32",   "ypass5_f64",
+      "zfwd_density_f32", "zfwd_density_f64", "ypass1_fwd_f32", "ypass1_fwd_f64", "ypass5_inv_f32", "ypass5_inv_f64",
       "xconv_fwd_f32",    "xconv_fwd_f64",    "zphi_energy_f32", "zphi_energy_f64", "zphi_grad_f32", "zphi_grad_f64",
       "xconv_bwd_f32",    "xconv_bwd_f64",    "zgrad_f32", "zgrad_f64", "cg_update", "cg_update_",
-      "cg_reset", "cg_reset_", "axpy", "axpy_", "reduce", "reduce_", "fft_seam_f32", "fft_seam_f64"};
+      "cg_reset", "cg_reset_", "axpy", "axpy_", "reduce", "reduce_", "fft_seam_f32", "fft_seam_f64",
+      "ypass5_fwd_f32", "ypass5_fwd_f64", "ypass1_inv_f32", "ypass1_inv_f64"};
   return (id >= 0 && id < NUM_KERNELS) ? names[id] : "?";
 }
 

@@ -12,6 +12,13 @@
 //                                                                            (src/equation-of-state.cpp:164-192)
 //   Minimize(&f).improve_energy(v)                Minimize(&f).improve_energy(v)
 //                                                                            (src/new/Minimize.h:13-199)
+//   Precision(p, PreconditionedConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer))
+//                                                 the same call               (src/Minimizer.h:16-81,
+//                                                                             src/ConjugateGradient.cpp:59-171,
+//                                                                             src/Precision.cpp:23-156)
+//   radial_plot, radial_plot2, z_plot             the same calls             (papers/contact/figs/
+//                                                                             inner-sphere.cpp:78-172,
+//                                                                             walls.cpp:55-80)
 // The functional is OfEffectivePotential(HardSpheresWBnotensor(R) + IdealGas +
 // ChemicalPotential(mu)) with an external potential (src/HardSpheres.cpp:137-159,
 // src/EffectivePotentialToDensity.cpp:28-90); x is Veff, stored on the GPU.
@@ -21,8 +28,10 @@
 #pragma once
 
 #include <cmath>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "deft_amd.h"
@@ -62,8 +71,10 @@ struct Lattice {
 };
 
 // GridDescription (src/GridDescription.cpp:6-50).  The spacing constructor uses
-// the reference's rule N = 1 + int(e/100 + |a|/spacing), rounded up to the next
-// edge the FFT kernels support (the reference's FFTW takes any N).
+// the reference's rule N = 1 + int(e/100 + |a|/spacing) (:26-30) and throws
+// (DEFT_ERR_UNSUPPORTED) when that N is not an edge the FFT kernels support,
+// since a different N would change the spacing and dV; at_least() rounds up to
+// the next supported edge for callers that accept that.
 struct GridDescription {
   Lattice Lat;
   int Nx, Ny, Nz;
@@ -77,11 +88,27 @@ struct GridDescription {
   GridDescription(const Lattice &lat, double spacing)
       : GridDescription(lat, edge(lat.length(0), spacing), edge(lat.length(1), spacing), edge(lat.length(2), spacing)) {}
 
+  // the reference's edge for this lattice vector length and spacing
+  static int reference_edge(double a, double spacing) { return 1 + int(std::exp(1.0) / 100 + a / spacing); }
   static int edge(double a, double spacing) {
-    int n = 1 + int(std::exp(1.0) / 100 + a / spacing);
-    while (!deft_supported_length(n) && n < 4096) ++n;
+    const int n = reference_edge(a, spacing);
+    if (!deft_supported_length(n))
+      throw Error(DEFT_ERR_UNSUPPORTED, "deft_amd: GridDescription(lat, " + std::to_string(spacing) + ") gives N = " +
+                                            std::to_string(n) +
+                                            ", which the FFT kernels do not support (GridDescription::at_least "
+                                            "rounds up to the next supported edge)");
     return n;
   }
+  // The spacing constructor's grid with each N rounded up to the next supported edge.
+  static GridDescription at_least(const Lattice &lat, double spacing) {
+    int n[3];
+    for (int i = 0; i < 3; ++i) {
+      n[i] = reference_edge(lat.length(i), spacing);
+      while (!deft_supported_length(n[i]) && n[i] < 4096) ++n[i];
+    }
+    return GridDescription(lat, n[0], n[1], n[2]);
+  }
+  long long index(int x, int y, int z) const { return ((long long)x * Ny + y) * Nz + z; }
   // Cartesian position of grid point (x, y, z) (src/Grid.cpp:43-52 before Wigner-Seitz wrapping)
   void position(int x, int y, int z, double r[3]) const {
     for (int i = 0; i < 3; ++i)
@@ -146,19 +173,41 @@ class WhiteBearFluid {
     check(deft_functional_energy(plan_, x.data(), &e, &terms_));
     return e;
   }
-  // Functional::integralgrad(kT, gd, x, g, pg): g += dF/dVeff, pg += preconditioned
+  // Functional::integralgrad(kT, gd, x, g, pg): g += dF/dVeff, pg += (dF/dn)/(Veff - kT),
+  // the old API's OfEffectivePotential pgrad (src/EffectivePotentialToDensity.cpp:62-71)
   void integralgrad(double, const GridDescription &, const std::vector<double> &x, std::vector<double> *g,
                     std::vector<double> *pg = nullptr) {
+    grad_of_kind(x, g, pg, DEFT_PGRAD_EFFECTIVE_POTENTIAL);
+  }
+  // the new API's energy_grad_and_precond: g += dF/dVeff, pg += grad/n; returns the energy
+  double energy_grad_and_precond(const std::vector<double> &x, std::vector<double> *g, std::vector<double> *pg) {
+    return grad_of_kind(x, g, pg, DEFT_PGRAD_NEW);
+  }
+  // the seven contact-paper fields of the resident density: n, n0, gSigmaS, YuWu_S,
+  // nA, gSigmaA, Gross (the columns radial_plot writes, inner-sphere.cpp:220-233)
+  std::vector<std::vector<double>> contact_fields() const {
+    const size_t N = (size_t)gd_.NxNyNz;
+    std::vector<double> all(7 * N);
+    check(deft_contact_values(plan_, all.data()));
+    std::vector<std::vector<double>> out;
+    out.push_back(density());
+    for (int k = 0; k < 6; ++k) out.emplace_back(all.begin() + k * N, all.begin() + (k + 1) * N);
+    return out;
+  }
+  // per-term energies of the last evaluation (print_summary, src/Functional.cpp:281-316)
+  const deft_energy_terms &last_terms() const { return terms_; }
+  double kT() const { return kT_; }
+
+ private:
+  double grad_of_kind(const std::vector<double> &x, std::vector<double> *g, std::vector<double> *pg, int kind) {
     size_check(x);
     if (!g) throw Error(DEFT_ERR_ARG, "deft_amd: null gradient");
     size_check(*g);
     if (pg) size_check(*pg);
-    check(deft_functional_grad(plan_, x.data(), g->data(), pg ? pg->data() : nullptr, nullptr));
+    double e;
+    check(deft_functional_grad_kind(plan_, x.data(), g->data(), pg ? pg->data() : nullptr, &e, kind));
+    return e;
   }
-  // per-term energies of the last evaluation (print_summary, src/Functional.cpp:281-316)
-  const deft_energy_terms &last_terms() const { return terms_; }
-
- private:
   void size_check(const std::vector<double> &v) const {
     if ((long long)v.size() != gd_.NxNyNz) throw Error(DEFT_ERR_ARG, "deft_amd: field size does not match the grid");
   }
@@ -205,5 +254,250 @@ class Minimize {
  private:
   deft_minimize m_ = nullptr;
 };
+
+// ---------------------------------------------------------------------------
+// old-API minimizers (MinimizerInterface, src/Minimizer.h:16-81)
+// ---------------------------------------------------------------------------
+// The line minimizer argument of the reference's factories; quadratic is the
+// one the hot path uses (src/LineMinimizer.h:8-21).
+enum LineMinimizerKind { QuadraticLineMinimizer };
+
+// A MinimizerInterface acting on the fluid's resident Veff.  As in the
+// reference the minimizer does not own x: the constructor uploads *x, and the
+// minimizer writes the current Veff back into *x whenever improve_energy
+// returns false, whenever energy() is asked for, and when it is destroyed (the
+// reference's drivers read x after their loop; a copy per iteration would cost
+// a device-to-host transfer each time).
+class Minimizer {
+ public:
+  Minimizer(WhiteBearFluid &f, double kT, std::vector<double> *x, bool preconditioned, double stepsize)
+      : f_(&f), x_(x) {
+    if (std::fabs(kT - f.kT()) > 1e-15 * std::fabs(kT))
+      throw Error(DEFT_ERR_ARG, "deft_amd: the minimizer's kT differs from the fluid's");
+    if (x) f.set_Veff(*x);
+    check(deft_cg_create(f.plan(), preconditioned ? 1 : 0, stepsize, &m_));
+  }
+  Minimizer(Minimizer &&o) noexcept : m_(o.m_), f_(o.f_), x_(o.x_) { o.m_ = nullptr; }
+  Minimizer &operator=(Minimizer &&o) noexcept {
+    std::swap(m_, o.m_);
+    std::swap(f_, o.f_);
+    std::swap(x_, o.x_);
+    return *this;
+  }
+  Minimizer(const Minimizer &) = delete;
+  Minimizer &operator=(const Minimizer &) = delete;
+  ~Minimizer() {
+    if (!m_) return;
+    try {
+      sync();
+    } catch (...) {
+    }
+    deft_cg_destroy(m_);
+  }
+
+  // MinimizerInterface::improve_energy: false once the outermost wrapper stops
+  bool improve_energy(bool verbose = false) {
+    int more = 0;
+    check(deft_cg_improve_energy(m_, verbose ? 1 : 0, &more));
+    if (!more) sync();
+    return more != 0;
+  }
+  double energy() {
+    double e;
+    check(deft_cg_energy(m_, &e));
+    sync();
+    return e;
+  }
+  std::vector<double> grad() const { return vec(&deft_cg_grad); }
+  std::vector<double> pgrad() const { return vec(&deft_cg_pgrad); }
+  // minimize(f, gd, newx): start over (iteration count, step, direction), from *newx if given
+  void minimize(std::vector<double> *newx = nullptr) {
+    if (newx) {
+      x_ = newx;
+      f_->set_Veff(*newx);
+    }
+    check(deft_cg_minimize(m_));
+    check(deft_cg_invalidate(m_));
+  }
+  void print_info(const char *prefix = "") const {
+    const deft_minimize_stats s = stats();
+    std::printf("%s==== Iteration %d ====\n%sEnergy = %.16g\n%sstep = %g\n", prefix, s.iter, prefix, s.energy, prefix,
+                s.step);
+  }
+  deft_minimize_stats stats() const {
+    deft_minimize_stats s;
+    check(deft_cg_stats_get(m_, &s));
+    return s;
+  }
+  // the current Veff into *x
+  void sync() {
+    if (x_) *x_ = f_->Veff();
+  }
+  void set_precision(double p) { check(deft_cg_set_precision(m_, p)); }
+  void set_maxiter(int n) { check(deft_cg_set_maxiter(m_, n)); }
+
+ private:
+  std::vector<double> vec(int (*fn)(deft_cg, double *)) const {
+    std::vector<double> v((size_t)f_->description().NxNyNz);
+    check(fn(m_, v.data()));
+    return v;
+  }
+  deft_cg m_ = nullptr;
+  WhiteBearFluid *f_ = nullptr;
+  std::vector<double> *x_ = nullptr;
+};
+
+// ConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer, stepsize) (src/ConjugateGradient.cpp:164-166)
+inline Minimizer ConjugateGradient(WhiteBearFluid &f, const GridDescription &, double kT, std::vector<double> *x,
+                                   LineMinimizerKind = QuadraticLineMinimizer, double stepsize = 10.0) {
+  return Minimizer(f, kT, x, false, stepsize);
+}
+// PreconditionedConjugateGradient(f, gd, kT, &x, QuadraticLineMinimizer, stepsize) (:168-171)
+inline Minimizer PreconditionedConjugateGradient(WhiteBearFluid &f, const GridDescription &, double kT,
+                                                 std::vector<double> *x, LineMinimizerKind = QuadraticLineMinimizer,
+                                                 double stepsize = 10.0) {
+  return Minimizer(f, kT, x, true, stepsize);
+}
+// Precision(precision, m) (src/Precision.cpp:154-156) and MaxIter(maxiter, m) (src/Minimizer.cpp)
+inline Minimizer Precision(double precision, Minimizer m) {
+  m.set_precision(precision);
+  return m;
+}
+inline Minimizer MaxIter(int maxiter, Minimizer m) {
+  m.set_maxiter(maxiter);
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// driver output formats (papers/contact/figs/inner-sphere.cpp, walls.cpp)
+// ---------------------------------------------------------------------------
+namespace detail {
+inline double cartesian(const GridDescription &gd, int x, int y, int z, int axis) {
+  return x * gd.Lat.a[axis] / gd.Nx + y * gd.Lat.a[3 + axis] / gd.Ny + z * gd.Lat.a[6 + axis] / gd.Nz;
+}
+inline double at(const GridDescription &gd, const std::vector<double> &f, int x, int y, int z) {
+  x %= gd.Nx;
+  y %= gd.Ny;
+  z %= gd.Nz;
+  if (x < 0) x += gd.Nx;
+  if (y < 0) y += gd.Ny;
+  if (z < 0) z += gd.Nz;
+  return f[(size_t)gd.index(x, y, z)];
+}
+// closest_value (inner-sphere.cpp:78-101) for an orthorhombic cell: the value at the
+// grid point nearest to t = delta among those with (p - t).delta > 0; 0 when there is
+// none (delta = 0).  Such a point lies within 2.9 spacings of t, so the window of
+// grid points from floor(t/h) - 2 to floor(t/h) + 3 holds it; where that window
+// reaches past the reference's scan range [-N/2, N/2) the whole range is scanned
+// instead.  The order is the reference's (x, y, z ascending), so ties resolve alike.
+inline double closest_value(const GridDescription &gd, const std::vector<double> &f, const double delta[3]) {
+  const double h[3] = {gd.Lat.a[0] / gd.Nx, gd.Lat.a[4] / gd.Ny, gd.Lat.a[8] / gd.Nz};
+  const int N[3] = {gd.Nx, gd.Ny, gd.Nz};
+  int lo[3], hi[3];
+  bool clamped = false;
+  for (int i = 0; i < 3; ++i) {
+    const int c = (int)std::floor(delta[i] / h[i]);
+    lo[i] = c - 2;
+    hi[i] = c + 3;
+    if (lo[i] < -N[i] / 2 || hi[i] > N[i] / 2 - 1) clamped = true;
+  }
+  if (clamped)
+    for (int i = 0; i < 3; ++i) {
+      lo[i] = -N[i] / 2;
+      hi[i] = N[i] / 2 - 1;
+    }
+  double closest = 1e300, best = 0.0;
+  for (int x = lo[0]; x <= hi[0]; ++x)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int z = lo[2]; z <= hi[2]; ++z) {
+        const double p[3] = {x * h[0], y * h[1], z * h[2]};
+        double dot = 0.0, d2 = 0.0;
+        for (int i = 0; i < 3; ++i) {
+          const double r = p[i] - delta[i];
+          dot += r * delta[i];
+          d2 += r * r;
+        }
+        const double dist = std::sqrt(d2);
+        if (dot > 0 && dist < closest) {
+          best = at(gd, f, x, y, z);
+          closest = dist;
+        }
+      }
+  return best;
+}
+// radial_average (inner-sphere.cpp:104-111): mean of closest_value over the
+// twenty icosahedron directions at this radius
+inline double radial_average(const GridDescription &gd, const std::vector<double> &f, double radius) {
+  const double g = (1 + std::sqrt(5.0)) / 2;
+  static const double ico[20][3] = {{1, 1, 1},      {-1, 1, 1},     {1, -1, 1},     {1, 1, -1},     {1, -1, -1},
+                                    {-1, 1, -1},    {-1, -1, 1},    {-1, -1, -1},   {0, 1 / g, g},  {0, 1 / g, -g},
+                                    {0, -1 / g, g}, {0, -1 / g, -g}, {g, 0, 1 / g}, {-g, 0, 1 / g}, {g, 0, -1 / g},
+                                    {-g, 0, -1 / g}, {1 / g, g, 0}, {1 / g, -g, 0}, {-1 / g, g, 0}, {-1 / g, -g, 0}};
+  double total = 0.0;
+  for (int i = 0; i < 20; ++i) {
+    const double n = std::sqrt(ico[i][0] * ico[i][0] + ico[i][1] * ico[i][1] + ico[i][2] * ico[i][2]);
+    const double d[3] = {radius * ico[i][0] / n, radius * ico[i][1] / n, radius * ico[i][2] / n};
+    total += closest_value(gd, f, d);
+  }
+  return total / 20;
+}
+}  // namespace detail
+
+// radial_plot (inner-sphere.cpp:149-172): y of grid point (0, y, 0) and the seven
+// fields there, for y < Ny/2, "%g" tab-separated.  Returns false if the file
+// cannot be created (the reference prints a message and returns).
+inline bool radial_plot(const char *fname, const GridDescription &gd, const std::vector<std::vector<double>> &f) {
+  if (f.size() != 7) throw Error(DEFT_ERR_ARG, "deft_amd: radial_plot takes seven fields");
+  FILE *out = std::fopen(fname, "w");
+  if (!out) {
+    std::fprintf(stderr, "Unable to create file %s!\n", fname);
+    return false;
+  }
+  for (int y = 0; y < gd.Ny / 2; y++) {
+    std::fprintf(out, "%g", detail::cartesian(gd, 0, y, 0, 1));
+    for (const auto &v : f) std::fprintf(out, "\t%g", v[(size_t)gd.index(0, y, 0)]);
+    std::fprintf(out, "\n");
+  }
+  std::fclose(out);
+  return true;
+}
+
+// radial_plot2 (inner-sphere.cpp:114-147): the seven fields averaged over twenty
+// directions at radii r = 0, dr, ... below Nx dr / 2, dr = diameter /
+// floor(diameter / |a2/Ny|) -- the driver's "-mean.dat" file.  Orthorhombic cells.
+inline bool radial_plot2(const char *fname, const GridDescription &gd, double diameter,
+                         const std::vector<std::vector<double>> &f) {
+  if (f.size() != 7) throw Error(DEFT_ERR_ARG, "deft_amd: radial_plot2 takes seven fields");
+  FILE *out = std::fopen(fname, "w");
+  if (!out) {
+    std::fprintf(stderr, "Unable to create file %s!\n", fname);
+    return false;
+  }
+  const double dr = diameter / std::floor(diameter / (gd.Lat.length(1) / gd.Ny));
+  for (double r = 0; r < gd.Nx * dr / 2; r += dr) {
+    std::fprintf(out, "%g", r);
+    for (const auto &v : f) std::fprintf(out, "\t%g", detail::radial_average(gd, v, r));
+    std::fprintf(out, "\n");
+  }
+  std::fclose(out);
+  return true;
+}
+
+// z_plot (walls.cpp:55-80): z of grid point (0, 0, z) and eight fields there, z < Nz/2.
+inline bool z_plot(const char *fname, const GridDescription &gd, const std::vector<std::vector<double>> &f) {
+  if (f.size() != 8) throw Error(DEFT_ERR_ARG, "deft_amd: z_plot takes eight fields");
+  FILE *out = std::fopen(fname, "w");
+  if (!out) {
+    std::fprintf(stderr, "Unable to create file %s!\n", fname);
+    return false;
+  }
+  for (int z = 0; z < gd.Nz / 2; z++) {
+    std::fprintf(out, "%g", detail::cartesian(gd, 0, 0, z, 2));
+    for (const auto &v : f) std::fprintf(out, "\t%g", v[(size_t)gd.index(0, 0, z)]);
+    std::fprintf(out, "\n");
+  }
+  std::fclose(out);
+  return true;
+}
 
 }  // namespace deft_amd

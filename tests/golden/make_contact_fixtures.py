@@ -2,7 +2,8 @@
 
     python tests/golden/make_contact_fixtures.py
 
-Source: papers/contact/figs/inner-sphere{WB,WBm2}-04.0-{eta}.dat of the reference, the
+Source: papers/contact/figs/inner-sphere{WB,WBm2}-04.0-{eta}.dat of the reference (and
+their -mean.dat companions where published), the
 output of papers/contact/figs/inner-sphere.cpp:177-249 (radial_plot, :143-172): a hard
 solute of excluded-centre radius 2 (diameter argument 4, notinwall at :30-38) in a
 periodic 20^3 cell (padding 16), grid 201^3 (GridDescription(lat, 0.1),
@@ -41,6 +42,20 @@ def main():
             for c, key in enumerate(("n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross")):
                 prof[key] = [r[2 + c] for r in rows]
             out["profiles"]["%s-%s" % (name, eta)] = prof
+            # radial_plot2's companion (:114-147, written at :250-252): the same seven
+            # fields averaged over twenty directions at radii 0, 0.1, ... (first column r)
+            mean = os.path.join(REF, "inner-sphere%s-04.0-%s-mean.dat" % (name, eta))
+            if os.path.exists(mean):
+                rows = []
+                with open(mean) as fh:
+                    for line in fh:
+                        cols = line.split()
+                        if len(cols) >= 8:
+                            rows.append([float(c) for c in cols[:8]])
+                m = {"r": [r[0] for r in rows], "density": [r[1] for r in rows]}
+                for c, key in enumerate(("n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross")):
+                    m[key] = [r[2 + c] for r in rows]
+                out.setdefault("means", {})["%s-%s" % (name, eta)] = m
     with open(os.path.join(HERE, "contact_inner_sphere.json"), "w") as fh:
         json.dump(out, fh, indent=0)
 

@@ -43,3 +43,121 @@ def test_solute_driver_matches_oracle(gpu):
         assert abs(r["energy"] / E - 1) < 1e-9
         assert abs(r["N"] / (n.sum() * g.dV) - 1) < 1e-6
         assert np.max(np.abs(np.array(r["density_x"]) - n[:, 0, 0])) < 1e-5 * n.max()
+
+
+def test_spacing_constructor_refuses_unsupported_edges(tmp_path):
+    """GridDescription(lat, spacing) keeps the reference's N = 1 + int(e/100 + |a|/spacing)
+    (src/GridDescription.cpp:26-30) or throws; at_least() is the explicit rounding."""
+    src = tmp_path / "g.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "deft_amd.hpp"
+using namespace deft_amd;
+int main() {
+  const GridDescription a(Lattice::orthorhombic(20, 20, 20), 0.1);   // 201, supported
+  int thrown = 0;
+  try { GridDescription b(Lattice::orthorhombic(6.4, 6.4, 6.4), 0.1); } catch (const Error &e) { thrown = e.status; }
+  const GridDescription c = GridDescription::at_least(Lattice::orthorhombic(6.4, 6.4, 6.4), 0.1);
+  std::printf("%d %d %d %d\n", a.Nx, thrown, c.Nx, GridDescription::reference_edge(6.4, 0.1));
+  return 0;
+}
+''')
+    exe = tmp_path / "g"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), str(src),
+                    "-L" + os.path.join(ROOT, "deft_amd"), "-ldeft_amd", "-Wl,-rpath," + os.path.join(ROOT, "deft_amd"),
+                    "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["201", "2", "96", "65"]
+
+
+def _closest_value_reference(f, L, delta):
+    """closest_value of papers/contact/figs/inner-sphere.cpp:78-101, scanning the whole cell."""
+    N = f.shape
+    h = [L[i] / N[i] for i in range(3)]
+    closest, best = 1e300, 0.0
+    for x in range(-(N[0] // 2), N[0] // 2):
+        for y in range(-(N[1] // 2), N[1] // 2):
+            for z in range(-(N[2] // 2), N[2] // 2):
+                r = np.array([x * h[0], y * h[1], z * h[2]]) - delta
+                if r @ delta > 0 and np.sqrt(r @ r) < closest:
+                    best = f[x % N[0], y % N[1], z % N[2]]
+                    closest = np.sqrt(r @ r)
+    return best
+
+
+def test_radial_plot2_matches_the_reference_scan(tmp_path):
+    """The windowed closest-grid-point search of radial_plot2 against the reference's scan
+    over the whole cell (radial_average, inner-sphere.cpp:104-111), on random fields."""
+    exe = tmp_path / "r"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "host", "radial_harness.cpp"), "-o", str(exe)], check=True)
+    shape, L, diameter = (15, 17, 13), (1.5, 1.7, 1.3), 0.6
+    rng = np.random.default_rng(0)
+    fields = [rng.standard_normal(shape) for _ in range(7)]
+    inp = "%d %d %d %r %r %r %r\n" % (shape + L + (diameter,)) + "\n".join(
+        " ".join(format(float(v), ".17g") for v in a.ravel()) for a in fields)
+    out = tmp_path / "mean.dat"
+    subprocess.run([str(exe), str(out)], input=inp, text=True, check=True)
+    rows = [list(map(float, l.split())) for l in open(out)]
+    dr = diameter / math.floor(diameter / (L[1] / shape[1]))
+    gold = (1 + math.sqrt(5)) / 2
+    ico = np.array([(1, 1, 1), (-1, 1, 1), (1, -1, 1), (1, 1, -1), (1, -1, -1), (-1, 1, -1), (-1, -1, 1),
+                    (-1, -1, -1), (0, 1 / gold, gold), (0, 1 / gold, -gold), (0, -1 / gold, gold),
+                    (0, -1 / gold, -gold), (gold, 0, 1 / gold), (-gold, 0, 1 / gold), (gold, 0, -1 / gold),
+                    (-gold, 0, -1 / gold), (1 / gold, gold, 0), (1 / gold, -gold, 0), (-1 / gold, gold, 0),
+                    (-1 / gold, -gold, 0)])
+    ico = ico / np.linalg.norm(ico, axis=1)[:, None]
+    radii = []
+    r = 0.0
+    while r < shape[0] * dr / 2:
+        radii.append(r)
+        r += dr
+    assert len(rows) == len(radii)
+    for row, r in zip(rows, radii):
+        assert row[0] == pytest.approx(r, abs=1e-6)
+        for k, f in enumerate(fields):
+            want = np.mean([_closest_value_reference(f, L, r * u) for u in ico])
+            assert row[1 + k] == pytest.approx(want, rel=2e-6, abs=1e-6), (r, k)
+
+
+FIXTURES = json.load(open(os.path.join(ROOT, "tests", "golden", "contact_inner_sphere.json")))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("eta", ["0.40", "0.10"])
+def test_inner_sphere_driver_writes_the_published_files(gpu, tmp_path, eta):
+    """deft_amd/examples/inner_sphere.cpp (the reference driver's flow through the C++ layer
+    and the old-API minimizer) writes inner-sphereWB-04.0-<eta>.dat and its -mean.dat;
+    both match the reference's published files column by column (density and the
+    estimators the published WB files pin: n0, YuWu_S, nA, Gross)."""
+    print("inner_sphere driver", eta, flush=True)
+    out = subprocess.run([os.path.join(ROOT, "deft_amd", "examples", "inner_sphere"), "4", eta, str(tmp_path)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout)
+    print(r, flush=True)
+    assert r["grid"] == [201, 201, 201] and r["calls"] <= 200
+    cols = ("density", "n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross")
+    pinned = ("density", "n0", "YuWu_S", "nA", "Gross")
+    dat = np.loadtxt(tmp_path / ("inner-sphereWB-04.0-%s.dat" % eta))
+    fx = FIXTURES["profiles"]["WB-" + eta]
+    assert np.allclose(dat[:, 0], fx["y"], rtol=1e-5, atol=1e-6)
+    for k, c in enumerate(cols):
+        if c not in pinned:
+            continue
+        ref = np.array(fx[c])
+        rel = np.abs(dat[:, 1 + k] - ref) / np.abs(ref)
+        print(eta, ".dat", c, "max rel", rel.max(), flush=True)
+        assert rel.max() < 4e-4, c
+    mean = np.loadtxt(tmp_path / ("inner-sphereWB-04.0-%s-mean.dat" % eta))
+    mx = FIXTURES["means"]["WB-" + eta]
+    assert mean.shape[0] == len(mx["r"])
+    assert np.allclose(mean[:, 0], mx["r"], rtol=1e-5, atol=1e-9)
+    for k, c in enumerate(cols):
+        if c not in pinned:
+            continue
+        ref = np.array(mx[c])
+        err = np.abs(mean[:, 1 + k] - ref)
+        print(eta, "-mean.dat", c, "max abs", err.max(), "max rel", (err / np.maximum(np.abs(ref), 1e-300)).max(),
+              flush=True)
+        assert np.all(err <= 4e-4 * np.abs(ref) + 1e-9), c
