@@ -79,12 +79,15 @@ _SIGNATURES = {
     "deft_last_error": ([], ctypes.c_char_p),
     "deft_version": ([], _I),
     "deft_supported_length": ([_I], _I),
+    "deft_fused_length": ([_I], _I),
     "deft_device_count": ([_IP], _I),
     "deft_plan_create": ([_I, _I, _I, _DP, _D, _D, _D, _I, ctypes.POINTER(_P)], _I),
     "deft_plan_destroy": ([_P], _I),
     "deft_plan_autotune": ([_P], _I),
     "deft_plan_tuning": ([_P, _I, _IP, _IP, _IP], _I),
     "deft_plan_set_tuning": ([_P, _I, _I, _I, _I], _I),
+    "deft_plan_radix_schedule": ([_P, _I, _I, _IP, _I, _IP], _I),
+    "deft_plan_set_radix_schedule": ([_P, _I, _I, _IP, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
     "deft_plan_set_mu": ([_P, _D], _I),
     "deft_plan_set_functional": ([_P, _I], _I),
@@ -210,6 +213,12 @@ def supported_length(n):
     return bool(lib().deft_supported_length(int(n)))
 
 
+def fused_length(n):
+    """True when the fused per-edge kernels serve this edge (other supported
+    edges run the generic line passes)."""
+    return bool(lib().deft_fused_length(int(n)))
+
+
 # ---------------------------------------------------------------------------
 # bulk path
 # ---------------------------------------------------------------------------
@@ -259,9 +268,12 @@ def bulk_free_energy_density(n, R=1.0, kT=1.0):
 class WhiteBearFluidVeff:
     """White-Bear fluid of Veff on one grid, resident on the GPU.
 
-    ``shape`` = (Nx, Ny, Nz) (even, supported FFT lengths); ``cell`` = 3x3 lattice
-    vectors as rows (or three edge lengths of an orthorhombic cell).  ``Vext`` is
-    FP64 on the host, stored FP32 on the device; +inf marks hard walls.
+    ``shape`` = (Nx, Ny, Nz), each edge 1..4096 with no prime factor above 512
+    (``supported_length``; edges with ``fused_length`` run the fused passes);
+    ``cell`` = 3x3 lattice vectors as rows (or three edge lengths of an
+    orthorhombic cell).  ``Vext`` is FP64 on the host, kept in FP64 for the FP64
+    passes and rounded to FP32 for the FP32 ones; +inf marks hard walls, NaN
+    constrained points.
     """
 
     def __init__(self, shape, cell, R=1.0, kT=1.0, mu=0.0, fft_precision=FFT_FP32, functional=WHITE_BEAR):
@@ -422,6 +434,17 @@ class WhiteBearFluidVeff:
 
     def set_tuning(self, fft_precision, zblock, ytile, xtile):
         _check(lib().deft_plan_set_tuning(self._h, int(fft_precision), int(zblock), int(ytile), int(xtile)))
+
+    def radix_schedule(self, fft_precision, axis):
+        """The radix order (Stockham stages) of one axis's transforms."""
+        buf = (ctypes.c_int * 16)()
+        n = ctypes.c_int()
+        _check(lib().deft_plan_radix_schedule(self._h, int(fft_precision), int(axis), buf, 16, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
+
+    def set_radix_schedule(self, fft_precision, axis, radices):
+        arr = (ctypes.c_int * max(1, len(radices)))(*[int(r) for r in radices])
+        _check(lib().deft_plan_set_radix_schedule(self._h, int(fft_precision), int(axis), arr, len(radices)))
 
     def set_profiling(self, on=True):
         _check(lib().deft_plan_set_profiling(self._h, 1 if on else 0))

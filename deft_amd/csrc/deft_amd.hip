@@ -41,8 +41,24 @@ namespace {
 
 constexpr int kVersion = 2;
 
-bool length_ok(int n) {
+// edges with fused per-edge kernels; any other supported edge runs the generic passes
+bool fused_ok(int n) {
   return with_length(n, [](auto) {});
+}
+
+// room for the generic passes' real work fields (plan_fields(p) of N doubles)
+int ensure_gwork(deft_plan_s *p) {
+  const int nf = plan_fields(p);
+  if (!p->generic || p->gwork_fields >= nf) return DEFT_OK;
+  double *w = nullptr;
+  if (p->stream && hipStreamSynchronize(p->stream) != hipSuccess)
+    return set_error(DEFT_ERR_HIP, "stream synchronisation failed");
+  if (hipMalloc(&w, sizeof(double) * (size_t)nf * (size_t)p->N) != hipSuccess)
+    return set_error(DEFT_ERR_ALLOC, "hipMalloc of the generic work fields failed");
+  (void)hipFree(p->gwork);
+  p->gwork = w;
+  p->gwork_fields = nf;
+  return DEFT_OK;
 }
 
 int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, int precond);
@@ -128,6 +144,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->twbuf);
   (void)hipFree(p->tmp);
   (void)hipFree(p->wtab);
+  (void)hipFree(p->gwork);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -413,7 +430,9 @@ const char *deft_last_error(void) { return g_last_error.c_str(); }
 
 int deft_version(void) { return kVersion; }
 
-int deft_supported_length(int n) { return (n > 0 && length_ok(n)) ? 1 : 0; }
+int deft_supported_length(int n) { return (n > 0 && (fused_ok(n) || generic_length_ok(n))) ? 1 : 0; }
+
+int deft_fused_length(int n) { return (n > 0 && fused_ok(n)) ? 1 : 0; }
 
 int deft_device_count(int *count) {
   if (!count) return set_error(DEFT_ERR_ARG, "null count");
@@ -428,7 +447,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   if (!out || !cell) return set_error(DEFT_ERR_ARG, "null argument");
   *out = nullptr;
   if (!deft_supported_length(nx) || !deft_supported_length(ny) || !deft_supported_length(nz))
-    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must be in {16,20,21,32,48,64,96,128,201,256}");
+    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must be 1..4096 with no prime factor above 512");
   double det;
   if (!det3(cell, det)) return set_error(DEFT_ERR_ARG, "cell matrix is singular or not finite");
   if (!(R > 0) || !(kT > 0) || !std::isfinite(R) || !std::isfinite(kT) || !std::isfinite(mu))
@@ -455,6 +474,11 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
   p->kT = kT;
   p->mu = mu;
   p->prec = fft_precision;
+  p->generic = !(fused_ok(nx) && fused_ok(ny) && fused_ok(nz));
+  {
+    const int lens[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) p->sched[0][a] = p->sched[1][a] = make_sched(lens[a], radix_candidates(lens[a])[0]);
+  }
   p->eblocks = (int)std::min<long long>(1024, (p->N + kThreads - 1) / kThreads);
   p->pstride = std::max(std::max(zblocks_for(p, kZBlocks[0]), zblocks_for(p, kZBlocks[1])), p->eblocks);
 
@@ -475,6 +499,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->dcount, sizeof(unsigned)) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
+  if (ensure_gwork(p) != DEFT_OK) return fail(DEFT_ERR_ALLOC, "device allocation failed");
   if (hipMemsetAsync(p->dcount, 0, sizeof(unsigned), p->stream) != hipSuccess)
     return fail(DEFT_ERR_HIP, "hipMemsetAsync failed");
   if (hipHostMalloc((void **)&p->hflag, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent) !=
@@ -544,6 +569,8 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
     if (plans[i]->batch) return set_error(DEFT_ERR_STATE, "a plan is already in a batch");
     if (plans[i]->functional != DEFT_WHITE_BEAR)
       return set_error(DEFT_ERR_UNSUPPORTED, "batches take White-Bear plans only");
+    if (plans[i]->generic)
+      return set_error(DEFT_ERR_UNSUPPORTED, "batches take plans whose edges have fused kernels (deft_fused_length)");
     for (int j = 0; j < i; ++j)
       if (plans[j] == plans[i]) return set_error(DEFT_ERR_ARG, "a plan appears twice");
   }
@@ -670,6 +697,33 @@ int deft_plan_set_tuning(deft_plan p, int fft_precision, int zblock, int ytile, 
   return DEFT_OK;
 }
 
+int deft_plan_radix_schedule(deft_plan p, int fft_precision, int axis, int *radices, int capacity, int *nstages) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (axis < 0 || axis > 2) return set_error(DEFT_ERR_ARG, "axis must be 0, 1 or 2");
+  if (!nstages || (capacity > 0 && !radices)) return set_error(DEFT_ERR_ARG, "null output");
+  const LineSched &s = p->sched[fft_precision][axis];
+  *nstages = s.nst;
+  for (int i = 0; i < s.nst && i < capacity; ++i) radices[i] = s.rad[i];
+  return DEFT_OK;
+}
+
+int deft_plan_set_radix_schedule(deft_plan p, int fft_precision, int axis, const int *radices, int nstages) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (axis < 0 || axis > 2) return set_error(DEFT_ERR_ARG, "axis must be 0, 1 or 2");
+  if (!p->generic)
+    return set_error(DEFT_ERR_UNSUPPORTED, "the fused kernels of this plan's edges fix their radix order at compile time");
+  if (nstages > 0 && !radices) return set_error(DEFT_ERR_ARG, "null radices");
+  const int L = axis == 0 ? p->Nx : (axis == 1 ? p->Ny : p->Nz);
+  if (!sched_valid(L, radices, nstages))
+    return set_error(DEFT_ERR_ARG, "radices must be 4, 8 or primes up to 512 whose product is the edge (at most 12)");
+  p->sched[fft_precision][axis] = make_sched(L, std::vector<int>(radices, radices + nstages));
+  return DEFT_OK;
+}
+
 int deft_plan_set_fft_precision(deft_plan p, int prec) {
   TRY(check_plan(p));
   if (prec != DEFT_FFT_FP32 && prec != DEFT_FFT_FP64) return set_error(DEFT_ERR_ARG, "unknown FFT precision");
@@ -701,8 +755,11 @@ int deft_plan_set_functional(deft_plan p, int functional) {
   // eleven fields per z line: the smaller z blocks keep the tensor Phi pass at
   // two waves per SIMD
   if (functional == DEFT_WHITE_BEAR_TENSOR) p->tz[0] = p->tz[1] = kZBlocks[0];
+  const int was = p->functional;
   p->functional = functional;
-  return DEFT_OK;
+  const int rc = ensure_gwork(p);
+  if (rc != DEFT_OK) p->functional = was;
+  return rc;
 }
 
 int deft_plan_set_mu(deft_plan p, double mu) {

@@ -215,6 +215,158 @@ static void launch_density_to_weighted(deft_plan_s *p, const GridArgs &g) {
   launch_ypass<T>(p, g, spec_ptr<T>(p, 1), true, plan_fields(p), p->ty[ti]);
 }
 
+// --- edges without fused kernels (generic.hpp) --------------------------------
+
+template <typename T>
+inline LineSched line_sched(const deft_plan_s *p, int axis) {
+  LineSched s = p->sched[prec_index<T>()][axis];
+  s.SS = rt_lds_stride<T>(s.L);
+  return s;
+}
+
+template <typename T, typename Fn>
+inline void with_cap(int L, Fn &&fn) {
+  if (gen_large(L))
+    fn(ic<gen_cap<T>(true)>{});
+  else
+    fn(ic<gen_cap<T>(false)>{});
+}
+
+inline int kblocks_for(const deft_plan_s *p) {
+  return (int)std::min<long long>(4096, (p->Nk + kThreads - 1) / kThreads);
+}
+
+// line pairs per z block, columns per y/x block
+inline int gen_pairs(const LineSched &s, int cap) { return std::max(1, cap / std::max(s.SS, s.L + 2)); }
+inline int gen_cols(const LineSched &s, int cap) { return std::max(1, std::min(kThreads, cap / s.SS)); }
+
+template <typename T, bool INV>
+static void gen_col(deft_plan_s *p, const GridArgs &g, int axis, cx<T> *data, int nf) {
+  const LineSched s = line_sched<T>(p, axis);
+  if (s.L == 1) return;
+  with_cap<T>(s.L, [&](auto CAPc) {
+    constexpr int CAP = decltype(CAPc)::value;
+    const int C = gen_cols(s, CAP);
+    const long long ncols = (long long)(axis == 1 ? p->Nx : p->Ny) * p->Nkz;
+    ProfScope ps(p, kid<T>(K_GEN_LINE), 2.0 * nf * sizeof(cx<T>) * p->Nk);
+    hipLaunchKernelGGL((kg_col<T, CAP, INV>), dim3((unsigned)((ncols + C - 1) / C), nf), dim3(kThreads), 0, p->stream,
+                       g, s, C, axis, data, tw_ptr<T>(p, axis));
+  });
+}
+
+// nf real FP64 fields (fs_in apart) -> half spectra out + f*ks, times scl
+template <typename T>
+static void gen_forward(deft_plan_s *p, const GridArgs &g, const double *in, long long fs_in, int nf, cx<T> *out,
+                        double scl) {
+  const LineSched s = line_sched<T>(p, 2);
+  with_cap<T>(s.L, [&](auto CAPc) {
+    constexpr int CAP = decltype(CAPc)::value;
+    const int P = gen_pairs(s, CAP);
+    const long long lines = (long long)p->Nx * p->Ny;
+    ProfScope ps(p, kid<T>(K_GEN_LINE), nf * (8.0 * p->N + sizeof(cx<T>) * p->Nk));
+    hipLaunchKernelGGL((kg_zr2c<T, CAP>), dim3((unsigned)((lines + 2 * P - 1) / (2 * P)), nf), dim3(kThreads), 0,
+                       p->stream, g, s, P, in, fs_in, out, scl, tw_ptr<T>(p, 2));
+  });
+  gen_col<T, false>(p, g, 1, out, nf);
+  gen_col<T, false>(p, g, 0, out, nf);
+}
+
+// half spectra in + f*ks (consumed) -> nf real FP64 fields (fs_out apart), times scl
+template <typename T>
+static void gen_inverse(deft_plan_s *p, const GridArgs &g, cx<T> *in, int nf, double *out, long long fs_out,
+                        double scl) {
+  gen_col<T, true>(p, g, 0, in, nf);
+  gen_col<T, true>(p, g, 1, in, nf);
+  const LineSched s = line_sched<T>(p, 2);
+  with_cap<T>(s.L, [&](auto CAPc) {
+    constexpr int CAP = decltype(CAPc)::value;
+    const int P = gen_pairs(s, CAP);
+    const long long lines = (long long)p->Nx * p->Ny;
+    ProfScope ps(p, kid<T>(K_GEN_LINE), nf * (8.0 * p->N + sizeof(cx<T>) * p->Nk));
+    hipLaunchKernelGGL((kg_zc2r<T, CAP>), dim3((unsigned)((lines + 2 * P - 1) / (2 * P)), nf), dim3(kThreads), 0,
+                       p->stream, g, s, P, in, out, fs_out, scl, tw_ptr<T>(p, 2));
+  });
+}
+
+template <typename Fn>
+inline void with_nw(int nw, Fn &&fn) {
+  if (nw == 11)
+    fn(ic<11>{});
+  else
+    fn(ic<5>{});
+}
+
+// density -> the plan's weighted densities in gwork (nw fields), moves applied
+template <typename T>
+static void gen_density_to_weighted(deft_plan_s *p, const GridArgs &g, double *w) {
+  double m1, m2;
+  int nm;
+  take_moves(p, m1, m2, nm);
+  const int nw = plan_fields(p);
+  {
+    ProfScope ps(p, kid<T>(K_GEN_POINT), (nm > 0 ? 32.0 : 16.0) * p->N + (double)sizeof(T) * p->N);
+    hipLaunchKernelGGL(kg_density<T>, dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, p->x, p->d, m1, m2, nm,
+                       vext_ptr<T>(p), p->gwork, p->partial, p->pstride);
+  }
+  gen_forward<T>(p, g, p->gwork, p->N, 1, spec_ptr<T>(p, 0), 1.0);
+  with_nw(nw, [&](auto NWc) {
+    constexpr int NW = decltype(NWc)::value;
+    ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
+    hipLaunchKernelGGL((kg_wmul_fwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+                       spec_ptr<T>(p, 0), spec_ptr<T>(p, 1));
+  });
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 1), nw, w, p->N, 1.0);
+}
+
+template <typename T>
+static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nb) {
+  const GridArgs g = grid_args(p);
+  const int nw = plan_fields(p);
+  double *w = p->gwork;
+  gen_density_to_weighted<T>(p, g, w);
+  const int fn = p->functional == DEFT_WHITE_BEAR ? 0 : (nw == 11 ? 3 : 1);
+  {
+    ProfScope ps(p, kid<T>(K_GEN_POINT), (grad ? 16.0 : 8.0) * nw * p->N);
+    auto phi = [&](auto FNc) {
+      constexpr int FN = decltype(FNc)::value;
+      if (grad)
+        hipLaunchKernelGGL((kg_phi<T, true, FN>), dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->partial,
+                           p->pstride);
+      else
+        hipLaunchKernelGGL((kg_phi<T, false, FN>), dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->partial,
+                           p->pstride);
+    };
+    if (fn == 0)
+      phi(ic<0>{});
+    else if (fn == 1)
+      phi(ic<1>{});
+    else
+      phi(ic<3>{});
+  }
+  if (grad) {
+    gen_forward<T>(p, g, w, p->N, nw, spec_ptr<T>(p, 1), 1.0);
+    with_nw(nw, [&](auto NWc) {
+      constexpr int NW = decltype(NWc)::value;
+      ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
+      hipLaunchKernelGGL((kg_wmul_bwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 1), spec_ptr<T>(p, 0));
+    });
+    gen_inverse<T>(p, g, spec_ptr<T>(p, 0), 1, w, p->N, 1.0);
+    ProfScope ps(p, kid<T>(K_GEN_POINT), (have_old ? 48.0 : 40.0) * p->N + (double)sizeof(T) * p->N);
+    hipLaunchKernelGGL(kg_grad<T>, dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->x, vext_ptr<T>(p), p->og,
+                       have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride);
+  }
+  *nb = p->eblocks;
+  DEFT_HIP_CHECK(hipGetLastError());
+  return DEFT_OK;
+}
+
+// Times the candidate radix schedules of every axis longer than 1 (the
+// forward and inverse line passes of that axis over the plan's weighted
+// fields) and keeps the fastest.
+template <typename T>
+static int gen_autotune(deft_plan_s *p);
+
 template <typename T>
 int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
   int nzb = 0;
@@ -225,6 +377,7 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
 
 template <typename T>
 int Launch<T>::enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nzb_out) {
+  if (p->generic) return gen_enqueue_evaluation<T>(p, grad, have_old, precond, nzb_out);
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
   const int zb = p->tz[ti];
@@ -616,6 +769,11 @@ template <typename T>
 int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
+  if (p->generic) {
+    gen_forward<T>(p, g, in_dev, p->N, 1, spec_ptr<T>(p, 0), scl);
+    DEFT_HIP_CHECK(hipGetLastError());
+    return DEFT_OK;
+  }
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
     with_zb(p->tz[ti], [&](auto ZBc) {
@@ -641,6 +799,11 @@ template <typename T>
 int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
+  if (p->generic) {
+    gen_inverse<T>(p, g, spec_ptr<T>(p, 0), 1, out_dev, p->N, scl);
+    DEFT_HIP_CHECK(hipGetLastError());
+    return DEFT_OK;
+  }
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
     with_tb(p->tx[ti], [&](auto TBc) {
@@ -666,6 +829,11 @@ template <typename T>
 int Launch<T>::weighted_densities(deft_plan_s *p, double *out_dev) {
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
+  if (p->generic) {
+    gen_density_to_weighted<T>(p, g, out_dev);
+    DEFT_HIP_CHECK(hipGetLastError());
+    return DEFT_OK;
+  }
   launch_density_to_weighted<T>(p, g);
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
@@ -706,6 +874,7 @@ static float time_launches(deft_plan_s *p, Fn &&fn, int reps = 5) {
 // partial sums are overwritten, Veff and the CG vectors are not touched.
 template <typename T>
 int Launch<T>::autotune(deft_plan_s *p) {
+  if (p->generic) return gen_autotune<T>(p);
   const GridArgs g = grid_args(p);
   const int ti = prec_index<T>();
   const bool was_prof = p->prof;
@@ -736,6 +905,41 @@ int Launch<T>::autotune(deft_plan_s *p) {
       best = t;
       p->tx[ti] = tb;
     }
+  }
+  p->prof = was_prof;
+  DEFT_HIP_CHECK(hipGetLastError());
+  return plan_sync(p);
+}
+
+template <typename T>
+static int gen_autotune(deft_plan_s *p) {
+  const GridArgs g = grid_args(p);
+  const bool was_prof = p->prof;
+  p->prof = false;
+  const int nw = plan_fields(p);
+  for (int axis = 0; axis < 3; ++axis) {
+    const int L = axis == 0 ? p->Nx : (axis == 1 ? p->Ny : p->Nz);
+    if (L == 1) continue;
+    float best = 1e30f;
+    LineSched &cur = p->sched[prec_index<T>()][axis];
+    LineSched keep = cur;
+    for (const std::vector<int> &cand : radix_candidates(L)) {
+      cur = make_sched(L, cand);
+      const float t = time_launches(p, [&] {
+        if (axis == 2) {
+          gen_forward<T>(p, g, p->gwork, p->N, nw, spec_ptr<T>(p, 1), 1.0);
+          gen_inverse<T>(p, g, spec_ptr<T>(p, 1), nw, p->gwork, p->N, 1.0);
+        } else {
+          gen_col<T, false>(p, g, axis, spec_ptr<T>(p, 1), nw);
+          gen_col<T, true>(p, g, axis, spec_ptr<T>(p, 1), nw);
+        }
+      });
+      if (t < best) {
+        best = t;
+        keep = cur;
+      }
+    }
+    cur = keep;
   }
   p->prof = was_prof;
   DEFT_HIP_CHECK(hipGetLastError());

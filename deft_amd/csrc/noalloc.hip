@@ -111,8 +111,8 @@ extern "C" int deft_functional_grad_noalloc(int nx, int ny, int nz, const double
                                             const double *vext_dev, double *outgrad_dev, double *outpgrad_dev,
                                             double *energy_dev, void *stream) {
   if (!cell || !veff_dev || !outgrad_dev) return set_error(DEFT_ERR_ARG, "null argument");
-  if (!deft_supported_length(nx) || !deft_supported_length(ny) || !deft_supported_length(nz))
-    return set_error(DEFT_ERR_UNSUPPORTED, "grid edge not supported by the FFT kernels");
+  if (!deft_fused_length(nx) || !deft_fused_length(ny) || !deft_fused_length(nz))
+    return set_error(DEFT_ERR_UNSUPPORTED, "grid edges must have fused kernels (deft_fused_length)");
   const long long N = (long long)nx * ny * nz;
   if (N > kMaxPoints)
     return set_error(DEFT_ERR_ARG, "deft_functional_grad_noalloc takes grids of at most 64^3 = 262144 points");

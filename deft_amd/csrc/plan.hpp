@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "../../include/deft_amd.h"
-#include "passes.hpp"
+#include "generic.hpp"
 
 namespace deft {
 
@@ -34,6 +34,8 @@ enum KernelId : int {
   K_FFT_SEAM,
   K_YPASS5_FWD,  // forward y pass of the five partial-derivative spectra
   K_YPASS1_INV,  // inverse y pass of the back-convolution
+  K_GEN_LINE,    // line FFT passes of edges without fused kernels
+  K_GEN_POINT,   // pointwise passes of edges without fused kernels
   NUM_FAMILIES
 };
 constexpr int NUM_KERNELS = 2 * NUM_FAMILIES;  // entry 2*family + (FP64 ? 1 : 0)
@@ -49,7 +51,8 @@ inline const char *kernel_name(int id) {
       "xconv_fwd_f32",    "xconv_fwd_f64",    "zphi_energy_f32", "zphi_energy_f64", "zphi_grad_f32", "zphi_grad_f64",
       "xconv_bwd_f32",    "xconv_bwd_f64",    "zgrad_f32", "zgrad_f64", "cg_update", "cg_update_",
       "cg_reset", "cg_reset_", "axpy", "axpy_", "reduce", "reduce_", "fft_seam_f32", "fft_seam_f64",
-      "ypass5_fwd_f32", "ypass5_fwd_f64", "ypass1_inv_f32", "ypass1_inv_f64"};
+      "ypass5_fwd_f32", "ypass5_fwd_f64", "ypass1_inv_f32", "ypass1_inv_f64", "gen_line_f32", "gen_line_f64",
+      "gen_point_f32", "gen_point_f64"};
   return (id >= 0 && id < NUM_KERNELS) ? names[id] : "?";
 }
 
@@ -128,6 +131,12 @@ struct deft_plan_s {
   int tz[2] = {1024, 512};  // (FP64 at 1024 exceeds the Phi pass register budget)
   int ty[2] = {16384, 16384};
   int tx[2] = {8192, 8192};
+  // edges without fused kernels (generic.hpp): line-pass radix schedules per
+  // FFT precision and axis (SS is filled at launch) and the real work fields
+  bool generic = false;
+  deft::LineSched sched[2][3] = {};
+  double *gwork = nullptr;  // gwork_fields real FP64 fields of N values
+  int gwork_fields = 0;
 };
 
 namespace deft {

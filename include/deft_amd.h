@@ -14,10 +14,13 @@
  * Conventions (the reference's, kept):
  *   - real fields are row-major, index (x*Ny + y)*Nz + z (src/Grid.h:37-42);
  *     half spectra (x*Ny + y)*(Nz/2+1) + kz, interleaved re/im doubles;
- *   - grid edges come from the supported list (deft_supported_length): the
- *     even ones the new API asserts (src/new/Vector.h:328-330) plus the odd
- *     edges of the old-API drivers (201 = 1 + int(20/0.1 + e/100),
- *     src/GridDescription.cpp:28, papers/contact/figs/inner-sphere.cpp:183-184);
+ *   - any grid edge from 1 to 4096 whose prime factors are at most 512, as FFTW
+ *     takes any N (src/GridDescription.cpp:26-30): the edges of the old-API
+ *     drivers (65 = 1 + int(6.4/0.1 + e/100), 201, the 1 x 1 x 3601 wall lines
+ *     of papers/contact/figs/walls.cpp:91-93) as well as the even ones the new
+ *     API asserts (src/new/Vector.h:328-330).  Edges with fused per-edge
+ *     kernels (deft_fused_length) run the fused passes; the others run line
+ *     passes with a runtime radix schedule;
  *   - gradients are ACCUMULATED into caller buffers (+=), and the pgrad output
  *     may be NULL (src/Functional.h:407-417, src/EffectivePotentialToDensity.cpp:62-79);
  *   - gradients are derivatives of the discrete energy sum, so they carry the dV
@@ -129,6 +132,11 @@ const char *deft_last_error(void);
 int deft_version(void);
 /* 1 if an FFT edge of this length is supported, else 0. */
 int deft_supported_length(int n);
+/* 1 if the fused per-edge kernels are compiled for this edge
+ * (16, 20, 21, 32, 48, 64, 96, 128, 201, 256), else 0.  A plan whose three
+ * edges are all fused runs the fused passes and may join a batch; any other
+ * plan runs the generic line passes. */
+int deft_fused_length(int n);
 /* Number of visible HIP devices (0 without a GPU). */
 int deft_device_count(int *count);
 
@@ -154,6 +162,16 @@ int deft_plan_autotune(deft_plan plan);
  * (zblock 512 or 1024, ytile and xtile 8192 or 16384). */
 int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile, int *xtile);
 int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
+/* Radix schedule of one axis (0 x, 1 y, 2 z) for one FFT precision, in
+ * Stockham stage order: radix 8, 4, 2, 3, 5 butterflies in registers, other
+ * primes as direct-DFT stages.  Plans with a non-fused edge take any valid
+ * schedule (4, 8 or primes up to 512, product = the edge, at most 12 stages),
+ * and deft_plan_autotune times the candidate orders of every axis; fused
+ * plans report their compile-time order and refuse a change
+ * (DEFT_ERR_UNSUPPORTED).  *nstages is the stage count; at most `capacity`
+ * radices are written. */
+int deft_plan_radix_schedule(deft_plan plan, int fft_precision, int axis, int *radices, int capacity, int *nstages);
+int deft_plan_set_radix_schedule(deft_plan plan, int fft_precision, int axis, const int *radices, int nstages);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
 /* One of the deft_functional_kind values, default DEFT_WHITE_BEAR; drops nothing else. */

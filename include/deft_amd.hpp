@@ -72,9 +72,10 @@ struct Lattice {
 
 // GridDescription (src/GridDescription.cpp:6-50).  The spacing constructor uses
 // the reference's rule N = 1 + int(e/100 + |a|/spacing) (:26-30) and throws
-// (DEFT_ERR_UNSUPPORTED) when that N is not an edge the FFT kernels support,
-// since a different N would change the spacing and dV; at_least() rounds up to
-// the next supported edge for callers that accept that.
+// (DEFT_ERR_UNSUPPORTED) when that N is not an edge the FFT passes support
+// (above 4096, or a prime factor above 512), since a different N would change
+// the spacing and dV; at_least() rounds up to the next supported edge for
+// callers that accept that.
 struct GridDescription {
   Lattice Lat;
   int Nx, Ny, Nz;
@@ -105,6 +106,8 @@ struct GridDescription {
     for (int i = 0; i < 3; ++i) {
       n[i] = reference_edge(lat.length(i), spacing);
       while (!deft_supported_length(n[i]) && n[i] < 4096) ++n[i];
+      if (!deft_supported_length(n[i]))
+        throw Error(DEFT_ERR_UNSUPPORTED, "deft_amd: GridDescription::at_least finds no supported edge up to 4096");
     }
     return GridDescription(lat, n[0], n[1], n[2]);
   }

@@ -45,20 +45,24 @@ def test_solute_driver_matches_oracle(gpu):
         assert np.max(np.abs(np.array(r["density_x"]) - n[:, 0, 0])) < 1e-5 * n.max()
 
 
-def test_spacing_constructor_refuses_unsupported_edges(tmp_path):
+def test_spacing_constructor_keeps_the_reference_edge(tmp_path):
     """GridDescription(lat, spacing) keeps the reference's N = 1 + int(e/100 + |a|/spacing)
-    (src/GridDescription.cpp:26-30) or throws; at_least() is the explicit rounding."""
+    (src/GridDescription.cpp:26-30) -- 65 for 6.4/0.1, 3601 for walls.cpp's 36/0.01 -- and
+    throws only for an edge no pass supports (above 4096 here); at_least() rounds up."""
     src = tmp_path / "g.cpp"
     src.write_text(r'''
 #include <cstdio>
 #include "deft_amd.hpp"
 using namespace deft_amd;
 int main() {
-  const GridDescription a(Lattice::orthorhombic(20, 20, 20), 0.1);   // 201, supported
+  const GridDescription a(Lattice::orthorhombic(20, 20, 20), 0.1);
+  const GridDescription b(Lattice::orthorhombic(6.4, 6.4, 6.4), 0.1);
+  const GridDescription w(Lattice::orthorhombic(0.001, 0.001, 36), 0.01);
   int thrown = 0;
-  try { GridDescription b(Lattice::orthorhombic(6.4, 6.4, 6.4), 0.1); } catch (const Error &e) { thrown = e.status; }
-  const GridDescription c = GridDescription::at_least(Lattice::orthorhombic(6.4, 6.4, 6.4), 0.1);
-  std::printf("%d %d %d %d\n", a.Nx, thrown, c.Nx, GridDescription::reference_edge(6.4, 0.1));
+  try { GridDescription c(Lattice::orthorhombic(50, 1, 1), 0.01); } catch (const Error &e) { thrown = e.status; }
+  const GridDescription d = GridDescription::at_least(Lattice::orthorhombic(10.3, 1, 1), 0.01);
+  std::printf("%d %d %d %d %d %d %d %d\n", a.Nx, b.Nx, w.Nx, w.Ny, w.Nz, thrown, d.Nx,
+              GridDescription::reference_edge(10.3, 0.01));
   return 0;
 }
 ''')
@@ -67,7 +71,8 @@ int main() {
                     "-L" + os.path.join(ROOT, "deft_amd"), "-ldeft_amd", "-Wl,-rpath," + os.path.join(ROOT, "deft_amd"),
                     "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    assert out == ["201", "2", "96", "65"]
+    # 1031 (10.3/0.01 + 1) is prime above 512: at_least() takes 1032 = 8 * 3 * 43
+    assert out == ["201", "65", "1", "1", "3601", "2", "1032", "1031"]
 
 
 def _closest_value_reference(f, L, delta):

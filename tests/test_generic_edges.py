@@ -1,0 +1,173 @@
+"""Grid edges without fused per-edge kernels: the generic line passes
+(deft_amd/csrc/generic.hpp) against the FP64 oracle.
+
+The reference takes any N (FFTW, src/GridDescription.cpp:26-30): its old-API drivers
+build 65^3 grids (6.4/0.1), odd 201^3 ones and 1 x 1 x 3601 planar-wall lines
+(papers/contact/figs/walls.cpp:91-93, 36/0.01 + 1 = 3601 = 13 * 277).  Tolerances as
+the fused passes' (tests/test_gpu_parity.py): FP64 FFTs to round-off (1e-12 on
+energies, 1e-10 of the field scale), FP32 to FP32 round-off of the transforms.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fmt_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GEN_SHAPES = [((65, 65, 65), np.diag([6.4, 6.4, 6.4])),
+              ((18, 25, 30), np.diag([2.1, 2.6, 3.3])),
+              ((7, 22, 13), np.array([[1.9, 0.0, 0.2], [0.1, 2.5, 0.0], [0.0, 0.3, 1.6]])),
+              ((12, 1, 45), np.diag([2.4, 0.01, 4.5])),
+              ((1, 1, 3601), np.diag([0.001, 0.001, 36.0]))]
+# 3601 = 13 * 277 (direct-DFT stages), 65 = 5 * 13, 22 = 2 * 11, 7 and 13 prime
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _fluid(dm, shape, cell, eta=0.3, vext=None, prec=1, seed=None, amp=0.3):
+    g = O.Grid(shape, cell)
+    of = O.make_fluid(g, eta, vext=vext)
+    if seed is not None:
+        of.x = of.x + amp * np.random.default_rng(seed).standard_normal(g.shape)
+    f = dm.WhiteBearFluidVeff(shape, cell, R=of.R, kT=of.kT, mu=of.mu, fft_precision=prec)
+    f.Vext = of.vext
+    f.Veff = of.x
+    return g, of, f
+
+
+def _wall_vext(g, width):
+    """Hard wall |z| < width on the periodic cell (nearest image), sin bumps elsewhere."""
+    z = g.positions()[..., 2]
+    Lz = float(np.linalg.norm(g.cell[2]))
+    z = z - Lz * np.round(z / Lz)
+    v = 0.3 * np.sin(2 * math.pi * g.positions()[..., 2] / Lz * 3)
+    return np.where(np.abs(z) < width, np.inf, v)
+
+
+def test_edges_are_supported_and_generic(gpu):
+    for n in (1, 7, 13, 22, 25, 65, 191, 277, 3300, 3601, 4096):
+        assert gpu.supported_length(n), n
+        assert not gpu.fused_length(n), n
+    for n in (16, 20, 21, 32, 48, 64, 96, 128, 201, 256):
+        assert gpu.supported_length(n) and gpu.fused_length(n), n
+    for n in (0, 4097, 2 * 521, 1031):  # beyond 4096, prime factors above 512
+        assert not gpu.supported_length(n), n
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-13), (0, 3e-6)])
+@pytest.mark.parametrize("shape,cell", GEN_SHAPES)
+def test_fft_seam_matches_numpy(gpu, shape, cell, prec, tol):
+    print("generic fft", shape, prec, flush=True)
+    g = O.Grid(shape, cell)
+    f = gpu.WhiteBearFluidVeff(shape, cell, fft_precision=prec)
+    a = np.random.default_rng(7).standard_normal(shape)
+    F = f.fft(a)
+    assert _rel(F, O.fft(g, a)) < tol
+    assert _rel(f.ifft(F), a) < 2 * tol
+    f.close()
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+@pytest.mark.parametrize("shape,cell", GEN_SHAPES)
+def test_energy_and_gradient(gpu, shape, cell, prec, etol, gtol):
+    print("generic energy/grad", shape, prec, flush=True)
+    g = O.Grid(shape, cell)
+    vext = _wall_vext(g, 0.6) if shape[2] >= 30 else None
+    if vext is not None and prec == 0:
+        vext = vext.astype(np.float32).astype(np.float64)
+    g, of, f = _fluid(gpu, shape, cell, vext=vext, prec=prec, seed=11)
+    got = f.weighted_densities()
+    ref = of.weighted_densities(of.density())
+    # FP32: the direct-DFT stages (radix 7, 11, 13, 277) sum up to p terms in FP32;
+    # the vector densities, which cancel to ~1e-4 of n3, see 5e-6 of their maximum
+    for a in range(5):
+        assert _rel(got[a], ref[a]) < (1e-11 if prec == 1 else 1e-5), a
+    E = f.energy()
+    Eo = of.energy()
+    # relative to the terms' magnitudes: at eta 0.3 on the wall line they cancel to 1/40
+    scale = sum(abs(v) for v in of.last_terms.values())
+    assert abs(E - Eo) <= etol * scale
+    for k, v in of.last_terms.items():
+        assert abs(f.last_terms[k] - v) <= etol * scale, k
+    E2, gr, pg = f.energy_grad_and_precond()
+    Eo2, gro, pgo = of.energy_grad_and_precond()
+    assert abs(E2 - Eo2) <= etol * scale
+    assert _rel(gr, gro) < gtol
+    assert _rel(pg, pgo) < gtol
+    if vext is not None:
+        wall = ~np.isfinite(vext)
+        assert np.all(gr[wall] == 0) and np.all(pg[wall] == 0)
+    f.close()
+
+
+@pytest.mark.parametrize("functional", ["mark2", "tensor"])
+def test_other_functionals(gpu, functional):
+    shape, cell = GEN_SHAPES[1]
+    print("generic", functional, flush=True)
+    g = O.Grid(shape, cell)
+    of = O.make_fluid(g, 0.35, functional="wb2" if functional == "mark2" else "wbt")
+    of.x = of.x + 0.2 * np.random.default_rng(5).standard_normal(g.shape)
+    f = gpu.WhiteBearFluidVeff(shape, cell, R=of.R, kT=of.kT, mu=of.mu, fft_precision=1,
+                               functional=gpu.WHITE_BEAR_MARK_II if functional == "mark2" else gpu.WHITE_BEAR_TENSOR)
+    f.Veff = of.x
+    E, gr, pg = f.energy_grad_and_precond()
+    Eo, gro, pgo = of.energy_grad_and_precond()
+    assert abs(E - Eo) <= 1e-12 * abs(Eo)
+    assert _rel(gr, gro) < 1e-10
+    f.close()
+
+
+def test_every_radix_schedule_gives_the_same_answer(gpu):
+    """65 = 5 * 13, 30 = 2 * 3 * 5, 3601 = 13 * 277 in every candidate order: FP64
+    results agree to round-off; an invalid order is refused; autotune keeps one of them."""
+    for shape, cell in (GEN_SHAPES[0], GEN_SHAPES[1], GEN_SHAPES[4]):
+        g, of, f = _fluid(gpu, shape, cell, seed=3)
+        E0, g0, _ = f.energy_grad_and_precond()
+        for axis in range(3):
+            L = shape[axis]
+            if L == 1:
+                continue
+            cands = [f.radix_schedule(1, axis)]
+            fac = O_factors(L)
+            cands.append(sorted(fac, reverse=True))
+            cands.append(sorted(fac))
+            for c in cands:
+                f.set_radix_schedule(1, axis, c)
+                assert f.radix_schedule(1, axis) == c
+                E, gr, _ = f.energy_grad_and_precond()
+                assert abs(E - E0) <= 1e-13 * abs(E0), (shape, axis, c)
+                assert _rel(gr, g0) < 1e-11, (shape, axis, c)
+            with pytest.raises(gpu.DeftError):
+                f.set_radix_schedule(1, axis, [L + 1] if L > 1 else [2])
+        f.autotune()
+        for axis in range(3):
+            s = f.radix_schedule(1, axis)
+            assert int(np.prod(s)) == shape[axis] if s else shape[axis] == 1
+        E, gr, _ = f.energy_grad_and_precond()
+        assert abs(E - E0) <= 1e-13 * abs(E0)
+        f.close()
+
+
+def O_factors(n):
+    out, p = [], 2
+    while p * p <= n:
+        while n % p == 0:
+            out.append(p)
+            n //= p
+        p += 1
+    if n > 1:
+        out.append(n)
+    return out
+
+
+def test_fused_plans_refuse_a_radix_change(gpu):
+    f = gpu.WhiteBearFluidVeff((32, 32, 32), np.diag([3.2] * 3), fft_precision=1)
+    assert f.radix_schedule(1, 2) == [8, 4]
+    with pytest.raises(gpu.DeftError):
+        f.set_radix_schedule(1, 2, [4, 8])
+    f.close()

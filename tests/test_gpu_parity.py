@@ -292,10 +292,13 @@ def test_c1_wall_matches_oracle_golden(gpu):
 
 
 def test_bad_sizes_raise(gpu):
+    """Edges above 4096 or with a prime factor above 512 have no FFT pass."""
     with pytest.raises(gpu.DeftError):
-        gpu.WhiteBearFluidVeff((18, 16, 16), np.diag([1.0] * 3))
+        gpu.WhiteBearFluidVeff((1031, 1, 1), np.diag([1.0] * 3))
     with pytest.raises(gpu.DeftError):
-        gpu.WhiteBearFluidVeff((15, 16, 16), np.diag([1.0] * 3))
+        gpu.WhiteBearFluidVeff((1, 1, 4097), np.diag([1.0] * 3))
+    with pytest.raises(gpu.DeftError):
+        gpu.WhiteBearFluidVeff((0, 16, 16), np.diag([1.0] * 3))
 
 
 def test_nan_veff_stops_minimizer(gpu):

@@ -21,10 +21,14 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_supported_lengths():
+    """Fused per-edge kernels for ten edges; the generic line passes for any other edge
+    up to 4096 whose prime factors are at most 512 (FFTW's any-N in the reference)."""
     for n in (16, 20, 21, 32, 48, 64, 96, 128, 201, 256):
-        assert deft_amd.supported_length(n)
-    for n in (0, 15, 18, 100, 512):
-        assert not deft_amd.supported_length(n)
+        assert deft_amd.supported_length(n) and deft_amd.fused_length(n)
+    for n in (1, 15, 18, 65, 100, 512, 3601, 4096):
+        assert deft_amd.supported_length(n) and not deft_amd.fused_length(n)
+    for n in (0, -3, 4097, 1031, 2 * 521):
+        assert not deft_amd.supported_length(n) and not deft_amd.fused_length(n)
 
 
 @pytest.mark.parametrize("eta", [0.1, 0.3, 0.4, 0.45])
@@ -53,9 +57,10 @@ def test_bulk_state_of_every_functional(eta, functional, name):
 def test_bad_arguments_raise_without_device():
     with pytest.raises(deft_amd.DeftError):
         deft_amd.find_chemical_potential(-1.0)
-    # without a GPU plan creation fails loudly; with one, odd edges are rejected
+    # without a GPU plan creation fails loudly; with one, an edge with a prime factor
+    # above 512 is rejected
     with pytest.raises(deft_amd.DeftError):
-        deft_amd.WhiteBearFluidVeff((15, 16, 16), np.diag([1.0, 1.0, 1.0]))
+        deft_amd.WhiteBearFluidVeff((1031, 16, 16), np.diag([1.0, 1.0, 1.0]))
 
 
 def test_product_path_has_no_oracle_import():
