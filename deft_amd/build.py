@@ -55,7 +55,7 @@ def build(verbose=False):
     return LIB
 
 
-EXAMPLES = ["solute", "inner_sphere"]
+EXAMPLES = ["solute", "inner_sphere", "walls"]
 
 
 def build_examples(verbose=False):

@@ -171,3 +171,50 @@ def test_fused_plans_refuse_a_radix_change(gpu):
     with pytest.raises(gpu.DeftError):
         f.set_radix_schedule(1, 2, [4, 8])
     f.close()
+
+
+@pytest.mark.parametrize("eta,functional", [(0.3, "wb"), (0.4, "wb2"), (0.2, "wbt")])
+def test_wall_line_minimum_matches_oracle(gpu, eta, functional):
+    """The 1 x 1 x 3601 planar-wall line of papers/contact/figs/walls.cpp (constrained
+    slab |z| <= 3, Veff from -ln(eta constraint/(4 pi/3) + 1e-4 n_b)) minimized to
+    relative 1e-13 with the new-API Minimize at FP64 against the oracle's minimum of the
+    same problem: energy to 1e-11, density to 1e-6 of its maximum, and the contact
+    density at the slab's edge against the bulk pressure."""
+    print("wall line", eta, functional, flush=True)
+    g = O.Grid((1, 1, 3601), np.diag([0.001, 0.001, 36.0]))
+    z = g.positions()[..., 2]
+    z = z - 36.0 * np.floor(z / 36.0 + 0.5)
+    constraint = (np.abs(z) > 3).astype(float)
+    nb = eta / (4 * math.pi / 3)
+    mu = O.chemical_potential(nb, 1.0, 1.0, functional)
+    vext = np.where(constraint == 0, np.nan, 0.0)
+    x0 = -np.log((eta * constraint + 1e-4 * eta) / (4 * math.pi / 3))
+    of = O.WhiteBearFluidVeff(g, 1.0, 1.0, mu, vext, x0.copy(), functional)
+    om = O.Minimize(of)
+    om.precondition(True)
+    om.set_relative_precision(1e-13)
+    while om.improve_energy():
+        pass
+    kind = {"wb": gpu.WHITE_BEAR, "wb2": gpu.WHITE_BEAR_MARK_II, "wbt": gpu.WHITE_BEAR_TENSOR}[functional]
+    f = gpu.WhiteBearFluidVeff((1, 1, 3601), np.diag([0.001, 0.001, 36.0]), mu=mu, fft_precision=1, functional=kind)
+    f.Vext = vext
+    f.Veff = x0
+    m = gpu.Minimize(f)
+    m.precondition(True)
+    m.set_relative_precision(1e-13)
+    m.set_maxiter(5000)
+    m.run()
+    s = m.stats()
+    print("wall line", functional, s["iter"], s["stop_reason"], m.energy(), om.energy(), flush=True)
+    assert s["stop_reason"] == "converged"
+    assert abs(m.energy() / om.energy() - 1) < 1e-11
+    # the density is first order in the remaining error, the energy second order
+    n, no = f.density(), of.density()
+    assert np.max(np.abs(n - no)) < 1e-6 * no.max()
+    # hard-wall contact theorem n(contact) = p/kT, to the grid's first order (dz = 0.01)
+    k = int(np.argmax(z[0, 0] > 3.0))
+    p = O.pressure(nb, 1.0, functional=functional)
+    print("contact", z[0, 0, k], n[0, 0, k], "p/kT", p, flush=True)
+    assert abs(n[0, 0, k] / p - 1) < 0.03
+    m.close()
+    f.close()
