@@ -7,7 +7,9 @@ inference included): hard-sphere fluid, R = 1, kT = 1, bulk packing fraction
 periodic) in a 12.8^3 cell on a 128^3 grid (dx = 0.1); mu from the bulk
 equation of state; start Veff = -kT (ln n_b + u) with u from the committed
 network (deft_amd/models/unet_warmstart.npz), or Veff = -kT ln n_b (plain);
-preconditioned Polak-Ribiere CG with deft's quadratic line search; FP32 FFTs /
+Polak-Ribiere CG with deft's quadratic line search, preconditioned with the
+structure factor of the bulk fluid (value) or with the reference's grad/n
+(reported beside it); FP32 FFTs /
 FP64 pointwise and reductions until the error estimate is below SWITCH_RP |F|,
 then FP64 throughout to the reference driver's absolute precision |p_bulk V| 1e-10
 (about 1e-10 |F|), with the quadratic step's
@@ -60,7 +62,7 @@ def gpu_run(steps, warmup):
     gold = json.load(open(os.path.join(HERE, "tests", "golden", "oracle_c2.json")))
     precision = abs(D.pressure(nb, R, KT) * L ** 3) * PRECISION_FACTOR
 
-    def one(warm):
+    def one(warm, sk):
         f.fft_precision = D.FFT_FP32
         t0 = time.perf_counter()
         if warm:
@@ -70,6 +72,8 @@ def gpu_run(steps, warmup):
         t1 = time.perf_counter()
         m = D.Minimize(f)
         m.precondition(True)
+        if sk:
+            m.set_preconditioner(D.PGRAD_STRUCTURE_FACTOR)
         m.set_precision(precision)
         m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
         m.run()
@@ -83,25 +87,27 @@ def gpu_run(steps, warmup):
         return s
 
     out = {}
-    for warm in (True, False):
-        for _ in range(warmup):
-            one(warm)
-        results = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            results.append(one(warm))
-        elapsed = time.perf_counter() - t0
-        n = f.density()
-        dn = float(np.max(np.abs(n[:, 0, 0] - np.array(gold["density_x_axis"]))) / gold["density_max"])
-        errs = [abs(r["energy"] - gold["energy"]) / abs(gold["energy"]) for r in results]
-        out["warm" if warm else "plain"] = dict(results=results, elapsed=elapsed, err=max(errs), dn=dn)
-    # the same warm-start steps again with HIP events around every launch (per-kernel
-    # times for the roofline; kept out of the throughput timing above)
+    for sk in (True, False):
+        for warm in (True, False):
+            for _ in range(warmup):
+                one(warm, sk)
+            results = []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                results.append(one(warm, sk))
+            elapsed = time.perf_counter() - t0
+            n = f.density()
+            dn = float(np.max(np.abs(n[:, 0, 0] - np.array(gold["density_x_axis"]))) / gold["density_max"])
+            errs = [abs(r["energy"] - gold["energy"]) / abs(gold["energy"]) for r in results]
+            out[("warm" if warm else "plain") + ("" if sk else "_ref")] = dict(results=results, elapsed=elapsed,
+                                                                               err=max(errs), dn=dn)
+    # the value's steps again with HIP events around every launch (per-kernel times
+    # for the roofline; kept out of the throughput timing above)
     f.set_profiling(True)
     f.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
-        one(True)
+        one(True, True)
     out["profiled_elapsed"] = time.perf_counter() - t0
     out["tuning"] = out_tuning
     out["prof"] = f.profile()
@@ -208,7 +214,9 @@ def whole_path_bytes(prof, steps):
     one energy B_E = 6 B_fft + 12N (+ 24N for the line-search move it follows), one
     energy + gradient B_EG = 12 B_fft + 20N, and 40N of CG vectors per iteration; the
     evaluations are counted from the profiled steps (energy-only = Phi energy passes,
-    energy + gradient = Phi gradient passes, iterations = CG updates)."""
+    energy + gradient = Phi gradient passes, iterations = CG updates), plus per
+    structure-factor preconditioned gradient one forward and one inverse transform,
+    the spectral scale (2 x 8Nk FP32 / 2 x 16Nk FP64 + 8Nk) and the last pass (48N)."""
     Nr = float(N) ** 3
     Nk = float(N) * N * (N // 2 + 1)
     total = 0.0
@@ -216,6 +224,8 @@ def whole_path_bytes(prof, steps):
         nE = prof.get("zphi_energy_" + prec, (0,))[0]
         nEG = prof.get("zphi_grad_" + prec, (0,))[0]
         total += nE * (6 * bfft + 12 * Nr + 24 * Nr) + nEG * (12 * bfft + 20 * Nr)
+        nSK = prof.get("sk_finish_" + prec, (0,))[0]
+        total += nSK * (2 * bfft + (bfft - (4 if prec == "f32" else 8) * Nr) + 8 * Nk + 48 * Nr)
     total += prof.get("cg_update", (0,))[0] * 40 * Nr
     return total / steps
 
@@ -314,8 +324,10 @@ def main():
     # 256 MB last-level cache), next to the spec peak
     copy_gbps = D.copy_bandwidth(1 << 31, 10)
     w, pl = runs["warm"], runs["plain"]
+    wr, plr = runs["warm_ref"], runs["plain_ref"]
     value = args.steps / w["elapsed"]
     plain_value = args.steps / pl["elapsed"]
+    ref_value, ref_plain_value = args.steps / wr["elapsed"], args.steps / plr["elapsed"]
     prof = runs["prof"]
     # dominant kernel: the kernel instance (one profiler entry: pass, direction and FFT
     # precision at the autotuned variant) with the most event time over the profiled steps
@@ -327,7 +339,15 @@ def main():
     path_gbps = path_bytes / (w["elapsed"] / args.steps) / 1e9
     c3 = None if args.no_c3 else c3_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
-    r0, p0 = w["results"][-1], pl["results"][-1]
+    def block(run, start):
+        r = run["results"][-1]
+        d = {"minimizations_per_s": args.steps / run["elapsed"], "ms_per_minimization": run["elapsed"] / args.steps * 1e3,
+             "iterations": r["iter"], "fp32_iterations": r["fp32_iterations"], "energy_calcs": r["num_energy_calcs"],
+             "grad_calcs": r["num_grad_calcs"], "dF_over_F_vs_fp64_oracle": run["err"],
+             "max_dn_over_nmax_x_axis": run["dn"]}
+        if start == "warm":
+            d["network_ms"] = r["start_seconds"] * 1e3
+        return d
     line = {
         "metric": METRIC, "value": value, "unit": "minimizations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": w["elapsed"] / args.steps * 1e3,
@@ -336,21 +356,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": "hard-sphere fluid eta=0.40 around a radius-3 hard solute, 128^3 grid, "
                                "12.8^3 cell, network warm start (value) vs plain start",
+                   "preconditioner": "structure factor of the bulk fluid (DEFT_PGRAD_STRUCTURE_FACTOR); "
+                                     "'reference_preconditioner' repeats both starts with deft's grad/n",
                    "grid": [N, N, N], "cell": L, "eta": ETA, "R": R,
                    "switch_relative_precision": SWITCH_RP, "precision": precision,
                    "precision_rule": "|p_bulk V| * 1e-10 (papers/contact/figs/inner-sphere.cpp:201-203)",
                    "polish_gradients": "fp64" if POLISH_GRAD == D.FFT_FP64 else "fp32",
-                   "warm": {"minimizations_per_s": value, "ms_per_minimization": w["elapsed"] / args.steps * 1e3,
-                            "network_ms": r0["start_seconds"] * 1e3, "iterations": r0["iter"],
-                            "fp32_iterations": r0["fp32_iterations"], "energy_calcs": r0["num_energy_calcs"],
-                            "grad_calcs": r0["num_grad_calcs"], "dF_over_F_vs_fp64_oracle": w["err"],
-                            "max_dn_over_nmax_x_axis": w["dn"]},
-                   "plain": {"minimizations_per_s": plain_value,
-                             "ms_per_minimization": pl["elapsed"] / args.steps * 1e3, "iterations": p0["iter"],
-                             "fp32_iterations": p0["fp32_iterations"], "energy_calcs": p0["num_energy_calcs"],
-                             "grad_calcs": p0["num_grad_calcs"], "dF_over_F_vs_fp64_oracle": pl["err"],
-                             "max_dn_over_nmax_x_axis": pl["dn"]},
+                   "warm": block(w, "warm"),
+                   "plain": block(pl, "plain"),
                    "warm_over_plain": value / plain_value,
+                   "reference_preconditioner": {"warm": block(wr, "warm"), "plain": block(plr, "plain"),
+                                                "warm_over_plain": ref_value / ref_plain_value},
+                   "structure_factor_over_reference_preconditioner": {"warm": value / ref_value,
+                                                                      "plain": plain_value / ref_plain_value},
                    "c3_batch": c3,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -370,13 +388,18 @@ def main():
                      "measured": f"HIP events around every launch over {args.steps} further warm-start steps "
                                  f"({runs['profiled_elapsed'] / args.steps * 1e3:.2f} ms per step with events)"},
         "cpu_baseline": cpu,
-        "gpu_over_cpu_baseline": None if not cpu else {"warm": value / cpu["value"], "plain": plain_value / cpu["value"]},
+        "gpu_over_cpu_baseline": None if not cpu else {
+            "warm": value / cpu["value"], "plain": plain_value / cpu["value"],
+            "plain_reference_preconditioner": ref_plain_value / cpu["value"],
+            "note": "the CPU baseline runs deft's grad/n preconditioner; plain_reference_preconditioner is "
+                    "the like-for-like algorithm"},
         "kernels": {k: {"launches": v[0], "ms": round(v[1], 3), "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1)}
                     for k, v in prof.items() if v[0] > 0},
     }
-    ok = w["err"] < 1e-9 and pl["err"] < 1e-9
+    ok = all(runs[k]["err"] < 1e-9 for k in ("warm", "plain", "warm_ref", "plain_ref"))
     if not ok:
-        print(f"convergence check failed: |dF/F| warm {w['err']:.2e}, plain {pl['err']:.2e}", file=sys.stderr)
+        print("convergence check failed: |dF/F| " + ", ".join(f"{k} {runs[k]['err']:.2e}" for k in
+                                                            ("warm", "plain", "warm_ref", "plain_ref")), file=sys.stderr)
     print(json.dumps(line), flush=True)
     if not ok:
         sys.exit(1)

@@ -41,6 +41,7 @@ WHITE_BEAR_TENSOR = 3
 
 PGRAD_NEW = 1
 PGRAD_EFFECTIVE_POTENTIAL = 2
+PGRAD_STRUCTURE_FACTOR = 3
 
 STOP_REASONS = {
     0: "none", 1: "maxiter", 2: "nan-energy", 3: "infinite-energy", 4: "nan-slope",
@@ -90,6 +91,7 @@ _SIGNATURES = {
     "deft_plan_set_radix_schedule": ([_P, _I, _I, _IP, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
     "deft_plan_set_mu": ([_P, _D], _I),
+    "deft_plan_bulk_density": ([_P, _DP], _I),
     "deft_plan_set_functional": ([_P, _I], _I),
     "deft_contact_values": ([_P, _DP], _I),
     "deft_copy_bandwidth": ([ctypes.c_longlong, _I, _DP], _I),
@@ -134,6 +136,7 @@ _SIGNATURES = {
     "deft_minimize_set_maxiter": ([_P, _I], _I),
     "deft_minimize_set_miniter": ([_P, _I], _I),
     "deft_minimize_precondition": ([_P, _I], _I),
+    "deft_minimize_set_preconditioner": ([_P, _I], _I),
     "deft_minimize_improve_energy": ([_P, _I, _IP], _I),
     "deft_minimize_run": ([_P, _I, _IP], _I),
     "deft_minimize_energy": ([_P, _DP], _I),
@@ -413,6 +416,24 @@ class WhiteBearFluidVeff:
                                                PGRAD_EFFECTIVE_POTENTIAL))
         return e.value, g, pg
 
+    def sk_precond(self, Veff=None):
+        """(energy, grad, pgrad) with the structure-factor preconditioned gradient
+        (DEFT_PGRAD_STRUCTURE_FACTOR; no counterpart in the reference)."""
+        if Veff is not None:
+            self.Veff = Veff
+        g = np.zeros(self.shape)
+        pg = np.zeros(self.shape)
+        e = ctypes.c_double()
+        _check(lib().deft_functional_grad_kind(self._h, None, _dptr(g), _dptr(pg), ctypes.byref(e),
+                                               PGRAD_STRUCTURE_FACTOR))
+        return e.value, g, pg
+
+    def bulk_density(self):
+        """The bulk density whose chemical potential is this fluid's mu."""
+        v = ctypes.c_double()
+        _check(lib().deft_plan_bulk_density(self._h, ctypes.byref(v)))
+        return v.value
+
     def energy_grad_and_precond(self, Veff=None):
         if Veff is not None:
             self.Veff = Veff
@@ -526,6 +547,10 @@ class Minimize:
 
     def precondition(self, u):
         _check(lib().deft_minimize_precondition(self._h, 1 if u else 0))
+
+    def set_preconditioner(self, kind):
+        """PGRAD_NEW (grad/n, the reference's) or PGRAD_STRUCTURE_FACTOR."""
+        _check(lib().deft_minimize_set_preconditioner(self._h, int(kind)))
 
     def improve_energy(self, verbose=False):
         more = ctypes.c_int()

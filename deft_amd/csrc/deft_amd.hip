@@ -63,8 +63,13 @@ int ensure_gwork(deft_plan_s *p) {
 
 int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, int precond);
 int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir);
+int plan_ensure_sk(deft_plan_s *p);
 
 int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, int precond) {
+  if (grad && precond == PGRAD_SK) {
+    if (p->batch) return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner runs on plans outside batches");
+    TRY(plan_ensure_sk(p));
+  }
   if (p->batch) return batch_evaluate(p, prec, grad, have_old, precond);
   if (prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
   return Launch<float>::evaluate(p, grad, have_old, precond);
@@ -145,6 +150,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->tmp);
   (void)hipFree(p->wtab);
   (void)hipFree(p->gwork);
+  (void)hipFree(p->sktab);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -757,6 +763,7 @@ int deft_plan_set_functional(deft_plan p, int functional) {
   if (functional == DEFT_WHITE_BEAR_TENSOR) p->tz[0] = p->tz[1] = kZBlocks[0];
   const int was = p->functional;
   p->functional = functional;
+  p->sk_ok = false;
   const int rc = ensure_gwork(p);
   if (rc != DEFT_OK) p->functional = was;
   return rc;
@@ -766,6 +773,7 @@ int deft_plan_set_mu(deft_plan p, double mu) {
   TRY(check_plan(p));
   if (!std::isfinite(mu)) return set_error(DEFT_ERR_ARG, "mu must be finite");
   p->mu = mu;
+  p->sk_ok = false;
   return DEFT_OK;
 }
 
@@ -854,7 +862,8 @@ int deft_functional_grad_device_kind(deft_plan p, const double *veff_dev, double
                                      double *energy, int pgrad_kind) {
   TRY(check_plan(p));
   if (!outgrad_dev) return set_error(DEFT_ERR_ARG, "null outgrad");
-  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL)
+  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL &&
+      pgrad_kind != DEFT_PGRAD_STRUCTURE_FACTOR)
     return set_error(DEFT_ERR_ARG, "unknown pgrad kind");
   if (veff_dev) {
     p->npend = 0;
@@ -879,7 +888,8 @@ int deft_functional_grad_kind(deft_plan p, const double *veff, double *outgrad, 
                               int pgrad_kind) {
   TRY(check_plan(p));
   if (!outgrad) return set_error(DEFT_ERR_ARG, "null outgrad");
-  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL)
+  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_EFFECTIVE_POTENTIAL &&
+      pgrad_kind != DEFT_PGRAD_STRUCTURE_FACTOR)
     return set_error(DEFT_ERR_ARG, "unknown pgrad kind");
   if (veff) TRY(deft_plan_set_veff(p, veff));
   TRY(evaluate(p, true, false, pgrad_kind));
@@ -1094,6 +1104,85 @@ int deft_bulk_state(int functional, double n, double R, double kT, double *f, do
   return DEFT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Bulk density of the plan's mu and functional: mu(n) rises monotonically from
+// -inf (n -> 0) towards +inf (n3 -> 1); bisection in ln n.
+bool bulk_density_of_mu(const deft_plan_s *p, double &nb) {
+  const double nmax = 1.0 / (4 * M_PI / 3 * p->R * p->R * p->R);
+  auto mu_of = [&](double n) {
+    double fe, dfe;
+    bulk_terms(n, p->R, p->kT, &fe, &dfe, p->functional);
+    return p->kT * std::log(n / water_nQ(p->kT)) + dfe;
+  };
+  double lo = std::log(1e-30 * nmax), hi = std::log((1 - 1e-9) * nmax);
+  if (!(mu_of(std::exp(lo)) < p->mu) || !(mu_of(std::exp(hi)) > p->mu)) return false;
+  for (int i = 0; i < 200; ++i) {
+    const double mid = 0.5 * (lo + hi);
+    (mu_of(std::exp(mid)) < p->mu ? lo : hi) = mid;
+  }
+  nb = std::exp(0.5 * (lo + hi));
+  return std::isfinite(nb) && nb > 0;
+}
+
+}  // namespace
+
+namespace {
+
+// The structure-factor table of the plan's current mu, functional and radius:
+// Phi's second derivatives at the bulk weighted densities by central
+// differences (relative step 1e-5) of the FP64 partials, then S(k)/N per
+// spectral element on the device.
+int plan_ensure_sk(deft_plan_s *p) {
+  if (p->sk_ok) return DEFT_OK;
+  double nb;
+  if (!bulk_density_of_mu(p, nb))
+    return set_error(DEFT_ERR_ARG, "no bulk density has this chemical potential (structure-factor preconditioner)");
+  const WBConst c = make_wb_const(p->R, p->functional == DEFT_WHITE_BEAR_TENSOR ? DEFT_WHITE_BEAR : p->functional);
+  const double n3 = 4 * M_PI / 3 * p->R * p->R * p->R * nb, n2 = 4 * M_PI * p->R * p->R * nb;
+  auto part = [&](double a3, double a2, double vx, double *d) {
+    double p1, p2, p3;
+    wb_phi_partials(a3, a2, vx, 0.0, 0.0, c, p1, p2, p3, d);
+  };
+  double dp[5], dm[5];
+  const double h3 = 1e-5 * n3, h2 = 1e-5 * n2;
+  part(n3 + h3, n2, 0.0, dp);
+  part(n3 - h3, n2, 0.0, dm);
+  const double P33 = (dp[0] - dm[0]) / (2 * h3), P32a = (dp[1] - dm[1]) / (2 * h3);
+  part(n3, n2 + h2, 0.0, dp);
+  part(n3, n2 - h2, 0.0, dm);
+  const double P22 = (dp[1] - dm[1]) / (2 * h2), P32b = (dp[0] - dm[0]) / (2 * h2);
+  part(n3, n2, h2, dp);
+  part(n3, n2, -h2, dm);
+  const double cv = (dp[2] - dm[2]) / (2 * h2);
+  if (!p->sktab && hipMalloc(&p->sktab, sizeof(double) * (size_t)p->ks) != hipSuccess) {
+    p->sktab = nullptr;
+    return set_error(DEFT_ERR_ALLOC, "hipMalloc of the structure-factor table failed");
+  }
+  const int kb = (int)std::min<long long>(4096, (p->Nk + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_sk_table, dim3(kb), dim3(kThreads), 0, p->stream, grid_args(p), P33, 0.5 * (P32a + P32b), P22,
+                     cv, nb, 1e3, p->sktab);
+  DEFT_HIP_CHECK(hipGetLastError());
+  TRY(plan_sync(p));
+  p->sk_nb = nb;
+  p->sk_floor = 1e-3 * nb;
+  p->sk_ok = true;
+  return DEFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int deft_plan_bulk_density(deft_plan p, double *nb) {
+  TRY(check_plan(p));
+  if (!nb) return set_error(DEFT_ERR_ARG, "null output");
+  if (!bulk_density_of_mu(p, *nb)) return set_error(DEFT_ERR_ARG, "no bulk density has this chemical potential");
+  return DEFT_OK;
+}
+
 int deft_bulk_pressure(double n, double R, double kT, double *pr) {
   if (!pr) return set_error(DEFT_ERR_ARG, "null output");
   double mu, f;
@@ -1139,6 +1228,23 @@ int deft_minimize_set_maxiter(deft_minimize m, int v) {
 int deft_minimize_set_miniter(deft_minimize m, int v) {
   if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
   m->miniter = v;
+  return DEFT_OK;
+}
+
+int deft_minimize_set_preconditioner(deft_minimize m, int pgrad_kind) {
+  if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
+  if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_STRUCTURE_FACTOR)
+    return set_error(DEFT_ERR_ARG, "the new-API minimizer takes DEFT_PGRAD_NEW or DEFT_PGRAD_STRUCTURE_FACTOR");
+  if (pgrad_kind == DEFT_PGRAD_STRUCTURE_FACTOR) {
+    if (m->p->batch)
+      return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner runs on plans outside batches");
+    TRY(plan_ensure_sk(m->p));
+  }
+  if (pgrad_kind != m->precond_kind) {
+    m->precond_kind = pgrad_kind;
+    min_invalidate(*m);
+    m->have_dir = m->have_old = false;
+  }
   return DEFT_OK;
 }
 

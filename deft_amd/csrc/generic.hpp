@@ -510,7 +510,7 @@ __global__ __launch_bounds__(kThreads) void kg_grad(GridArgs g, const double* __
       const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu;
       const double dfdn = (g.kT * cv + local) * g.dV;
       gr = -dfdn * n / g.kT;
-      pg = precond == PGRAD_NEW ? -dfdn / g.kT : (precond == PGRAD_OLD ? dfdn / (xv - g.kT) : gr);
+      pg = point_pgrad(precond, dfdn, n, xv, gr, g);
     }
     grad[i] = gr;
     pgrad[i] = pg;

@@ -319,6 +319,9 @@ static void gen_density_to_weighted(deft_plan_s *p, const GridArgs &g, double *w
 }
 
 template <typename T>
+static int sk_apply(deft_plan_s *p, const GridArgs &g, int nb, bool have_old);
+
+template <typename T>
 static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nb) {
   const GridArgs g = grid_args(p);
   const int nw = plan_fields(p);
@@ -356,6 +359,7 @@ static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
     hipLaunchKernelGGL(kg_grad<T>, dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->x, vext_ptr<T>(p), p->og,
                        have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride);
   }
+  if (grad && precond == PGRAD_SK) TRY_LAUNCH(sk_apply<T>(p, g, p->eblocks, have_old));
   *nb = p->eblocks;
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
@@ -366,6 +370,26 @@ static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
 // fields) and keeps the fastest.
 template <typename T>
 static int gen_autotune(deft_plan_s *p);
+
+// The structure-factor preconditioner's transforms after a gradient pass that
+// stored grad/sqrt(n) in p->pg (PGRAD_SK): pg <- F^-1 [S(k)/N F pg], then
+// k_sk_finish; nb blocks write the pgrad.grad partial sums.
+template <typename T>
+static int sk_apply(deft_plan_s *p, const GridArgs &g, int nb, bool have_old) {
+  if (!p->sktab) return set_error(DEFT_ERR_STATE, "structure-factor table missing");
+  TRY_LAUNCH(Launch<T>::fft_forward(p, p->pg, 1.0));
+  {
+    ProfScope ps(p, kid<T>(K_FFT_SEAM), 2.0 * sizeof(cx<T>) * p->Nk + 8.0 * p->Nk);
+    hipLaunchKernelGGL(k_spec_scale<T>, dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0),
+                       p->sktab);
+  }
+  TRY_LAUNCH(Launch<T>::fft_inverse(p, p->pg, 1.0));
+  ProfScope ps(p, kid<T>(K_SK_FINISH), (have_old ? 48.0 : 40.0) * p->N + (double)sizeof(T) * p->N);
+  hipLaunchKernelGGL(k_sk_finish<T>, dim3(nb), dim3(kThreads), 0, p->stream, g, p->x, vext_ptr<T>(p), p->g, p->og,
+                     have_old ? 1 : 0, p->pg, p->partial, p->pstride);
+  DEFT_HIP_CHECK(hipGetLastError());
+  return DEFT_OK;
+}
 
 template <typename T>
 int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
@@ -400,6 +424,7 @@ int Launch<T>::enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
                            tw_ptr<T>(p, 2));
       });
     });
+    if (precond == PGRAD_SK) TRY_LAUNCH(sk_apply<T>(p, g, nzb, have_old));
   }
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;
@@ -778,6 +803,7 @@ int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
     constexpr int L = decltype(LZ)::value;
     with_zb(p->tz[ti], [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
+      ProfScope ps(p, kid<T>(K_FFT_SEAM), 8.0 * p->N + (double)sizeof(cx<T>) * p->Nk);
       hipLaunchKernelGGL((k_zfwd_plain<T, L, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g, in_dev,
                          spec_ptr<T>(p, 0), tw_ptr<T>(p, 2));
     });
@@ -787,6 +813,7 @@ int Launch<T>::fft_forward(deft_plan_s *p, const double *in_dev, double scl) {
     constexpr int L = decltype(LX)::value;
     with_tb(p->tx[ti], [&](auto TBc) {
       constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(K_FFT_SEAM), 2.0 * sizeof(cx<T>) * p->Nk);
       hipLaunchKernelGGL((k_xpass_plain<T, L, false, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
                          spec_ptr<T>(p, 0), scl, tw_ptr<T>(p, 0));
     });
@@ -808,6 +835,7 @@ int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
     constexpr int L = decltype(LX)::value;
     with_tb(p->tx[ti], [&](auto TBc) {
       constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(K_FFT_SEAM), 2.0 * sizeof(cx<T>) * p->Nk);
       hipLaunchKernelGGL((k_xpass_plain<T, L, true, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
                          spec_ptr<T>(p, 0), scl, tw_ptr<T>(p, 0));
     });
@@ -817,6 +845,7 @@ int Launch<T>::fft_inverse(deft_plan_s *p, double *out_dev, double scl) {
     constexpr int L = decltype(LZ)::value;
     with_zb(p->tz[ti], [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
+      ProfScope ps(p, kid<T>(K_FFT_SEAM), 8.0 * p->N + (double)sizeof(cx<T>) * p->Nk);
       hipLaunchKernelGGL((k_zinv_plain<T, L, ZB>), dim3(zblocks_for(p, ZB), 1), dim3(kThreads), 0, p->stream, g,
                          spec_ptr<T>(p, 0), out_dev, p->N, 1.0, tw_ptr<T>(p, 2));
     });

@@ -46,7 +46,7 @@ namespace deft {
   } while (0)
 
 // kinds of preconditioned gradient the backend computes (the device's PgradKind)
-enum MinPgrad : int { MIN_PGRAD_NONE = 0, MIN_PGRAD_NEW = 1, MIN_PGRAD_OLD = 2 };
+enum MinPgrad : int { MIN_PGRAD_NONE = 0, MIN_PGRAD_NEW = 1, MIN_PGRAD_OLD = 2, MIN_PGRAD_SK = 3 };
 
 inline bool min_better(double a, double b) { return a < b || b != b; }
 
@@ -68,6 +68,7 @@ struct MinState {
   int iter = 0, maxiter = 10000000, miniter = 6;
   double precision = 0.0, relative_precision = 1e-15;
   bool use_precond = false;
+  int precond_kind = MIN_PGRAD_NEW;  // with use_precond: grad/n (the reference's) or MIN_PGRAD_SK
   double step = 0.1, orig_stepsize = 0.1, oldgradsqr = 1.0;
   double deltaE = 0.0, dEdn = 0.0, log_avg = 0.0, error_estimate = 0.0;
   bool have_dir = false, have_old = false;
@@ -139,7 +140,7 @@ int min_pgrad(MinState &m, Ops &ops) {
   const bool keep = !same && m.E_valid;
   if (m.use_precond) min_invalidate(m);
   double Eg;
-  DEFT_MIN_TRY(ops.gradient(min_grad_prec(m, pp), m.have_old, m.use_precond ? MIN_PGRAD_NEW : MIN_PGRAD_NONE, &Eg,
+  DEFT_MIN_TRY(ops.gradient(min_grad_prec(m, pp), m.have_old, m.use_precond ? m.precond_kind : MIN_PGRAD_NONE, &Eg,
                             &m.pgg, &m.pgo));
   if (same && (m.use_precond || !m.E_valid)) {
     m.E = Eg;

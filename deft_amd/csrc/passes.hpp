@@ -52,6 +52,7 @@ struct GridArgs {
   double b[9];    // reciprocal lattice vectors as rows: k = mx*b0 + my*b1 + mz*b2
   double R, six_dr2, invN;
   double kT, mu, logNQ, dV;
+  double sk_floor;  // structure-factor preconditioner: densities below keep grad/n
   WBConst wb;
 };
 
@@ -837,8 +838,26 @@ __global__ __launch_bounds__(kThreads) void k_zphi11(GridArgs g, cx<T>* __restri
 //   PGRAD_NEW  1   grad/n = -(dF/dn)/kT, the new API's (src/haskell/NewCode.hs:229-242)
 //   PGRAD_OLD  2   (dF/dn)/(Veff - kT), the old API's OfEffectivePotential
 //                  (src/EffectivePotentialToDensity.cpp:62-71)
+//   PGRAD_SK   3   the structure-factor preconditioner: this pass stores
+//                  grad/sqrt(n) (0 below sk_floor), which k_sk_finish turns
+//                  into n^-1/2 F^-1 S(k) F (grad/sqrt(n)) after the transforms
 // ---------------------------------------------------------------------------
-enum PgradKind : int { PGRAD_NONE = 0, PGRAD_NEW = 1, PGRAD_OLD = 2 };
+enum PgradKind : int { PGRAD_NONE = 0, PGRAD_NEW = 1, PGRAD_OLD = 2, PGRAD_SK = 3 };
+
+// pgrad of one point from dF/dn (finite Vext)
+__device__ __forceinline__ double point_pgrad(int precond, double dfdn, double n, double xv, double gr,
+                                              const GridArgs& g) {
+  switch (precond) {
+    case PGRAD_NEW:
+      return -dfdn / g.kT;
+    case PGRAD_OLD:
+      return dfdn / (xv - g.kT);
+    case PGRAD_SK:
+      return n >= g.sk_floor ? -dfdn * sqrt(n) / g.kT : 0.0;
+    default:
+      return gr;
+  }
+}
 
 template <typename T, int L, int ZB>
 __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T>* __restrict__ accs,
@@ -870,7 +889,7 @@ __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T
         const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu;
         const double dfdn = (g.kT * conv + local) * g.dV;
         gr = -dfdn * n / g.kT;
-        pg = precond == PGRAD_NEW ? -dfdn / g.kT : (precond == PGRAD_OLD ? dfdn / (xv - g.kT) : gr);
+        pg = point_pgrad(precond, dfdn, n, xv, gr, g);
       }
       grad[idx] = gr;
       pgrad[idx] = pg;
@@ -888,6 +907,82 @@ __global__ __launch_bounds__(kThreads) void k_zgrad(GridArgs g, const cx<T>* __r
                                                     double* __restrict__ partial, int pstride,
                                                     const cx<T>* __restrict__ tw) {
   zgrad_body<T, L, ZB>(blockIdx.x, g, accs, x, vext, oldgrad, have_old, precond, grad, pgrad, partial, pstride, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Structure-factor preconditioner (PGRAD_SK): pg = n^-1/2 F^-1 S(k) F n^-1/2 g
+// on points with n >= sk_floor, grad/n on the dilute ones, 0 on hard-wall and
+// constrained points.  S(k) = (kT/n_b)/(kT/n_b + H_ex(k)) is the uniform
+// fluid's structure factor, H_ex(k) = kT [P33 w3^2 + 2 P32 w3 w2 + P22 w2^2
+// + cv k^2 w3^2] with Phi's second derivatives at the bulk weighted
+// densities; in the uniform fluid the operator is (dV/kT) times the inverse
+// Hessian over Veff, and with S = 1 it is the new API's grad/n.
+// ---------------------------------------------------------------------------
+
+// S(k)/N on the padded half spectrum from the plan's weight table (w/N)
+static __global__ __launch_bounds__(kThreads) void k_sk_table(GridArgs g, double P33, double P32, double P22,
+                                                              double cv, double nb, double clamp,
+                                                              double* __restrict__ out) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  const double N = (double)g.N;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    const int kz = (int)(i % g.Nkz);
+    const long long r = i / g.Nkz;
+    const int y = (int)(r % g.Ny), x = (int)(r / g.Ny);
+    const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+    const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
+    const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
+    const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
+    const long long idx = r * g.ldk + kz;
+    const double2 w = reinterpret_cast<const double2*>(g.wtab)[idx];
+    const double a3 = w.x * N, a2 = w.y * N;
+    const double hex = g.kT * (P33 * a3 * a3 + 2.0 * P32 * a3 * a2 + P22 * a2 * a2 +
+                               cv * (kx * kx + ky * ky + kz_ * kz_) * a3 * a3);
+    double S = (g.kT / nb) / (g.kT / nb + hex);
+    if (!(S > 0.0) || !isfinite(S)) S = clamp;
+    S = fmin(fmax(S, 1.0 / clamp), clamp);
+    out[idx] = S * g.invN;
+  }
+}
+
+// spectrum *= S(k)/N
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spec_scale(GridArgs g, cx<T>* __restrict__ spec,
+                                                         const double* __restrict__ sk) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    const long long r = i / g.Nkz;
+    const long long idx = r * g.ldk + (i - r * g.Nkz);
+    spec[idx] = scale(spec[idx], (T)sk[idx]);
+  }
+}
+
+// pg = conv/sqrt(n) (dense), grad/n (dilute), 0 (wall, constrained), with the
+// pgrad.grad and pgrad.oldgrad partial sums; gridDim.x = the block count the
+// evaluation's reduction reads
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sk_finish(GridArgs g, const double* __restrict__ x,
+                                                        const T* __restrict__ vext, const double* __restrict__ grad,
+                                                        const double* __restrict__ oldgrad, int have_old,
+                                                        double* __restrict__ pgrad, double* __restrict__ partial,
+                                                        int pstride) {
+  double dots[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < g.N; i += (long long)gridDim.x * kThreads) {
+    const T ve = vext[i];
+    const double gr = grad[i];
+    double pg = 0.0;
+    if (isfinite(ve)) {
+      const double n = exp(-x[i] / g.kT);
+      if (n >= g.sk_floor)
+        pg = pgrad[i] / sqrt(n);
+      else if (n > 0.0)
+        pg = gr / n;
+    }
+    pgrad[i] = pg;
+    dots[0] += pg * gr;
+    if (have_old) dots[1] += pg * oldgrad[i];
+  }
+  block_sum_store<2>(dots, partial, SLOT_PGG, pstride, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

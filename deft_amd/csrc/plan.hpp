@@ -36,6 +36,7 @@ enum KernelId : int {
   K_YPASS1_INV,  // inverse y pass of the back-convolution
   K_GEN_LINE,    // line FFT passes of edges without fused kernels
   K_GEN_POINT,   // pointwise passes of edges without fused kernels
+  K_SK_FINISH,   // last pass of the structure-factor preconditioner
   NUM_FAMILIES
 };
 constexpr int NUM_KERNELS = 2 * NUM_FAMILIES;  // entry 2*family + (FP64 ? 1 : 0)
@@ -52,7 +53,7 @@ inline const char *kernel_name(int id) {
       "xconv_bwd_f32",    "xconv_bwd_f64",    "zgrad_f32", "zgrad_f64", "cg_update", "cg_update_",
       "cg_reset", "cg_reset_", "axpy", "axpy_", "reduce", "reduce_", "fft_seam_f32", "fft_seam_f64",
       "ypass5_fwd_f32", "ypass5_fwd_f64", "ypass1_inv_f32", "ypass1_inv_f64", "gen_line_f32", "gen_line_f64",
-      "gen_point_f32", "gen_point_f64"};
+      "gen_point_f32", "gen_point_f64", "sk_finish_f32", "sk_finish_f64"};
   return (id >= 0 && id < NUM_KERNELS) ? names[id] : "?";
 }
 
@@ -137,6 +138,11 @@ struct deft_plan_s {
   deft::LineSched sched[2][3] = {};
   double *gwork = nullptr;  // gwork_fields real FP64 fields of N values
   int gwork_fields = 0;
+  // structure-factor preconditioner: S(k)/N on the padded half spectrum (ks
+  // doubles), valid for the current mu, functional and radius while sk_ok
+  double *sktab = nullptr;
+  bool sk_ok = false;
+  double sk_nb = 0.0, sk_floor = 0.0;
 };
 
 namespace deft {
@@ -187,6 +193,7 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.mu = p->mu;
   a.logNQ = std::log(water_nQ(p->kT));
   a.dV = p->dV;
+  a.sk_floor = p->sk_floor;
   a.wb = make_wb_const(p->R, p->functional);
   return a;
 }

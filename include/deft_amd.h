@@ -100,10 +100,20 @@ typedef enum {
  *                                  energy_grad_and_precond (src/haskell/NewCode.hs:229-242)
  *   DEFT_PGRAD_EFFECTIVE_POTENTIAL (dF/dn)/(Veff - kT), what the old API's
  *                                  OfEffectivePotential::grad accumulates into outpgrad
- *                                  (src/EffectivePotentialToDensity.cpp:62-71) */
+ *                                  (src/EffectivePotentialToDensity.cpp:62-71)
+ *   DEFT_PGRAD_STRUCTURE_FACTOR    no counterpart in the reference:
+ *                                  n^-1/2 F^-1 S(k) F n^-1/2 grad with S(k) the
+ *                                  structure factor of the uniform fluid at the bulk
+ *                                  density of the plan's mu (deft_plan_bulk_density),
+ *                                  on points with n >= 1e-3 n_b; grad/n below that,
+ *                                  zero on hard-wall and constrained points.  In the
+ *                                  uniform fluid it is dV/kT times the inverse Hessian
+ *                                  over Veff; with S = 1 it is DEFT_PGRAD_NEW.  Costs one
+ *                                  extra forward and inverse 3-D FFT per gradient. */
 typedef enum {
   DEFT_PGRAD_NEW = 1,
-  DEFT_PGRAD_EFFECTIVE_POTENTIAL = 2
+  DEFT_PGRAD_EFFECTIVE_POTENTIAL = 2,
+  DEFT_PGRAD_STRUCTURE_FACTOR = 3
 } deft_pgrad_kind;
 
 typedef struct deft_plan_s *deft_plan;
@@ -174,6 +184,9 @@ int deft_plan_radix_schedule(deft_plan plan, int fft_precision, int axis, int *r
 int deft_plan_set_radix_schedule(deft_plan plan, int fft_precision, int axis, const int *radices, int nstages);
 int deft_plan_set_fft_precision(deft_plan plan, int fft_precision);
 int deft_plan_set_mu(deft_plan plan, double mu);
+/* The bulk density whose chemical potential (ideal gas + the plan's
+ * functional, deft_bulk_state) is the plan's mu. */
+int deft_plan_bulk_density(deft_plan plan, double *nb);
 /* One of the deft_functional_kind values, default DEFT_WHITE_BEAR; drops nothing else. */
 int deft_plan_set_functional(deft_plan plan, int functional);
 /* Vext (N doubles, host); kept on the device in FP64 for the FP64 passes and
@@ -283,6 +296,11 @@ int deft_minimize_set_relative_precision(deft_minimize m, double relative_precis
 int deft_minimize_set_maxiter(deft_minimize m, int maxiter);
 int deft_minimize_set_miniter(deft_minimize m, int miniter);
 int deft_minimize_precondition(deft_minimize m, int use_preconditioning);
+/* Which preconditioned gradient precondition(1) uses: DEFT_PGRAD_NEW (the
+ * reference's grad/n, default) or DEFT_PGRAD_STRUCTURE_FACTOR (no counterpart
+ * in the reference; plans outside batches).  The minimum is the same; the
+ * search directions, and so the iteration count, are not. */
+int deft_minimize_set_preconditioner(deft_minimize m, int pgrad_kind);
 /* Minimize::improve_energy: *more = 1 while the minimizer wants to continue. */
 int deft_minimize_improve_energy(deft_minimize m, int verbose, int *more);
 /* Loops improve_energy until it returns false or max_calls iterations ran. */

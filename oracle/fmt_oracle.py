@@ -35,6 +35,9 @@ It restates, in numpy/scipy FP64, the reference algorithm of deft
 * the new-API Polak-Ribiere conjugate gradient with quadratic
   line search, cached energies and the geometric error
   estimate                                                      (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
+* the structure-factor preconditioner of the product (no reference
+  counterpart): grad -> n^-1/2 F^-1 S(k) F n^-1/2 grad with the uniform
+  fluid's S(k); its pins are mathematical (tests/test_sk_precond.py)
 * the old-API minimizer stack: ConjugateGradient and
   PreconditionedConjugateGradient (pgrad (dF/dn)/(Veff - kT))
   with QuadraticLineMinimizer, Precision and MaxIter            (src/ConjugateGradient.cpp:59-171,
@@ -534,6 +537,97 @@ class WhiteBearFluidVeff:
         with np.errstate(divide="ignore", invalid="ignore"):
             pg = np.where(self.finite, g / (x - self.kT), 0.0)
         return -g * n / self.kT, pg
+
+
+# ---------------------------------------------------------------------------
+# Structure-factor preconditioner (no counterpart in the reference; the
+# product's DEFT_PGRAD_STRUCTURE_FACTOR).  In a uniform fluid at n_b the
+# Hessian of F over Veff is (n_b/kT) (kT/n_b + H_ex(k)) (n_b/kT) per
+# Fourier mode, H_ex(k) = kT [P33 w3^2 + 2 P32 w3 w2 + P22 w2^2 + cv k^2 w3^2]
+# with the second derivatives P_ab of Phi at the bulk weighted densities
+# and cv the n2v.n2v coefficient (the vector weight is -i k w3).  Its inverse
+# is D F^-1 S(k) F D with D = n^-1/2 and S(k) = (kT/n_b)/(kT/n_b + H_ex(k)),
+# the fluid's structure factor; with S = 1 it is the new API's grad/n.  On a
+# non-uniform fluid D takes the local density.  Points below 1e-3 n_b (the
+# dilute side of walls and soft repulsions) keep grad/n, where S ~ 1 and
+# 1/sqrt(n) would be large; hard-wall and constrained points get zero.  The
+# operator is symmetric positive definite, so Polak-Ribiere CG stays valid.
+# ---------------------------------------------------------------------------
+
+SK_FLOOR = 1e-3   # fraction of n_b below which a point keeps grad/n
+SK_CLAMP = 1e3    # S(k) is kept in [1/SK_CLAMP, SK_CLAMP]
+
+
+def phi_bulk_hessian(nb, R, functional="wb"):
+    """(P33, P32, P22, cv) at the bulk weighted densities n3 = 4/3 pi R^3 nb,
+    n2 = 4 pi R^2 nb, n2v = 0, by central differences (relative step 1e-5) of
+    the partials; cv = d(dPhi/dn2v_x)/dn2v_x."""
+    n3 = 4 * math.pi / 3 * R ** 3 * nb
+    n2 = 4 * math.pi * R * R * nb
+    fn = "wb" if functional == "wbt" else functional
+
+    def part(a3, a2, vx):
+        _, _, _, d = whitebear_phi(np.float64(a3), np.float64(a2), np.float64(vx), np.float64(0.0),
+                                   np.float64(0.0), R, functional=fn)
+        return [float(v) for v in d]
+
+    h3, h2, hv = 1e-5 * n3, 1e-5 * n2, 1e-5 * n2
+    p, m = part(n3 + h3, n2, 0.0), part(n3 - h3, n2, 0.0)
+    P33 = (p[0] - m[0]) / (2 * h3)
+    P32a = (p[1] - m[1]) / (2 * h3)
+    p, m = part(n3, n2 + h2, 0.0), part(n3, n2 - h2, 0.0)
+    P22 = (p[1] - m[1]) / (2 * h2)
+    P32b = (p[0] - m[0]) / (2 * h2)
+    p, m = part(n3, n2, hv), part(n3, n2, -hv)
+    cv = (p[2] - m[2]) / (2 * hv)
+    return P33, 0.5 * (P32a + P32b), P22, cv
+
+
+def structure_factor(grid, nb, R, kT=1.0, functional="wb"):
+    """S(k) of the uniform fluid on the half spectrum, with the grid's smeared weights."""
+    w3, w2, _ = weights(grid, R)
+    P33, P32, P22, cv = phi_bulk_hessian(nb, R, functional)
+    hex_ = kT * (P33 * w3 * w3 + 2 * P32 * w3 * w2 + P22 * w2 * w2 + cv * grid.k ** 2 * w3 * w3)
+    S = (kT / nb) / (kT / nb + hex_)
+    S = np.where(np.isfinite(S) & (S > 0), S, SK_CLAMP)
+    return np.clip(S, 1.0 / SK_CLAMP, SK_CLAMP)
+
+
+def sk_precondition(fluid, grad, S, nb, x=None):
+    """The structure-factor preconditioned gradient of ``grad`` (dF/dVeff) at ``x``."""
+    x = fluid.x if x is None else x
+    n = fluid.density(x)
+    dense = fluid.finite & (n >= SK_FLOOR * nb)
+    rn = np.where(dense, 1.0 / np.sqrt(np.where(dense, n, 1.0)), 0.0)
+    conv = _irfftn(S * _rfftn(grad * rn), fluid.grid.shape)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dilute = np.where(fluid.finite & ~dense & (n > 0), grad / np.where(n > 0, n, 1.0), 0.0)
+    return np.where(dense, rn * conv, dilute)
+
+
+class StructureFactorFluid:
+    """A fluid whose energy_grad_and_precond returns the structure-factor
+    preconditioned gradient (for the oracle's Minimize)."""
+
+    def __init__(self, fluid, nb):
+        self.fluid = fluid
+        self.nb = float(nb)
+        self.S = structure_factor(fluid.grid, nb, fluid.R, fluid.kT, fluid.functional)
+
+    def __getattr__(self, name):
+        return getattr(self.fluid, name)
+
+    @property
+    def x(self):
+        return self.fluid.x
+
+    @x.setter
+    def x(self, v):
+        self.fluid.x = v
+
+    def energy_grad_and_precond(self, x=None):
+        E, g, _ = self.fluid.energy_grad_and_precond(x)
+        return E, g, sk_precondition(self.fluid, g, self.S, self.nb, x)
 
 
 # ---------------------------------------------------------------------------
