@@ -76,7 +76,7 @@ static __global__ __launch_bounds__(kThreads) void k_contact_point(long long N, 
   const double fpR = 4 * pi * R, fpR2 = 4 * pi * R * R;
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long long)gridDim.x * kThreads) {
     const double n3 = w[i], n2 = w[N + i], vx = w[2 * N + i], vy = w[3 * N + i], vz = w[4 * N + i];
-    const double om = 1.0 - n3, io = 1.0 / om, L = log1p(-n3);
+    const double om = 1.0 - n3, io = 1.0 / om, L = log1m(n3);
     double g2, g2p, g3, g3p;
     if (s3 > 0.0) {
       markii_of(n3, io, L, s3, g2, g2p, g3, g3p);

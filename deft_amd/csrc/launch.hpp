@@ -391,6 +391,48 @@ static int sk_apply(deft_plan_s *p, const GridArgs &g, int nb, bool have_old) {
   return DEFT_OK;
 }
 
+// The structure-factor preconditioner fused into the passes of a fixed-edge
+// plan (k_zgrad_sk, the y passes, k_xsk, k_zsk_finish), after the
+// back-convolution's inverse y pass; nzb z blocks write the partial sums.
+template <typename T>
+static int sk_apply_fused(deft_plan_s *p, const GridArgs &g, int zb, int nzb, bool have_old) {
+  if (!p->sktab) return set_error(DEFT_ERR_STATE, "structure-factor table missing");
+  const int ti = prec_index<T>();
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  with_length(p->Nz, [&](auto LZ) {
+    constexpr int L = decltype(LZ)::value;
+    with_zb(zb, [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      ProfScope ps(p, kid<T>(K_ZGRAD), 2.0 * cb + 24.0 * p->N + (double)sizeof(T) * p->N);
+      hipLaunchKernelGGL((k_zgrad_sk<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0), p->x,
+                         vext_ptr<T>(p), p->g, tw_ptr<T>(p, 2));
+    });
+  });
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 0), false, 1, p->ty[ti]);
+  with_length(p->Nx, [&](auto LX) {
+    constexpr int L = decltype(LX)::value;
+    with_tb(p->tx[ti], [&](auto TBc) {
+      constexpr int TB = decltype(TBc)::value;
+      ProfScope ps(p, kid<T>(K_FFT_SEAM), 2.0 * cb + 8.0 * p->Nk);
+      hipLaunchKernelGGL((k_xsk<T, L, TB>), dim3(x_blocks<T>(p, TB)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), p->sktab, tw_ptr<T>(p, 0));
+    });
+  });
+  launch_ypass<T>(p, g, spec_ptr<T>(p, 0), true, 1, p->ty[ti]);
+  with_length(p->Nz, [&](auto LZ) {
+    constexpr int L = decltype(LZ)::value;
+    with_zb(zb, [&](auto ZBc) {
+      constexpr int ZB = decltype(ZBc)::value;
+      ProfScope ps(p, kid<T>(K_SK_FINISH), cb + (have_old ? 40.0 : 32.0) * p->N + (double)sizeof(T) * p->N);
+      hipLaunchKernelGGL((k_zsk_finish<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0),
+                         p->x, vext_ptr<T>(p), p->g, p->og, have_old ? 1 : 0, p->pg, p->partial, p->pstride,
+                         tw_ptr<T>(p, 2));
+    });
+  });
+  DEFT_HIP_CHECK(hipGetLastError());
+  return DEFT_OK;
+}
+
 template <typename T>
 int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
   int nzb = 0;
@@ -413,18 +455,21 @@ int Launch<T>::enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
     launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, plan_fields(p), p->ty[ti]);
     launch_xconv_bwd<T>(p, g, p->tx[ti]);
     launch_ypass<T>(p, g, spec_ptr<T>(p, 0), true, 1, p->ty[ti]);
-    with_length(p->Nz, [&](auto LZ) {
-      constexpr int L = decltype(LZ)::value;
-      with_zb(zb, [&](auto ZBc) {
-        constexpr int ZB = decltype(ZBc)::value;
-        ProfScope ps(p, kid<T>(K_ZGRAD),
-                     (double)sizeof(cx<T>) * p->Nk + (have_old ? 32.0 : 24.0) * p->N + (double)sizeof(T) * p->N);
-        hipLaunchKernelGGL((k_zgrad<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0), p->x,
-                           vext_ptr<T>(p), p->og, have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride,
-                           tw_ptr<T>(p, 2));
+    if (precond == PGRAD_SK) {
+      TRY_LAUNCH(sk_apply_fused<T>(p, g, zb, nzb, have_old));
+    } else {
+      with_length(p->Nz, [&](auto LZ) {
+        constexpr int L = decltype(LZ)::value;
+        with_zb(zb, [&](auto ZBc) {
+          constexpr int ZB = decltype(ZBc)::value;
+          ProfScope ps(p, kid<T>(K_ZGRAD),
+                       (double)sizeof(cx<T>) * p->Nk + (have_old ? 32.0 : 24.0) * p->N + (double)sizeof(T) * p->N);
+          hipLaunchKernelGGL((k_zgrad<T, L, ZB>), dim3(nzb), dim3(kThreads), 0, p->stream, g, spec_ptr<T>(p, 0),
+                             p->x, vext_ptr<T>(p), p->og, have_old ? 1 : 0, precond, p->g, p->pg, p->partial,
+                             p->pstride, tw_ptr<T>(p, 2));
+        });
       });
-    });
-    if (precond == PGRAD_SK) TRY_LAUNCH(sk_apply<T>(p, g, nzb, have_old));
+    }
   }
   DEFT_HIP_CHECK(hipGetLastError());
   return DEFT_OK;

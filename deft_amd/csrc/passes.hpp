@@ -985,6 +985,158 @@ __global__ __launch_bounds__(kThreads) void k_sk_finish(GridArgs g, const double
   block_sum_store<2>(dots, partial, SLOT_PGG, pstride, blockIdx.x);
 }
 
+// The same preconditioner fused into the passes of a fixed-edge plan (the
+// operations per element are those of the split path above, so the results
+// are identical; no real-space round trip between the passes):
+//   k_zgrad_sk     c2r of the back-convolution -> grad (stored) and
+//                  grad/sqrt(n) (0 below sk_floor) -> r2c along z, in place
+//   y pass         forward, one spectrum
+//   k_xsk          FFT along x, times S(k)/N, inverse FFT along x
+//   y pass         inverse, one spectrum
+//   k_zsk_finish   c2r -> pg as k_sk_finish, with the pgrad.grad and
+//                  pgrad.oldgrad partial sums
+template <typename T, int L, int ZB>
+__device__ __forceinline__ void zgrad_sk_body(int bx, const GridArgs& g, cx<T>* __restrict__ accs,
+                                              const double* __restrict__ x, const T* __restrict__ vext,
+                                              double* __restrict__ grad, const cx<T>* __restrict__ tw) {
+  constexpr int P = zpairs(L, ZB);
+  constexpr int NPT = 2 * P * L;
+  constexpr int PT = (NPT + kThreads - 1) / kThreads;
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[zbuf_size<T, L, P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const long long lines = (long long)g.Nx * g.Ny;
+  const long long line0 = (long long)bx * 2 * P;
+  load_c2r_packed<T, L, P>(accs, line0, lines, g.ldk, A, B);
+  cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  T val[PT];
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const int e = threadIdx.x + i * kThreads;
+    val[i] = (T)0;
+    if (e < NPT) {
+      const int l = e / L, z = e - l * L;
+      if (line0 + l < lines) {
+        const long long idx = (line0 + l) * L + z;
+        const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
+        const double conv = (double)((l & 1) ? v.im : v.re);
+        const double xv = x[idx];
+        const T ve = vext[idx];
+        double gr = 0.0, pg = 0.0;
+        if (isfinite(ve)) {
+          const double n = exp(-xv / g.kT);
+          const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu;
+          const double dfdn = (g.kT * conv + local) * g.dV;
+          gr = -dfdn * n / g.kT;
+          pg = point_pgrad(PGRAD_SK, dfdn, n, xv, gr, g);
+        }
+        grad[idx] = gr;
+        val[i] = (T)pg;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PT; ++i) {
+    const int e = threadIdx.x + i * kThreads;
+    if (e < NPT) {
+      const int l = e / L, z = e - l * L;
+      cx<T>* c = &R[(l >> 1) * SS + skew<T>(z)];
+      if (l & 1)
+        c->im = val[i];
+      else
+        c->re = val[i];
+    }
+  }
+  __syncthreads();
+  const cx<T>* Z = fft_lds<T, L, SS, false>(R, R == A ? B : A, P, tw);
+  store_r2c_halves<T, L, P>(Z, accs, line0, lines, g.ldk);
+}
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zgrad_sk(GridArgs g, cx<T>* __restrict__ accs,
+                                                       const double* __restrict__ x, const T* __restrict__ vext,
+                                                       double* __restrict__ grad, const cx<T>* __restrict__ tw) {
+  zgrad_sk_body<T, L, ZB>(blockIdx.x, g, accs, x, vext, grad, tw);
+}
+
+template <typename T, int L, int TB>
+__device__ __forceinline__ void xsk_body(int bx, const GridArgs& g, cx<T>* __restrict__ data,
+                                         const double* __restrict__ sk, const cx<T>* __restrict__ tw) {
+  constexpr int C = ccols<T>(L, TB);
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[C * SS];
+  __shared__ cx<T> B[C * SS];
+  const long long plane = (long long)g.Ny * g.ldk;
+  const int myc = threadIdx.x % C;
+  int cy, ckz;
+  bool cok;
+  const int ncol = x_tile_column<C>(g, bx, myc, cy, ckz, cok);  // cok == (myc < ncol)
+  const long long off = (long long)cy * g.ldk + ckz;
+  for (int e = threadIdx.x; e < C * L; e += kThreads) {
+    const int xx = e / C;
+    A[myc * SS + skew<T>(xx)] = load_or_zero(data, (long long)xx * plane + off, 0, cok);
+  }
+  __syncthreads();
+  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
+  for (int e = threadIdx.x; e < C * L; e += kThreads) {
+    const int xx = e / C;
+    if (cok) {
+      cx<T>* c = &S[myc * SS + skew<T>(xx)];
+      *c = scale(*c, (T)sk[(long long)xx * plane + off]);
+    }
+  }
+  __syncthreads();
+  const cx<T>* R = fft_lds<T, L, SS, true>(S, S == A ? B : A, ncol, tw);
+  for (int e = threadIdx.x; e < C * L; e += kThreads) {
+    const int xx = e / C;
+    if (cok) data[(long long)xx * plane + off] = R[myc * SS + skew<T>(xx)];
+  }
+}
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void k_xsk(GridArgs g, cx<T>* __restrict__ data, const double* __restrict__ sk,
+                                                  const cx<T>* __restrict__ tw) {
+  xsk_body<T, L, TB>(xcd_pair_tile(blockIdx.x, gridDim.x), g, data, sk, tw);
+}
+
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zsk_finish(GridArgs g, const cx<T>* __restrict__ spec,
+                                                         const double* __restrict__ x, const T* __restrict__ vext,
+                                                         const double* __restrict__ grad,
+                                                         const double* __restrict__ oldgrad, int have_old,
+                                                         double* __restrict__ pgrad, double* __restrict__ partial,
+                                                         int pstride, const cx<T>* __restrict__ tw) {
+  constexpr int P = zpairs(L, ZB);
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> A[zbuf_size<T, L, P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const long long lines = (long long)g.Nx * g.Ny;
+  const long long line0 = (long long)blockIdx.x * 2 * P;
+  load_c2r_packed<T, L, P>(spec, line0, lines, g.ldk, A, B);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  double dots[2] = {0.0, 0.0};
+  for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
+    const int l = e / L, z = e - l * L;
+    if (line0 + l < lines) {
+      const long long idx = (line0 + l) * L + z;
+      const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
+      const T ve = vext[idx];
+      const double gr = grad[idx];
+      double pg = 0.0;
+      if (isfinite(ve)) {
+        const double n = exp(-x[idx] / g.kT);
+        if (n >= g.sk_floor)
+          pg = (double)((l & 1) ? v.im : v.re) / sqrt(n);
+        else if (n > 0.0)
+          pg = gr / n;
+      }
+      pgrad[idx] = pg;
+      dots[0] += pg * gr;
+      if (have_old) dots[1] += pg * oldgrad[idx];
+    }
+  }
+  block_sum_store<2>(dots, partial, SLOT_PGG, pstride, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // CG vector kernels (FP64)
 // ---------------------------------------------------------------------------

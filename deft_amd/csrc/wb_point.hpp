@@ -57,6 +57,40 @@ inline WBConst make_wb_const(double R, int functional = 0) {
                  mark2 == 2 ? 1.0 : 1.5, R};
 }
 
+// log(1 - x) in FP64 for the pointwise kernels (the library's log1p costs
+// about 110 FP64 instructions per call).  1 - x = 2^k (1 + f) with 1 + f in
+// [sqrt(1/2), sqrt(2)); for |x| < 1 - sqrt(1/2) this is k = 0 and f = -x
+// exactly, so small arguments keep full relative accuracy (no 1 - x rounding).
+// log(1 + f) = 2 atanh(s), s = f/(2 + f), is evaluated as f - s (f - R) with
+// R = 2s^2/3 + 2s^4/5 + ... + 2s^20/21 (the atanh Taylor series; |s| <= 0.1716,
+// so the first omitted term is below 2e-17 relative), keeping the exact f as
+// the leading term.  k ln 2 is added with ln 2 split in two, the high part
+// having 32 significant bits so that k ln2_hi is exact.  x = 1 gives -inf,
+// x > 1 and NaN give NaN.  Error within 2 ulp of log1p(-x) over the range the
+// kernels use (pinned against numpy in the host tests).
+__host__ __device__ inline double log1m(double x) {
+#pragma clang fp contract(off)
+  constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 0x3fe62e42fee00000
+  constexpr double kLn2Lo = 1.90821492927058770002e-10;  // ln 2 - kLn2Hi
+  const double u = 1.0 - x;
+  int k = 0;
+  double m = frexp(u, &k);  // u = m 2^k, m in [1/2, 1)
+  const bool low = m < 0.70710678118654752440;
+  m = low ? 2.0 * m : m;
+  k = low ? k - 1 : k;
+  const bool near = fabs(x) < 0.29289321881345247560;  // 1 - sqrt(1/2)
+  const double f = near ? -x : m - 1.0;
+  const double kk = near ? 0.0 : (double)k;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double P = 2.0 / 21.0;
+#pragma unroll
+  for (int j = 8; j >= 0; --j) P = fma(P, z, 2.0 / (2 * j + 3));
+  const double lg = f - s * (f - z * P);
+  const double r = fma(kk, kLn2Hi, fma(kk, kLn2Lo, lg));
+  return u > 0.0 ? r : (u == 0.0 ? -__builtin_huge_val() : __builtin_nan(""));
+}
+
 constexpr double kWB2SeriesCutoff = 0.05;
 constexpr int kWB2SeriesTerms = 24;
 
@@ -67,22 +101,24 @@ __host__ __device__ inline void markii_of(double n, double io, double L, double 
 #pragma clang fp contract(off)
   double S2, S2p, S3, S3p;
   if (fabs(n) < kWB2SeriesCutoff) {
-    S2 = 0.0;
-    S2p = 0.0;
-    double p = n;  // n^(m-2)
-    for (int m = 3; m < kWB2SeriesTerms + 3; ++m) {
-      S2p += 6.0 * p / m;
-      S2 += 6.0 * (p * n) / (m * (m - 1));
-      p *= n;
+    // S2 = sum_{m>=3} 6 n^(m-1)/(m(m-1)), S3 = n - sum_{m>=4} 3 n^(m-2)/(m(m-1)(m-2)),
+    // Horner in n with explicit FMAs (terms m < kWB2SeriesTerms + 3)
+    double a = 0.0, ap = 0.0, b = 0.0, bp = 0.0;
+#pragma unroll
+    for (int m = kWB2SeriesTerms + 2; m >= 3; --m) {
+      a = fma(a, n, 6.0 / (m * (m - 1.0)));         // sum 6 n^(m-3)/(m(m-1))
+      ap = fma(ap, n, 6.0 / m);                      // sum 6 n^(m-3)/m
     }
-    S3 = n;
-    S3p = 1.0;
-    double q = n;  // n^(m-3)
-    for (int m = 4; m < kWB2SeriesTerms + 4; ++m) {
-      S3p -= 3.0 * q / (m * (m - 1));
-      S3 -= 3.0 * (q * n) / (m * (m - 1) * (m - 2));
-      q *= n;
+#pragma unroll
+    for (int m = kWB2SeriesTerms + 3; m >= 4; --m) {
+      b = fma(b, n, 3.0 / (m * (m - 1.0) * (m - 2.0)));  // sum 3 n^(m-4)/(m(m-1)(m-2))
+      bp = fma(bp, n, 3.0 / (m * (m - 1.0)));            // sum 3 n^(m-4)/(m(m-1))
     }
+    const double n2_ = n * n;
+    S2 = a * n2_;
+    S2p = ap * n;
+    S3 = n - b * n2_;
+    S3p = 1.0 - bp * n;
   } else {
     const double om = 1.0 - n;
     const double N2 = 6.0 * n - 3.0 * n * n + 6.0 * om * L;
@@ -103,7 +139,7 @@ __host__ __device__ inline void markii_of(double n, double io, double L, double 
 
 constexpr double kInv36Pi = 1.0 / (36.0 * 3.14159265358979323846);
 constexpr double kF3SeriesCutoff = 0.05;
-constexpr int kF3SeriesTerms = 18;
+constexpr int kF3SeriesTerms = 14;
 
 // f3 and f3' given om = 1-n, io = 1/(1-n) and L = ln(1-n).
 __host__ __device__ inline void f3_of(double n, double om, double io, double L, double& f3, double& df3) {
@@ -111,14 +147,15 @@ __host__ __device__ inline void f3_of(double n, double om, double io, double L, 
   // round the energy density identically (the line search compares them)
 #pragma clang fp contract(off)
   if (fabs(n) < kF3SeriesCutoff) {
-    double g = 1.5, gp = 0.0, p = 1.0;  // p = n^(k-3)
+    // g = 3/2 - sum_{k>=3} c_k n^(k-2), g' = -sum c_k (k-2) n^(k-3), c_k = 2/(k(k-1)(k-2)),
+    // Horner in n with explicit FMAs
+    double t = 0.0, tp = 0.0;
 #pragma unroll
-    for (int k = 3; k < kF3SeriesTerms + 3; ++k) {
-      const double c = 2.0 / (double(k) * (k - 1) * (k - 2));
-      gp -= c * (k - 2) * p;
-      p = p * n;
-      g -= c * p;
+    for (int k = kF3SeriesTerms + 2; k >= 3; --k) {
+      t = fma(t, n, 2.0 / (k * (k - 1.0) * (k - 2.0)));
+      tp = fma(tp, n, 2.0 / (k * (k - 1.0)));
     }
+    const double g = fma(-n, t, 1.5), gp = -tp;
     const double io2 = io * io;
     f3 = g * io2;
     df3 = (gp + 2.0 * g * io) * io2;
@@ -134,7 +171,7 @@ __host__ __device__ inline void f3_of(double n, double om, double io, double L, 
 
 __host__ __device__ inline void f3_and_derivative(double n, double& f3, double& df3) {
   const double om = 1.0 - n;
-  f3_of(n, om, 1.0 / om, log1p(-n), f3, df3);
+  f3_of(n, om, 1.0 / om, log1m(n), f3, df3);
 }
 
 // Energy density terms only; M2 selects the Mark II factors at compile time
@@ -147,7 +184,7 @@ __host__ __device__ inline void wb_phi_k(double n3, double n2, double vx, double
 #pragma clang fp contract(off)
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
-  const double L = log1p(-n3);
+  const double L = log1m(n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3, g2 = io;
   if constexpr (M2) {
@@ -169,7 +206,7 @@ __host__ __device__ inline void wb_phi_partials_k(double n3, double n2, double v
                                                   double* d) {
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
-  const double L = log1p(-n3);
+  const double L = log1m(n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
   const double n2s = n2 * n2;
@@ -220,7 +257,7 @@ __host__ __device__ inline void wbt_phi(const double* v, const WBConst& c, doubl
   const double txx = v[5], tyy = v[6], tzz = v[7], txy = v[8], tyz = v[9], tzx = v[10];
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
-  const double L = log1p(-n3);
+  const double L = log1m(n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
   f3_of(n3, om, io, L, f3, df3);
@@ -243,7 +280,7 @@ __host__ __device__ inline void wbt_phi_partials(const double* v, const WBConst&
   const double txx = v[5], tyy = v[6], tzz = v[7], txy = v[8], tyz = v[9], tzx = v[10];
   const double om = 1.0 - n3;
   const double io = 1.0 / om;
-  const double L = log1p(-n3);
+  const double L = log1m(n3);
   const double v2 = vx * vx + vy * vy + vz * vz;
   double f3, df3;
   f3_of(n3, om, io, L, f3, df3);

@@ -1,7 +1,8 @@
 // Host build of the HIP path's pointwise White-Bear code (deft_amd/csrc/wb_point.hpp):
 // reads lines "functional v..." on stdin (5 weighted densities n3 n2 vx vy vz, or 11
 // for the tensor functional 3: ... Txx Tyy Tzz Txy Tyz Tzx) and prints p1 p2 p3, the
-// partials and the energy-only p1 p2 p3 with 17 significant digits.
+// partials and the energy-only p1 p2 p3 with 17 significant digits.  A line "-1 x"
+// prints log1m(x) = log(1 - x) instead.
 #include <cstdio>
 
 #include "../../deft_amd/csrc/wb_point.hpp"
@@ -9,6 +10,12 @@
 int main() {
   int fn;
   while (std::scanf("%d", &fn) == 1) {
+    if (fn == -1) {
+      double x;
+      if (std::scanf("%lf", &x) != 1) return 1;
+      std::printf("%.17g\n", deft::log1m(x));
+      continue;
+    }
     const int nw = fn == 3 ? 11 : 5;
     double v[11];
     for (int i = 0; i < nw; ++i)

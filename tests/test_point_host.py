@@ -62,3 +62,26 @@ def test_tensor_point_code_matches_oracle(harness):
     got = np.array([[float(x) for x in line.split()] for line in out.splitlines()])
     p1, p2, p3, P = O.whitebear_tensor_phi(n3, n2, v[:, 0], v[:, 1], v[:, 2], *t.T, 1.0)
     _check(got, np.stack([p1, p2, p3, *P], axis=1))
+
+
+def test_log1m_matches_log1p(harness):
+    """log1m(x) = log(1 - x), the kernels' replacement for log1p(-x): within 2 ulp of
+    numpy's log1p over small, moderate and near-1 arguments, both sides of the k = 0
+    region's edge |x| = 1 - sqrt(1/2), negative arguments, and the special values."""
+    rng = np.random.default_rng(11)
+    edge = 1 - np.sqrt(0.5)
+    x = np.concatenate([
+        rng.uniform(-0.05, 0.05, 400), rng.uniform(0.05, 0.999, 400), rng.uniform(-3.0, 0.0, 200),
+        10.0 ** rng.uniform(-300, -1, 300), -(10.0 ** rng.uniform(-300, -1, 200)),
+        1 - 10.0 ** rng.uniform(-16, -1, 200), -(10.0 ** rng.uniform(0, 300, 100)),
+        np.nextafter(edge, 0) * np.array([1, -1]), np.nextafter(edge, 1) * np.array([1, -1]),
+        [0.0, -0.0, 5e-324, 0.5, 0.75, 1 - 2.0 ** -53, np.nextafter(1.0, 0.0)]])
+    text = "".join("-1 %.17g\n" % v for v in x)
+    out = subprocess.run([harness], input=text.encode(), stdout=subprocess.PIPE, check=True).stdout.decode()
+    got = np.array([float(v) for v in out.split()])
+    want = np.log1p(-x)
+    ulp = np.spacing(np.abs(want))
+    assert np.max(np.abs(got - want) / np.maximum(ulp, 5e-324)) <= 2.0
+    sp = "-1 1\n-1 1.5\n-1 nan\n-1 2\n"
+    out = subprocess.run([harness], input=sp.encode(), stdout=subprocess.PIPE, check=True).stdout.decode().split()
+    assert float(out[0]) == -np.inf and all(np.isnan(float(v)) for v in out[1:])
