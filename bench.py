@@ -176,6 +176,75 @@ def c3_run(count=64, seed=12345, reps=2):
             "all_converged": all(b[2] == "converged" for b in bat)}
 
 
+def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
+    """The learned warm start on held-out potentials (C4): `count` potentials from the
+    training generator (deft_amd.nn.training_vext: hard spheres, slabs and Gaussian
+    bumps, or the C3 Gaussians and wall) at 64^3, dx = 0.1, eta ~ U(0.1, 0.45), drawn
+    from a seed the training never used; each minimized from the plain (bulk) start and
+    from the network's start, in batches of `chunk`, with the headline's ladder and
+    precision rule and the reference's grad/n preconditioner (the one batches run).
+    Times count the start (bulk fill or network inference) and the minimization."""
+    rng = np.random.default_rng(seed)
+    cases = [(NN.training_vext(M, L, rng), float(rng.uniform(0.1, 0.45))) for _ in range(count)]
+    net = NN.WarmStartNet.load()
+    fluids = [D.WhiteBearFluidVeff((M, M, M), np.diag([L] * 3), R=R, kT=KT, mu=0.0, fft_precision=D.FFT_FP32)
+              for _ in range(chunk)]
+    out = {k: {"iterations": [], "fp32_iterations": [], "seconds": 0.0, "converged": 0, "energies": []}
+           for k in ("plain", "warm")}
+    for c0 in range(0, count, chunk):
+        part = cases[c0:c0 + chunk]
+        fl = fluids[:len(part)]
+        precs, nbs = [], []
+        for f, (V, eta) in zip(fl, part):
+            nb = eta / (4 * math.pi / 3 * R ** 3)
+            f.mu = D.find_chemical_potential(nb, R, KT)
+            f.Vext = V
+            nbs.append(nb)
+            precs.append(abs(D.pressure(nb, R, KT) * L ** 3) * PRECISION_FACTOR)
+        for start in ("plain", "warm"):
+            t0 = time.perf_counter()
+            for f, (V, eta), nb in zip(fl, part, nbs):
+                f.fft_precision = D.FFT_FP32
+                if start == "warm":
+                    net.warm_start(f, eta)
+                else:
+                    f.fill_Veff(-KT * math.log(nb))
+            batch = D.Batch(fl)  # after the warm starts: their safety check evaluates outside a batch
+            ms = []
+            for f, prec in zip(fl, precs):
+                m = D.Minimize(f)
+                m.precondition(True)
+                m.set_precision(prec)
+                m.set_maxiter(1000)
+                m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
+                ms.append(m)
+            batch.minimize(ms)
+            out[start]["seconds"] += time.perf_counter() - t0
+            for m in ms:
+                st = m.stats()
+                out[start]["iterations"].append(st["iter"])
+                out[start]["fp32_iterations"].append(m.phase_iterations()[0])
+                out[start]["converged"] += st["stop_reason"] == "converged"
+                out[start]["energies"].append(m.energy())
+                m.close()
+            batch.close()
+    net.close()
+    for f in fluids:
+        f.close()
+    res = {"workload": f"{count} held-out potentials of the training generator at {M}^3, dx = {L / M:g}, "
+                       f"eta ~ U(0.1, 0.45), seed {seed}; batches of {chunk}; grad/n preconditioner"}
+    for k in ("plain", "warm"):
+        it = np.array(out[k]["iterations"])
+        res[k] = {"mean_iterations": float(it.mean()), "median_iterations": float(np.median(it)),
+                  "mean_fp32_iterations": float(np.mean(out[k]["fp32_iterations"])),
+                  "minimizations_per_s": count / out[k]["seconds"], "converged": out[k]["converged"]}
+    ep, ew = np.array(out["plain"]["energies"]), np.array(out["warm"]["energies"])
+    res["iteration_ratio_plain_over_warm"] = res["plain"]["mean_iterations"] / res["warm"]["mean_iterations"]
+    res["time_ratio_plain_over_warm"] = res["warm"]["minimizations_per_s"] / res["plain"]["minimizations_per_s"]
+    res["max_dF_over_F_between_starts"] = float(np.max(np.abs(ew - ep) / np.abs(ep)))
+    return res
+
+
 def rocprof_instance(family, tuning):
     """rocprof name of the kernel instance behind a profiler family at the autotuned
     variants (`tuning` as returned by autotune()); None when there is no single one."""
@@ -301,6 +370,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=float, default=15.0)
     ap.add_argument("--no-c3", action="store_true", help="skip the 64-grid batch measurement")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 512 held-out warm-start measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -338,6 +408,7 @@ def main():
     path_bytes = whole_path_bytes(prof, args.steps)
     path_gbps = path_bytes / (w["elapsed"] / args.steps) / 1e9
     c3 = None if args.no_c3 else c3_run()
+    c4 = None if args.no_c4 else c4_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
     def block(run, start):
         r = run["results"][-1]
@@ -370,6 +441,7 @@ def main():
                    "structure_factor_over_reference_preconditioner": {"warm": value / ref_value,
                                                                       "plain": plain_value / ref_plain_value},
                    "c3_batch": c3,
+                   "c4_warm_start_heldout": c4,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname, runs["tuning"]),
@@ -377,7 +449,13 @@ def main():
                                        "copy-calibrated, bytes per dispatch)",
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
-                     "kernel_time_fraction_of_wall": total_ms * 1e-3 / runs["profiled_elapsed"],
+                     # the plan's kernel time per minimization (HIP events; they add ~1-3% to each
+                     # kernel) over the wall time of the minimizations without events (network
+                     # inference excluded from both); the event-timed steps themselves run ~40%
+                     # longer, so their own wall time would understate the fraction
+                     "kernel_time_fraction_of_wall": min(1.0, total_ms / args.steps / (1e3 * np.mean(
+                         [r["seconds"] - r["start_seconds"] for r in w["results"]]))),
+                     "kernel_ms_per_step": total_ms / args.steps,
                      "streaming_copy_GBps": copy_gbps, "frac_of_streaming_copy": achieved / copy_gbps,
                      "whole_path": {"bytes_per_minimization": path_bytes, "achieved_GBps": path_gbps,
                                     "frac": path_gbps / HBM_PEAK_GBS,

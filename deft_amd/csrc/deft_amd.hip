@@ -267,6 +267,13 @@ struct DeviceOps {
   }
   void move(double s) { queue_move(p, s); }
   double newton_scale() const { return p->kT / p->dV; }
+  // the structure-factor pgrad: plans outside batches whose mu has a bulk density
+  bool sk_available() {
+    if (p->batch) return false;
+    if (plan_ensure_sk(p) == DEFT_OK) return true;
+    g_last_error.clear();
+    return false;
+  }
 };
 
 int improve_energy_mixed(deft_minimize_s *m, int verbose, int *more) {

@@ -15,6 +15,11 @@
 
 namespace deft {
 
+// Every kernel that runs these stages launches blocks of kBlockThreads
+// threads; the stage loops stride by the constant rather than by blockDim.x
+// (which the compiler would read from the dispatch packet in every loop).
+constexpr int kBlockThreads = 256;
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cx {
   T re, im;
@@ -204,11 +209,11 @@ constexpr bool supported_length(int L) {
 // One radix-R stage: sub-transforms of length NS are combined into length NS*R.
 template <typename T, int L, int SS, int R, int NS, bool INV>
 __device__ __forceinline__ void stockham_stage(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int nseq,
-                                               const cx<T>* __restrict__ tw) {
+                                               const cx<T>* __restrict__ tw, int stride) {
   constexpr int M = L / R;
   constexpr int TWS = L / (NS * R);
   const int total = nseq * M;
-  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+  for (int b = threadIdx.x; b < total; b += stride) {
     const int s = b / M;
     const int j = b - s * M;
     const int k = j % NS;
@@ -237,11 +242,11 @@ __device__ __forceinline__ void stockham_stage(const cx<T>* __restrict__ src, cx
 // stockham_stage.
 template <typename T, int L, int SS, int R, int NS, bool INV>
 __device__ __forceinline__ void stockham_stage_direct(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst,
-                                                      int nseq, const cx<T>* __restrict__ tw) {
+                                                      int nseq, const cx<T>* __restrict__ tw, int stride) {
   constexpr int M = L / R;
   constexpr int TWS = L / (NS * R);
   const int total = nseq * L;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = threadIdx.x; e < total; e += stride) {
     const int s = e / L;
     const int rem = e - s * L;
     const int q = rem / M;
@@ -266,25 +271,41 @@ __device__ __forceinline__ void stockham_stage_direct(const cx<T>* __restrict__ 
 }
 
 template <typename T, int L, int SS, int NS, bool INV>
-__device__ __forceinline__ void stockham_run(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void stockham_run(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw,
+                                             int stride) {
   if constexpr (NS < L) {
     constexpr int R = pick_radix(L / NS);
     if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
-      stockham_stage<T, L, SS, R, NS, INV>(a, b, nseq, tw);
+      stockham_stage<T, L, SS, R, NS, INV>(a, b, nseq, tw, stride);
     else
-      stockham_stage_direct<T, L, SS, R, NS, INV>(a, b, nseq, tw);
+      stockham_stage_direct<T, L, SS, R, NS, INV>(a, b, nseq, tw, stride);
     __syncthreads();
-    stockham_run<T, L, SS, NS * R, INV>(b, a, nseq, tw);
+    stockham_run<T, L, SS, NS * R, INV>(b, a, nseq, tw, stride);
   }
+}
+
+// Copies the length-L twiddle table into LDS (one static array per T and L)
+// and returns it; the caller's next __syncthreads() makes it visible.  The
+// stages of a pass then read their twiddles from LDS instead of from global
+// memory, whose latency every stage would otherwise wait out.
+template <typename T, int L>
+__device__ __forceinline__ const cx<T>* lds_twiddles(const cx<T>* __restrict__ tw) {
+  __shared__ cx<T> t[L];
+  for (int i = threadIdx.x; i < L; i += kBlockThreads) t[i] = tw[i];
+  return t;
 }
 
 // Transforms the nseq sequences held in `a` (stride SS, elements at skew(i));
 // `b` is scratch of the same size.  Returns the buffer holding the result.  All threads of the block
 // must call it; the data in `a` must be visible (a __syncthreads() before).
+// `stride` is the loop stride over butterflies, kBlockThreads unless the
+// caller passes blockDim.x to keep the compiler from scheduling on a known
+// trip count (see zphi_body).
 template <typename T, int L, int SS, bool INV>
-__device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, int nseq, const cx<T>* __restrict__ tw,
+                                          int stride = kBlockThreads) {
   static_assert(supported_length(L), "FFT length has a prime factor above kMaxDirectRadix");
-  stockham_run<T, L, SS, 1, INV>(a, b, nseq, tw);
+  stockham_run<T, L, SS, 1, INV>(a, b, nseq, tw, stride);
   return (num_stages<L>() % 2 == 0) ? a : b;
 }
 

@@ -148,7 +148,7 @@ __device__ __forceinline__ void stage_rt(const cx<T>* __restrict__ src, cx<T>* _
   const int M = L / R;
   const int TWS = L / (NS * R);
   const int total = nseq * M;
-  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+  for (int b = threadIdx.x; b < total; b += kBlockThreads) {
     const int s = b / M;
     const int j = b - s * M;
     const int k = j % NS;
@@ -178,7 +178,7 @@ __device__ __forceinline__ void stage_direct_rt(const cx<T>* __restrict__ src, c
   const int M = L / R;
   const int TWS = L / (NS * R);
   const int total = nseq * L;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = threadIdx.x; e < total; e += kBlockThreads) {
     const int s = e / L;
     const int rem = e - s * L;
     const int q = rem / M;

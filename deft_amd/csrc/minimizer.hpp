@@ -16,6 +16,7 @@
 //   int  cg_reset(double *gd);                    d = -grad, gd = grad.d
 //   void move(double s);                          x += s d
 //   double newton_scale() const;                  kT/dV (see grad_error below)
+//   bool sk_available();                          whether gradient() takes MIN_PGRAD_SK
 //
 // Two minimizers live here:
 //   - the new API's Minimize (src/new/Minimize.cpp:9-313, src/new/Minimize.h:13-199)
@@ -247,7 +248,9 @@ int improve_energy(MinState &m, Ops &ops, int verbose, int *more) {
       step1 *= 2;
       ops.move(step1);
       DEFT_MIN_TRY(min_energy(m, ops, &E));
-    } while (E == E0);
+      // (the reference's loop has no exit when step1 is 0: a zero step never
+      // changes the energy; stop doubling there, and at an infinite step)
+    } while (E == E0 && step1 != 0 && std::isfinite(step1));
     if (E > E0) {
       ops.move(-step1);
       min_invalidate(m);
@@ -327,12 +330,15 @@ int improve_energy(MinState &m, Ops &ops, int verbose, int *more) {
 }
 
 // Restarts the CG direction from steepest descent at the step of the last
-// iteration that clearly lowered the energy.
+// iteration that clearly lowered the energy; without one (a start so close to
+// the minimum that no iteration's decrease stood clear of the round-off), at
+// the reference's first step 0.001 |E0| / slope (orig_stepsize selects it):
+// the step the line searches left behind has collapsed to round-off size.
 inline void min_restart_direction(MinState &m) {
   m.have_dir = false;
   m.have_old = false;
   m.oldgradsqr = 1.0;
-  if (m.good_step > 0) m.step = m.good_step;
+  m.step = m.good_step > 0 ? m.good_step : m.orig_stepsize;
 }
 
 // Newton estimate of the energy still to be gained from the current point:
@@ -342,6 +348,11 @@ inline void min_restart_direction(MinState &m) {
 template <class Ops>
 double grad_error(const MinState &m, const Ops &ops) {
   return 0.5 * ops.newton_scale() * m.pgg;
+}
+
+inline bool min_grad_check_on(const MinState &m) {
+  const double factor = m.grad_check < 0 ? kDefaultGradCheck : m.grad_check;
+  return factor > 0 && m.use_precond;
 }
 
 // The ladder's final verdict: a stop that the ladder would report as
@@ -355,12 +366,24 @@ int min_confirm_by_gradient(MinState &m, Ops &ops, int verbose, int *verdict) {
   if (!(factor > 0) || !m.use_precond) return DEFT_OK;
   const double keepE = m.E;
   const bool keepEv = m.E_valid;
-  DEFT_MIN_TRY(min_pgrad(m, ops));  // cached when the fused line search evaluated it
+  double ge;
+  if (m.precond_kind != MIN_PGRAD_SK && ops.sk_available()) {
+    // The structure-factor pgrad gives the Newton estimate with the bulk
+    // fluid's Hessian (exact in the uniform fluid); grad/n's ideal-gas Hessian
+    // is 1/S(0) times too soft at long wavelengths (15x at eta = 0.35, 24x at
+    // 0.40), which exceeds the factor below at minima the energy has reached.
+    double Eg, pgg, pgo;
+    DEFT_MIN_TRY(ops.gradient(min_grad_prec(m, ops.plan_prec()), false, MIN_PGRAD_SK, &Eg, &pgg, &pgo));
+    m.G_valid = false;  // the backend's pgrad is the structure-factor one now
+    ge = 0.5 * ops.newton_scale() * pgg;
+  } else {
+    DEFT_MIN_TRY(min_pgrad(m, ops));  // cached when the fused line search evaluated it
+    ge = grad_error(m, ops);
+  }
   if (keepEv) {
     m.E = keepE;  // the verdict's energy (same point, same precision: bit-identical)
     m.E_valid = true;
   }
-  const double ge = grad_error(m, ops);
   m.last_grad_error = ge;
   const double target = std::max(m.precision, m.relative_precision * std::fabs(m.E));
   if (ge <= factor * target) return DEFT_OK;
@@ -402,10 +425,16 @@ int improve_energy_mixed(MinState &m, Ops &ops, int verbose, int *more) {
   // the verdict of the iteration before it, as the reference would have
   // stopped there; so does the first iteration after a collapse restart.
   // Either way the gradient has the last word below.
+  // A final-phase line search that can no longer lower the FP64 energy (it
+  // rose, did not change, or met negative curvature at round-off level) also
+  // goes to the gradient when the previous iteration had not met the
+  // precision yet: the FP32 iterates can reach the round-off floor one
+  // iteration before the energy-based estimate does.  Without the gradient
+  // confirmation such a stop stays unconverged.
   bool candidate = false;  // a final-phase stop the ladder reports as converged
-  if (final_phase && met && !*more &&
-      (m.stop == DEFT_STOP_ENERGY_ROSE || m.stop == DEFT_STOP_NO_CHANGE ||
-       m.stop == DEFT_STOP_NEGATIVE_CURVATURE)) {
+  const bool stalled = m.stop == DEFT_STOP_ENERGY_ROSE || m.stop == DEFT_STOP_NO_CHANGE ||
+                       m.stop == DEFT_STOP_NEGATIVE_CURVATURE;
+  if (final_phase && !*more && stalled && (met || min_grad_check_on(m))) {
     m.stop = DEFT_STOP_CONVERGED;
     candidate = true;
   } else if (!*more && final_phase && m.stop == DEFT_STOP_CONVERGED) {
@@ -463,7 +492,10 @@ int improve_energy_mixed(MinState &m, Ops &ops, int verbose, int *more) {
   // A phase usually ends on a line search that ran into the round-off of its
   // energies and shrank the step to nothing; the next phase starts from the
   // step of the last iteration that clearly lowered the energy.
-  if (m.good_step > 0 && std::fabs(m.step) < 1e-3 * std::fabs(m.good_step)) m.step = m.good_step;
+  if (m.good_step > 0 && std::fabs(m.step) < 1e-3 * std::fabs(m.good_step))
+    m.step = m.good_step;
+  else if (m.good_step <= 0)
+    m.step = m.orig_stepsize;  // no clear decrease yet: the reference's first step
   if (m.g_prec != old_g) {
     m.have_dir = false;
     m.have_old = false;

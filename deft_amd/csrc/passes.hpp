@@ -27,7 +27,7 @@
 
 namespace deft {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = kBlockThreads;
 
 // reduction slots of the per-block partial-sum buffer
 enum Slot : int {
@@ -261,6 +261,7 @@ __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, dou
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[P * SS];
   __shared__ cx<T> B[P * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -294,7 +295,7 @@ __device__ __forceinline__ void zfwd_density_body(int bx, const GridArgs& g, dou
       c->re = (T)nv;
   }
   __syncthreads();
-  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, twl);
   store_r2c_halves<T, L, P>(Z, out, line0, lines, g.ldk);
   block_sum_store<3>(acc, partial, SLOT_IDEAL, pstride, bx);
 }
@@ -316,6 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_plain(GridArgs g, const doubl
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[P * SS];
   __shared__ cx<T> B[P * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_zfwd_plain(GridArgs g, const doubl
       c->re = v;
   }
   __syncthreads();
-  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+  const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, twl);
   store_r2c_halves<T, L, P>(Z, out, line0, lines, g.ldk);
 }
 
@@ -342,10 +344,11 @@ __global__ __launch_bounds__(kThreads) void k_zinv_plain(GridArgs g, const cx<T>
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   load_c2r_packed<T, L, P>(in + blockIdx.y * g.ks, line0, lines, g.ldk, A, B);
-  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl);
   double* o = out + blockIdx.y * fstride_out;
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
@@ -375,6 +378,7 @@ __device__ __forceinline__ void ypass_body(int bx, int by, const GridArgs& g, cx
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const int nfull = g.Nkz / C;
   const int ntail = g.Nkz - nfull * C;
   const int nmain = g.Nx * nfull;
@@ -399,7 +403,7 @@ __device__ __forceinline__ void ypass_body(int bx, int by, const GridArgs& g, cx
     A[myc * SS + skew<T>(y)] = load_or_zero(base, off_c + (long long)y * g.ldk, (long long)y * g.ldk, myc < nseq);
   }
   __syncthreads();
-  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, nseq, tw);
+  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, nseq, twl);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int y = e / C;
     if (myc < nseq) base[off_c + (long long)y * g.ldk] = R[myc * SS + skew<T>(y)];
@@ -420,6 +424,7 @@ __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __r
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long ncols = (long long)g.Ny * g.Nkz;
   const long long plane = (long long)g.Ny * g.ldk;
   const long long col0 = (long long)blockIdx.x * C;
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void k_xpass_plain(GridArgs g, cx<T>* __r
     A[myc * SS + skew<T>(xx)] = load_or_zero(data, (long long)xx * plane + off, 0, myc < ncol);
   }
   __syncthreads();
-  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, ncol, tw);
+  const cx<T>* R = fft_lds<T, L, SS, INV>(A, B, ncol, twl);
   const T s = (T)scl;
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
@@ -524,6 +529,7 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
   __shared__ cx<T> S2[C * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long plane = (long long)g.Ny * g.ldk;
   const int myc = threadIdx.x % C;
   int cy, ckz;
@@ -536,7 +542,7 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
     A[myc * SS + skew<T>(xx)] = load_or_zero(in, (long long)xx * plane + off, 0, myc < ncol);
   }
   __syncthreads();
-  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
+  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, twl);
   cx<T>* W1 = (S == A) ? B : A;
   T w3[PT], w2[PT], wk[PT][3], wt[PT][6];
   {
@@ -564,7 +570,7 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
       }
     }
     __syncthreads();
-    const cx<T>* R = fft_lds<T, L, SS, true>(W1, S2, ncol, tw);
+    const cx<T>* R = fft_lds<T, L, SS, true>(W1, S2, ncol, twl);
     cx<T>* o = out + f * g.ks;
     for (int e = threadIdx.x; e < C * L; e += kThreads) {
       const int xx = e / C;
@@ -593,6 +599,7 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
   constexpr int PT = (C * L + kThreads - 1) / kThreads;
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long plane = (long long)g.Ny * g.ldk;
   const int myc = threadIdx.x % C;
   int cy, ckz;
@@ -650,7 +657,7 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
       }
       __syncthreads();
     }
-    const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
+    const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, twl);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
@@ -668,7 +675,7 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
     if (e < C * L) A[myc * SS + skew<T>(e / C)] = acc[i];
   }
   __syncthreads();
-  const cx<T>* R = fft_lds<T, L, SS, true>(A, B, ncol, tw);
+  const cx<T>* R = fft_lds<T, L, SS, true>(A, B, ncol, twl);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
     if (myc < ncol) out[(long long)xx * plane + off] = R[myc * SS + skew<T>(xx)];
@@ -695,6 +702,12 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  // The FP64 variants read their twiddles from global memory and stride their
+  // butterfly loops by blockDim.x: on a compile-time stride the compiler keeps
+  // every stage's twiddles in registers across the field loop and the pass no
+  // longer fits 128 VGPRs; their LDS copy measured 13% slower at 128^3.
+  const cx<T>* twl = sizeof(T) == 4 ? lds_twiddles<T, L>(tw) : tw;
+  const int fstride = sizeof(T) == 8 ? (int)blockDim.x : kBlockThreads;
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
   constexpr int NW = FN == 3 ? 11 : 5;
@@ -723,7 +736,7 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
     __syncthreads();
     if (f + 1 < NW) fetch(f + 1);
     pack_c2r<T, L, P>(A, B);
-    const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+    const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl, fstride);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int e = threadIdx.x + i * kThreads;
@@ -797,7 +810,7 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
         }
       }
       __syncthreads();
-      const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, tw);
+      const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, twl, fstride);
       store_r2c_halves<T, L, P>(Z, fields + f * g.ks, line0, lines, g.ldk);
       __syncthreads();
     }
@@ -870,10 +883,11 @@ __device__ __forceinline__ void zgrad_body(int bx, const GridArgs& g, const cx<T
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
   load_c2r_packed<T, L, P>(accs, line0, lines, g.ldk, A, B);
-  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl);
   double dots[2] = {0.0, 0.0};
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;
@@ -1005,10 +1019,11 @@ __device__ __forceinline__ void zgrad_sk_body(int bx, const GridArgs& g, cx<T>* 
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
   load_c2r_packed<T, L, P>(accs, line0, lines, g.ldk, A, B);
-  cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl);
   T val[PT];
 #pragma unroll
   for (int i = 0; i < PT; ++i) {
@@ -1049,7 +1064,7 @@ __device__ __forceinline__ void zgrad_sk_body(int bx, const GridArgs& g, cx<T>* 
     }
   }
   __syncthreads();
-  const cx<T>* Z = fft_lds<T, L, SS, false>(R, R == A ? B : A, P, tw);
+  const cx<T>* Z = fft_lds<T, L, SS, false>(R, R == A ? B : A, P, twl);
   store_r2c_halves<T, L, P>(Z, accs, line0, lines, g.ldk);
 }
 template <typename T, int L, int ZB>
@@ -1066,6 +1081,7 @@ __device__ __forceinline__ void xsk_body(int bx, const GridArgs& g, cx<T>* __res
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[C * SS];
   __shared__ cx<T> B[C * SS];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long plane = (long long)g.Ny * g.ldk;
   const int myc = threadIdx.x % C;
   int cy, ckz;
@@ -1077,7 +1093,7 @@ __device__ __forceinline__ void xsk_body(int bx, const GridArgs& g, cx<T>* __res
     A[myc * SS + skew<T>(xx)] = load_or_zero(data, (long long)xx * plane + off, 0, cok);
   }
   __syncthreads();
-  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, tw);
+  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, twl);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
     if (cok) {
@@ -1086,7 +1102,7 @@ __device__ __forceinline__ void xsk_body(int bx, const GridArgs& g, cx<T>* __res
     }
   }
   __syncthreads();
-  const cx<T>* R = fft_lds<T, L, SS, true>(S, S == A ? B : A, ncol, tw);
+  const cx<T>* R = fft_lds<T, L, SS, true>(S, S == A ? B : A, ncol, twl);
   for (int e = threadIdx.x; e < C * L; e += kThreads) {
     const int xx = e / C;
     if (cok) data[(long long)xx * plane + off] = R[myc * SS + skew<T>(xx)];
@@ -1109,10 +1125,11 @@ __global__ __launch_bounds__(kThreads) void k_zsk_finish(GridArgs g, const cx<T>
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)blockIdx.x * 2 * P;
   load_c2r_packed<T, L, P>(spec, line0, lines, g.ldk, A, B);
-  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, tw);
+  const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl);
   double dots[2] = {0.0, 0.0};
   for (int e = threadIdx.x; e < 2 * P * L; e += kThreads) {
     const int l = e / L, z = e - l * L;

@@ -92,6 +92,7 @@ struct HostOps {
     for (int i = 0; i < n; ++i) x[i] = x[i] + s * d[i];
   }
   double newton_scale() const { return 1.0; }
+  bool sk_available() const { return false; }
 };
 
 static void read_vec(std::vector<double> &v, size_t count) {

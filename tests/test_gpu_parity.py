@@ -221,7 +221,7 @@ def test_mixed_precision_from_near_converged_starts(gpu, polish_grad, amp):
         m.set_mixed_precision(1e-6, polish_grad)
         m.run()
         s = m.stats()
-        assert s["stop_reason"] == "converged", (trial, s)
+        assert s["stop_reason"] == "converged", (trial, s, m.gradient_check(), m.energy() / E_star - 1)
         assert abs(m.energy() / E_star - 1) < 1e-9, (trial, m.energy(), E_star)
         m.close()
 
