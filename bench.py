@@ -277,6 +277,15 @@ def committed_traffic(family, tuning):
     return None if t is None else t["fetch_bytes"] + t["write_bytes"]
 
 
+def committed_busy_fraction():
+    """GPU busy fraction of the timed warm-start steps in the committed rocprofv3 kernel trace
+    summary of this workload (profiles/c2_128_solute_pmc.json), or None."""
+    path = os.path.join(HERE, "profiles", "c2_128_solute_pmc.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get("warm_step_busy_fraction")
+
+
 def whole_path_bytes(prof, steps):
     """Algorithmic HBM bytes of one minimization by the whole-path rule: every 3-D FFT
     reads its input and writes its output once (B_fft = 4N + 8Nk FP32, 8N + 16Nk FP64),
@@ -449,12 +458,14 @@ def main():
                                        "copy-calibrated, bytes per dispatch)",
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
-                     # the plan's kernel time per minimization (HIP events; they add ~1-3% to each
-                     # kernel) over the wall time of the minimizations without events (network
-                     # inference excluded from both); the event-timed steps themselves run ~40%
-                     # longer, so their own wall time would understate the fraction
-                     "kernel_time_fraction_of_wall": min(1.0, total_ms / args.steps / (1e3 * np.mean(
-                         [r["seconds"] - r["start_seconds"] for r in w["results"]]))),
+                     # GPU busy fraction of the warm-start steps from the committed rocprofv3 kernel
+                     # trace of this workload (kernel time over the span of each step; a lower
+                     # bound, the profiler adds gaps of its own)
+                     "kernel_time_fraction_of_wall": committed_busy_fraction(),
+                     # the plan's HIP-event kernel time over the minimizations' own wall time without
+                     # events: events lengthen every kernel a little, so this overstates the fraction
+                     "kernel_event_ms_over_wall": total_ms / args.steps / (1e3 * np.mean(
+                         [r["seconds"] - r["start_seconds"] for r in w["results"]])),
                      "kernel_ms_per_step": total_ms / args.steps,
                      "streaming_copy_GBps": copy_gbps, "frac_of_streaming_copy": achieved / copy_gbps,
                      "whole_path": {"bytes_per_minimization": path_bytes, "achieved_GBps": path_gbps,

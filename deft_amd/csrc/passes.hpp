@@ -702,12 +702,13 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
-  // The FP64 variants read their twiddles from global memory and stride their
-  // butterfly loops by blockDim.x: on a compile-time stride the compiler keeps
-  // every stage's twiddles in registers across the field loop and the pass no
-  // longer fits 128 VGPRs; their LDS copy measured 13% slower at 128^3.
+  // The butterfly loops stride by blockDim.x here: on a compile-time stride
+  // the compiler keeps every stage's twiddles in registers across the field
+  // loop and the pass no longer fits 128 VGPRs (it spilled 156 bytes in FP32,
+  // 244 in FP64).  The FP64 variants read their twiddles from global memory:
+  // their LDS copy measured 13% slower at 128^3.
   const cx<T>* twl = sizeof(T) == 4 ? lds_twiddles<T, L>(tw) : tw;
-  const int fstride = sizeof(T) == 8 ? (int)blockDim.x : kBlockThreads;
+  const int fstride = (int)blockDim.x;
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
   constexpr int NW = FN == 3 ? 11 : 5;
