@@ -245,6 +245,57 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
     return res
 
 
+def c5_run(iters=40, seed=2024):
+    """C5 at its real shape: instance 0 of the seed-2024 porous batch (256^3, dx = 0.05,
+    30 hard spherical obstacles of excluded radius U(1, 2.5), eta = 0.45), started from the
+    network's prediction on the 2x coarser grid, upsampled, then the headline's ladder for
+    `iters` CG iterations.  C5 minimizations do not converge within 2000 iterations with
+    either preconditioner (slow relaxation in the pores), so C5 is reported as iteration
+    throughput and the FP64 energy it reaches, not as minimizations per second."""
+    shape, cell, V, eta = W.c5_batch(1, seed=seed)[0]
+    nb = W.bulk_density(eta)
+    f = D.WhiteBearFluidVeff(shape, np.diag(cell), R=R, kT=KT, mu=D.find_chemical_potential(nb, R, KT),
+                             fft_precision=D.FFT_FP32)
+    f.Vext = V
+    f.autotune()
+    net = NN.WarmStartNet.load()
+    precision = abs(D.pressure(nb, R, KT) * np.prod(cell)) * PRECISION_FACTOR
+
+    def one():
+        f.fft_precision = D.FFT_FP32
+        t0 = time.perf_counter()
+        net.warm_start(f, eta, factor=2)
+        t1 = time.perf_counter()
+        m = D.Minimize(f)
+        m.precondition(True)
+        m.set_precision(precision)
+        m.set_maxiter(iters)
+        m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
+        m.run()
+        t2 = time.perf_counter()
+        st = m.stats()
+        m.close()
+        return t1 - t0, t2 - t1, st
+
+    one()
+    tn, tm, st = one()
+    f.fft_precision = D.FFT_FP64
+    E_end = f.energy()
+    net.warm_start(f, eta, factor=2)
+    f.fft_precision = D.FFT_FP64
+    E_start = f.energy()
+    net.close()
+    f.close()
+    return {"workload": f"C5 instance 0 of seed {seed}: 256^3, dx = 0.05, 30 hard obstacles, eta = 0.45, "
+                        f"network start on the 2x coarser grid, {iters} ladder iterations (grad/n)",
+            "excluded_fraction": float(np.mean(np.isinf(V))), "iterations": st["iter"],
+            "ms_per_iteration": tm / st["iter"] * 1e3,
+            "iterations_per_s": st["iter"] / tm, "network_ms": tn * 1e3,
+            "energy_calcs": st["num_energy_calcs"], "grad_calcs": st["num_grad_calcs"],
+            "fp64_energy_at_start": E_start, "fp64_energy_after": E_end, "stop_reason": st["stop_reason"],
+            "converged": False}
+
+
 def rocprof_instance(family, tuning):
     """rocprof name of the kernel instance behind a profiler family at the autotuned
     variants (`tuning` as returned by autotune()); None when there is no single one."""
@@ -380,6 +431,7 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=15.0)
     ap.add_argument("--no-c3", action="store_true", help="skip the 64-grid batch measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512 held-out warm-start measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 256^3 porous iteration measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -418,6 +470,7 @@ def main():
     path_gbps = path_bytes / (w["elapsed"] / args.steps) / 1e9
     c3 = None if args.no_c3 else c3_run()
     c4 = None if args.no_c4 else c4_run()
+    c5 = None if args.no_c5 else c5_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
     def block(run, start):
         r = run["results"][-1]
@@ -451,6 +504,7 @@ def main():
                                                                       "plain": plain_value / ref_plain_value},
                    "c3_batch": c3,
                    "c4_warm_start_heldout": c4,
+                   "c5_porous_256": c5,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname, runs["tuning"]),
