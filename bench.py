@@ -183,7 +183,8 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
     from a seed the training never used; each minimized from the plain (bulk) start and
     from the network's start, in batches of `chunk`, with the headline's ladder and
     precision rule and the reference's grad/n preconditioner (the one batches run).
-    Times count the start (bulk fill or network inference) and the minimization."""
+    Times count the start (bulk fill, or one network inference over the chunk) and the
+    minimization."""
     rng = np.random.default_rng(seed)
     cases = [(NN.training_vext(M, L, rng), float(rng.uniform(0.1, 0.45))) for _ in range(count)]
     net = NN.WarmStartNet.load()
@@ -203,11 +204,13 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
             precs.append(abs(D.pressure(nb, R, KT) * L ** 3) * PRECISION_FACTOR)
         for start in ("plain", "warm"):
             t0 = time.perf_counter()
-            for f, (V, eta), nb in zip(fl, part, nbs):
+            for f in fl:
                 f.fft_precision = D.FFT_FP32
-                if start == "warm":
-                    net.warm_start(f, eta)
-                else:
+            if start == "warm":
+                # one inference over the whole chunk (deft_plans_warm_start)
+                NN.warm_start_batch(net, fl, [eta for _, eta in part])
+            else:
+                for f, nb in zip(fl, nbs):
                     f.fill_Veff(-KT * math.log(nb))
             batch = D.Batch(fl)  # after the warm starts: their safety check evaluates outside a batch
             ms = []

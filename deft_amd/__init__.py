@@ -171,6 +171,7 @@ _SIGNATURES = {
     "deft_plan_warm_start": ([_P, _P, _D], _I),
     "deft_plan_warm_start_scaled": ([_P, _P, _D, _I], _I),
     "deft_net_last_blend": ([_P, _DP], _I),
+    "deft_plans_warm_start": ([ctypes.POINTER(_P), _DP, _I, _P, _I, _DP], _I),
     "deft_buffer_create": ([ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_buffer_destroy": ([_P], _I),
     "deft_buffer_upload": ([_P, ctypes.c_ulonglong, _P, ctypes.c_ulonglong], _I),

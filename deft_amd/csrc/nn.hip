@@ -1202,17 +1202,24 @@ int deft_net_make_batch(const float *data_dev, long long count, int M, int batch
   return DEFT_OK;
 }
 
-int deft_plan_warm_start_scaled(deft_plan p, deft_net n, double eta, int factor) {
-  if (!p) return set_error(DEFT_ERR_ARG, "null plan");
+int deft_plans_warm_start(const deft_plan *plans, const double *etas, int count, deft_net n, int factor,
+                          double *lambdas) {
+  if (!plans || !etas || count < 1) return set_error(DEFT_ERR_ARG, "need at least one plan and its eta");
   NN_TRY(check_net(n));
-  if (!(eta > 0 && eta < 0.74)) return set_error(DEFT_ERR_ARG, "eta must lie in (0, 0.74)");
   if (factor != 1 && factor != 2 && factor != 4) return set_error(DEFT_ERR_ARG, "factor must be 1, 2 or 4");
-  if (p->Nx != p->Ny || p->Nx != p->Nz) return set_error(DEFT_ERR_UNSUPPORTED, "warm start needs a cubic grid");
-  if (p->Nx % factor) return set_error(DEFT_ERR_ARG, "the grid edge must be a multiple of the factor");
-  const int M = p->Nx, Mc = M / factor;
-  NN_TRY(check_shape(Mc, 1));
+  const int M = plans[0] ? plans[0]->Nx : 0;
+  for (int i = 0; i < count; ++i) {
+    const deft_plan p = plans[i];
+    if (!p) return set_error(DEFT_ERR_ARG, "null plan");
+    if (!(etas[i] > 0 && etas[i] < 0.74)) return set_error(DEFT_ERR_ARG, "eta must lie in (0, 0.74)");
+    if (p->Nx != p->Ny || p->Nx != p->Nz) return set_error(DEFT_ERR_UNSUPPORTED, "warm start needs a cubic grid");
+    if (p->Nx != M) return set_error(DEFT_ERR_ARG, "the plans of one warm start must share their grid edge");
+    if (p->Nx % factor) return set_error(DEFT_ERR_ARG, "the grid edge must be a multiple of the factor");
+  }
+  const int Mc = M / factor;
+  NN_TRY(check_shape(Mc, count));
   const long long Nc = (long long)Mc * Mc * Mc;
-  const size_t need = sizeof(float) * 2 * (size_t)Nc;
+  const size_t need = sizeof(float) * 2 * (size_t)Nc * count;
   if (n->ws_bytes < need) {
     (void)hipFree(n->ws);
     n->ws = nullptr;
@@ -1220,41 +1227,56 @@ int deft_plan_warm_start_scaled(deft_plan p, deft_net n, double eta, int factor)
     NN_HIP(hipMalloc(&n->ws, need));
     n->ws_bytes = need;
   }
-  float *boltz = n->ws, *u = n->ws + Nc;
-  if (factor == 1)
-    hipLaunchKernelGGL(k_boltz, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT), boltz,
-                       p->N);
-  else
-    hipLaunchKernelGGL(k_boltz_coarse, dim3(blocks_for(Nc)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT),
-                       boltz, M, factor);
-  NN_HIP(hipGetLastError());
-  NN_HIP(hipStreamSynchronize(p->stream));
-  const float e = (float)eta;
-  NN_TRY(deft_net_forward(n, Mc, 1, boltz, &e, u));
-  const double nb = eta / (4 * M_PI / 3 * p->R * p->R * p->R);
-  // Veff = -kT (ln n_b + lambda u): lambda = 1 unless the prediction puts the
-  // weighted packing fraction at or past 1 somewhere (infinite or NaN energy);
-  // then the start is pulled toward the bulk one (lambda = 0, always finite)
-  double lambda = 1.0;
-  for (int attempt = 0;; ++attempt) {
-    p->npend = 0;
+  float *boltz = n->ws, *u = n->ws + Nc * count;
+  // exp(-Vext/kT) of every plan (on the finer grid smoothed onto the coarse
+  // one) into one batch, then one inference for all of them
+  for (int i = 0; i < count; ++i) {
+    const deft_plan p = plans[i];
     if (factor == 1)
-      hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, p->kT, std::log(nb),
-                         lambda, p->x, p->N);
+      hipLaunchKernelGGL(k_boltz, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, p->vext, (float)(1.0 / p->kT),
+                         boltz + Nc * i, p->N);
     else
-      hipLaunchKernelGGL(k_veff_from_u_fine, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, u, M, factor, p->kT,
-                         std::log(nb), lambda, p->x);
+      hipLaunchKernelGGL(k_boltz_coarse, dim3(blocks_for(Nc)), dim3(kNT), 0, p->stream, p->vext,
+                         (float)(1.0 / p->kT), boltz + Nc * i, M, factor);
     NN_HIP(hipGetLastError());
-    NN_HIP(hipStreamSynchronize(p->stream));
-    if (lambda == 0.0) break;
-    double E = 0;
-    deft_energy_terms terms;
-    NN_TRY(deft_functional_energy(p, nullptr, &E, &terms));
-    if (std::isfinite(E)) break;
-    lambda = (attempt < 8) ? 0.5 * lambda : 0.0;
   }
-  n->last_lambda = lambda;
+  for (int i = 0; i < count; ++i) NN_HIP(hipStreamSynchronize(plans[i]->stream));
+  std::vector<float> ef(count);
+  for (int i = 0; i < count; ++i) ef[i] = (float)etas[i];
+  NN_TRY(deft_net_forward(n, Mc, count, boltz, ef.data(), u));
+  for (int i = 0; i < count; ++i) {
+    const deft_plan p = plans[i];
+    const float *ui = u + Nc * i;
+    const double nb = etas[i] / (4 * M_PI / 3 * p->R * p->R * p->R);
+    // Veff = -kT (ln n_b + lambda u): lambda = 1 unless the prediction puts the
+    // weighted packing fraction at or past 1 somewhere (infinite or NaN energy);
+    // then the start is pulled toward the bulk one (lambda = 0, always finite)
+    double lambda = 1.0;
+    for (int attempt = 0;; ++attempt) {
+      p->npend = 0;
+      if (factor == 1)
+        hipLaunchKernelGGL(k_veff_from_u, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, ui, p->kT,
+                           std::log(nb), lambda, p->x, p->N);
+      else
+        hipLaunchKernelGGL(k_veff_from_u_fine, dim3(blocks_for(p->N)), dim3(kNT), 0, p->stream, ui, M, factor,
+                           p->kT, std::log(nb), lambda, p->x);
+      NN_HIP(hipGetLastError());
+      NN_HIP(hipStreamSynchronize(p->stream));
+      if (lambda == 0.0) break;
+      double E = 0;
+      deft_energy_terms terms;
+      NN_TRY(deft_functional_energy(p, nullptr, &E, &terms));
+      if (std::isfinite(E)) break;
+      lambda = (attempt < 8) ? 0.5 * lambda : 0.0;
+    }
+    if (lambdas) lambdas[i] = lambda;
+    n->last_lambda = lambda;
+  }
   return DEFT_OK;
+}
+
+int deft_plan_warm_start_scaled(deft_plan p, deft_net n, double eta, int factor) {
+  return deft_plans_warm_start(&p, &eta, 1, n, factor, nullptr);
 }
 
 int deft_plan_warm_start(deft_plan p, deft_net n, double eta) { return deft_plan_warm_start_scaled(p, n, eta, 1); }

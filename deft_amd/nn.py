@@ -191,6 +191,17 @@ class WarmStartNet:
         return lam.value
 
 
+def warm_start_batch(net, fluids, etas, factor=1):
+    """WarmStartNet.warm_start for several fluids of one cubic grid edge with a single
+    inference over all of them (deft_plans_warm_start); returns the blend factors."""
+    n = len(fluids)
+    arr = (ctypes.c_void_p * n)(*[f.handle for f in fluids])
+    e = (ctypes.c_double * n)(*[float(x) for x in etas])
+    lam = (ctypes.c_double * n)()
+    _check(lib().deft_plans_warm_start(arr, e, n, net._h, int(factor), lam))
+    return [lam[i] for i in range(n)]
+
+
 def mfma_probe(a, b):
     """c = a @ b.T for a 16x32 and b 16x32 through the convolution kernel's MFMA path."""
     a = np.ascontiguousarray(a, dtype=np.float32)

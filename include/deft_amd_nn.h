@@ -97,6 +97,15 @@ int deft_plan_warm_start_scaled(deft_plan plan, deft_net net, double eta, int fa
 /* lambda of the last warm start (1 = the network's prediction unchanged). */
 int deft_net_last_blend(deft_net net, double *lambda);
 
+/* deft_plan_warm_start_scaled for `count` plans of one cubic grid edge at once:
+ * their Boltzmann factors form one batch of a single inference (no counterpart
+ * in the reference; for batches of minimizations, where one inference per
+ * plan would cost more host-driven launches than the start saves).  Each
+ * plan's start is checked by its own energy evaluation as in the single-plan
+ * call; lambdas (may be null) receives each plan's blend factor. */
+int deft_plans_warm_start(const deft_plan *plans, const double *etas, int count, deft_net net, int factor,
+                          double *lambdas);
+
 /* One layer on its own, for checking each kernel against a reference (host
  * buffers, channels-last, M^3 grids, batch of them):
  *   op 0: out = act(conv(in) + bias), in B M^3 x cin, w cout x kp (FP32,

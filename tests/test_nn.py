@@ -387,6 +387,37 @@ def test_warm_start_sets_veff_from_prediction(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("factor", [1, 2])
+def test_batched_warm_start_equals_one_by_one(gpu, factor):
+    """deft_plans_warm_start: one inference over several plans gives each plan the Veff
+    (and blend factor) its own warm start gives, including a plan whose prediction has to
+    be pulled toward the bulk start; mixed etas, both grid factors."""
+    net = deft_amd.nn.WarmStartNet(SMALL, seed=5)
+    M, L = 32 * factor, 3.2
+    rng = np.random.default_rng(21)
+    cases = [(deft_amd.nn.random_vext(M, L, rng), eta) for eta in (0.15, 0.3, 0.42)]
+    fl = []
+    for V, eta in cases:
+        f = deft_amd.WhiteBearFluidVeff((M, M, M), np.diag([L] * 3), R=1, kT=1.0,
+                                        mu=deft_amd.find_chemical_potential(eta / (4 * math.pi / 3)))
+        f.Vext = V
+        fl.append(f)
+    alone = []
+    for f, (V, eta) in zip(fl, cases):
+        lam = net.warm_start(f, eta, factor=factor)
+        alone.append((lam, f.Veff.copy()))
+    lams = deft_amd.nn.warm_start_batch(net, fl, [eta for _, eta in cases], factor=factor)
+    for f, lam, (lam1, v1) in zip(fl, lams, alone):
+        assert lam == lam1
+        v = f.Veff
+        fin = np.isfinite(v1)
+        assert np.array_equal(np.isfinite(v), fin)
+        assert np.max(np.abs(v[fin] - v1[fin])) <= 1e-6 * np.max(np.abs(v1[fin]))
+        f.close()
+    net.close()
+
+
+@pytest.mark.gpu
 def test_scaled_warm_start_matches_numpy(gpu):
     """factor 2: the network runs on the 2x coarser grid (tent-smoothed Boltzmann factor)
     and its u is interpolated back trilinearly."""
