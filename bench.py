@@ -55,7 +55,8 @@ def gpu_run(steps, warmup):
     f = D.WhiteBearFluidVeff((N, N, N), np.diag([L] * 3), R=R, kT=KT, mu=mu, fft_precision=D.FFT_FP32)
     f.Vext = W.c2_solute_vext(N, L, SOLUTE)
     tuning = f.autotune()
-    print("autotuned pass variants:", tuning, file=sys.stderr, flush=True)
+    eng = f.engine()
+    print("autotuned pass variants:", tuning, "engine:", eng, file=sys.stderr, flush=True)
     out_tuning = tuning
     net = NN.WarmStartNet.load()
     v0 = -KT * math.log(nb)
@@ -110,6 +111,7 @@ def gpu_run(steps, warmup):
         one(True, True)
     out["profiled_elapsed"] = time.perf_counter() - t0
     out["tuning"] = out_tuning
+    out["engine"] = {"chosen": eng[0], "fused_eg_ms_fp32_plus_fp64": eng[1], "generic_eg_ms_fp32_plus_fp64": eng[2]}
     out["prof"] = f.profile()
     f.set_profiling(False)
     return out, gold, precision
@@ -505,6 +507,7 @@ def main():
                                                 "warm_over_plain": ref_value / ref_plain_value},
                    "structure_factor_over_reference_preconditioner": {"warm": value / ref_value,
                                                                       "plain": plain_value / ref_plain_value},
+                   "autotuned_engine": runs["engine"],
                    "c3_batch": c3,
                    "c4_warm_start_heldout": c4,
                    "c5_porous_256": c5,

@@ -171,6 +171,8 @@ _SIGNATURES = {
     "deft_plan_warm_start": ([_P, _P, _D], _I),
     "deft_plan_warm_start_scaled": ([_P, _P, _D, _I], _I),
     "deft_net_last_blend": ([_P, _DP], _I),
+    "deft_plan_engine": ([_P, _IP, _DP, _DP], _I),
+    "deft_plan_set_engine": ([_P, _I], _I),
     "deft_plans_warm_start": ([ctypes.POINTER(_P), _DP, _I, _P, _I, _DP], _I),
     "deft_buffer_create": ([ctypes.c_ulonglong, ctypes.POINTER(_P)], _I),
     "deft_buffer_destroy": ([_P], _I),
@@ -448,6 +450,16 @@ class WhiteBearFluidVeff:
         """Times the pass variants on this grid and keeps the fastest (like FFTW_MEASURE)."""
         _check(lib().deft_plan_autotune(self._h))
         return {p: self.tuning(p) for p in (FFT_FP32, FFT_FP64)}
+
+    def engine(self):
+        """("generic" or "fused", fused ms, generic ms): the engine the plan runs and the
+        energy+gradient evaluation times (FP32 + FP64) the last autotune measured."""
+        g, a, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().deft_plan_engine(self._h, ctypes.byref(g), ctypes.byref(a), ctypes.byref(b)))
+        return ("generic" if g.value else "fused"), a.value, b.value
+
+    def set_engine(self, generic):
+        _check(lib().deft_plan_set_engine(self._h, 1 if generic else 0))
 
     def tuning(self, fft_precision):
         z, y, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -678,12 +678,79 @@ int deft_batch_stats(deft_batch b, long long *rounds, long long *launches, long 
   return DEFT_OK;
 }
 
+// Variant tuning per engine, then (for a plan whose edges have fused kernels,
+// outside batches) the engine itself: one FP32 and one FP64 energy+gradient
+// evaluation with the fused passes (their radix order fixed per edge) against
+// the generic line passes at their own autotuned radix orders; the faster
+// engine stays.  The gradient buffers are saved around the timing.
 int deft_plan_autotune(deft_plan p) {
   TRY(check_plan(p));
   TRY(flush_moves(p));
   TRY(plan_sync(p));
   TRY(Launch<float>::autotune(p));
   TRY(Launch<double>::autotune(p));
+  p->engine_fused_ms = p->engine_generic_ms = -1.0;
+  if (p->generic || p->batch || p->functional == DEFT_WHITE_BEAR_TENSOR) return DEFT_OK;
+  const size_t nb = sizeof(double) * (size_t)p->N;
+  double *keep = nullptr;
+  if (hipMalloc(&keep, 2 * nb) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "hipMalloc of the tuning copy failed");
+  auto restore = [&](int rc) {
+    if (hipMemcpyAsync(p->g, keep, nb, hipMemcpyDeviceToDevice, p->stream) != hipSuccess ||
+        hipMemcpyAsync(p->pg, keep + p->N, nb, hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+      rc = set_error(DEFT_ERR_HIP, "restoring the gradient buffers failed");
+    if (plan_sync(p) != DEFT_OK && rc == DEFT_OK) rc = DEFT_ERR_HIP;
+    (void)hipFree(keep);
+    return rc;
+  };
+  if (hipMemcpyAsync(keep, p->g, nb, hipMemcpyDeviceToDevice, p->stream) != hipSuccess ||
+      hipMemcpyAsync(keep + p->N, p->pg, nb, hipMemcpyDeviceToDevice, p->stream) != hipSuccess) {
+    (void)hipFree(keep);
+    return set_error(DEFT_ERR_HIP, "saving the gradient buffers failed");
+  }
+  const float fused = Launch<float>::time_evaluation(p) + Launch<double>::time_evaluation(p);
+  LineSched sched_keep[2][3];
+  std::memcpy(sched_keep, p->sched, sizeof(sched_keep));
+  p->generic = true;
+  int rc = ensure_gwork(p);
+  if (rc == DEFT_OK) rc = Launch<float>::autotune(p);
+  if (rc == DEFT_OK) rc = Launch<double>::autotune(p);
+  const float gen = rc == DEFT_OK ? Launch<float>::time_evaluation(p) + Launch<double>::time_evaluation(p) : 1e30f;
+  p->engine_fused_ms = fused;
+  p->engine_generic_ms = gen;
+  if (!(gen < fused)) {
+    p->generic = false;
+    std::memcpy(p->sched, sched_keep, sizeof(sched_keep));  // the fused kernels' compile-time orders
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(p->gwork);
+    p->gwork = nullptr;
+    p->gwork_fields = 0;
+    if (rc != DEFT_OK) g_last_error.clear();
+    rc = DEFT_OK;
+  }
+  return restore(rc);
+}
+
+int deft_plan_set_engine(deft_plan p, int generic) {
+  TRY(check_plan(p));
+  if (p->batch) return set_error(DEFT_ERR_STATE, "the plan is in a batch");
+  if (!generic && !(fused_ok(p->Nx) && fused_ok(p->Ny) && fused_ok(p->Nz)))
+    return set_error(DEFT_ERR_UNSUPPORTED, "this plan's edges have no fused kernels");
+  if (!generic == !p->generic) return DEFT_OK;
+  TRY(flush_moves(p));
+  TRY(plan_sync(p));
+  if (!generic) {
+    const int lens[3] = {p->Nx, p->Ny, p->Nz};
+    for (int a = 0; a < 3; ++a) p->sched[0][a] = p->sched[1][a] = make_sched(lens[a], radix_candidates(lens[a])[0]);
+  }
+  p->generic = generic != 0;
+  return ensure_gwork(p);
+}
+
+int deft_plan_engine(deft_plan p, int *generic, double *fused_ms, double *generic_ms) {
+  TRY(check_plan(p));
+  if (generic) *generic = p->generic ? 1 : 0;
+  if (fused_ms) *fused_ms = p->engine_fused_ms;
+  if (generic_ms) *generic_ms = p->engine_generic_ms;
   return DEFT_OK;
 }
 

@@ -986,6 +986,18 @@ int Launch<T>::autotune(deft_plan_s *p) {
 }
 
 template <typename T>
+float Launch<T>::time_evaluation(deft_plan_s *p) {
+  const bool was_prof = p->prof;
+  p->prof = false;
+  int nzb = 0, rc = DEFT_OK;
+  const float ms = time_launches(p, [&] {
+    if (rc == DEFT_OK) rc = enqueue_evaluation(p, true, false, PGRAD_NEW, &nzb);
+  });
+  p->prof = was_prof;
+  return rc == DEFT_OK ? ms : 1e30f;
+}
+
+template <typename T>
 static int gen_autotune(deft_plan_s *p) {
   const GridArgs g = grid_args(p);
   const bool was_prof = p->prof;

@@ -143,6 +143,8 @@ struct deft_plan_s {
   double *sktab = nullptr;
   bool sk_ok = false;
   double sk_nb = 0.0, sk_floor = 0.0;
+  // evaluation times of both engines from the last autotune (-1: not timed)
+  double engine_fused_ms = -1.0, engine_generic_ms = -1.0;
 };
 
 namespace deft {
@@ -388,6 +390,9 @@ struct Launch {
   static int enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nzb);
   // times the tunable kernel variants on this grid and keeps the fastest
   static int autotune(deft_plan_s *p);
+  // milliseconds of one energy + gradient evaluation (grad/n) with the plan's
+  // current engine and tuning; the gradient buffers are overwritten
+  static float time_evaluation(deft_plan_s *p);
   // fft seam: real FP64 (device, N) -> spectrum in field 0 (scaled by scl)
   static int fft_forward(deft_plan_s *p, const double *in_dev, double scl);
   // ifft seam: spectrum in field 0 -> real FP64 (device, N) times scl

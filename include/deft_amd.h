@@ -166,8 +166,20 @@ int deft_plan_destroy(deft_plan plan);
  * src/GridDescription.cpp:19-25, plays the same role): times the instantiated
  * variants of the z pass (block of 512 or 1024 complex values), the y pass and
  * the x passes (tiles of 8 or 16 KiB) on this grid for both FFT precisions and
- * keeps the fastest.  Overwrites the plan's spectral scratch, not Veff. */
+ * keeps the fastest; times the candidate radix orders of every axis for the
+ * generic line passes; and, for a plan whose edges have fused kernels
+ * (outside batches, not the tensor functional), times one FP32 and one FP64
+ * energy+gradient evaluation with each engine -- the fused passes with their
+ * compile-time radix order against the generic passes at their tuned orders
+ * -- and keeps the faster.  Overwrites the plan's spectral scratch, not Veff
+ * or the gradient buffers. */
 int deft_plan_autotune(deft_plan plan);
+/* The plan's engine (1: generic line passes, 0: fused per-edge passes) and the
+ * two evaluation times of the last autotune (-1 when not timed); and an
+ * explicit choice (0 needs fused kernels for every edge; plans outside
+ * batches).  No counterpart in the reference. */
+int deft_plan_engine(deft_plan plan, int *generic, double *fused_ms, double *generic_ms);
+int deft_plan_set_engine(deft_plan plan, int generic);
 /* The chosen variants for one FFT precision, and an explicit choice
  * (zblock 512 or 1024, ytile and xtile 8192 or 16384). */
 int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile, int *xtile);

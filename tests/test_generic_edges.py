@@ -218,3 +218,55 @@ def test_wall_line_minimum_matches_oracle(gpu, eta, functional):
     assert abs(n[0, 0, k] / p - 1) < 0.03
     m.close()
     f.close()
+
+
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+def test_generic_engine_on_fused_edges_meets_oracle(gpu, prec, etol, gtol):
+    """The engine choice of the autotuner: on edges that have fused kernels the generic
+    line passes (forced by deft_plan_set_engine) give the oracle's energy and gradient
+    as the fused passes do."""
+    shape, cell = (32, 48, 20), np.diag([3.1, 4.9, 2.3])
+    g, of, f = _fluid(gpu, shape, cell, vext=_wall_vext(O.Grid(shape, cell), 0.5), prec=prec, seed=3)
+    E_f, g_f, _ = f.energy_grad_and_precond()
+    f.set_engine(True)
+    assert f.engine()[0] == "generic"
+    E_g, g_g, _ = f.energy_grad_and_precond()
+    Eo, go, _ = of.energy_grad_and_precond()
+    for E, gr in ((E_f, g_f), (E_g, g_g)):
+        assert abs(E - Eo) <= etol * abs(Eo)
+        assert np.max(np.abs(gr - go)) <= gtol * np.max(np.abs(go))
+    f.set_engine(False)
+    assert f.engine()[0] == "fused"
+    f.close()
+
+
+def test_autotune_keeps_the_faster_engine_and_the_minimizer_state(gpu):
+    """deft_plan_autotune times one FP32 and one FP64 energy+gradient evaluation with each
+    engine and keeps the faster; run in the middle of a minimization it leaves Veff and
+    the gradient buffers as they were, so the iterates that follow are the same as
+    without it."""
+    shape, cell = (64, 64, 64), np.diag([6.4, 6.4, 6.4])
+    g = O.Grid(shape, cell)
+    runs = []
+    for tune in (False, True):
+        _, of, f = _fluid(gpu, shape, cell, vext=O.planar_wall_vext(g, 1.0), prec=1, seed=2, amp=0.2)
+        m = gpu.Minimize(f)
+        m.precondition(True)
+        for _ in range(3):
+            m.improve_energy()
+        if tune:
+            eng, tf, tg = f.engine()
+            assert tf == -1.0 and tg == -1.0
+            f.autotune()
+            eng, tf, tg = f.engine()
+            print("engines: fused", tf, "generic", tg, "->", eng, flush=True)
+            assert tf > 0 and tg > 0
+            assert eng == ("generic" if tg < tf else "fused")
+        Es = []
+        for _ in range(4):
+            m.improve_energy()
+            Es.append(m.energy())
+        runs.append(Es)
+        m.close()
+        f.close()
+    assert np.allclose(runs[0], runs[1], rtol=1e-11, atol=0)
