@@ -117,10 +117,11 @@ def gpu_run(steps, warmup):
     return out, gold, precision
 
 
-def c3_run(count=64, seed=12345, reps=2):
+def c3_run(count=64, seed=12345, reps=2, precond=D.PGRAD_STRUCTURE_FACTOR, sequential=True):
     """64 uneven grids (edges per axis from {32, 48, 64, 96}, dx = 0.1, 8 Gaussians and a
     planar hard wall, eta ~ U(0.1, 0.45)): one batch against the same minimizations run
-    one after another; both from the bulk start, same ladder and precision rule as above."""
+    one after another; both from the bulk start, same ladder and precision rule as above,
+    preconditioned with `precond` (the structure factor, or deft's grad/n)."""
     insts = W.c3_batch(count, seed=seed)
     fluids = []
     for shape, cell, V, eta in insts:
@@ -136,6 +137,8 @@ def c3_run(count=64, seed=12345, reps=2):
             f.fill_Veff(v0)
             m = D.Minimize(f)
             m.precondition(True)
+            if precond != D.PGRAD_NEW:
+                m.set_preconditioner(precond)
             m.set_precision(prec)
             m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
             ms.append(m)
@@ -147,8 +150,8 @@ def c3_run(count=64, seed=12345, reps=2):
             m.close()
         return r
 
-    best_seq = best_bat = None
-    for _ in range(reps):
+    best_seq = best_bat = seq = None
+    for _ in range(reps if sequential else 0):
         ms = minimizers()
         t0 = time.perf_counter()
         for m in ms:
@@ -168,14 +171,17 @@ def c3_run(count=64, seed=12345, reps=2):
     batch.close()
     for f, _, _ in fluids:
         f.close()
-    return {"workload": f"{count} uneven grids, edges per axis from {{32, 48, 64, 96}}, seed {seed}",
-            "points": int(sum(np.prod(s) for s, _, _, _ in insts)),
-            "batched_minimizations_per_s": count / best_bat, "sequential_minimizations_per_s": count / best_seq,
-            "batched_over_sequential": best_seq / best_bat,
-            "rounds_per_batch": st["rounds"] // reps, "launches_per_batch": st["launches"] // reps,
-            "iterations": [b[0] for b in bat],
-            "identical_to_sequential": all(a[:2] == b[:2] for a, b in zip(seq, bat)),
-            "all_converged": all(b[2] == "converged" for b in bat)}
+    res = {"workload": f"{count} uneven grids, edges per axis from {{32, 48, 64, 96}}, seed {seed}",
+           "preconditioner": "grad/n" if precond == D.PGRAD_NEW else "structure factor",
+           "points": int(sum(np.prod(s) for s, _, _, _ in insts)),
+           "batched_minimizations_per_s": count / best_bat,
+           "rounds_per_batch": st["rounds"] // reps, "launches_per_batch": st["launches"] // reps,
+           "iterations": [b[0] for b in bat], "total_iterations": int(sum(b[0] for b in bat)),
+           "all_converged": all(b[2] == "converged" for b in bat)}
+    if sequential:
+        res.update({"sequential_minimizations_per_s": count / best_seq, "batched_over_sequential": best_seq / best_bat,
+                    "identical_to_sequential": all(a[:2] == b[:2] for a, b in zip(seq, bat))})
+    return res
 
 
 def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
@@ -184,7 +190,7 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
     bumps, or the C3 Gaussians and wall) at 64^3, dx = 0.1, eta ~ U(0.1, 0.45), drawn
     from a seed the training never used; each minimized from the plain (bulk) start and
     from the network's start, in batches of `chunk`, with the headline's ladder and
-    precision rule and the reference's grad/n preconditioner (the one batches run).
+    precision rule and the structure-factor preconditioner.
     Times count the start (bulk fill, or one network inference over the chunk) and the
     minimization."""
     rng = np.random.default_rng(seed)
@@ -219,6 +225,7 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
             for f, prec in zip(fl, precs):
                 m = D.Minimize(f)
                 m.precondition(True)
+                m.set_preconditioner(D.PGRAD_STRUCTURE_FACTOR)
                 m.set_precision(prec)
                 m.set_maxiter(1000)
                 m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
@@ -237,7 +244,7 @@ def c4_run(count=512, chunk=64, seed=4242, M=64, L=6.4):
     for f in fluids:
         f.close()
     res = {"workload": f"{count} held-out potentials of the training generator at {M}^3, dx = {L / M:g}, "
-                       f"eta ~ U(0.1, 0.45), seed {seed}; batches of {chunk}; grad/n preconditioner"}
+                       f"eta ~ U(0.1, 0.45), seed {seed}; batches of {chunk}; structure-factor preconditioner"}
     for k in ("plain", "warm"):
         it = np.array(out[k]["iterations"])
         res[k] = {"mean_iterations": float(it.mean()), "median_iterations": float(np.median(it)),
@@ -474,6 +481,8 @@ def main():
     path_bytes = whole_path_bytes(prof, args.steps)
     path_gbps = path_bytes / (w["elapsed"] / args.steps) / 1e9
     c3 = None if args.no_c3 else c3_run()
+    if c3 is not None:
+        c3["grad_n_preconditioner"] = c3_run(precond=D.PGRAD_NEW, sequential=False)
     c4 = None if args.no_c4 else c4_run()
     c5 = None if args.no_c5 else c5_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None

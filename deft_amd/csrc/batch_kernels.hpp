@@ -37,6 +37,7 @@ struct BatchArgs {
   double* grad;
   double* pgrad;
   double* dscal;
+  const double* sktab;  // S(k)/N of the structure-factor preconditioner (gradient requests that use it)
   int kind;  // 0 no request this round, 1 energy, 2 energy + gradient
 };
 
@@ -122,6 +123,32 @@ __global__ __launch_bounds__(kThreads) void kb_zgrad(BatchGroup grp, const Batch
   if (batch_skip(a.kind, sel)) return;
   zgrad_body<T, L, ZB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.x, a.vext, a.og, a.have_old, a.precond, a.grad,
                        a.pgrad, a.partial, a.pstride, a.twz);
+}
+
+// The structure-factor preconditioner's passes (gradient requests with
+// precond == PGRAD_SK): the fused z gradient pass, then after the one-field y
+// pass the x pass with S(k)/N, and after the inverse y pass the finishing z pass.
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void kb_zgrad_sk(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  zgrad_sk_body<T, L, ZB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.x, a.vext, a.grad, a.twz);
+}
+
+template <typename T, int L, int TB>
+__global__ __launch_bounds__(kThreads) void kb_xsk(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int b = xcd_pair_tile(blockIdx.x, gridDim.x);
+  const int i = batch_find(grp, b);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  xsk_body<T, L, TB>(b - grp.off[i], a.g, a.spec0, a.sktab, a.twx);
+}
+
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void kb_zsk_finish(BatchGroup grp, const BatchArgs<T>* __restrict__ args) {
+  const int i = batch_find(grp, blockIdx.x);
+  const BatchArgs<T>& a = args[grp.idx[i]];
+  zsk_finish_body<T, L, ZB>(blockIdx.x - grp.off[i], a.g, a.spec0, a.x, a.vext, a.grad, a.og, a.have_old, a.pgrad,
+                            a.partial, a.pstride, a.twz);
 }
 
 // nslots blocks per grid: block b reduces slot b % nslots of grid b / nslots

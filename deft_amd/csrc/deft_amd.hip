@@ -67,8 +67,15 @@ int plan_ensure_sk(deft_plan_s *p);
 
 int evaluate_at(deft_plan_s *p, int prec, bool grad, bool have_old, int precond) {
   if (grad && precond == PGRAD_SK) {
-    if (p->batch) return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner runs on plans outside batches");
-    TRY(plan_ensure_sk(p));
+    // batched plans build their table when the batch is created or the
+    // preconditioner is chosen (deft_minimize_set_preconditioner), never
+    // inside the batch's rounds
+    if (p->batch) {
+      if (!p->sk_ok)
+        return set_error(DEFT_ERR_STATE, "structure-factor table missing (choose the preconditioner after the last mu change)");
+    } else {
+      TRY(plan_ensure_sk(p));
+    }
   }
   if (p->batch) return batch_evaluate(p, prec, grad, have_old, precond);
   if (prec == DEFT_FFT_FP64) return Launch<double>::evaluate(p, grad, have_old, precond);
@@ -267,9 +274,11 @@ struct DeviceOps {
   }
   void move(double s) { queue_move(p, s); }
   double newton_scale() const { return p->kT / p->dV; }
-  // the structure-factor pgrad: plans outside batches whose mu has a bulk density
+  // the structure-factor pgrad: plans whose mu has a bulk density
   bool sk_available() {
-    if (p->batch) return false;
+    // batched plans: built when the batch was created; graph-captured rounds
+    // run grad/n only
+    if (p->batch) return p->sk_ok && !p->batch->graphs;
     if (plan_ensure_sk(p) == DEFT_OK) return true;
     g_last_error.clear();
     return false;
@@ -616,6 +625,9 @@ int deft_batch_create(deft_plan *plans, int n, deft_batch *out) {
   int rc = DEFT_OK;
   for (int i = 0; i < n && rc == DEFT_OK; ++i) {
     rc = flush_moves(plans[i]);
+    // the structure-factor table, where the plan's mu has a bulk density: the
+    // gradient confirmation of batched minimizers uses it, as unbatched ones do
+    if (rc == DEFT_OK && plan_ensure_sk(plans[i]) != DEFT_OK) g_last_error.clear();
     if (rc == DEFT_OK && hipStreamSynchronize(plans[i]->stream) != hipSuccess)
       rc = set_error(DEFT_ERR_HIP, "hipStreamSynchronize of a plan failed");
   }
@@ -1309,11 +1321,7 @@ int deft_minimize_set_preconditioner(deft_minimize m, int pgrad_kind) {
   if (!m) return set_error(DEFT_ERR_ARG, "null minimizer");
   if (pgrad_kind != DEFT_PGRAD_NEW && pgrad_kind != DEFT_PGRAD_STRUCTURE_FACTOR)
     return set_error(DEFT_ERR_ARG, "the new-API minimizer takes DEFT_PGRAD_NEW or DEFT_PGRAD_STRUCTURE_FACTOR");
-  if (pgrad_kind == DEFT_PGRAD_STRUCTURE_FACTOR) {
-    if (m->p->batch)
-      return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner runs on plans outside batches");
-    TRY(plan_ensure_sk(m->p));
-  }
+  if (pgrad_kind == DEFT_PGRAD_STRUCTURE_FACTOR) TRY(plan_ensure_sk(m->p));
   if (pgrad_kind != m->precond_kind) {
     m->precond_kind = pgrad_kind;
     min_invalidate(*m);
@@ -1340,6 +1348,8 @@ int deft_batch_minimize(deft_batch b, deft_minimize *mins, int n) {
     if (!mins[i] || mins[i]->p->batch != b) return set_error(DEFT_ERR_ARG, "every minimizer's plan must be in the batch");
     for (int j = 0; j < i; ++j)
       if (mins[j]->p == mins[i]->p) return set_error(DEFT_ERR_ARG, "two minimizers share a plan");
+    if (b->graphs && mins[i]->precond_kind == PGRAD_SK)
+      return set_error(DEFT_ERR_UNSUPPORTED, "graph-captured batch rounds take the new API's grad/n only");
   }
   if (b->active > 0) return set_error(DEFT_ERR_STATE, "the batch is already minimizing");
   constexpr size_t kFiberStack = 512 * 1024;

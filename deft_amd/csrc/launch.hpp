@@ -539,11 +539,16 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
     a.grad = p->g;
     a.pgrad = p->pg;
     a.dscal = p->dscal;
+    a.sktab = p->sktab;
     a.kind = reqs[i].grad ? 2 : 1;
+    if (reqs[i].grad && reqs[i].precond == PGRAD_SK && !(p->sk_ok && p->sktab))
+      return set_error(DEFT_ERR_STATE, "structure-factor table missing");
   }
   auto P = [&](int i) { return reqs[i].p; };
   auto all = [&](int) { return true; };
   auto gr = [&](int i) { return reqs[i].grad; };
+  auto gr_plain = [&](int i) { return reqs[i].grad && reqs[i].precond != PGRAD_SK; };
+  auto gr_sk = [&](int i) { return reqs[i].grad && reqs[i].precond == PGRAD_SK; };
   auto zkey = [&](int i) { return std::make_tuple(P(i)->Nz, P(i)->tz[ti], 0); };
   auto zkey_g = [&](int i) { return std::make_tuple(P(i)->Nz, P(i)->tz[ti], reqs[i].grad ? 1 : 0); };
   auto ykey = [&](int i) { return std::make_tuple(P(i)->Ny, P(i)->ty[ti], 0); };
@@ -558,9 +563,12 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
   std::vector<Group> g_phi = make_groups(n, zkey_g, zbl, all);
   std::vector<Group> g_yg = make_groups(n, ykey, ybl, gr);
   std::vector<Group> g_xb = make_groups(n, xkey, xbl, gr);
-  std::vector<Group> g_zg = make_groups(n, zkey, zbl, gr);
+  std::vector<Group> g_zg = make_groups(n, zkey, zbl, gr_plain);
+  std::vector<Group> g_zsk = make_groups(n, zkey, zbl, gr_sk);
+  std::vector<Group> g_ysk = make_groups(n, ykey, ybl, gr_sk);
+  std::vector<Group> g_xsk = make_groups(n, xkey, xbl, gr_sk);
   std::vector<Group> g_red = make_groups(n, rkey, [&](int i) { return reqs[i].grad ? 8 : 6; }, all);
-  std::vector<Group> *stages[] = {&g_dens, &g_y1, &g_xf, &g_phi, &g_yg, &g_xb, &g_zg, &g_red};
+  std::vector<Group> *stages[] = {&g_dens, &g_y1, &g_xf, &g_phi, &g_yg, &g_xb, &g_zg, &g_zsk, &g_ysk, &g_xsk, &g_red};
   // staging layout: args, then offsets and member indices of every group
   size_t at = align_up(S.used, 256);
   const size_t args_at = at;
@@ -646,6 +654,27 @@ int Launch<T>::evaluate_batch(hipStream_t st, BatchStage &S, const BatchReq *req
     constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
     hipLaunchKernelGGL((kb_zgrad<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs, 0);
   });
+  // structure-factor preconditioner (sk_apply_fused's passes, batched)
+  zpass(g_zsk, [&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zgrad_sk<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_ysk, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, false, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 0);
+  });
+  tpass(g_xsk, [&](auto LX, auto TBc, const Group &g) {
+    constexpr int L = decltype(LX)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_xsk<T, L, TB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
+  tpass(g_ysk, [&](auto LY, auto TBc, const Group &g) {
+    constexpr int L = decltype(LY)::value, TB = decltype(TBc)::value;
+    hipLaunchKernelGGL((kb_ypass<T, L, true, TB>), dim3(g.total(), 1), blk, 0, st, grp(g), dargs, 0, 0);
+  });
+  zpass(g_zsk, [&](auto LZ, auto ZBc, const Group &g) {
+    constexpr int L = decltype(LZ)::value, ZB = decltype(ZBc)::value;
+    hipLaunchKernelGGL((kb_zsk_finish<T, L, ZB>), dim3(g.total()), blk, 0, st, grp(g), dargs);
+  });
   for (const Group &g : g_red) {
     hipLaunchKernelGGL(kb_reduce<T>, dim3(g.total()), blk, 0, st, grp(g), dargs, g.flag, 0);
     ++nl;
@@ -693,6 +722,8 @@ int Launch<T>::evaluate_batch_graph(hipStream_t st, BatchStage &S, size_t region
     a.have_old = reqs[r].have_old ? 1 : 0;
     a.precond = reqs[r].precond;
     a.kind = reqs[r].grad ? 2 : 1;
+    if (reqs[r].grad && reqs[r].precond == PGRAD_SK)
+      return set_error(DEFT_ERR_UNSUPPORTED, "graph-captured batch rounds take the new API's grad/n only");
   }
   std::vector<int> sig;
   for (int i = 0; i < nplans; ++i) {

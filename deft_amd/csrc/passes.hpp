@@ -1116,19 +1116,19 @@ __global__ __launch_bounds__(kThreads) void k_xsk(GridArgs g, cx<T>* __restrict_
 }
 
 template <typename T, int L, int ZB>
-__global__ __launch_bounds__(kThreads) void k_zsk_finish(GridArgs g, const cx<T>* __restrict__ spec,
-                                                         const double* __restrict__ x, const T* __restrict__ vext,
-                                                         const double* __restrict__ grad,
-                                                         const double* __restrict__ oldgrad, int have_old,
-                                                         double* __restrict__ pgrad, double* __restrict__ partial,
-                                                         int pstride, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void zsk_finish_body(int bx, const GridArgs& g, const cx<T>* __restrict__ spec,
+                                                const double* __restrict__ x, const T* __restrict__ vext,
+                                                const double* __restrict__ grad,
+                                                const double* __restrict__ oldgrad, int have_old,
+                                                double* __restrict__ pgrad, double* __restrict__ partial,
+                                                int pstride, const cx<T>* __restrict__ tw) {
   constexpr int P = zpairs(L, ZB);
   constexpr int SS = lds_stride<T, L>();
   __shared__ cx<T> A[zbuf_size<T, L, P>()];
   __shared__ cx<T> B[zbuf_size<T, L, P>()];
   const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long lines = (long long)g.Nx * g.Ny;
-  const long long line0 = (long long)blockIdx.x * 2 * P;
+  const long long line0 = (long long)bx * 2 * P;
   load_c2r_packed<T, L, P>(spec, line0, lines, g.ldk, A, B);
   const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl);
   double dots[2] = {0.0, 0.0};
@@ -1152,7 +1152,16 @@ __global__ __launch_bounds__(kThreads) void k_zsk_finish(GridArgs g, const cx<T>
       if (have_old) dots[1] += pg * oldgrad[idx];
     }
   }
-  block_sum_store<2>(dots, partial, SLOT_PGG, pstride, blockIdx.x);
+  block_sum_store<2>(dots, partial, SLOT_PGG, pstride, bx);
+}
+template <typename T, int L, int ZB>
+__global__ __launch_bounds__(kThreads) void k_zsk_finish(GridArgs g, const cx<T>* __restrict__ spec,
+                                                         const double* __restrict__ x, const T* __restrict__ vext,
+                                                         const double* __restrict__ grad,
+                                                         const double* __restrict__ oldgrad, int have_old,
+                                                         double* __restrict__ pgrad, double* __restrict__ partial,
+                                                         int pstride, const cx<T>* __restrict__ tw) {
+  zsk_finish_body<T, L, ZB>(blockIdx.x, g, spec, x, vext, grad, oldgrad, have_old, pgrad, partial, pstride, tw);
 }
 
 // ---------------------------------------------------------------------------

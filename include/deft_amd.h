@@ -310,7 +310,7 @@ int deft_minimize_set_miniter(deft_minimize m, int miniter);
 int deft_minimize_precondition(deft_minimize m, int use_preconditioning);
 /* Which preconditioned gradient precondition(1) uses: DEFT_PGRAD_NEW (the
  * reference's grad/n, default) or DEFT_PGRAD_STRUCTURE_FACTOR (no counterpart
- * in the reference; plans outside batches).  The minimum is the same; the
+ * in the reference; in batches too, not with graph-captured rounds).  The minimum is the same; the
  * search directions, and so the iteration count, are not. */
 int deft_minimize_set_preconditioner(deft_minimize m, int pgrad_kind);
 /* Minimize::improve_energy: *more = 1 while the minimizer wants to continue. */

@@ -105,3 +105,53 @@ def test_batch_misuse_is_rejected(gpu):
     b.close()
     for f in (f1, f2, f3):
         f.close()
+
+
+@pytest.mark.parametrize("ladder", [True, False])
+def test_batch_structure_factor_follows_unbatched(gpu, ladder):
+    """The structure-factor preconditioner in batched rounds: the same fused
+    passes per grid, so every grid follows its unbatched trajectory bit for bit."""
+    insts = small_c3(4, seed=11)
+    prec = deft_amd.FFT_FP32 if ladder else deft_amd.FFT_FP64
+    alone = []
+    for inst in insts:
+        f = make_fluid(inst, prec)
+        m = make_min(f, ladder)
+        m.set_preconditioner(deft_amd.PGRAD_STRUCTURE_FACTOR)
+        m.run()
+        alone.append((m.stats(), m.energy()))
+        m.close()
+        f.close()
+    fluids = [make_fluid(inst, prec) for inst in insts]
+    mins = [make_min(f, ladder) for f in fluids]
+    for m in mins:
+        m.set_preconditioner(deft_amd.PGRAD_STRUCTURE_FACTOR)
+    b = deft_amd.Batch(fluids, graphs=False)
+    b.minimize(mins)
+    for i, m in enumerate(mins):
+        s, (sa, ea) = m.stats(), alone[i]
+        assert s["iter"] == sa["iter"], (i, s, sa)
+        assert s["stop_reason"] == sa["stop_reason"]
+        assert s["num_energy_calcs"] == sa["num_energy_calcs"] and s["num_grad_calcs"] == sa["num_grad_calcs"]
+        assert m.energy() == ea, (i, m.energy(), ea)
+    b.close()
+    for m in mins:
+        m.close()
+    for f in fluids:
+        f.close()
+
+
+def test_batch_graph_rounds_refuse_structure_factor(gpu):
+    insts = small_c3(2, seed=5)
+    fluids = [make_fluid(inst, deft_amd.FFT_FP64) for inst in insts]
+    mins = [make_min(f, False) for f in fluids]
+    for m in mins:
+        m.set_preconditioner(deft_amd.PGRAD_STRUCTURE_FACTOR)
+    b = deft_amd.Batch(fluids, graphs=True)
+    with pytest.raises(deft_amd.DeftError):
+        b.minimize(mins)
+    b.close()
+    for m in mins:
+        m.close()
+    for f in fluids:
+        f.close()
