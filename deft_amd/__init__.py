@@ -38,6 +38,7 @@ WHITE_BEAR = 0
 WHITE_BEAR_MARK_II = 1
 WHITE_BEAR_MARK_II_SLOW = 2
 WHITE_BEAR_TENSOR = 3
+SFMT = 4
 
 PGRAD_NEW = 1
 PGRAD_EFFECTIVE_POTENTIAL = 2
@@ -93,6 +94,8 @@ _SIGNATURES = {
     "deft_plan_set_mu": ([_P, _D], _I),
     "deft_plan_bulk_density": ([_P, _DP], _I),
     "deft_plan_set_functional": ([_P, _I], _I),
+    "deft_plan_set_sfmt": ([_P, _D, _D, _D], _I),
+    "deft_sfmt_homogeneous": ([_D, _D, _D, _D, _D, _D, _DP, _DP], _I),
     "deft_contact_values": ([_P, _DP], _I),
     "deft_copy_bandwidth": ([ctypes.c_longlong, _I, _DP], _I),
     "deft_bulk_state": ([_I, _D, _D, _D, _DP, _DP, _DP], _I),
@@ -265,6 +268,34 @@ def bulk_free_energy_density(n, R=1.0, kT=1.0):
     out = ctypes.c_double()
     _check(lib().deft_bulk_free_energy_density(n, R, kT, ctypes.byref(out)))
     return out.value
+
+
+class HomogeneousSFMTFluid:
+    """The generated HomogeneousSFMTFluid (src/haskell/SFMT.hs:39-41): a uniform
+    soft-FMT fluid with settable sigma, epsilon, Xi, kT, n and mu; ``energy()`` is
+    the free energy per volume kT Phi + kT n (ln(n/nQ) - 1) - mu n and
+    ``d_by_dn()`` its derivative (drivers set mu = d_by_dn() at mu = 0,
+    papers/fuzzy-fmt/figs/new-radial-wca.cpp:121-123)."""
+
+    def __init__(self, sigma=1.0, epsilon=1.0, Xi=0.1, kT=1.0, n=0.0, mu=0.0):
+        self.sigma, self.epsilon, self.Xi, self.kT, self.n, self.mu = sigma, epsilon, Xi, kT, n, mu
+
+    def _eval(self):
+        e, d = ctypes.c_double(), ctypes.c_double()
+        _check(lib().deft_sfmt_homogeneous(float(self.sigma), float(self.epsilon), float(self.Xi), float(self.kT),
+                                           float(self.n), float(self.mu), ctypes.byref(e), ctypes.byref(d)))
+        return e.value, d.value
+
+    def energy(self):
+        return self._eval()[0]
+
+    def d_by_dn(self):
+        return self._eval()[1]
+
+    def pressure(self):
+        """n d_by_dn - energy (papers/fuzzy-fmt/figs/homogeneous.cpp:57), at the set mu."""
+        e, d = self._eval()
+        return self.n * d - e
 
 
 # ---------------------------------------------------------------------------
@@ -507,7 +538,8 @@ class WhiteBearFluidVeff:
         return {k: out[i] for i, k in enumerate(("n0", "gSigmaS", "YuWu_S", "nA", "gSigmaA", "Gross", "gSigmaS_cpp"))}
 
     def weighted_densities(self):
-        out = np.empty((11 if self._functional == WHITE_BEAR_TENSOR else 5,) + self.shape)
+        nw = {WHITE_BEAR_TENSOR: 11, SFMT: 16}.get(self._functional, 5)
+        out = np.empty((nw,) + self.shape)
         _check(lib().deft_weighted_densities(self._h, _dptr(out)))
         return out
 
@@ -524,6 +556,54 @@ class WhiteBearFluidVeff:
         out = np.empty(self.shape)
         _check(lib().deft_ifft(self._h, a.ctypes.data_as(_DP), _dptr(out)))
         return out
+
+
+class SFMTFluidVeff(WhiteBearFluidVeff):
+    """The generated SFMTFluidVeff (src/haskell/SFMT.hs:35-37,
+    src/haskell/create_generators.py:25-26): soft FMT of Veff on one grid,
+    with the sigma, epsilon and Xi accessors of the generated class (alpha
+    follows from sigma, epsilon and kT).  Runs on the generic line passes."""
+
+    def __init__(self, shape, cell, sigma=1.0, epsilon=1.0, Xi=0.1, kT=1.0, mu=0.0, fft_precision=FFT_FP32):
+        super().__init__(shape, cell, R=sigma / 2, kT=kT, mu=mu, fft_precision=fft_precision)
+        self._sigma, self._epsilon, self._Xi = float(sigma), float(epsilon), float(Xi)
+        _check(lib().deft_plan_set_sfmt(self._h, self._sigma, self._epsilon, self._Xi))
+        self.functional = SFMT
+
+    def _set(self):
+        _check(lib().deft_plan_set_sfmt(self._h, self._sigma, self._epsilon, self._Xi))
+
+    @property
+    def sigma(self):
+        return self._sigma
+
+    @sigma.setter
+    def sigma(self, v):
+        self._sigma = float(v)
+        self._set()
+
+    @property
+    def epsilon(self):
+        return self._epsilon
+
+    @epsilon.setter
+    def epsilon(self, v):
+        self._epsilon = float(v)
+        self._set()
+
+    @property
+    def Xi(self):
+        return self._Xi
+
+    @Xi.setter
+    def Xi(self, v):
+        self._Xi = float(v)
+        self._set()
+
+    @property
+    def alpha(self):
+        """sigma (2/(1 + sqrt(ln 2 kT/epsilon)))^(1/6) (src/haskell/SFMT.hs:55-56)."""
+        return self._sigma * (2.0 / (1.0 + math.sqrt(math.log(2.0) * self._kT / self._epsilon))) ** (1.0 / 6.0)
 
 
 class Minimize:

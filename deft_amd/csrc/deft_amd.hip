@@ -61,6 +61,40 @@ int ensure_gwork(deft_plan_s *p) {
   return DEFT_OK;
 }
 
+// Soft FMT: the radial weight table of the plan's cell, kT and softness
+// (k_sfmt_table), built whenever one of them is set.
+int plan_build_sfmt_table(deft_plan_s *p) {
+  if (!p->stab) {
+    if (p->stream && hipStreamSynchronize(p->stream) != hipSuccess)
+      return set_error(DEFT_ERR_HIP, "stream synchronisation failed");
+    if (hipMalloc(&p->stab, 8 * sizeof(double) * (size_t)p->ks) != hipSuccess) {
+      p->stab = nullptr;
+      return set_error(DEFT_ERR_ALLOC, "hipMalloc of the soft-FMT weight table failed");
+    }
+  }
+  const SFMTConst c = make_sfmt_const(p->sfmt_sigma, p->sfmt_epsilon, p->kT, p->sfmt_xi);
+  const GridArgs g = grid_args(p);
+  const long long nk = (long long)p->Nx * p->Ny * p->Nkz;
+  const int blocks = (int)std::min<long long>(4096, (nk + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_sfmt_table, dim3(blocks), dim3(kThreads), 0, p->stream, g, c, p->stab);
+  DEFT_HIP_CHECK(hipGetLastError());
+  DEFT_HIP_CHECK(hipStreamSynchronize(p->stream));
+  p->stab_ok = true;
+  return DEFT_OK;
+}
+
+// Uniform soft-FMT fluid at density n: kT Phi + kT n (ln(n/nQ) - 1) and its d/dn
+// (the k = 0 weights; HomogeneousSFMTFluid of src/haskell/SFMT.hs:39-41 at mu = 0).
+void sfmt_bulk(const SFMTConst &c, double n, double kT, double *f, double *d) {
+  double v[16], w0[4], p1, p2, p3, dd[16];
+  sfmt_uniform_fields(n, c, v, w0);
+  sfmt_phi(v, p1, p2, p3);
+  sfmt_phi_partials(v, dd);
+  const double lg = std::log(n / water_nQ(kT));
+  *f = kT * (p1 + p2 + p3) + kT * n * (lg - 1.0);
+  *d = kT * (dd[0] * w0[0] + dd[1] * w0[1] + dd[2] * w0[2] + dd[3] * w0[3]) + kT * lg;
+}
+
 int batch_evaluate(deft_plan_s *p, int prec, bool grad, bool have_old, int precond);
 int batch_cg(deft_plan_s *p, int reset, double beta, int have_dir);
 int plan_ensure_sk(deft_plan_s *p);
@@ -158,6 +192,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->wtab);
   (void)hipFree(p->gwork);
   (void)hipFree(p->sktab);
+  (void)hipFree(p->stab);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -747,6 +782,8 @@ int deft_plan_set_engine(deft_plan p, int generic) {
   if (p->batch) return set_error(DEFT_ERR_STATE, "the plan is in a batch");
   if (!generic && !(fused_ok(p->Nx) && fused_ok(p->Ny) && fused_ok(p->Nz)))
     return set_error(DEFT_ERR_UNSUPPORTED, "this plan's edges have no fused kernels");
+  if (!generic && p->functional == DEFT_SFMT)
+    return set_error(DEFT_ERR_UNSUPPORTED, "soft FMT runs on the generic passes");
   if (!generic == !p->generic) return DEFT_OK;
   TRY(flush_moves(p));
   TRY(plan_sync(p));
@@ -826,14 +863,15 @@ int deft_plan_set_fft_precision(deft_plan p, int prec) {
 int deft_plan_set_functional(deft_plan p, int functional) {
   TRY(check_plan(p));
   if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
-      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR)
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR && functional != DEFT_SFMT)
     return set_error(DEFT_ERR_ARG, "unknown functional");
   if (functional != DEFT_WHITE_BEAR && p->batch)
     return set_error(DEFT_ERR_STATE, "plans in a batch keep the White-Bear functional");
-  if (functional == DEFT_WHITE_BEAR_TENSOR && p->spec_fields < 12) {
-    // eleven weighted-density spectra plus the density spectrum
+  // the weighted-density spectra plus the density spectrum
+  const int need = functional == DEFT_SFMT ? 17 : (functional == DEFT_WHITE_BEAR_TENSOR ? 12 : 6);
+  if (p->spec_fields < need) {
     void *spec = nullptr;
-    const size_t bytes = 12 * sizeof(cx<double>) * (size_t)p->ks;
+    const size_t bytes = need * sizeof(cx<double>) * (size_t)p->ks;
     if (hipStreamSynchronize(p->stream) != hipSuccess) return set_error(DEFT_ERR_HIP, "stream synchronisation failed");
     if (hipMalloc(&spec, bytes) != hipSuccess) return set_error(DEFT_ERR_ALLOC, "hipMalloc of the tensor spectra failed");
     if (hipMemset(spec, 0, bytes) != hipSuccess) {
@@ -842,17 +880,53 @@ int deft_plan_set_functional(deft_plan p, int functional) {
     }
     (void)hipFree(p->spec);
     p->spec = spec;
-    p->spec_fields = 12;
+    p->spec_fields = need;
   }
   // eleven fields per z line: the smaller z blocks keep the tensor Phi pass at
   // two waves per SIMD
   if (functional == DEFT_WHITE_BEAR_TENSOR) p->tz[0] = p->tz[1] = kZBlocks[0];
   const int was = p->functional;
+  const bool was_generic = p->generic;
   p->functional = functional;
   p->sk_ok = false;
-  const int rc = ensure_gwork(p);
-  if (rc != DEFT_OK) p->functional = was;
+  // soft FMT runs on the generic passes; leaving it restores the fused ones
+  // where the edges have them
+  if (functional == DEFT_SFMT)
+    p->generic = true;
+  else if (was == DEFT_SFMT)
+    p->generic = !(fused_ok(p->Nx) && fused_ok(p->Ny) && fused_ok(p->Nz));
+  int rc = ensure_gwork(p);
+  if (rc == DEFT_OK && functional == DEFT_SFMT) rc = plan_build_sfmt_table(p);
+  if (rc != DEFT_OK) {
+    p->functional = was;
+    p->generic = was_generic;
+  }
   return rc;
+}
+
+int deft_plan_set_sfmt(deft_plan p, double sigma, double epsilon, double xi) {
+  TRY(check_plan(p));
+  auto ok = [](double v) { return std::isfinite(v) && v > 0.0; };
+  if (!ok(sigma) || !ok(epsilon) || !ok(xi)) return set_error(DEFT_ERR_ARG, "sigma, epsilon and Xi must be positive");
+  p->sfmt_sigma = sigma;
+  p->sfmt_epsilon = epsilon;
+  p->sfmt_xi = xi;
+  p->stab_ok = false;
+  p->sk_ok = false;
+  if (p->functional == DEFT_SFMT) TRY(plan_build_sfmt_table(p));
+  return DEFT_OK;
+}
+
+int deft_sfmt_homogeneous(double sigma, double epsilon, double xi, double kT, double n, double mu, double *energy,
+                          double *d_by_dn) {
+  auto ok = [](double v) { return std::isfinite(v) && v > 0.0; };
+  if (!ok(sigma) || !ok(epsilon) || !ok(xi) || !ok(kT) || !ok(n) || !std::isfinite(mu))
+    return set_error(DEFT_ERR_ARG, "sigma, epsilon, Xi, kT and n must be positive, mu finite");
+  double f, d;
+  sfmt_bulk(make_sfmt_const(sigma, epsilon, kT, xi), n, kT, &f, &d);
+  if (energy) *energy = f - mu * n;
+  if (d_by_dn) *d_by_dn = d - mu;
+  return DEFT_OK;
 }
 
 int deft_plan_set_mu(deft_plan p, double mu) {
@@ -996,7 +1070,7 @@ int deft_functional_grad_kind(deft_plan p, const double *veff, double *outgrad, 
 int deft_weighted_densities(deft_plan p, double *out5) {
   TRY(check_plan(p));
   if (!out5) return set_error(DEFT_ERR_ARG, "null output");
-  const int nw = p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5;
+  const int nw = plan_fields(p);
   TRY(ensure_tmp(p, nw * sizeof(double) * (size_t)p->N));
   if (p->prec == DEFT_FFT_FP64)
     TRY(Launch<double>::weighted_densities(p, p->tmp));
@@ -1064,7 +1138,7 @@ int deft_copy_bandwidth(long long bytes, int reps, double *gbps) {
 int deft_contact_values(deft_plan p, double *out7) {
   TRY(check_plan(p));
   if (!out7) return set_error(DEFT_ERR_ARG, "null output");
-  if (p->functional == DEFT_WHITE_BEAR_TENSOR)
+  if (p->functional == DEFT_WHITE_BEAR_TENSOR || p->functional == DEFT_SFMT)
     return set_error(DEFT_ERR_UNSUPPORTED, "contact values are defined for White-Bear and Mark II");
   TRY(flush_moves(p));
   const size_t N = (size_t)p->N;
@@ -1197,8 +1271,17 @@ namespace {
 // Bulk density of the plan's mu and functional: mu(n) rises monotonically from
 // -inf (n -> 0) towards +inf (n3 -> 1); bisection in ln n.
 bool bulk_density_of_mu(const deft_plan_s *p, double &nb) {
-  const double nmax = 1.0 / (4 * M_PI / 3 * p->R * p->R * p->R);
+  const bool soft = p->functional == DEFT_SFMT;
+  const SFMTConst sc = make_sfmt_const(p->sfmt_sigma, p->sfmt_epsilon, p->kT, p->sfmt_xi);
+  double r0[7];
+  sfmt_radial(0.0, sc, r0);  // soft FMT: n3 = w3(0) n
+  const double nmax = soft ? 1.0 / r0[0] : 1.0 / (4 * M_PI / 3 * p->R * p->R * p->R);
   auto mu_of = [&](double n) {
+    if (soft) {
+      double f, d;
+      sfmt_bulk(sc, n, p->kT, &f, &d);
+      return d;
+    }
     double fe, dfe;
     bulk_terms(n, p->R, p->kT, &fe, &dfe, p->functional);
     return p->kT * std::log(n / water_nQ(p->kT)) + dfe;
@@ -1223,6 +1306,8 @@ namespace {
 // spectral element on the device.
 int plan_ensure_sk(deft_plan_s *p) {
   if (p->sk_ok) return DEFT_OK;
+  if (p->functional == DEFT_SFMT)
+    return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner is built for the White-Bear functionals");
   double nb;
   if (!bulk_density_of_mu(p, nb))
     return set_error(DEFT_ERR_ARG, "no bulk density has this chemical potential (structure-factor preconditioner)");

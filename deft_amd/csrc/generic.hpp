@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "passes.hpp"
+#include "sfmt.hpp"
 
 namespace deft {
 
@@ -456,19 +457,121 @@ __global__ __launch_bounds__(kThreads) void kg_wmul_bwd(GridArgs g, const cx<T>*
   }
 }
 
+// --- soft FMT (sfmt.hpp): sixteen weighted fields from a per-plan table ----
+
+// The table: the seven radial functions of |k| (w3, w2, w1, w0, W2v, W1v, W2t)
+// times 1/N for every valid spectral element, 8 doubles each, rows padded to ldk.
+static __global__ __launch_bounds__(kThreads) void k_sfmt_table(GridArgs g, SFMTConst c, double* __restrict__ tab) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    const int kz = (int)(i % g.Nkz);
+    const long long r = i / g.Nkz;
+    const int y = (int)(r % g.Ny), x = (int)(r / g.Ny);
+    const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+    const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
+    const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
+    const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
+    double rad[7];
+    sfmt_radial(sqrt(kx * kx + ky * ky + kz_ * kz_), c, rad);
+    double* t = tab + 8 * (r * g.ldk + kz);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) t[j] = rad[j] * g.invN;
+    t[7] = 0.0;
+  }
+}
+
+// The sixteen weights of one spectral element: scalars (w3, w2, w1, w0), the
+// vector radial factors times k_j (the weights are i k_j W), and the tensor
+// weights (W2t (1 - 3 kh_i^2) on the diagonal, 3 W2t kh_i kh_j off it).
+template <typename T>
+__device__ __forceinline__ void sfmt_element(const GridArgs& g, long long i, long long& idx, T (&wt)[16]) {
+  const int kz = (int)(i % g.Nkz);
+  const long long r = i / g.Nkz;
+  const int y = (int)(r % g.Ny), x = (int)(r / g.Ny);
+  idx = r * g.ldk + kz;
+  const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+  const double k[3] = {mx * g.b[0] + my * g.b[3] + mz * g.b[6], mx * g.b[1] + my * g.b[4] + mz * g.b[7],
+                       mx * g.b[2] + my * g.b[5] + mz * g.b[8]};
+  const double* t = g.stab + 8 * idx;
+  const double kk = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+  const double ik2 = kk > 0.0 ? 1.0 / kk : 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) wt[j] = (T)t[j];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    wt[4 + j] = (T)(t[4] * k[j]);
+    wt[7 + j] = (T)(t[5] * k[j]);
+  }
+  const double w2t = t[6];
+  wt[10] = (T)(w2t * (1.0 - 3.0 * k[0] * k[0] * ik2));
+  wt[11] = (T)(w2t * (1.0 - 3.0 * k[1] * k[1] * ik2));
+  wt[12] = (T)(w2t * (1.0 - 3.0 * k[2] * k[2] * ik2));
+  wt[13] = (T)(w2t * 3.0 * k[0] * k[1] * ik2);
+  wt[14] = (T)(w2t * 3.0 * k[1] * k[2] * ik2);
+  wt[15] = (T)(w2t * 3.0 * k[2] * k[0] * ik2);
+}
+
+// vector fields (4..9) carry i k_j: odd under k -> -k
+__device__ __forceinline__ constexpr bool sfmt_odd(int f) { return f >= 4 && f < 10; }
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kg_wmul_fwd_sfmt(GridArgs g, const cx<T>* __restrict__ in,
+                                                             cx<T>* __restrict__ out) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    long long idx;
+    T wt[16];
+    sfmt_element<T>(g, i, idx, wt);
+    const cx<T> s = in[idx];
+#pragma unroll
+    for (int f = 0; f < 16; ++f) out[f * g.ks + idx] = sfmt_odd(f) ? mul_i(scale(s, wt[f])) : scale(s, wt[f]);
+  }
+}
+
+// sum over f of w_f(-k) in[f] -> out
+template <typename T>
+__global__ __launch_bounds__(kThreads) void kg_wmul_bwd_sfmt(GridArgs g, const cx<T>* __restrict__ in,
+                                                             cx<T>* __restrict__ out) {
+  const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
+    long long idx;
+    T wt[16];
+    sfmt_element<T>(g, i, idx, wt);
+    cx<T> acc = {(T)0, (T)0};
+#pragma unroll
+    for (int f = 0; f < 16; ++f) {
+      const cx<T> s = in[f * g.ks + idx];
+      if (sfmt_odd(f))
+        acc = acc - mul_i(scale(s, wt[f]));
+      else
+        acc = acc + scale(s, wt[f]);
+    }
+    out[idx] = acc;
+  }
+}
+
 // Phi sums of the NW weighted densities in w (fields N apart) and, with GRAD,
-// their partials written over them (the arithmetic of zphi_body).
+// their partials written over them (the arithmetic of zphi_body; FN = 4 is
+// the soft-FMT Phi of sfmt.hpp).
 template <typename T, bool GRAD, int FN>
 __global__ __launch_bounds__(kThreads) void kg_phi(GridArgs g, double* __restrict__ w, double* __restrict__ partial,
                                                    int pstride) {
-  constexpr int NW = FN == 3 ? 11 : 5;
+  constexpr int NW = FN == 4 ? 16 : (FN == 3 ? 11 : 5);
   double acc[3] = {0.0, 0.0, 0.0};
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < g.N; i += (long long)gridDim.x * kThreads) {
     double v[NW];
 #pragma unroll
     for (int f = 0; f < NW; ++f) v[f] = w[f * g.N + i];
     double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-    if constexpr (NW == 11) {
+    if constexpr (NW == 16) {
+      sfmt_phi(v, p1, p2, p3);
+      if constexpr (GRAD) {
+        double dd[16];
+        sfmt_phi_partials(v, dd);
+#pragma unroll
+        for (int f = 0; f < 16; ++f) w[f * g.N + i] = (double)(T)dd[f];
+      }
+    } else if constexpr (NW == 11) {
       wbt_phi(v, g.wb, p1, p2, p3);
       if constexpr (GRAD) {
         double dd[11], q1, q2, q3;

@@ -124,7 +124,9 @@ static void launch_ypass(deft_plan_s *p, const GridArgs &g, cx<T> *data, bool in
 // (Mark II: compile-time factors; tensor White-Bear: eleven weighted fields,
 // whose x passes use the first tile size only).  Those launches are compiled
 // in their own translation units (tensor_f32.hip, tensor_f64.hip).
-inline int plan_fields(const deft_plan_s *p) { return p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5; }
+inline int plan_fields(const deft_plan_s *p) {
+  return p->functional == DEFT_SFMT ? 16 : (p->functional == DEFT_WHITE_BEAR_TENSOR ? 11 : 5);
+}
 
 template <typename T>
 struct TensorLaunch {
@@ -309,12 +311,18 @@ static void gen_density_to_weighted(deft_plan_s *p, const GridArgs &g, double *w
                        vext_ptr<T>(p), p->gwork, p->partial, p->pstride);
   }
   gen_forward<T>(p, g, p->gwork, p->N, 1, spec_ptr<T>(p, 0), 1.0);
-  with_nw(nw, [&](auto NWc) {
-    constexpr int NW = decltype(NWc)::value;
-    ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
-    hipLaunchKernelGGL((kg_wmul_fwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+  if (p->functional == DEFT_SFMT) {
+    ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + nw) * sizeof(cx<T>) * p->Nk + 64.0 * p->Nk);
+    hipLaunchKernelGGL((kg_wmul_fwd_sfmt<T>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
                        spec_ptr<T>(p, 0), spec_ptr<T>(p, 1));
-  });
+  } else {
+    with_nw(nw, [&](auto NWc) {
+      constexpr int NW = decltype(NWc)::value;
+      ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
+      hipLaunchKernelGGL((kg_wmul_fwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+                         spec_ptr<T>(p, 0), spec_ptr<T>(p, 1));
+    });
+  }
   gen_inverse<T>(p, g, spec_ptr<T>(p, 1), nw, w, p->N, 1.0);
 }
 
@@ -327,7 +335,7 @@ static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
   const int nw = plan_fields(p);
   double *w = p->gwork;
   gen_density_to_weighted<T>(p, g, w);
-  const int fn = p->functional == DEFT_WHITE_BEAR ? 0 : (nw == 11 ? 3 : 1);
+  const int fn = p->functional == DEFT_WHITE_BEAR ? 0 : (nw == 16 ? 4 : (nw == 11 ? 3 : 1));
   {
     ProfScope ps(p, kid<T>(K_GEN_POINT), (grad ? 16.0 : 8.0) * nw * p->N);
     auto phi = [&](auto FNc) {
@@ -343,17 +351,25 @@ static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
       phi(ic<0>{});
     else if (fn == 1)
       phi(ic<1>{});
-    else
+    else if (fn == 3)
       phi(ic<3>{});
+    else
+      phi(ic<4>{});
   }
   if (grad) {
     gen_forward<T>(p, g, w, p->N, nw, spec_ptr<T>(p, 1), 1.0);
-    with_nw(nw, [&](auto NWc) {
-      constexpr int NW = decltype(NWc)::value;
-      ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
-      hipLaunchKernelGGL((kg_wmul_bwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+    if (p->functional == DEFT_SFMT) {
+      ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + nw) * sizeof(cx<T>) * p->Nk + 64.0 * p->Nk);
+      hipLaunchKernelGGL((kg_wmul_bwd_sfmt<T>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
                          spec_ptr<T>(p, 1), spec_ptr<T>(p, 0));
-    });
+    } else {
+      with_nw(nw, [&](auto NWc) {
+        constexpr int NW = decltype(NWc)::value;
+        ProfScope ps(p, kid<T>(K_GEN_POINT), (1.0 + NW) * sizeof(cx<T>) * p->Nk);
+        hipLaunchKernelGGL((kg_wmul_bwd<T, NW>), dim3(kblocks_for(p)), dim3(kThreads), 0, p->stream, g,
+                           spec_ptr<T>(p, 1), spec_ptr<T>(p, 0));
+      });
+    }
     gen_inverse<T>(p, g, spec_ptr<T>(p, 0), 1, w, p->N, 1.0);
     ProfScope ps(p, kid<T>(K_GEN_POINT), (have_old ? 48.0 : 40.0) * p->N + (double)sizeof(T) * p->N);
     hipLaunchKernelGGL(kg_grad<T>, dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->x, vext_ptr<T>(p), p->og,

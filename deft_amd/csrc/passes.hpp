@@ -45,6 +45,7 @@ enum Slot : int {
 
 struct GridArgs {
   const double* wtab;  // FP64 weight table: (w3/N, w2/N) per spectral element, rows padded to ldk
+  const double* stab;  // soft FMT: the seven radial weights /N per spectral element (8 doubles), rows padded to ldk
   int Nx, Ny, Nz, Nkz;
   int ldk;        // padded half-spectrum row stride (complex values)
   long long N;    // real points

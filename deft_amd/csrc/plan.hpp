@@ -82,7 +82,7 @@ struct deft_plan_s {
   double recip[9] = {0};  // rows b1, b2, b3 with a_i . b_j = 2 pi delta_ij
   double volume = 0, dV = 0;
   double R = 1, kT = 1, mu = 0;
-  int functional = DEFT_WHITE_BEAR;  // DEFT_WHITE_BEAR or DEFT_WHITE_BEAR_MARK_II
+  int functional = DEFT_WHITE_BEAR;  // a deft_functional_kind
   int prec = DEFT_FFT_FP32;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -145,6 +145,11 @@ struct deft_plan_s {
   double sk_nb = 0.0, sk_floor = 0.0;
   // evaluation times of both engines from the last autotune (-1: not timed)
   double engine_fused_ms = -1.0, engine_generic_ms = -1.0;
+  // soft FMT (DEFT_SFMT): sigma, epsilon, Xi and the radial weight table
+  // (8 doubles per element of the padded half spectrum), valid while stab_ok
+  double sfmt_sigma = 1.0, sfmt_epsilon = 1.0, sfmt_xi = 0.1;
+  double *stab = nullptr;
+  bool stab_ok = false;
 };
 
 namespace deft {
@@ -179,6 +184,7 @@ inline void reciprocal(const double *c, double det, double *b) {
 inline GridArgs grid_args(const deft_plan_s *p) {
   GridArgs a{};
   a.wtab = p->wtab;
+  a.stab = p->stab;
   a.Nx = p->Nx;
   a.Ny = p->Ny;
   a.Nz = p->Nz;
@@ -196,7 +202,7 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.logNQ = std::log(water_nQ(p->kT));
   a.dV = p->dV;
   a.sk_floor = p->sk_floor;
-  a.wb = make_wb_const(p->R, p->functional);
+  a.wb = make_wb_const(p->R, p->functional == DEFT_SFMT ? DEFT_WHITE_BEAR : p->functional);
   return a;
 }
 

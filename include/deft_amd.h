@@ -71,7 +71,13 @@ typedef enum {
   /* TensorWhiteBear (src/haskell/WhiteBear.hs:44-51, src/HardSpheres.cpp:30-67):
    * Phi3 with the shell tensor density (six more weighted fields); same bulk as
    * White-Bear */
-  DEFT_WHITE_BEAR_TENSOR = 3
+  DEFT_WHITE_BEAR_TENSOR = 3,
+  /* Soft fundamental-measure theory (SFMTFluidVeff, src/haskell/SFMT.hs:13-133):
+   * Gaussian-smeared independent weights w3, w2, w1, w0, w2v, w1v and the
+   * transverse tensor weight (sixteen weighted fields), softness from
+   * deft_plan_set_sfmt.  Runs on the generic line passes; no batches, no
+   * structure-factor preconditioner, no contact estimators. */
+  DEFT_SFMT = 4
 } deft_functional_kind;
 
 /* Why a minimizer stopped (mirrors the return-false branches of
@@ -201,6 +207,16 @@ int deft_plan_set_mu(deft_plan plan, double mu);
 int deft_plan_bulk_density(deft_plan plan, double *nb);
 /* One of the deft_functional_kind values, default DEFT_WHITE_BEAR; drops nothing else. */
 int deft_plan_set_functional(deft_plan plan, int functional);
+/* Soft FMT parameters of the generated SFMTFluidVeff's sigma(), epsilon() and
+ * Xi() (src/haskell/SFMT.hs:43-56): alpha = sigma (2/(1 + sqrt(ln 2 kT/epsilon)))^(1/6)
+ * with the plan's kT.  Defaults sigma = epsilon = 1, Xi = 0.1.  All three must
+ * be positive and finite. */
+int deft_plan_set_sfmt(deft_plan plan, double sigma, double epsilon, double xi);
+/* HomogeneousSFMTFluid (src/haskell/SFMT.hs:39-41): free energy per volume
+ * kT Phi(n) + kT n (ln(n/nQ) - 1) - mu n of the uniform fluid at density n and
+ * its derivative d_by_dn; either output may be NULL.  Host FP64. */
+int deft_sfmt_homogeneous(double sigma, double epsilon, double xi, double kT, double n, double mu,
+                          double *energy, double *d_by_dn);
 /* Vext (N doubles, host); kept on the device in FP64 for the FP64 passes and
  * rounded to FP32 for the FP32 passes.  +inf marks a hard wall
  * (density exactly 0); NaN marks a constrained point as the old API's

@@ -1,0 +1,336 @@
+"""Soft fundamental-measure theory (src/haskell/SFMT.hs): oracle pins, the
+product's point code on the host, and GPU parity.
+
+The oracle (oracle/sfmt_oracle.py) is pinned by
+* the reference's own Faddeeva package (src/Faddeeva.cpp, built into
+  oracle/_ref by oracle/Makefile) for the complex erf the weights use;
+* the hard-sphere limit Xi -> 0: every weight against its textbook Rosenfeld
+  form with R = alpha/2 (step, shell, 1/(4 pi R) shell, vector shells, the
+  shell tensor (4 pi R^2/3) j2(kR)), and w0 against half the Rosenfeld n0
+  weight, as the reference writes it;
+* the k -> 0 limits against the k != 0 formulas;
+* finite differences of Phi and of the whole functional;
+* the reference's own check that a uniform fluid's energy is its volume times
+  HomogeneousSFMTFluid's energy (tests/new-generated.cpp:261-280, 2e-11).
+No published soft-FMT profile is checked here: the reference's radial
+profiles (papers/fuzzy-fmt/figs/new-data/radial-wca-*.dat) come from 800^3
+grids and WCA softness fits, not from a workload of this size.
+"""
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import special
+
+from oracle import fmt_oracle as O
+from oracle import sfmt_oracle as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libfaddeeva_ref.so")
+
+
+# ---------------------------------------------------------------------------
+# oracle pins (CPU)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.skipif(not os.path.isfile(REF_LIB), reason="oracle/_ref not built (needs the reference's tree)")
+def test_scipy_erf_is_the_reference_faddeeva():
+    lib = ctypes.CDLL(REF_LIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.faddeeva_ref_erf.argtypes = [dp, dp, ctypes.c_long, dp, dp]
+    u = np.linspace(0.0, 8.0, 81)
+    v = np.linspace(0.5, 24.0, 48)
+    U, V = np.meshgrid(u, v)
+    re, im = np.ascontiguousarray(U.ravel()), np.ascontiguousarray(V.ravel())
+    ore, oim = np.empty_like(re), np.empty_like(re)
+    lib.faddeeva_ref_erf(re.ctypes.data_as(dp), im.ctypes.data_as(dp), re.size, ore.ctypes.data_as(dp),
+                         oim.ctypes.data_as(dp))
+    ref = ore + 1j * oim
+    got = special.erf(re + 1j * im)
+    assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-13
+
+
+def test_erf_term_forms_agree():
+    """exp(-v^2) Re erf(u + iv): the generated code's product and the Faddeeva-w form."""
+    u = np.linspace(0.0, 6.0, 121)
+    for v in (2.0, 4.0, 7.0, 12.0, 20.0):
+        direct = np.exp(-v * v) * special.erf(u + 1j * v).real
+        w = special.wofz(v + 1j * u)
+        stable = np.exp(-v * v) - np.exp(-u * u) * (np.cos(2 * u * v) * w.real - np.sin(2 * u * v) * w.imag)
+        assert np.max(np.abs(direct - stable)) < 1e-14 * max(1.0, np.max(np.abs(direct))), v
+
+
+def test_hard_sphere_limit():
+    alpha, xi = 1.1, 1e-4
+    R = alpha / 2
+    k = np.linspace(0.05, 30.0, 600)
+    w3, w2, w1, w0, W2v, W1v, W2t = S.sfmt_radial(k, alpha, xi)
+    x = k * R
+    step = 4 * math.pi * (np.sin(x) - x * np.cos(x)) / k ** 3
+    shell = 4 * math.pi * R * np.sin(x) / k
+    tensor = 4 * math.pi * R * R / 3 * special.spherical_jn(2, x)
+    scale3, scale2 = 4 * math.pi / 3 * R ** 3, 4 * math.pi * R * R
+    assert np.max(np.abs(w3 - step)) < 1e-6 * scale3
+    assert np.max(np.abs(w2 - shell)) < 1e-6 * scale2
+    assert np.max(np.abs(w1 - shell / (4 * math.pi * R))) < 1e-6 * R
+    assert np.max(np.abs(w0 - np.sin(x) / (2 * x))) < 1e-6  # half the Rosenfeld n0 weight
+    assert np.max(np.abs(W2v - step)) < 1e-6 * scale3
+    assert np.max(np.abs(W1v - step / (4 * math.pi * R))) < 1e-6 * R * R
+    assert np.max(np.abs(W2t - tensor)) < 1e-6 * scale2
+
+
+def test_k_to_zero_limits():
+    for alpha, xi in ((1.0, 0.1), (1.1, 0.3), (0.9, 0.05)):
+        lim = S.sfmt_radial_at_zero(alpha, xi)
+        vals = S.sfmt_radial(np.array([2e-3]), alpha, xi)
+        for j in range(4):
+            assert abs(vals[j][0] - lim[j]) < 1e-5 * abs(lim[j]), (alpha, xi, j)
+        assert abs(vals[6][0]) < 1e-5, (alpha, xi)  # the tensor weight vanishes at k = 0
+
+
+def test_w2v_is_the_gradient_of_n3():
+    k = np.linspace(0.1, 40.0, 400)
+    for alpha, xi in ((1.0, 0.1), (1.1, 0.3)):
+        w3, _, _, _, W2v, _, _ = S.sfmt_radial(k, alpha, xi)
+        # the two closed forms differ only in round-off, which the 4 pi/k^2 prefactor scales
+        assert np.all(np.abs(W2v - w3) <= 1e-14 * np.max(np.abs(w3)) + 1e-15 * 4 * math.pi / k ** 2)
+
+
+def _random_fields(rng, n):
+    v = [rng.uniform(0.05, 0.5, n), rng.uniform(0.5, 3.0, n), rng.uniform(0.1, 0.6, n), rng.uniform(0.1, 1.0, n)]
+    v += [rng.normal(scale=0.2, size=n) for _ in range(6)]
+    v += [rng.normal(scale=0.1, size=n) for _ in range(6)]
+    return v
+
+
+def test_phi_partials_match_finite_differences():
+    rng = np.random.default_rng(3)
+    v = _random_fields(rng, 50)
+    p1, p2, p3, d = S.sfmt_phi(v)
+    for f in range(16):
+        h = 1e-6 * np.maximum(np.abs(v[f]), 0.1)
+        vp = list(v)
+        vm = list(v)
+        vp[f] = v[f] + h
+        vm[f] = v[f] - h
+        fp = sum(S.sfmt_phi(vp, False)[:3])
+        fm = sum(S.sfmt_phi(vm, False)[:3])
+        fd = (fp - fm) / (2 * h)
+        assert np.max(np.abs(fd - d[f]) / np.maximum(np.abs(d[f]), 1.0)) < 1e-7, S.FIELDS[f]
+
+
+def test_homogeneous_derivative_matches_finite_difference():
+    for n, kT in ((0.05, 1.0), (0.3, 0.5), (0.6, 2.0)):
+        f, d = S.homogeneous_sfmt(n, 1.0, 1.0, 0.2, kT)
+        h = 1e-6 * n
+        fd = (S.homogeneous_sfmt(n + h, 1.0, 1.0, 0.2, kT)[0] - S.homogeneous_sfmt(n - h, 1.0, 1.0, 0.2, kT)[0]) / (2 * h)
+        assert abs(fd - d) < 1e-7 * max(1.0, abs(d))
+
+
+def _uniform_case():
+    """tests/new-generated.cpp:261-280: SFMTFluid(2, 2, 100) with a1 = a2 = 2a/Nx,
+    a3 = a = 2, kT = 0.001, n = 0.3/(4 pi/3), mu = d_by_dn (Xi is our choice: the
+    test does not set it)."""
+    kT, n, xi = 0.001, 0.3 / (4 * math.pi / 3), 0.1
+    grid = O.Grid((2, 2, 100), np.diag([0.04, 0.04, 2.0]))
+    mu = S.sfmt_chemical_potential(n, 1.0, 1.0, xi, kT)
+    return grid, kT, n, xi, mu
+
+
+def test_uniform_fluid_is_volume_times_homogeneous():
+    grid, kT, n, xi, mu = _uniform_case()
+    f = S.SFMTFluidVeff(grid, 1.0, 1.0, xi, kT, mu, np.zeros(grid.shape), np.full(grid.shape, -kT * math.log(n)))
+    E = f.energy()
+    hf = S.homogeneous_sfmt(n, 1.0, 1.0, xi, kT, mu)[0]
+    assert abs(E / (grid.volume * hf) - 1) < 2e-11
+
+
+def _nonuniform_oracle(shape=(12, 10, 14), prec_cell=None, xi=0.15, kT=0.8):
+    rng = np.random.default_rng(11)
+    cell = np.array([[3.0, 0.0, 0.0], [0.4, 2.8, 0.0], [0.0, 0.3, 3.2]]) if prec_cell is None else prec_cell
+    grid = O.Grid(shape, cell)
+    nb = 0.5
+    mu = S.sfmt_chemical_potential(nb, 1.0, 1.0, xi, kT)
+    pos = grid.positions()
+    vext = 0.6 * kT * np.cos(2 * math.pi * pos[..., 2] / 3.2) + 0.3 * kT * np.sin(2 * math.pi * pos[..., 0] / 3.0)
+    vext[0, :, :4] = np.inf  # a hard wall patch
+    x = -kT * math.log(nb) + 0.3 * kT * rng.normal(size=shape)
+    return S.SFMTFluidVeff(grid, 1.0, 1.0, xi, kT, mu, vext, x), grid, vext, x, mu
+
+
+def test_functional_gradient_matches_finite_differences():
+    f, grid, _, x, _ = _nonuniform_oracle()
+    g = f.grad(x)
+    rng = np.random.default_rng(5)
+    for _ in range(6):
+        i = tuple(int(rng.integers(0, s)) for s in grid.shape)
+        if not np.isfinite(f.vext[i]):
+            continue
+        h = 1e-5
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (f.energy(xp) - f.energy(xm)) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * max(abs(g[i]), 1e-3 * np.max(np.abs(g))), i
+
+
+# ---------------------------------------------------------------------------
+# the product's point code on the host (deft_amd/csrc/sfmt.hpp)
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("sfmt") / "sfmt_harness")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", os.path.join(ROOT, "tests", "host", "sfmt_harness.cpp"),
+                    "-o", exe], check=True)
+    return exe
+
+
+def _run(exe, text):
+    return subprocess.run([exe], input=text, capture_output=True, text=True, check=True).stdout.split("\n")
+
+
+def test_faddeeva_w_matches_scipy(harness):
+    xs = np.concatenate([np.linspace(0, 12, 49), [20.0, 40.0]])
+    ys = np.concatenate([np.linspace(0, 12, 49), [20.0]])
+    pts = [(x, y) for x in xs for y in ys]
+    out = _run(harness, "".join(f"w {float(x)!r} {float(y)!r}\n" for x, y in pts))
+    for (x, y), line in zip(pts, out):
+        re, im = map(float, line.split())
+        ref = special.wofz(x + 1j * y)
+        assert abs(complex(re, im) - ref) <= 3e-14 * abs(ref), (x, y)
+
+
+def test_radial_weights_match_oracle(harness):
+    cases = [(1.0, 1.0, 1.0, 0.1), (1.0, 1.0, 0.3, 0.07), (1.2, 0.8, 2.0, 0.35)]
+    ks = np.concatenate([[0.0], np.geomspace(0.05, 80.0, 200)])
+    for sigma, eps, kT, xi in cases:
+        text = f"c {sigma!r} {eps!r} {kT!r} {xi!r}\n" + "".join(f"r {float(k)!r}\n" for k in ks)
+        out = _run(harness, text)
+        alpha = S.sfmt_alpha(sigma, eps, kT)
+        assert float(out[0]) == pytest.approx(alpha, rel=1e-15)
+        got = np.array([[float(t) for t in l.split()] for l in out[1:1 + len(ks)]])
+        ref = np.stack(S.sfmt_radial(ks, alpha, xi), axis=1)
+        scale = np.max(np.abs(ref), axis=0)
+        # round-off of the closed forms at small k sets the floor: in W1v and W2t
+        # the erf term, times (2 pi)^1.5 2/(k^3 Xi), cancels against 4 pi/k^2-sized
+        # terms (as in the reference's formulas), so an error of ~1e-16 in
+        # exp(-v^2) Re erf becomes ~4e-14/(k^3 Xi)
+        tol = 1e-13 * scale + 4e-14 / (np.maximum(ks, 0.05) ** 3 * xi)[:, None]
+        assert np.all(np.abs(got - ref) <= tol), (sigma, eps, kT, xi, np.max(np.abs(got - ref) / scale))
+
+
+def test_phi_point_code_matches_oracle(harness):
+    rng = np.random.default_rng(9)
+    v = _random_fields(rng, 40)
+    text = "".join("p " + " ".join(repr(float(v[f][i])) for f in range(16)) + "\n" for i in range(40))
+    out = _run(harness, text)
+    p1, p2, p3, d = S.sfmt_phi(v)
+    for i in range(40):
+        vals = [float(t) for t in out[i].split()]
+        ref = [p1[i], p2[i], p3[i]] + [d[f][i] for f in range(16)]
+        for a, b in zip(vals, ref):
+            assert abs(a - b) <= 1e-14 * max(1.0, abs(b)), i
+
+
+def test_homogeneous_c_abi_matches_oracle():
+    import deft_amd
+    for n, kT, xi, mu in ((0.1, 1.0, 0.1, 0.0), (0.5, 0.3, 0.07, 1.5), (0.9, 2.0, 0.3, -0.4)):
+        hf = deft_amd.HomogeneousSFMTFluid(sigma=1.0, epsilon=1.0, Xi=xi, kT=kT, n=n, mu=mu)
+        f, d = S.homogeneous_sfmt(n, 1.0, 1.0, xi, kT, mu)
+        assert hf.energy() == pytest.approx(f, rel=1e-13, abs=1e-14)
+        assert hf.d_by_dn() == pytest.approx(d, rel=1e-13, abs=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# GPU parity
+# ---------------------------------------------------------------------------
+
+def _gpu_fluid(deft_amd, grid, xi, kT, mu, vext, x, prec):
+    f = deft_amd.SFMTFluidVeff(grid.shape, grid.cell, sigma=1.0, epsilon=1.0, Xi=xi, kT=kT, mu=mu,
+                               fft_precision=prec)
+    f.Vext = vext
+    f.Veff = x
+    return f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol_e,tol_g", [(1, 1e-12, 1e-10), (0, 2e-6, 2e-4)])
+def test_energy_and_gradient_match_oracle(gpu, prec, tol_e, tol_g):
+    fo, grid, vext, x, mu = _nonuniform_oracle()
+    f = _gpu_fluid(gpu, grid, fo.xi, fo.kT, mu, vext, x, prec)
+    E = f.energy()
+    Eo = fo.energy()
+    assert abs(E - Eo) <= tol_e * abs(Eo), (E, Eo)
+    for k, v in fo.last_terms.items():
+        assert abs(f.last_terms[k] - v) <= tol_e * max(abs(Eo), 1.0), k
+    g = f.grad()
+    go = fo.grad(x)
+    assert np.max(np.abs(g - go)) <= tol_g * np.max(np.abs(go))
+    w = f.weighted_densities()
+    wo = fo.weighted_densities(fo.density(x))
+    for j in range(16):
+        assert np.max(np.abs(w[j] - wo[j])) <= (1e-12 if prec == 1 else 1e-5) * max(np.max(np.abs(wo[j])), 1e-3), \
+            S.FIELDS[j]
+    f.close()
+
+
+@pytest.mark.gpu
+def test_uniform_fluid_is_volume_times_homogeneous_on_gpu(gpu):
+    grid, kT, n, xi, mu = _uniform_case()
+    f = _gpu_fluid(gpu, grid, xi, kT, mu, np.zeros(grid.shape), np.full(grid.shape, -kT * math.log(n)), 1)
+    hf = gpu.HomogeneousSFMTFluid(sigma=1.0, epsilon=1.0, Xi=xi, kT=kT, n=n, mu=mu)
+    assert abs(f.energy() / (grid.volume * hf.energy()) - 1) < 2e-11
+    f.close()
+
+
+@pytest.mark.gpu
+def test_wall_minimization_follows_oracle(gpu):
+    """The new-sfmt-walls set-up (tests/new-sfmt-walls.cpp:96-151: kT = 0.3,
+    n* = 0.8, a soft wall Vext = 10 kT for |z| < 1.5, the start 0.01 n inside)
+    on a 2 x 2 x 330 cell of 0.1 x 0.1 x 16.5 (dx = 0.05; the reference's 0.01
+    needs 1650 points per line): the new-API minimizer, preconditioned, in FP64
+    on the GPU against the oracle's, iteration by iteration."""
+    kT, xi = 0.3, 0.08
+    n = 0.8 * 2 ** -2.5
+    mu = S.sfmt_chemical_potential(n, 1.0, 1.0, xi, kT)
+    grid = O.Grid((2, 2, 330), np.diag([0.1, 0.1, 16.5]))
+    z = grid.positions()[..., 2]
+    z = np.where(z > 16.5 / 2, z - 16.5, z)
+    inside = np.abs(z) < 1.5
+    vext = np.where(inside, 10 * kT, 0.0)
+    x = np.where(inside, -kT * math.log(0.01 * n), -kT * math.log(n))
+    fo = S.SFMTFluidVeff(grid, 1.0, 1.0, xi, kT, mu, vext, x)
+    mo = O.Minimize(fo)
+    mo.precondition(True)
+    mo.set_relative_precision(1e-12)
+    mo.set_maxiter(200)
+    Eo = []
+    while True:
+        more = mo.improve_energy()
+        Eo.append(mo.energy())
+        if not more:
+            break
+    f = _gpu_fluid(gpu, grid, xi, kT, mu, vext, x, 1)
+    m = gpu.Minimize(f)
+    m.precondition(True)
+    m.set_relative_precision(1e-12)
+    m.set_maxiter(200)
+    E = []
+    while True:
+        more = m.improve_energy()
+        E.append(m.energy())
+        if not more:
+            break
+    for i in range(min(10, len(E), len(Eo))):
+        assert abs(E[i] - Eo[i]) <= 1e-11 * abs(Eo[i]), (i, E[i], Eo[i])
+    assert abs(E[-1] - Eo[-1]) <= 1e-10 * abs(Eo[-1])
+    assert m.stats()["stop_reason"] == "converged"
+    dens = f.density()
+    assert np.max(np.abs(dens - fo.density())) <= 1e-6 * n
+    m.close()
+    f.close()
