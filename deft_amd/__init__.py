@@ -39,6 +39,7 @@ WHITE_BEAR_MARK_II = 1
 WHITE_BEAR_MARK_II_SLOW = 2
 WHITE_BEAR_TENSOR = 3
 SFMT = 4
+SW_LIQUID = 5
 
 PGRAD_NEW = 1
 PGRAD_EFFECTIVE_POTENTIAL = 2
@@ -57,7 +58,7 @@ class DeftError(RuntimeError):
 
 class EnergyTerms(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
-                ("kTphi1", "kTphi2", "kTphi3", "ideal", "int_V_n", "minus_mu_N", "total")]
+                ("kTphi1", "kTphi2", "kTphi3", "ideal", "int_V_n", "minus_mu_N", "total", "sw")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -96,6 +97,8 @@ _SIGNATURES = {
     "deft_plan_set_functional": ([_P, _I], _I),
     "deft_plan_set_sfmt": ([_P, _D, _D, _D], _I),
     "deft_sfmt_homogeneous": ([_D, _D, _D, _D, _D, _D, _DP, _DP], _I),
+    "deft_plan_set_sw": ([_P, _D, _D], _I),
+    "deft_sw_homogeneous": ([_D, _D, _D, _D, _D, _D, _DP, _DP], _I),
     "deft_contact_values": ([_P, _DP], _I),
     "deft_copy_bandwidth": ([ctypes.c_longlong, _I, _DP], _I),
     "deft_bulk_state": ([_I, _D, _D, _D, _DP, _DP, _DP], _I),
@@ -294,6 +297,33 @@ class HomogeneousSFMTFluid:
 
     def pressure(self):
         """n d_by_dn - energy (papers/fuzzy-fmt/figs/homogeneous.cpp:57), at the set mu."""
+        e, d = self._eval()
+        return self.n * d - e
+
+
+class HomogeneousSW_liquid:
+    """The generated HomogeneousSW_liquid (src/haskell/SW_liquid.hs:45-47): the
+    uniform square-well liquid with settable R, epsilon, lambda, kT, n and mu;
+    ``energy()`` is the free energy per volume, ``d_by_dn()`` its derivative
+    (papers/square-well-fluid/figs/new-radial-sw.cpp:113-125 sets mu = d_by_dn()
+    at mu = 0)."""
+
+    def __init__(self, R=1.0, epsilon=1.0, lambda_=1.3, kT=1.0, n=0.0, mu=0.0):
+        self.R, self.epsilon, self.lambda_, self.kT, self.n, self.mu = R, epsilon, lambda_, kT, n, mu
+
+    def _eval(self):
+        e, d = ctypes.c_double(), ctypes.c_double()
+        _check(lib().deft_sw_homogeneous(float(self.R), float(self.epsilon), float(self.lambda_), float(self.kT),
+                                         float(self.n), float(self.mu), ctypes.byref(e), ctypes.byref(d)))
+        return e.value, d.value
+
+    def energy(self):
+        return self._eval()[0]
+
+    def d_by_dn(self):
+        return self._eval()[1]
+
+    def pressure(self):
         e, d = self._eval()
         return self.n * d - e
 
@@ -604,6 +634,37 @@ class SFMTFluidVeff(WhiteBearFluidVeff):
     def alpha(self):
         """sigma (2/(1 + sqrt(ln 2 kT/epsilon)))^(1/6) (src/haskell/SFMT.hs:55-56)."""
         return self._sigma * (2.0 / (1.0 + math.sqrt(math.log(2.0) * self._kT / self._epsilon))) ** (1.0 / 6.0)
+
+
+class SWLiquidVeff(WhiteBearFluidVeff):
+    """The generated SW_liquidVeff (src/haskell/SW_liquid.hs:13-27,
+    src/haskell/create_generators.py): White-Bear of Veff plus the square-well
+    attraction of depth ``epsilon`` and range ``lambda_`` sigma, sigma = 2R.
+    Runs on the generic line passes; ``last_terms['sw']`` is the attraction."""
+
+    def __init__(self, shape, cell, R=1.0, epsilon=1.0, lambda_=1.3, kT=1.0, mu=0.0, fft_precision=FFT_FP32):
+        super().__init__(shape, cell, R=R, kT=kT, mu=mu, fft_precision=fft_precision)
+        self._epsilon, self._lambda = float(epsilon), float(lambda_)
+        _check(lib().deft_plan_set_sw(self._h, self._epsilon, self._lambda))
+        self.functional = SW_LIQUID
+
+    @property
+    def epsilon(self):
+        return self._epsilon
+
+    @epsilon.setter
+    def epsilon(self, v):
+        self._epsilon = float(v)
+        _check(lib().deft_plan_set_sw(self._h, self._epsilon, self._lambda))
+
+    @property
+    def lambda_(self):
+        return self._lambda
+
+    @lambda_.setter
+    def lambda_(self, v):
+        self._lambda = float(v)
+        _check(lib().deft_plan_set_sw(self._h, self._epsilon, self._lambda))
 
 
 class Minimize:

@@ -130,7 +130,8 @@ deft_energy_terms terms_from_sums(const deft_plan_s *p) {
   t.ideal = s[SLOT_IDEAL] * p->dV;
   t.int_V_n = s[SLOT_EXT] * p->dV;
   t.minus_mu_N = s[SLOT_MUN] * p->dV;
-  t.total = t.kTphi1 + t.kTphi2 + t.kTphi3 + t.ideal + t.int_V_n + t.minus_mu_N;
+  t.sw = p->functional == DEFT_SW_LIQUID ? s[SLOT_SW] * p->dV : 0.0;
+  t.total = t.kTphi1 + t.kTphi2 + t.kTphi3 + t.ideal + t.int_V_n + t.minus_mu_N + t.sw;
   return t;
 }
 
@@ -193,6 +194,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->gwork);
   (void)hipFree(p->sktab);
   (void)hipFree(p->stab);
+  (void)hipFree(p->swbuf);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -782,8 +784,8 @@ int deft_plan_set_engine(deft_plan p, int generic) {
   if (p->batch) return set_error(DEFT_ERR_STATE, "the plan is in a batch");
   if (!generic && !(fused_ok(p->Nx) && fused_ok(p->Ny) && fused_ok(p->Nz)))
     return set_error(DEFT_ERR_UNSUPPORTED, "this plan's edges have no fused kernels");
-  if (!generic && p->functional == DEFT_SFMT)
-    return set_error(DEFT_ERR_UNSUPPORTED, "soft FMT runs on the generic passes");
+  if (!generic && (p->functional == DEFT_SFMT || p->functional == DEFT_SW_LIQUID))
+    return set_error(DEFT_ERR_UNSUPPORTED, "soft FMT and the square-well liquid run on the generic passes");
   if (!generic == !p->generic) return DEFT_OK;
   TRY(flush_moves(p));
   TRY(plan_sync(p));
@@ -863,12 +865,15 @@ int deft_plan_set_fft_precision(deft_plan p, int prec) {
 int deft_plan_set_functional(deft_plan p, int functional) {
   TRY(check_plan(p));
   if (functional != DEFT_WHITE_BEAR && functional != DEFT_WHITE_BEAR_MARK_II &&
-      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR && functional != DEFT_SFMT)
+      functional != DEFT_WHITE_BEAR_MARK_II_SLOW && functional != DEFT_WHITE_BEAR_TENSOR && functional != DEFT_SFMT &&
+      functional != DEFT_SW_LIQUID)
     return set_error(DEFT_ERR_ARG, "unknown functional");
   if (functional != DEFT_WHITE_BEAR && p->batch)
     return set_error(DEFT_ERR_STATE, "plans in a batch keep the White-Bear functional");
   // the weighted-density spectra plus the density spectrum
-  const int need = functional == DEFT_SFMT ? 17 : (functional == DEFT_WHITE_BEAR_TENSOR ? 12 : 6);
+  const int need = functional == DEFT_SFMT ? 17
+                   : functional == DEFT_SW_LIQUID ? 10
+                   : (functional == DEFT_WHITE_BEAR_TENSOR ? 12 : 6);
   if (p->spec_fields < need) {
     void *spec = nullptr;
     const size_t bytes = need * sizeof(cx<double>) * (size_t)p->ks;
@@ -889,14 +894,21 @@ int deft_plan_set_functional(deft_plan p, int functional) {
   const bool was_generic = p->generic;
   p->functional = functional;
   p->sk_ok = false;
-  // soft FMT runs on the generic passes; leaving it restores the fused ones
-  // where the edges have them
-  if (functional == DEFT_SFMT)
+  // soft FMT and the square-well liquid run on the generic passes; leaving
+  // them restores the fused ones where the edges have them
+  const bool gen_only = functional == DEFT_SFMT || functional == DEFT_SW_LIQUID;
+  if (gen_only)
     p->generic = true;
-  else if (was == DEFT_SFMT)
+  else if (was == DEFT_SFMT || was == DEFT_SW_LIQUID)
     p->generic = !(fused_ok(p->Nx) && fused_ok(p->Ny) && fused_ok(p->Nz));
   int rc = ensure_gwork(p);
   if (rc == DEFT_OK && functional == DEFT_SFMT) rc = plan_build_sfmt_table(p);
+  if (rc == DEFT_OK && functional == DEFT_SW_LIQUID && !p->swbuf) {
+    if (hipMalloc(&p->swbuf, 26 * sizeof(double) * (size_t)p->N) != hipSuccess) {
+      p->swbuf = nullptr;
+      rc = set_error(DEFT_ERR_ALLOC, "hipMalloc of the square-well work fields failed");
+    }
+  }
   if (rc != DEFT_OK) {
     p->functional = was;
     p->generic = was_generic;
@@ -914,6 +926,28 @@ int deft_plan_set_sfmt(deft_plan p, double sigma, double epsilon, double xi) {
   p->stab_ok = false;
   p->sk_ok = false;
   if (p->functional == DEFT_SFMT) TRY(plan_build_sfmt_table(p));
+  return DEFT_OK;
+}
+
+int deft_plan_set_sw(deft_plan p, double epsilon, double lambda) {
+  TRY(check_plan(p));
+  if (!std::isfinite(epsilon) || !(lambda > 1.0) || !std::isfinite(lambda))
+    return set_error(DEFT_ERR_ARG, "epsilon must be finite and lambda above 1");
+  p->sw_eps = epsilon;
+  p->sw_lambda = lambda;
+  p->sk_ok = false;
+  return DEFT_OK;
+}
+
+int deft_sw_homogeneous(double R, double epsilon, double lambda, double kT, double n, double mu, double *energy,
+                        double *d_by_dn) {
+  auto ok = [](double v) { return std::isfinite(v) && v > 0.0; };
+  if (!ok(R) || !ok(kT) || !ok(n) || !std::isfinite(epsilon) || !(lambda > 1.0) || !std::isfinite(mu))
+    return set_error(DEFT_ERR_ARG, "R, kT and n must be positive, lambda above 1, epsilon and mu finite");
+  double f, d;
+  sw_bulk(make_sw_const(R, epsilon, lambda), n, kT, std::log(water_nQ(kT)), &f, &d);
+  if (energy) *energy = f - mu * n;
+  if (d_by_dn) *d_by_dn = d - mu;
   return DEFT_OK;
 }
 
@@ -1138,7 +1172,7 @@ int deft_copy_bandwidth(long long bytes, int reps, double *gbps) {
 int deft_contact_values(deft_plan p, double *out7) {
   TRY(check_plan(p));
   if (!out7) return set_error(DEFT_ERR_ARG, "null output");
-  if (p->functional == DEFT_WHITE_BEAR_TENSOR || p->functional == DEFT_SFMT)
+  if (p->functional == DEFT_WHITE_BEAR_TENSOR || p->functional == DEFT_SFMT || p->functional == DEFT_SW_LIQUID)
     return set_error(DEFT_ERR_UNSUPPORTED, "contact values are defined for White-Bear and Mark II");
   TRY(flush_moves(p));
   const size_t N = (size_t)p->N;
@@ -1276,10 +1310,16 @@ bool bulk_density_of_mu(const deft_plan_s *p, double &nb) {
   double r0[7];
   sfmt_radial(0.0, sc, r0);  // soft FMT: n3 = w3(0) n
   const double nmax = soft ? 1.0 / r0[0] : 1.0 / (4 * M_PI / 3 * p->R * p->R * p->R);
+  const SWConst swc = make_sw_const(p->R, p->sw_eps, p->sw_lambda);
   auto mu_of = [&](double n) {
     if (soft) {
       double f, d;
       sfmt_bulk(sc, n, p->kT, &f, &d);
+      return d;
+    }
+    if (p->functional == DEFT_SW_LIQUID) {
+      double f, d;
+      sw_bulk(swc, n, p->kT, std::log(water_nQ(p->kT)), &f, &d);
       return d;
     }
     double fe, dfe;
@@ -1306,7 +1346,7 @@ namespace {
 // spectral element on the device.
 int plan_ensure_sk(deft_plan_s *p) {
   if (p->sk_ok) return DEFT_OK;
-  if (p->functional == DEFT_SFMT)
+  if (p->functional == DEFT_SFMT || p->functional == DEFT_SW_LIQUID)
     return set_error(DEFT_ERR_UNSUPPORTED, "the structure-factor preconditioner is built for the White-Bear functionals");
   double nb;
   if (!bulk_density_of_mu(p, nb))

@@ -23,6 +23,7 @@
 
 #include "passes.hpp"
 #include "sfmt.hpp"
+#include "sw_kernels.hpp"
 
 namespace deft {
 
@@ -595,13 +596,15 @@ __global__ __launch_bounds__(kThreads) void kg_phi(GridArgs g, double* __restric
   block_sum_store<3>(acc, partial, SLOT_PHI1, pstride, blockIdx.x);
 }
 
-// dF/dn chain rule of zgrad_body from the back-convolution `conv` (real).
+// dF/dn chain rule of zgrad_body from the back-convolution `conv` (real),
+// plus `extra` (dF/dn of another term, e.g. the square-well attraction) where given.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void kg_grad(GridArgs g, const double* __restrict__ conv,
                                                     const double* __restrict__ x, const T* __restrict__ vext,
                                                     const double* __restrict__ oldgrad, int have_old, int precond,
                                                     double* __restrict__ grad, double* __restrict__ pgrad,
-                                                    double* __restrict__ partial, int pstride) {
+                                                    double* __restrict__ partial, int pstride,
+                                                    const double* __restrict__ extra) {
   double dots[2] = {0.0, 0.0};
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < g.N; i += (long long)gridDim.x * kThreads) {
     const double cv = conv[i];
@@ -610,7 +613,7 @@ __global__ __launch_bounds__(kThreads) void kg_grad(GridArgs g, const double* __
     double gr = 0.0, pg = 0.0;
     if (isfinite(ve)) {
       const double n = exp(-xv / g.kT);
-      const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu;
+      const double local = -xv - g.kT * g.logNQ + (double)ve - g.mu + (extra ? extra[i] : 0.0);
       const double dfdn = (g.kT * cv + local) * g.dV;
       gr = -dfdn * n / g.kT;
       pg = point_pgrad(precond, dfdn, n, xv, gr, g);

@@ -329,13 +329,88 @@ static void gen_density_to_weighted(deft_plan_s *p, const GridArgs &g, double *w
 template <typename T>
 static int sk_apply(deft_plan_s *p, const GridArgs &g, int nb, bool have_old);
 
+// The square-well liquid's attraction (sw_kernels.hpp), with the White-Bear
+// weighted densities in w and n~ in spectrum 0 (as gen_density_to_weighted
+// leaves them): its energy partial sums (SLOT_SW) and, with grad, its dF/dn in
+// the plan's swg field.  Works in spectra 1..9 (spectrum 0 last) and in the
+// plan's 26 real SW fields:
+//   a/q (8) | num, den | f/kn (5) | g | c | swg | u, t | r (5) | scratch
+template <typename T>
+static void sw_enqueue(deft_plan_s *p, const GridArgs &g, const double *w, bool grad) {
+  const long long N = p->N;
+  double *B = p->swbuf;
+  double *A8 = B, *ND = B + 8 * N, *F5 = B + 10 * N, *G = B + 15 * N, *C = B + 16 * N, *SWG = B + 17 * N,
+         *UT = B + 18 * N, *R5 = B + 20 * N, *TMP = B + 25 * N;
+  const SWConst c = make_sw_const(p->R, p->sw_eps, p->sw_lambda);
+  const dim3 eb(p->eblocks), tb(kThreads), kb(kblocks_for(p));
+  const double cb = (double)sizeof(cx<T>) * p->Nk;
+  const int pk = kid<T>(K_GEN_POINT);
+  {
+    ProfScope ps(p, pk, 104.0 * N);
+    hipLaunchKernelGGL(k_sw_afields, eb, tb, 0, p->stream, N, p->R, w, A8);
+  }
+  gen_forward<T>(p, g, A8, N, 8, spec_ptr<T>(p, 1), 1.0);
+  {
+    ProfScope ps(p, pk, 11.0 * cb);
+    hipLaunchKernelGGL((kg_sw_num<T>), kb, tb, 0, p->stream, g, c, spec_ptr<T>(p, 1), spec_ptr<T>(p, 0));
+  }
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 1), 2, ND, N, 1.0);
+  {
+    ProfScope ps(p, pk, 64.0 * N + sizeof(T) * N);
+    hipLaunchKernelGGL((k_sw_point_e<T>), eb, tb, 0, p->stream, g, p->x, vext_ptr<T>(p), ND, G, F5);
+  }
+  gen_forward<T>(p, g, F5, N, 5, spec_ptr<T>(p, 1), 1.0);
+  {
+    ProfScope ps(p, pk, 6.0 * cb);
+    hipLaunchKernelGGL((kg_sw_xi_sum<T>), kb, tb, 0, p->stream, g, c, spec_ptr<T>(p, 1));
+  }
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 1), 1, C, N, 1.0);
+  {
+    ProfScope ps(p, pk, 16.0 * N + sizeof(T) * N);
+    hipLaunchKernelGGL((k_sw_energy<T>), eb, tb, 0, p->stream, g, c, p->x, vext_ptr<T>(p), C, p->partial, p->pstride);
+  }
+  if (!grad) return;
+  {
+    ProfScope ps(p, pk, 6.0 * cb);
+    hipLaunchKernelGGL((kg_sw_xi_split<T>), kb, tb, 0, p->stream, g, c, spec_ptr<T>(p, 0), spec_ptr<T>(p, 1));
+  }
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 1), 5, F5, N, 1.0);
+  {
+    ProfScope ps(p, pk, 104.0 * N + sizeof(T) * N);
+    hipLaunchKernelGGL((k_sw_point_g<T>), eb, tb, 0, p->stream, g, c, p->x, vext_ptr<T>(p), G, ND, C, F5, SWG, UT);
+  }
+  gen_forward<T>(p, g, UT, N, 2, spec_ptr<T>(p, 1), 1.0);
+  {
+    ProfScope ps(p, pk, 11.0 * cb);
+    hipLaunchKernelGGL((kg_sw_adj<T>), kb, tb, 0, p->stream, g, c, spec_ptr<T>(p, 1));
+  }
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 1), 8, A8, N, 1.0);
+  {
+    ProfScope ps(p, pk, 144.0 * N);
+    hipLaunchKernelGGL(k_sw_jac, eb, tb, 0, p->stream, N, p->R, w, A8, R5);
+  }
+  gen_forward<T>(p, g, R5, N, 5, spec_ptr<T>(p, 1), 1.0);
+  {
+    ProfScope ps(p, pk, 9.0 * cb);
+    hipLaunchKernelGGL((kg_wmul_bwd<T, 5>), kb, tb, 0, p->stream, g, spec_ptr<T>(p, 1), spec_ptr<T>(p, 0));
+    hipLaunchKernelGGL((kg_spec_add<T>), kb, tb, 0, p->stream, g, spec_ptr<T>(p, 0), spec_ptr<T>(p, 9));
+  }
+  gen_inverse<T>(p, g, spec_ptr<T>(p, 0), 1, TMP, N, 1.0);
+  {
+    ProfScope ps(p, pk, 24.0 * N);
+    hipLaunchKernelGGL(k_sw_add, eb, tb, 0, p->stream, N, SWG, TMP);
+  }
+}
+
 template <typename T>
 static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int precond, int *nb) {
   const GridArgs g = grid_args(p);
   const int nw = plan_fields(p);
   double *w = p->gwork;
   gen_density_to_weighted<T>(p, g, w);
-  const int fn = p->functional == DEFT_WHITE_BEAR ? 0 : (nw == 16 ? 4 : (nw == 11 ? 3 : 1));
+  const bool sw = p->functional == DEFT_SW_LIQUID;
+  if (sw) sw_enqueue<T>(p, g, w, grad);
+  const int fn = (p->functional == DEFT_WHITE_BEAR || sw) ? 0 : (nw == 16 ? 4 : (nw == 11 ? 3 : 1));
   {
     ProfScope ps(p, kid<T>(K_GEN_POINT), (grad ? 16.0 : 8.0) * nw * p->N);
     auto phi = [&](auto FNc) {
@@ -373,7 +448,8 @@ static int gen_enqueue_evaluation(deft_plan_s *p, bool grad, bool have_old, int 
     gen_inverse<T>(p, g, spec_ptr<T>(p, 0), 1, w, p->N, 1.0);
     ProfScope ps(p, kid<T>(K_GEN_POINT), (have_old ? 48.0 : 40.0) * p->N + (double)sizeof(T) * p->N);
     hipLaunchKernelGGL(kg_grad<T>, dim3(p->eblocks), dim3(kThreads), 0, p->stream, g, w, p->x, vext_ptr<T>(p), p->og,
-                       have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride);
+                       have_old ? 1 : 0, precond, p->g, p->pg, p->partial, p->pstride,
+                       sw ? (const double *)(p->swbuf + 17 * p->N) : (const double *)nullptr);
   }
   if (grad && precond == PGRAD_SK) TRY_LAUNCH(sk_apply<T>(p, g, p->eblocks, have_old));
   *nb = p->eblocks;
@@ -454,6 +530,7 @@ int Launch<T>::evaluate(deft_plan_s *p, bool grad, bool have_old, int precond) {
   int nzb = 0;
   TRY_LAUNCH(enqueue_evaluation(p, grad, have_old, precond, &nzb));
   // every pass wrote its partial sums from nzb blocks
+  if (p->functional == DEFT_SW_LIQUID) TRY_LAUNCH(plan_reduce_wait(p, SLOT_SW, 1, nzb));
   return plan_reduce_wait(p, SLOT_IDEAL, grad ? 8 : 6, nzb);
 }
 

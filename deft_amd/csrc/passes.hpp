@@ -40,7 +40,8 @@ enum Slot : int {
   SLOT_PGG = 6,    // pgrad . grad
   SLOT_PGO = 7,    // pgrad . oldgrad
   SLOT_GD = 8,     // grad . direction
-  NUM_SLOTS = 9
+  SLOT_SW = 9,     // square-well liquid: (eps/2) n (Xi * n P(g))
+  NUM_SLOTS = 10
 };
 
 struct GridArgs {

@@ -150,6 +150,10 @@ struct deft_plan_s {
   double sfmt_sigma = 1.0, sfmt_epsilon = 1.0, sfmt_xi = 0.1;
   double *stab = nullptr;
   bool stab_ok = false;
+  // square-well liquid (DEFT_SW_LIQUID): well depth, range lambda (in units of
+  // sigma = 2R) and its 26 real work fields of N values
+  double sw_eps = 1.0, sw_lambda = 1.3;
+  double *swbuf = nullptr;
 };
 
 namespace deft {
@@ -202,7 +206,8 @@ inline GridArgs grid_args(const deft_plan_s *p) {
   a.logNQ = std::log(water_nQ(p->kT));
   a.dV = p->dV;
   a.sk_floor = p->sk_floor;
-  a.wb = make_wb_const(p->R, p->functional == DEFT_SFMT ? DEFT_WHITE_BEAR : p->functional);
+  a.wb = make_wb_const(p->R, (p->functional == DEFT_SFMT || p->functional == DEFT_SW_LIQUID) ? DEFT_WHITE_BEAR
+                                                                                               : p->functional);
   return a;
 }
 

@@ -77,7 +77,15 @@ typedef enum {
    * transverse tensor weight (sixteen weighted fields), softness from
    * deft_plan_set_sfmt.  Runs on the generic line passes; no batches, no
    * structure-factor preconditioner, no contact estimators. */
-  DEFT_SFMT = 4
+  DEFT_SFMT = 4,
+  /* The square-well liquid (SW_liquidVeff, src/haskell/SW_liquid.hs): White-Bear
+   * plus the attraction (eps/2) int n [sum_i Xi_i * (n P_i(gSigmaA))] of a well
+   * of depth eps and range lambda sigma (sigma = 2R), with the White-Bear
+   * gSigmaA of the same density; parameters from deft_plan_set_sw.  Runs on
+   * the generic line passes; no batches, no structure-factor preconditioner,
+   * no contact estimators; Vext finite where the fluid is (the reference's
+   * drivers cap it, papers/square-well-fluid/figs/new-radial-sw.cpp:142-150). */
+  DEFT_SW_LIQUID = 5
 } deft_functional_kind;
 
 /* Why a minimizer stopped (mirrors the return-false branches of
@@ -131,6 +139,7 @@ typedef struct deft_cg_s *deft_cg;
  * kT*int Phi3, ideal gas, int Vext n, -mu N, and their sum. */
 typedef struct {
   double kTphi1, kTphi2, kTphi3, ideal, int_V_n, minus_mu_N, total;
+  double sw; /* square-well attraction (DEFT_SW_LIQUID; 0 otherwise), included in total */
 } deft_energy_terms;
 
 typedef struct {
@@ -217,6 +226,16 @@ int deft_plan_set_sfmt(deft_plan plan, double sigma, double epsilon, double xi);
  * its derivative d_by_dn; either output may be NULL.  Host FP64. */
 int deft_sfmt_homogeneous(double sigma, double epsilon, double xi, double kT, double n, double mu,
                           double *energy, double *d_by_dn);
+/* Square-well parameters of the generated SW_liquidVeff's epsilon() and
+ * lambda() (src/haskell/SW_liquid.hs:13-27); R() is the plan's radius.
+ * Defaults epsilon = 1, lambda = 1.3.  epsilon finite, lambda > 1. */
+int deft_plan_set_sw(deft_plan plan, double epsilon, double lambda);
+/* HomogeneousSW_liquid (src/haskell/SW_liquid.hs:45-47): free energy per
+ * volume of the uniform liquid at density n (White-Bear + ideal gas + the
+ * attraction with the bulk gSigmaA, - mu n) and d_by_dn; either output may be
+ * NULL.  Host FP64. */
+int deft_sw_homogeneous(double R, double epsilon, double lambda, double kT, double n, double mu, double *energy,
+                        double *d_by_dn);
 /* Vext (N doubles, host); kept on the device in FP64 for the FP64 passes and
  * rounded to FP32 for the FP32 passes.  +inf marks a hard wall
  * (density exactly 0); NaN marks a constrained point as the old API's
