@@ -463,6 +463,8 @@ class WhiteBearFluidVeff:
         t = EnergyTerms()
         _check(lib().deft_functional_energy(self._h, None, ctypes.byref(e), ctypes.byref(t)))
         self.last_terms = t.as_dict()
+        if self._functional != SW_LIQUID:
+            del self.last_terms["sw"]  # the White-Bear family's printme has six terms
         return e.value
 
     def grad(self, Veff=None):
