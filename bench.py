@@ -308,6 +308,98 @@ def c5_run(iters=40, seed=2024):
             "converged": False}
 
 
+def sw_run(kT=5.0, ff=0.3, lam=1.3):
+    """The square-well liquid around a test particle at its published shape
+    (papers/square-well-fluid/figs/new-radial-sw.cpp: 18^3 cell, dx = 0.1 ->
+    180^3, R = 1, epsilon = 1, lambda = 1.3, a 400 kT core, relative precision 0,
+    at most 500 iterations, preconditioned, FP64): one minimization, timed, and
+    its deviation from the reference's published profile of that case."""
+    R, eps, edge, dx = 1.0, 1.0, 18.0, 0.1
+    sigma = 2 * R
+    Ng = int(math.ceil(edge / dx / 2)) * 2
+    idx = np.arange(Ng)
+    c = np.where(idx > Ng // 2, idx - Ng, idx) * edge / Ng
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    nb = ff / (math.pi * sigma ** 3 / 6)
+    vmax = 400 * kT
+    vext = np.where(r <= sigma, vmax, np.where((r > sigma) & (r < sigma * lam), -eps, 0.0))
+    veff = -kT * math.log(nb) + np.where(r <= sigma, vmax, 0.0)
+    mu = D.HomogeneousSW_liquid(R=R, epsilon=eps, lambda_=lam, kT=kT, n=nb).d_by_dn()
+    f = D.SWLiquidVeff((Ng, Ng, Ng), np.diag([edge] * 3), R=R, epsilon=eps, lambda_=lam, kT=kT, mu=mu,
+                       fft_precision=D.FFT_FP64)
+    f.autotune()
+    f.Vext = vext
+
+    def one():
+        f.Veff = veff
+        m = D.Minimize(f)
+        m.set_relative_precision(0)
+        m.set_maxiter(500)
+        m.precondition(True)
+        t0 = time.perf_counter()
+        m.run()
+        dt = time.perf_counter() - t0
+        st = m.stats()
+        m.close()
+        return dt, st
+
+    one()
+    dt, st = one()
+    eta = f.density()[0, 0, :Ng // 2] * math.pi * sigma ** 3 / 6
+    gold = json.load(open(os.path.join(HERE, "tests", "golden", "sw_radial.json")))["cases"]
+    case = [g for g in gold if g["kT"] == kT and g["ff"] == ff][0]
+    pub = np.array(case["eta"])
+    out = np.array(case["r_over_sigma"]) > 1.0
+    f.close()
+    return {"workload": f"square-well liquid around a test particle, 180^3, 18^3 cell, kT = {kT:g}, "
+                        f"ff = {ff:g}, lambda = {lam:g}, the reference driver's protocol (FP64, grad/n)",
+            "minimizations_per_s": 1.0 / dt, "seconds": dt, "iterations": st["iter"],
+            "energy_calcs": st["num_energy_calcs"], "grad_calcs": st["num_grad_calcs"],
+            "ms_per_iteration": dt / st["iter"] * 1e3,
+            "max_rel_dev_from_published": float(np.max(np.abs(eta[out] - pub[out]) / pub[out]))}
+
+
+def sfmt_run(iters=30):
+    """Soft FMT (sigma = epsilon = 1, Xi = 0.1, kT = 1) around the headline's radius-3
+    hard solute at 128^3, 12.8^3 cell, bulk packing 0.40 of the sphere of diameter
+    alpha: `iters` CG iterations of the FP32 -> FP64 ladder with grad/n, timed."""
+    L, M, kT, xi = 12.8, 128, 1.0, 0.1
+    hf = D.HomogeneousSFMTFluid(sigma=1.0, epsilon=1.0, Xi=xi, kT=kT)
+    f = D.SFMTFluidVeff((M, M, M), np.diag([L] * 3), sigma=1.0, epsilon=1.0, Xi=xi, kT=kT,
+                        fft_precision=D.FFT_FP32)
+    alpha = f.alpha
+    nb = 0.40 / (math.pi * alpha ** 3 / 6)
+    hf.n = nb
+    f.mu = hf.d_by_dn()
+    f.Vext = W.sphere_vext((M, M, M), (L, L, L), (0.0, 0.0, 0.0), 3.0)
+    f.autotune()
+
+    def one():
+        f.fft_precision = D.FFT_FP32
+        f.fill_Veff(-kT * math.log(nb))
+        m = D.Minimize(f)
+        m.precondition(True)
+        m.set_relative_precision(1e-12)
+        m.set_maxiter(iters)
+        m.set_mixed_precision(SWITCH_RP, POLISH_GRAD)
+        t0 = time.perf_counter()
+        m.run()
+        dt = time.perf_counter() - t0
+        st = m.stats()
+        m.close()
+        return dt, st
+
+    one()
+    dt, st = one()
+    f.close()
+    return {"workload": f"soft FMT (sigma = epsilon = kT = 1, Xi = {xi:g}) around a radius-3 hard solute, "
+                        f"{M}^3, {L:g}^3 cell, eta = 0.40, {iters} ladder iterations (grad/n)",
+            "iterations": st["iter"], "ms_per_iteration": dt / st["iter"] * 1e3,
+            "iterations_per_s": st["iter"] / dt, "energy_calcs": st["num_energy_calcs"],
+            "grad_calcs": st["num_grad_calcs"]}
+
+
 def rocprof_instance(family, tuning):
     """rocprof name of the kernel instance behind a profiler family at the autotuned
     variants (`tuning` as returned by autotune()); None when there is no single one."""
@@ -444,6 +536,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 64-grid batch measurement")
     ap.add_argument("--no-c4", action="store_true", help="skip the 512 held-out warm-start measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the 256^3 porous iteration measurement")
+    ap.add_argument("--no-f4", action="store_true", help="skip the soft-FMT and square-well measurements")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -485,6 +578,8 @@ def main():
         c3["grad_n_preconditioner"] = c3_run(precond=D.PGRAD_NEW, sequential=False)
     c4 = None if args.no_c4 else c4_run()
     c5 = None if args.no_c5 else c5_run()
+    sw = None if args.no_f4 else sw_run()
+    sfmt = None if args.no_f4 else sfmt_run()
     cpu = cpu_baseline(args.cpu_sample) if args.cpu_sample > 0 else None
     def block(run, start):
         r = run["results"][-1]
@@ -519,7 +614,7 @@ def main():
                    "autotuned_engine": runs["engine"],
                    "c3_batch": c3,
                    "c4_warm_start_heldout": c4,
-                   "c5_porous_256": c5,
+                   "c5_porous_256": c5, "sw_radial_180": sw, "sfmt_solute_128": sfmt,
                    "oracle_iterations": gold["iterations"]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": committed_traffic(kname, runs["tuning"]),
