@@ -540,7 +540,13 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
+    ap.add_argument("--f4-only", action="store_true",
+                    help="only run the square-well and soft-FMT legs and print their results (for profiling)")
     args = ap.parse_args()
+    if args.f4_only:
+        print(json.dumps({"sw_radial_180": sw_run()}), flush=True)
+        print(json.dumps({"sfmt_solute_128": sfmt_run()}), flush=True)
+        return
     if args.copy_probe:
         import torch
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
