@@ -540,9 +540,14 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy kernel (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="only run the 64-grid batch once (structure factor, no sequential runs), for profiling")
     ap.add_argument("--f4-only", action="store_true",
                     help="only run the square-well and soft-FMT legs and print their results (for profiling)")
     args = ap.parse_args()
+    if args.c3_only:
+        print(json.dumps({"c3_batch": c3_run(reps=1, sequential=False)}), flush=True)
+        return
     if args.f4_only:
         print(json.dumps({"sw_radial_180": sw_run()}), flush=True)
         print(json.dumps({"sfmt_solute_128": sfmt_run()}), flush=True)
