@@ -249,6 +249,33 @@ __device__ __forceinline__ void pack_c2r(const cx<T>* raw, cx<T>* Z) {
   __syncthreads();
 }
 
+// pack_c2r for nf fields at once: field q's raw rows at raw + q * 2P(L/2+1),
+// its packed lines as sequences q*P .. q*P+P-1 of Z; one barrier.
+template <typename T, int L, int P>
+__device__ __forceinline__ void pack_c2r_fields(const cx<T>* raw, cx<T>* Z, int nf) {
+  constexpr int NKZ = L / 2 + 1;
+  constexpr int SS = lds_stride<T, L>();
+  for (int e = threadIdx.x; e < nf * P * L; e += kThreads) {
+    const int pq = e / L, z = e - pq * L;
+    const int q = pq / P, p = pq - q * P;
+    const cx<T>* r = raw + q * (2 * P * NKZ);
+    cx<T> a, b;
+    if (z <= L / 2) {
+      a = r[(2 * p) * NKZ + z];
+      b = r[(2 * p + 1) * NKZ + z];
+      if (z == 0 || (L % 2 == 0 && z == L / 2)) {
+        a.im = (T)0;
+        b.im = (T)0;
+      }
+    } else {
+      a = cconj(r[(2 * p) * NKZ + (L - z)]);
+      b = cconj(r[(2 * p + 1) * NKZ + (L - z)]);
+    }
+    Z[pq * SS + skew<T>(z)] = {a.re - b.im, a.im + b.re};
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // z pass 1: (optional CG moves) Veff -> n = exp(-Veff/kT) -> r2c along z.
 // Also accumulates the local energy terms (ideal gas, Vext n, -mu n).
@@ -702,8 +729,12 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   constexpr int NPT = 2 * P * L;
   constexpr int PT = (NPT + kThreads - 1) / kThreads;
   constexpr int SS = lds_stride<T, L>();
-  __shared__ cx<T> A[zbuf_size<T, L, P>()];
-  __shared__ cx<T> B[zbuf_size<T, L, P>()];
+  constexpr int NW = FN == 3 ? 11 : 5;
+  // Two fields per FFT round (the eleven tensor fields one at a time): twice
+  // the butterflies per stage and half the barriers of one field per round.
+  constexpr int FG = FN == 3 ? 1 : 2;
+  __shared__ cx<T> A[zbuf_size<T, L, FG * P>()];
+  __shared__ cx<T> B[zbuf_size<T, L, FG * P>()];
   // The butterfly loops stride by blockDim.x here: on a compile-time stride
   // the compiler keeps every stage's twiddles in registers across the field
   // loop and the pass no longer fits 128 VGPRs (it spilled 156 bytes in FP32,
@@ -713,41 +744,55 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   const int fstride = (int)blockDim.x;
   const long long lines = (long long)g.Nx * g.Ny;
   const long long line0 = (long long)bx * 2 * P;
-  constexpr int NW = FN == 3 ? 11 : 5;
   T val[NW][PT];
-  // the next field's half-spectrum rows are loaded into registers while the
-  // current field is packed and transformed (two fields' loads in flight)
+  // the next round's half-spectrum rows are loaded into registers while the
+  // current round is packed and transformed
   constexpr int NRAW = 2 * P * (L / 2 + 1);
   constexpr int PF = (NRAW + kThreads - 1) / kThreads;
-  cx<T> pre[PF];
-  auto fetch = [&](int f) {
+  cx<T> pre[FG][PF];
+  auto fetch = [&](int f0) {
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int e = threadIdx.x + i * kThreads;
-      const int l = e / (L / 2 + 1), k = e - l * (L / 2 + 1);
-      pre[i] = load_or_zero(fields + f * g.ks, (line0 + l) * g.ldk + k, 0, e < NRAW && line0 + l < lines);
+    for (int q = 0; q < FG; ++q) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        const int e = threadIdx.x + i * kThreads;
+        const int l = e / (L / 2 + 1), k = e - l * (L / 2 + 1);
+        pre[q][i] = {(T)0, (T)0};
+        if (f0 + q < NW)  // past the last field nothing is read, not even the base
+          pre[q][i] = load_or_zero(fields + (f0 + q) * g.ks, (line0 + l) * g.ldk + k, 0,
+                                   e < NRAW && line0 + l < lines);
+      }
     }
   };
   fetch(0);
 #pragma unroll
-  for (int f = 0; f < NW; ++f) {
+  for (int f0 = 0; f0 < NW; f0 += FG) {
+    const int nf = NW - f0 < FG ? NW - f0 : FG;
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int e = threadIdx.x + i * kThreads;
-      if (e < NRAW) A[e] = pre[i];
+    for (int q = 0; q < FG; ++q) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        const int e = threadIdx.x + i * kThreads;
+        if (q < nf && e < NRAW) A[q * NRAW + e] = pre[q][i];
+      }
     }
     __syncthreads();
-    if (f + 1 < NW) fetch(f + 1);
-    pack_c2r<T, L, P>(A, B);
-    const cx<T>* R = fft_lds<T, L, SS, true>(B, A, P, twl, fstride);
+    if (f0 + FG < NW) fetch(f0 + FG);
+    pack_c2r_fields<T, L, P>(A, B, nf);
+    const cx<T>* R = fft_lds<T, L, SS, true>(B, A, nf * P, twl, fstride);
 #pragma unroll
-    for (int i = 0; i < PT; ++i) {
-      const int e = threadIdx.x + i * kThreads;
-      val[f][i] = (T)0;
-      if (e < NPT) {
-        const int l = e / L, z = e - l * L;
-        const cx<T> v = R[(l >> 1) * SS + skew<T>(z)];
-        val[f][i] = (l & 1) ? v.im : v.re;
+    for (int q = 0; q < FG; ++q) {
+      if (f0 + q < NW) {
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+          const int e = threadIdx.x + i * kThreads;
+          val[f0 + q][i] = (T)0;
+          if (e < NPT) {
+            const int l = e / L, z = e - l * L;
+            const cx<T> v = R[(q * P + (l >> 1)) * SS + skew<T>(z)];
+            val[f0 + q][i] = (l & 1) ? v.im : v.re;
+          }
+        }
       }
     }
     __syncthreads();
@@ -799,22 +844,28 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   }
   if constexpr (GRAD) {
 #pragma unroll
-    for (int f = 0; f < NW; ++f) {
+    for (int f0 = 0; f0 < NW; f0 += FG) {
+      const int nf = NW - f0 < FG ? NW - f0 : FG;
 #pragma unroll
-      for (int i = 0; i < PT; ++i) {
-        const int e = threadIdx.x + i * kThreads;
-        if (e < NPT) {
-          const int l = e / L, z = e - l * L;
-          cx<T>* c = &A[(l >> 1) * SS + skew<T>(z)];
-          if (l & 1)
-            c->im = val[f][i];
-          else
-            c->re = val[f][i];
+      for (int q = 0; q < FG; ++q) {
+        if (f0 + q < NW) {
+#pragma unroll
+          for (int i = 0; i < PT; ++i) {
+            const int e = threadIdx.x + i * kThreads;
+            if (e < NPT) {
+              const int l = e / L, z = e - l * L;
+              cx<T>* c = &A[(q * P + (l >> 1)) * SS + skew<T>(z)];
+              if (l & 1)
+                c->im = val[f0 + q][i];
+              else
+                c->re = val[f0 + q][i];
+            }
+          }
         }
       }
       __syncthreads();
-      const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, P, twl, fstride);
-      store_r2c_halves<T, L, P>(Z, fields + f * g.ks, line0, lines, g.ldk);
+      const cx<T>* Z = fft_lds<T, L, SS, false>(A, B, nf * P, twl, fstride);
+      for (int q = 0; q < nf; ++q) store_r2c_halves<T, L, P>(Z + q * P * SS, fields + (f0 + q) * g.ks, line0, lines, g.ldk);
       __syncthreads();
     }
   }
