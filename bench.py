@@ -308,6 +308,22 @@ def c5_run(iters=40, seed=2024):
             "converged": False}
 
 
+def event_roofline(f, run):
+    """Runs `run()` with the plan's HIP-event profiler on and returns, per kernel
+    family with time, its launches, ms, algorithmic GB/s and fraction of the
+    8 TB/s peak (the families and byte counts of ProfScope, plan.hpp)."""
+    f.profile_reset()
+    f.set_profiling(True)
+    run()
+    f.set_profiling(False)
+    out = {}
+    for k, (n, ms, by) in f.profile().items():
+        if n > 0 and ms > 0:
+            gbs = by / (ms * 1e-3) / 1e9
+            out[k] = {"launches": n, "ms": round(ms, 3), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    return dict(sorted(out.items(), key=lambda kv: -kv[1]["ms"]))
+
+
 def sw_run(kT=5.0, ff=0.3, lam=1.3):
     """The square-well liquid around a test particle at its published shape
     (papers/square-well-fluid/figs/new-radial-sw.cpp: 18^3 cell, dx = 0.1 ->
@@ -346,6 +362,7 @@ def sw_run(kT=5.0, ff=0.3, lam=1.3):
 
     one()
     dt, st = one()
+    prof = event_roofline(f, one)
     eta = f.density()[0, 0, :Ng // 2] * math.pi * sigma ** 3 / 6
     gold = json.load(open(os.path.join(HERE, "tests", "golden", "sw_radial.json")))["cases"]
     case = [g for g in gold if g["kT"] == kT and g["ff"] == ff][0]
@@ -357,7 +374,8 @@ def sw_run(kT=5.0, ff=0.3, lam=1.3):
             "minimizations_per_s": 1.0 / dt, "seconds": dt, "iterations": st["iter"],
             "energy_calcs": st["num_energy_calcs"], "grad_calcs": st["num_grad_calcs"],
             "ms_per_iteration": dt / st["iter"] * 1e3,
-            "max_rel_dev_from_published": float(np.max(np.abs(eta[out] - pub[out]) / pub[out]))}
+            "max_rel_dev_from_published": float(np.max(np.abs(eta[out] - pub[out]) / pub[out])),
+            "kernel_families": prof}
 
 
 def sfmt_run(iters=30):
@@ -392,12 +410,13 @@ def sfmt_run(iters=30):
 
     one()
     dt, st = one()
+    prof = event_roofline(f, one)
     f.close()
     return {"workload": f"soft FMT (sigma = epsilon = kT = 1, Xi = {xi:g}) around a radius-3 hard solute, "
                         f"{M}^3, {L:g}^3 cell, eta = 0.40, {iters} ladder iterations (grad/n)",
             "iterations": st["iter"], "ms_per_iteration": dt / st["iter"] * 1e3,
             "iterations_per_s": st["iter"] / dt, "energy_calcs": st["num_energy_calcs"],
-            "grad_calcs": st["num_grad_calcs"]}
+            "grad_calcs": st["num_grad_calcs"], "kernel_families": prof}
 
 
 def rocprof_instance(family, tuning):
