@@ -343,24 +343,26 @@ __global__ __launch_bounds__(kThreads) void kg_col(GridArgs g, LineSched s, int 
   const long long stride = axis == 1 ? (long long)g.ldk : (long long)g.Ny * g.ldk;
   const long long outer = axis == 1 ? (long long)g.Ny * g.ldk : (long long)g.ldk;
   cx<T>* base = data + blockIdx.y * g.ks;
-  for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int c = e % C, i = e / C;
-    if (c < ncol) {
-      const long long v = col0 + c;
-      const long long o = v / g.Nkz;
-      A[c * SS + skew<T>(i)] = base[o * outer + (v - o * g.Nkz) + i * stride];
-    }
+  // each column's offset once (one 64-bit division per column, not per
+  // element), and a fixed thread -> (column, first row) mapping: the rows of
+  // a column are kThreads / C apart
+  __shared__ long long coff[kThreads];
+  if (threadIdx.x < ncol) {
+    const long long v = col0 + threadIdx.x;
+    const long long o = v / g.Nkz;
+    coff[threadIdx.x] = o * outer + (v - o * g.Nkz);
   }
   __syncthreads();
+  const int rows = kThreads / C;  // rows handled at once
+  const int c = threadIdx.x % C, i0 = threadIdx.x / C;
+  const bool act = i0 < rows && c < ncol;
+  const long long co = act ? coff[c] : 0;
+  if (act)
+    for (int i = i0; i < L; i += rows) A[c * SS + skew<T>(i)] = base[co + i * stride];
+  __syncthreads();
   const cx<T>* R = fft_lds_rt<T, INV>(A, B, ncol, tw, s);
-  for (int e = threadIdx.x; e < C * L; e += kThreads) {
-    const int c = e % C, i = e / C;
-    if (c < ncol) {
-      const long long v = col0 + c;
-      const long long o = v / g.Nkz;
-      base[o * outer + (v - o * g.Nkz) + i * stride] = R[c * SS + skew<T>(i)];
-    }
-  }
+  if (act)
+    for (int i = i0; i < L; i += rows) base[co + i * stride] = R[c * SS + skew<T>(i)];
 }
 
 // --- pointwise passes (gridDim.x = the plan's elementwise block count) -----------
