@@ -144,16 +144,33 @@ inline std::vector<std::vector<int>> radix_candidates(int L) {
 
 // --- runtime Stockham stages -------------------------------------------------
 
+// n / d for 0 <= n < 2^22 and a run-time d >= 1 without the integer division
+// sequence: a float estimate (relative error below 2^-23, so off by at most
+// one) and one correction each way.
+struct FastDiv {
+  int d;
+  float inv;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return FastDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int fdiv(int n, const FastDiv& f) {
+  int q = (int)((float)n * f.inv);
+  if (q * f.d > n) --q;
+  if ((q + 1) * f.d <= n) ++q;
+  return q;
+}
+
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void stage_rt(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int nseq,
                                          const cx<T>* __restrict__ tw, int L, int SS, int NS) {
   const int M = L / R;
   const int TWS = L / (NS * R);
   const int total = nseq * M;
+  const FastDiv dM = fast_div(M), dNS = fast_div(NS);
   for (int b = threadIdx.x; b < total; b += kBlockThreads) {
-    const int s = b / M;
+    const int s = fdiv(b, dM);
     const int j = b - s * M;
-    const int k = j % NS;
+    const int jn = fdiv(j, dNS);
+    const int k = j - jn * NS;
     const cx<T>* in = src + s * SS;
     cx<T> v[R];
 #pragma unroll
@@ -167,7 +184,7 @@ __device__ __forceinline__ void stage_rt(const cx<T>* __restrict__ src, cx<T>* _
     }
     dft<T, R, INV>(v);
     cx<T>* out = dst + s * SS;
-    const int o0 = (j / NS) * NS * R + k;
+    const int o0 = jn * NS * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) out[skew<T>(o0 + r * NS)] = v[r];
   }
