@@ -250,3 +250,40 @@ def test_published_radial_profile(gpu, case):
     assert err < 1e-4, err
     m.close()
     f.close()
+
+
+@pytest.mark.gpu
+def test_ladder_reaches_the_fp64_minimum(gpu):
+    """The FP32 -> FP64 precision ladder on the square-well liquid (a 64^3 test
+    particle set-up at dx = 0.2) ends at the pure-FP64 minimum: the same energy to
+    1e-9 relative and densities within 1e-6 of the largest."""
+    kT, lam, ff, R, eps = 3.0, 1.3, 0.3, 1.0, 1.0
+    sigma, edge, M = 2 * R, 12.8, 64
+    idx = np.arange(M)
+    c = np.where(idx > M // 2, idx - M, idx) * edge / M
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    nb = ff / (math.pi * sigma ** 3 / 6)
+    vext = np.where(r <= sigma, 400 * kT, np.where(r < sigma * lam, -eps, 0.0))
+    mu = gpu.HomogeneousSW_liquid(R=R, epsilon=eps, lambda_=lam, kT=kT, n=nb).d_by_dn()
+    out = []
+    for ladder in (False, True):
+        f = gpu.SWLiquidVeff((M,) * 3, np.diag([edge] * 3), R=R, epsilon=eps, lambda_=lam, kT=kT, mu=mu,
+                             fft_precision=0 if ladder else 1)
+        f.Vext = vext
+        f.fill_Veff(-kT * math.log(nb))
+        m = gpu.Minimize(f)
+        m.precondition(True)
+        m.set_relative_precision(1e-11)
+        m.set_maxiter(600)
+        if ladder:
+            m.set_mixed_precision(1e-5, gpu.FFT_FP64)
+        m.run()
+        assert m.stats()["stop_reason"] == "converged", (ladder, m.stats())
+        f.fft_precision = 1
+        out.append((f.energy(), f.density()))
+        m.close()
+        f.close()
+    (E64, n64), (El, nl) = out
+    assert abs(El - E64) <= 1e-9 * abs(E64), (El, E64)
+    assert np.max(np.abs(nl - n64)) <= 1e-6 * np.max(n64)
