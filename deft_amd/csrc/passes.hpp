@@ -558,11 +558,15 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
   // FP64: two weighted fields per inverse-FFT round (twice the butterflies
   // per stage, half the barriers; 3% faster at 128^3).  FP32 and the eleven
   // tensor fields take one field per round: the doubled buffers cost FP32 a
-  // third of its resident blocks and 14%.  The transformed input stays in
-  // registers between the rounds, so two buffers suffice.
-  constexpr int FG = (NW == 5 && sizeof(T) == 8) ? 2 : 1;
+  // third of its resident blocks and 14%.  Up to L = 128 the transformed
+  // input stays in registers between the rounds, so two buffers suffice;
+  // beyond, those registers (and the doubled buffers) cost resident blocks,
+  // so the input stays in LDS and a third buffer serves the inverse FFTs.
+  constexpr bool kSReg = L <= 128;
+  constexpr int FG = (NW == 5 && sizeof(T) == 8 && kSReg) ? 2 : 1;
   __shared__ cx<T> A[FG * C * SS];
   __shared__ cx<T> B[FG * C * SS];
+  __shared__ cx<T> S2[kSReg ? 1 : C * SS];
   const cx<T>* twl = lds_twiddles<T, L>(tw);
   const long long plane = (long long)g.Ny * g.ldk;
   const int myc = threadIdx.x % C;
@@ -576,9 +580,11 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
     A[myc * SS + skew<T>(xx)] = load_or_zero(in, (long long)xx * plane + off, 0, myc < ncol);
   }
   __syncthreads();
-  const cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, twl);
+  cx<T>* S = fft_lds<T, L, SS, false>(A, B, ncol, twl);
+  cx<T>* W1 = kSReg ? A : (S == A ? B : A);
+  cx<T>* W2 = kSReg ? B : S2;
   T w3[PT], w2[PT], wk[PT][3], wt[PT][6];
-  cx<T> sv[PT];
+  cx<T> sv[kSReg ? PT : 1];
   {
     const int y = cy, kz = ckz;
 #pragma unroll
@@ -587,15 +593,17 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
       w3[i] = w2[i] = wk[i][0] = wk[i][1] = wk[i][2] = (T)0;
 #pragma unroll
       for (int j = 0; j < 6; ++j) wt[i][j] = (T)0;
-      sv[i] = {(T)0, (T)0};
-      if (e < C * L) sv[i] = S[myc * SS + skew<T>(e / C)];
+      if constexpr (kSReg) {
+        sv[i] = {(T)0, (T)0};
+        if (e < C * L) sv[i] = S[myc * SS + skew<T>(e / C)];
+      }
       if (e < C * L && myc < ncol) {
         element_weights<T>(g, e / C, y, kz, w3[i], w2[i], wk[i]);
         if constexpr (NW == 11) element_tensor_weights<T>(g, e / C, y, kz, w3[i], w2[i], wt[i]);
       }
     }
   }
-  __syncthreads();  // S is read before the rounds overwrite both buffers
+  if constexpr (kSReg) __syncthreads();  // S is read before the rounds overwrite both buffers
 #pragma unroll
   for (int f0 = 0; f0 < NW; f0 += FG) {
     const int nf = NW - f0 < FG ? NW - f0 : FG;
@@ -605,13 +613,16 @@ __device__ __forceinline__ void xconv_fwd_body(int bx, const GridArgs& g, const 
 #pragma unroll
         for (int i = 0; i < PT; ++i) {
           const int e = threadIdx.x + i * kThreads;
-          if (e < C * L)
-            A[(q * C + myc) * SS + skew<T>(e / C)] = weight_mul<T, PT>(f0 + q, i, sv[i], w3, w2, wk, wt);
+          if (e < C * L) {
+            const int j = skew<T>(e / C);
+            const cx<T> v = kSReg ? sv[kSReg ? i : 0] : S[myc * SS + j];
+            W1[(q * C + myc) * SS + j] = weight_mul<T, PT>(f0 + q, i, v, w3, w2, wk, wt);
+          }
         }
       }
     }
     __syncthreads();
-    const cx<T>* R = fft_lds<T, L, SS, true>(A, B, nf * C, twl);
+    const cx<T>* R = fft_lds<T, L, SS, true>(W1, W2, FG == 1 ? ncol : nf * C, twl);
     for (int q = 0; q < nf; ++q) {
       cx<T>* o = out + (f0 + q) * g.ks;
       for (int e = threadIdx.x; e < C * L; e += kThreads) {
@@ -746,7 +757,9 @@ __device__ __forceinline__ void zphi_body(int bx, const GridArgs& g, cx<T>* __re
   constexpr int NW = FN == 3 ? 11 : 5;
   // Two fields per FFT round (the eleven tensor fields one at a time): twice
   // the butterflies per stage and half the barriers of one field per round.
-  constexpr int FG = FN == 3 ? 1 : 2;
+  // At L >= 256 the paired round needs 184 FP64 registers (two waves per
+  // SIMD instead of four), which costs more than the barriers it saves.
+  constexpr int FG = (FN == 3 || L > 128) ? 1 : 2;
   __shared__ cx<T> A[zbuf_size<T, L, FG * P>()];
   __shared__ cx<T> B[zbuf_size<T, L, FG * P>()];
   // The butterfly loops stride by blockDim.x here: on a compile-time stride
