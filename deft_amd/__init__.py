@@ -89,6 +89,8 @@ _SIGNATURES = {
     "deft_plan_autotune": ([_P], _I),
     "deft_plan_tuning": ([_P, _I, _IP, _IP, _IP], _I),
     "deft_plan_set_tuning": ([_P, _I, _I, _I, _I], _I),
+    "deft_plan_zphi_variant": ([_P, _I, _IP], _I),
+    "deft_plan_set_zphi_variant": ([_P, _I, _I], _I),
     "deft_plan_radix_schedule": ([_P, _I, _I, _IP, _I, _IP], _I),
     "deft_plan_set_radix_schedule": ([_P, _I, _I, _IP, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
@@ -527,10 +529,16 @@ class WhiteBearFluidVeff:
     def tuning(self, fft_precision):
         z, y, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().deft_plan_tuning(self._h, int(fft_precision), ctypes.byref(z), ctypes.byref(y), ctypes.byref(x)))
-        return {"zblock": z.value, "ytile": y.value, "xtile": x.value}
+        w = ctypes.c_int()
+        _check(lib().deft_plan_zphi_variant(self._h, int(fft_precision), ctypes.byref(w)))
+        return {"zblock": z.value, "ytile": y.value, "xtile": x.value, "zphi": w.value}
 
-    def set_tuning(self, fft_precision, zblock, ytile, xtile):
+    def set_tuning(self, fft_precision, zblock, ytile, xtile, zphi=None):
+        """Explicit pass variants; `zphi` (0 block-wide FFT stages, 1 one field per wave)
+        is left as it is when None."""
         _check(lib().deft_plan_set_tuning(self._h, int(fft_precision), int(zblock), int(ytile), int(xtile)))
+        if zphi is not None:
+            _check(lib().deft_plan_set_zphi_variant(self._h, int(fft_precision), int(zphi)))
 
     def radix_schedule(self, fft_precision, axis):
         """The radix order (Stockham stages) of one axis's transforms."""

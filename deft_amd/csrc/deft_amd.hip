@@ -828,6 +828,26 @@ int deft_plan_set_tuning(deft_plan p, int fft_precision, int zblock, int ytile, 
   return DEFT_OK;
 }
 
+int deft_plan_zphi_variant(deft_plan p, int fft_precision, int *variant) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (variant) *variant = p->zw[fft_precision];
+  return DEFT_OK;
+}
+
+int deft_plan_set_zphi_variant(deft_plan p, int fft_precision, int variant) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (variant != 0 && variant != 1) return set_error(DEFT_ERR_ARG, "the Phi pass variant is 0 or 1");
+  if (variant == 1 && (p->generic || p->functional != DEFT_WHITE_BEAR || !wave_fft_length(p->Nz)))
+    return set_error(DEFT_ERR_ARG,
+                     "the one-field-per-wave Phi pass takes fused White-Bear plans with a power-of-two z edge");
+  p->zw[fft_precision] = variant;
+  return DEFT_OK;
+}
+
 int deft_plan_radix_schedule(deft_plan p, int fft_precision, int axis, int *radices, int capacity, int *nstages) {
   TRY(check_plan(p));
   if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)

@@ -12,6 +12,7 @@
 
 #include "batch_kernels.hpp"
 #include "plan.hpp"
+#include "wave_fft.hpp"
 
 namespace deft {
 
@@ -178,10 +179,26 @@ static void launch_zphi(deft_plan_s *p, const GridArgs &g, bool grad, int zb) {
     TensorLaunch<T>::zphi(p, g, grad, zb);
     return;
   }
+  const bool wave = p->zw[prec_index<T>()] != 0;
   with_length(p->Nz, [&](auto LZ) {
     constexpr int L = decltype(LZ)::value;
     with_zb(zb, [&](auto ZBc) {
       constexpr int ZB = decltype(ZBc)::value;
+      if constexpr (wave_fft_length(L) && ZB == 512) {
+        if (wave) {
+          const dim3 grid(zblocks_for(p, ZB)), blk(kWaveThreads);
+          if (grad) {
+            ProfScope ps(p, kid<T>(K_ZPHI_EG), 10 * cb);
+            hipLaunchKernelGGL((k_zphi_wave<T, L, true>), grid, blk, 0, p->stream, g, spec_ptr<T>(p, 1),
+                               p->partial, p->pstride, tw_ptr<T>(p, 2));
+          } else {
+            ProfScope ps(p, kid<T>(K_ZPHI_E), 5 * cb);
+            hipLaunchKernelGGL((k_zphi_wave<T, L, false>), grid, blk, 0, p->stream, g, spec_ptr<T>(p, 1),
+                               p->partial, p->pstride, tw_ptr<T>(p, 2));
+          }
+          return;
+        }
+      }
       if (grad) {
         ProfScope ps(p, kid<T>(K_ZPHI_EG), 10 * cb);
         hipLaunchKernelGGL((k_zphi<T, L, true, ZB>), dim3(zblocks_for(p, ZB)), dim3(kThreads), 0, p->stream, g,
@@ -1078,13 +1095,21 @@ int Launch<T>::autotune(deft_plan_s *p) {
   const bool was_prof = p->prof;
   p->prof = false;
   float best = 1e30f;
-  for (int zb : kZBlocks) {
-    const float t = time_launches(p, [&] { launch_zphi<T>(p, g, true, zb); });
-    if (t < best) {
-      best = t;
-      p->tz[ti] = zb;
+  const int nzw = (p->functional == DEFT_WHITE_BEAR && wave_fft_length(p->Nz)) ? 2 : 1;
+  int best_zw = 0;
+  for (int zw = 0; zw < nzw; ++zw) {
+    for (int zb : kZBlocks) {
+      if (zw == 1 && zb != 512) continue;  // the wave variant exists at 512-value blocks
+      p->zw[ti] = zw;
+      const float t = time_launches(p, [&] { launch_zphi<T>(p, g, true, zb); });
+      if (t < best) {
+        best = t;
+        p->tz[ti] = zb;
+        best_zw = zw;
+      }
     }
   }
+  p->zw[ti] = best_zw;
   best = 1e30f;
   for (int tb : kTileBytes) {
     const float t = time_launches(p, [&] { launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, 5, tb); });

@@ -75,23 +75,27 @@ constexpr int ccols(int L, int TB) {
 static_assert(256 % ccols<float>(20, 16384) == 0 && ccols<double>(128, 8192) == 4, "column tiles");
 
 // Block-wide FP64 sums of NV values, stored to partial[(slot0+i)*stride + bx].
-template <int NV>
+// The first NWV waves of the block hold the values (any further waves' are
+// not counted); every thread of the block must call it.
+template <int NV, int NWV = kThreads / 64>
 __device__ __forceinline__ void block_sum_store(double (&v)[NV], double* __restrict__ partial, int slot0,
                                                 int stride, int bx) {
-  __shared__ double red[NV][kThreads / 64];
+  __shared__ double red[NV][NWV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave < NWV) {
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = v[i];
+    for (int i = 0; i < NV; ++i) {
+      double s = v[i];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) red[i][wave] = s;
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if (lane == 0) red[i][wave] = s;
+    }
   }
   __syncthreads();
   if (threadIdx.x < NV) {
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) s += red[threadIdx.x][w];
+    for (int w = 0; w < NWV; ++w) s += red[threadIdx.x][w];
     partial[(long long)(slot0 + threadIdx.x) * stride + bx] = s;
   }
 }

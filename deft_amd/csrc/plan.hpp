@@ -132,6 +132,9 @@ struct deft_plan_s {
   int tz[2] = {1024, 512};  // (FP64 at 1024 exceeds the Phi pass register budget)
   int ty[2] = {16384, 16384};
   int tx[2] = {8192, 8192};
+  // White-Bear Phi pass: 0 the block-wide FFT stages (k_zphi), 1 one field per
+  // wave (k_zphi_wave, power-of-two z edges); both give the same values
+  int zw[2] = {0, 0};
   // edges without fused kernels (generic.hpp): line-pass radix schedules per
   // FFT precision and axis (SS is filled at launch) and the real work fields
   bool generic = false;

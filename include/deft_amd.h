@@ -199,6 +199,13 @@ int deft_plan_set_engine(deft_plan plan, int generic);
  * (zblock 512 or 1024, ytile and xtile 8192 or 16384). */
 int deft_plan_tuning(deft_plan plan, int fft_precision, int *zblock, int *ytile, int *xtile);
 int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytile, int xtile);
+/* The White-Bear Phi pass variant for one FFT precision: 0 transforms a
+ * block's lines with block-wide FFT stages, 1 gives each field to one wave
+ * (fused plans of the White-Bear functional with a power-of-two z edge,
+ * 16..512); both compute the same values.  Set by deft_plan_autotune.  No
+ * counterpart in the reference. */
+int deft_plan_zphi_variant(deft_plan plan, int fft_precision, int *variant);
+int deft_plan_set_zphi_variant(deft_plan plan, int fft_precision, int variant);
 /* Radix schedule of one axis (0 x, 1 y, 2 z) for one FFT precision, in
  * Stockham stage order: radix 8, 4, 2, 3, 5 butterflies in registers, other
  * primes as direct-DFT stages.  Plans with a non-fused edge take any valid

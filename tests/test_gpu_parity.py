@@ -333,6 +333,41 @@ def test_every_tuning_variant_matches_oracle(gpu, prec, etol, gtol):
     assert np.array_equal(f.Veff, of.x)  # tuning never touches the state
 
 
+@pytest.mark.parametrize("prec,etol,gtol", [(1, 1e-12, 1e-10), (0, 2e-7, 2e-5)])
+@pytest.mark.parametrize("shape", [(48, 32, 64), (21, 20, 128), (16, 16, 256), (20, 21, 16), (32, 21, 32)])
+def test_wave_phi_pass_equals_block_pass(gpu, shape, prec, etol, gtol):
+    """The one-field-per-wave Phi pass runs the block pass's operations on the same
+    values: energy, gradient and preconditioned gradient agree bit for bit, and with
+    the oracle.  The line counts 420 and 672 leave the last block partly empty
+    (8, 64 and 32 lines per block at z edges 128, 16 and 32)."""
+    print("wave phi", shape, prec, flush=True)
+    cell = np.diag([0.1 * n for n in shape])
+    g, of, f = _fluid(gpu, shape, cell, prec=prec, seed=11)
+    Eo, gro, pgo = of.energy_grad_and_precond()
+    t = f.tuning(prec)
+    f.set_tuning(prec, 512, t["ytile"], t["xtile"], zphi=0)
+    Eb, grb, pgb = f.energy_grad_and_precond()
+    eb = f.energy()
+    f.set_tuning(prec, 512, t["ytile"], t["xtile"], zphi=1)
+    assert f.tuning(prec)["zphi"] == 1
+    Ew, grw, pgw = f.energy_grad_and_precond()
+    ew = f.energy()
+    assert Ew == Eb and ew == eb
+    assert np.array_equal(grw, grb) and np.array_equal(pgw, pgb)
+    assert abs(Ew - Eo) <= etol * abs(Eo)
+    assert _rel(grw, gro) < gtol
+
+
+def test_wave_phi_pass_is_refused_where_it_does_not_apply(gpu):
+    """Odd z edges and other functionals keep the block pass."""
+    g, of, f = _fluid(gpu, (16, 16, 21), np.diag([1.6, 1.6, 2.1]), prec=1)
+    with pytest.raises(gpu.DeftError):
+        f.set_tuning(1, 512, 16384, 8192, zphi=1)
+    with pytest.raises(gpu.DeftError):
+        f.set_tuning(1, 512, 16384, 8192, zphi=2)
+    assert f.tuning(1)["zphi"] == 0
+
+
 def test_copy_bandwidth_is_a_sane_yardstick(gpu):
     """The streaming-copy figure the bench reports next to the spec peak."""
     g = gpu.copy_bandwidth(1 << 28, 5)
