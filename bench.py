@@ -437,6 +437,12 @@ def rocprof_instance(family, tuning):
              "zphi_energy": f"deft::k_zphi<{T}, {N}, false, {zb}, 0>",
              "zphi_grad": f"deft::k_zphi<{T}, {N}, true, {zb}, 0>",
              "zgrad": f"deft::k_zgrad<{T}, {N}, {zb}>"}
+    # the one-per-wave variants of the Phi and forward x passes
+    if t.get("zphi"):
+        names["zphi_energy"] = f"deft::k_zphi_wave<{T}, {N}, false>"
+        names["zphi_grad"] = f"deft::k_zphi_wave<{T}, {N}, true>"
+    if t.get("xfwd"):
+        names["xconv_fwd"] = f"deft::k_xconv_fwd_wave<{T}, {N}>"
     return names.get(base)
 
 
@@ -651,6 +657,12 @@ def main():
                      "traffic_source": "profiles/c2_128_solute_pmc.json (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
                                        "copy-calibrated, bytes per dispatch)",
                      "avg_launch_us": kms / kl * 1e3, "bytes_per_launch": kby / kl,
+                     # the next kernel instances by event time, with their fractions of the
+                     # peak (the dominant one can change places with a close second)
+                     "top_kernels": [{"kernel": k, "ms": round(v[1], 3), "avg_launch_us": round(v[1] / v[0] * 1e3, 2),
+                                      "frac": round(v[2] / (v[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}
+                                     for k, v in sorted(((k, v) for k, v in prof.items() if v[0] > 0),
+                                                        key=lambda kv: -kv[1][1])[:5]],
                      "all_kernels_GBps": total_by / (total_ms * 1e-3) / 1e9,
                      # GPU busy fraction of the warm-start steps from the committed rocprofv3 kernel
                      # trace of this workload (kernel time over the span of each step; a lower

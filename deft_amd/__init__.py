@@ -91,6 +91,8 @@ _SIGNATURES = {
     "deft_plan_set_tuning": ([_P, _I, _I, _I, _I], _I),
     "deft_plan_zphi_variant": ([_P, _I, _IP], _I),
     "deft_plan_set_zphi_variant": ([_P, _I, _I], _I),
+    "deft_plan_xfwd_variant": ([_P, _I, _IP], _I),
+    "deft_plan_set_xfwd_variant": ([_P, _I, _I], _I),
     "deft_plan_radix_schedule": ([_P, _I, _I, _IP, _I, _IP], _I),
     "deft_plan_set_radix_schedule": ([_P, _I, _I, _IP, _I], _I),
     "deft_plan_set_fft_precision": ([_P, _I], _I),
@@ -529,16 +531,19 @@ class WhiteBearFluidVeff:
     def tuning(self, fft_precision):
         z, y, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().deft_plan_tuning(self._h, int(fft_precision), ctypes.byref(z), ctypes.byref(y), ctypes.byref(x)))
-        w = ctypes.c_int()
+        w, xw = ctypes.c_int(), ctypes.c_int()
         _check(lib().deft_plan_zphi_variant(self._h, int(fft_precision), ctypes.byref(w)))
-        return {"zblock": z.value, "ytile": y.value, "xtile": x.value, "zphi": w.value}
+        _check(lib().deft_plan_xfwd_variant(self._h, int(fft_precision), ctypes.byref(xw)))
+        return {"zblock": z.value, "ytile": y.value, "xtile": x.value, "zphi": w.value, "xfwd": xw.value}
 
-    def set_tuning(self, fft_precision, zblock, ytile, xtile, zphi=None):
-        """Explicit pass variants; `zphi` (0 block-wide FFT stages, 1 one field per wave)
-        is left as it is when None."""
+    def set_tuning(self, fft_precision, zblock, ytile, xtile, zphi=None, xfwd=None):
+        """Explicit pass variants; the Phi pass layout `zphi` and the forward x pass
+        layout `xfwd` (0 block-wide FFT stages, 1 per wave) are left as they are when None."""
         _check(lib().deft_plan_set_tuning(self._h, int(fft_precision), int(zblock), int(ytile), int(xtile)))
         if zphi is not None:
             _check(lib().deft_plan_set_zphi_variant(self._h, int(fft_precision), int(zphi)))
+        if xfwd is not None:
+            _check(lib().deft_plan_set_xfwd_variant(self._h, int(fft_precision), int(xfwd)))
 
     def radix_schedule(self, fft_precision, axis):
         """The radix order (Stockham stages) of one axis's transforms."""

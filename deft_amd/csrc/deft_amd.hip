@@ -848,6 +848,26 @@ int deft_plan_set_zphi_variant(deft_plan p, int fft_precision, int variant) {
   return DEFT_OK;
 }
 
+int deft_plan_xfwd_variant(deft_plan p, int fft_precision, int *variant) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (variant) *variant = p->xw[fft_precision];
+  return DEFT_OK;
+}
+
+int deft_plan_set_xfwd_variant(deft_plan p, int fft_precision, int variant) {
+  TRY(check_plan(p));
+  if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)
+    return set_error(DEFT_ERR_ARG, "unknown FFT precision");
+  if (variant != 0 && variant != 1) return set_error(DEFT_ERR_ARG, "the x pass variant is 0 or 1");
+  if (variant == 1 && (p->generic || plan_fields(p) != 5 || !wave_fft_length(p->Nx)))
+    return set_error(DEFT_ERR_ARG,
+                     "the one-tile-per-wave x pass takes fused five-field plans with a power-of-two x edge");
+  p->xw[fft_precision] = variant;
+  return DEFT_OK;
+}
+
 int deft_plan_radix_schedule(deft_plan p, int fft_precision, int axis, int *radices, int capacity, int *nstages) {
   TRY(check_plan(p));
   if (fft_precision != DEFT_FFT_FP32 && fft_precision != DEFT_FFT_FP64)

@@ -143,8 +143,20 @@ static void launch_xconv_fwd(deft_plan_s *p, const GridArgs &g, int tb) {
     TensorLaunch<T>::xconv_fwd(p, g);
     return;
   }
+  const bool wave = p->xw[prec_index<T>()] != 0;
   with_length(p->Nx, [&](auto LX) {
     constexpr int L = decltype(LX)::value;
+    if constexpr (wave_fft_length(L)) {
+      if (wave) {
+        constexpr int SPW = 64 / (L / 8);
+        const long long tiles = ((long long)p->Ny * p->Nkz + SPW - 1) / SPW;
+        const int blocks = (int)((tiles + kThreads / 64 - 1) / (kThreads / 64));
+        ProfScope ps(p, kid<T>(K_XCONV_FWD), 6 * cb);
+        hipLaunchKernelGGL((k_xconv_fwd_wave<T, L>), dim3(blocks), dim3(kThreads), 0, p->stream, g,
+                           spec_ptr<T>(p, 0), spec_ptr<T>(p, 1), tw_ptr<T>(p, 0));
+        return;
+      }
+    }
     with_tb(tb, [&](auto TBc) {
       constexpr int TB = decltype(TBc)::value;
       ProfScope ps(p, kid<T>(K_XCONV_FWD), 6 * cb);
@@ -1128,6 +1140,13 @@ int Launch<T>::autotune(deft_plan_s *p) {
       best = t;
       p->tx[ti] = tb;
     }
+  }
+  p->xw[ti] = 0;
+  if (plan_fields(p) == 5 && wave_fft_length(p->Nx)) {
+    const float t0 = time_launches(p, [&] { launch_xconv_fwd<T>(p, g, p->tx[ti]); });
+    p->xw[ti] = 1;
+    const float t1 = time_launches(p, [&] { launch_xconv_fwd<T>(p, g, p->tx[ti]); });
+    if (t1 >= t0) p->xw[ti] = 0;
   }
   p->prof = was_prof;
   DEFT_HIP_CHECK(hipGetLastError());

@@ -134,13 +134,22 @@ static __device__ __forceinline__ long long col_offset(const GridArgs& g, long l
 // vector weight's i-coefficient times k_j, which is -w3 k_j.  FP32 evaluates
 // them in registers; FP64 reads the plan's precomputed table (FP64 sincos/exp
 // per element had dominated the x passes' VALU time).
+// Component d of the k-vector (mx, my, mz) . b in R, with contraction off so
+// that its rounding does not depend on the kernel it is inlined into (the
+// x-pass variants recompute it and must agree bit for bit).
+template <typename R>
+__device__ __forceinline__ R kvec_component(const GridArgs& g, R mx, R my, R mz, int d) {
+#pragma clang fp contract(off)
+  return mx * (R)g.b[d] + my * (R)g.b[3 + d] + mz * (R)g.b[6 + d];
+}
+
 template <typename T>
 __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y, int kz, T& w3, T& w2, T (&wk)[3]) {
   if constexpr (sizeof(T) <= 4) {
     const float mx = (float)wrapf(x, g.Nx), my = (float)wrapf(y, g.Ny), mz = (float)kz;
-    const float kx = mx * (float)g.b[0] + my * (float)g.b[3] + mz * (float)g.b[6];
-    const float ky = mx * (float)g.b[1] + my * (float)g.b[4] + mz * (float)g.b[7];
-    const float kz_ = mx * (float)g.b[2] + my * (float)g.b[5] + mz * (float)g.b[8];
+    const float kx = kvec_component<float>(g, mx, my, mz, 0);
+    const float ky = kvec_component<float>(g, mx, my, mz, 1);
+    const float kz_ = kvec_component<float>(g, mx, my, mz, 2);
     const float k = sqrtf(kx * kx + ky * ky + kz_ * kz_);
     float a3, a2;
     fm_weights_f32(k, (float)g.R, (float)g.six_dr2, a3, a2);
@@ -152,9 +161,9 @@ __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y,
     wk[2] = -w3 * kz_;
   } else {
     const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
-    const double kx = mx * g.b[0] + my * g.b[3] + mz * g.b[6];
-    const double ky = mx * g.b[1] + my * g.b[4] + mz * g.b[7];
-    const double kz_ = mx * g.b[2] + my * g.b[5] + mz * g.b[8];
+    const double kx = kvec_component<double>(g, mx, my, mz, 0);
+    const double ky = kvec_component<double>(g, mx, my, mz, 1);
+    const double kz_ = kvec_component<double>(g, mx, my, mz, 2);
     const long long idx = ((long long)x * g.Ny + y) * g.ldk + kz;
     const double2 w = reinterpret_cast<const double2*>(g.wtab)[idx];
     const double a3 = w.x;

@@ -135,6 +135,9 @@ struct deft_plan_s {
   // White-Bear Phi pass: 0 the block-wide FFT stages (k_zphi), 1 one field per
   // wave (k_zphi_wave, power-of-two z edges); both give the same values
   int zw[2] = {0, 0};
+  // forward x pass (five fields): 0 block-wide FFT stages (k_xconv_fwd), 1 one
+  // tile of columns per wave (k_xconv_fwd_wave, power-of-two x edges)
+  int xw[2] = {0, 0};
   // edges without fused kernels (generic.hpp): line-pass radix schedules per
   // FFT precision and axis (SS is filled at launch) and the real work fields
   bool generic = false;

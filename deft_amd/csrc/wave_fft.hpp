@@ -229,4 +229,97 @@ __global__ __launch_bounds__(kWaveThreads) void k_zphi_wave(GridArgs g, cx<T>* _
   block_sum_store<3, kThreads / 64>(acc, partial, SLOT_PHI1, pstride, blockIdx.x);
 }
 
+// The vector weight -w3 k_d of one element (d = 0, 1, 2), the expression
+// element_weights evaluates, for a caller that keeps only w3 and w2.
+template <typename T>
+__device__ __forceinline__ T element_wk(const GridArgs& g, int x, int y, int kz, T w3, int d) {
+  if constexpr (sizeof(T) <= 4) {
+    const float mx = (float)wrapf(x, g.Nx), my = (float)wrapf(y, g.Ny), mz = (float)kz;
+    return -w3 * kvec_component<float>(g, mx, my, mz, d);
+  } else {
+    const double mx = wrapf(x, g.Nx), my = wrapf(y, g.Ny), mz = kz;
+    return (T)(-w3 * kvec_component<double>(g, mx, my, mz, d));
+  }
+}
+
+// Forward x pass with the weight multiply (the five White-Bear fields), one
+// wave per tile of SPW = 64/(L/8) columns: the wave loads its columns into its
+// LDS slice, transforms them, keeps each lane's eight spectral elements and
+// their w3, w2 in registers, and for every field multiplies, runs the inverse
+// transform (its first stage straight from registers) and stores.  Four waves
+// per block take four consecutive tiles (16 columns at L = 128, so a block
+// reads whole 128-byte rows).  Same stages and weights as k_xconv_fwd; the
+// compiler contracts some products into FMAs differently in the two kernels
+// (the weights' k-vector and, here, the products that feed the first inverse
+// stage in registers), so the results agree to rounding, not bit for bit.
+template <typename T, int L>
+__global__ __launch_bounds__(kThreads) void k_xconv_fwd_wave(GridArgs g, const cx<T>* __restrict__ in,
+                                                              cx<T>* __restrict__ out,
+                                                              const cx<T>* __restrict__ tw) {
+  static_assert(wave_fft_length(L), "the wave transforms take power-of-two lengths 16..512");
+  constexpr int LPS = L / 8, SPW = 64 / LPS, NWV = kThreads / 64;
+  constexpr int SS = lds_stride<T, L>();
+  __shared__ cx<T> S[NWV * SPW * SS];
+  __shared__ cx<T> twl[L];
+  for (int i = threadIdx.x; i < L; i += kThreads) twl[i] = tw[i];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long ntiles = ((long long)g.Ny * g.Nkz + SPW - 1) / SPW;
+  const int tile = blockIdx.x * NWV + wave;
+  if (tile >= ntiles) return;  // no block barrier follows
+  cx<T>* slice = S + wave * SPW * SS;
+  const long long plane = (long long)g.Ny * g.ldk;
+  // column lc = lane % SPW for the loads and stores (64 % SPW == 0, so every
+  // element e = lane + 64 i of the tile belongs to it)
+  const int lc = lane % SPW;
+  int ly, lkz;
+  bool lok;
+  x_tile_column<SPW>(g, tile, lc, ly, lkz, lok);
+  const long long loff = (long long)ly * g.ldk + lkz;
+  for (int e = lane; e < SPW * L; e += 64)
+    slice[lc * SS + skew<T>(e / SPW)] = load_or_zero(in, (long long)(e / SPW) * plane + loff, 0, lok);
+  wave_lds_sync();
+  // column sq = lane / (L/8) for the transforms; the lane holds its elements
+  // kx = t + r L/8
+  const int sq = lane / LPS, t = lane - sq * LPS;
+  cx<T>* seq = slice + sq * SS;
+  wave_stages<T, L, 1, false>(seq, t, twl);
+  int cy, ckz;
+  bool cok;
+  x_tile_column<SPW>(g, tile, sq, cy, ckz, cok);  // (cy, ckz) valid also where !cok
+  cx<T> sv[8];
+  T w3[8], w2[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    sv[r] = seq[skew<T>(t + r * LPS)];
+    T wk[3];
+    element_weights<T>(g, t + r * LPS, cy, ckz, w3[r], w2[r], wk);
+  }
+  wave_lds_sync();
+  // the n2 field first: its weights w2 are then dead for the other four
+#pragma unroll
+  for (int fo = 0; fo < 5; ++fo) {
+    const int f = fo == 0 ? 1 : (fo == 1 ? 0 : fo);
+    cx<T> v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (f == 0)
+        v[r] = scale(sv[r], w3[r]);
+      else if (f == 1)
+        v[r] = scale(sv[r], w2[r]);
+      else
+        v[r] = mul_i(scale(sv[r], element_wk<T>(g, t + r * LPS, cy, ckz, w3[r], f - 2)));
+    }
+    wave_butterflies<T, L, 8, 1, true>(v, t, twl);
+    wave_lds_sync();
+    wave_write<T, L, 8, 1>(v, seq, t);
+    wave_lds_sync();
+    wave_stages<T, L, 8, true>(seq, t, twl);
+    cx<T>* o = out + f * g.ks;
+    if (lok)
+      for (int e = lane; e < SPW * L; e += 64) o[(long long)(e / SPW) * plane + loff] = slice[lc * SS + skew<T>(e / SPW)];
+    wave_lds_sync();  // the next field's first stage overwrites the slice
+  }
+}
+
 }  // namespace deft

@@ -206,6 +206,13 @@ int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytil
  * counterpart in the reference. */
 int deft_plan_zphi_variant(deft_plan plan, int fft_precision, int *variant);
 int deft_plan_set_zphi_variant(deft_plan plan, int fft_precision, int variant);
+/* The forward x pass variant (the transform of the density and the five
+ * weighted densities): 0 block-wide FFT stages, 1 one tile of columns per wave
+ * (fused plans of the five-field functionals with a power-of-two x edge);
+ * both compute the same values.  Set by deft_plan_autotune.  No counterpart
+ * in the reference. */
+int deft_plan_xfwd_variant(deft_plan plan, int fft_precision, int *variant);
+int deft_plan_set_xfwd_variant(deft_plan plan, int fft_precision, int variant);
 /* Radix schedule of one axis (0 x, 1 y, 2 z) for one FFT precision, in
  * Stockham stage order: radix 8, 4, 2, 3, 5 butterflies in registers, other
  * primes as direct-DFT stages.  Plans with a non-fused edge take any valid
