@@ -191,6 +191,7 @@ void free_plan(deft_plan_s *p) {
   (void)hipFree(p->twbuf);
   (void)hipFree(p->tmp);
   (void)hipFree(p->wtab);
+  (void)hipFree(p->wtab32);
   (void)hipFree(p->gwork);
   (void)hipFree(p->sktab);
   (void)hipFree(p->stab);
@@ -556,6 +557,7 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
       hipMalloc(&p->spec, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->partial, sizeof(double) * NUM_SLOTS * (size_t)p->pstride) != hipSuccess ||
       hipMalloc(&p->wtab, 2 * sizeof(double) * (size_t)p->ks) != hipSuccess ||
+      hipMalloc(&p->wtab32, 2 * sizeof(float) * (size_t)p->ks) != hipSuccess ||
       hipMalloc(&p->dcount, sizeof(unsigned)) != hipSuccess)
     return fail(DEFT_ERR_ALLOC, "device allocation failed");
   if (ensure_gwork(p) != DEFT_OK) return fail(DEFT_ERR_ALLOC, "device allocation failed");
@@ -607,7 +609,8 @@ int deft_plan_create(int nx, int ny, int nz, const double cell[9], double R, dou
     return fail(DEFT_ERR_HIP, "initialisation failed");
   if (hipMemset(p->spec, 0, 6 * sizeof(cx<double>) * (size_t)p->ks) != hipSuccess)
     return fail(DEFT_ERR_HIP, "initialisation failed");
-  hipLaunchKernelGGL(k_weight_table, dim3(p->eblocks), dim3(kThreads), 0, p->stream, grid_args(p), p->wtab);
+  hipLaunchKernelGGL(k_weight_table, dim3(p->eblocks), dim3(kThreads), 0, p->stream, grid_args(p), p->wtab,
+                     p->wtab32);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
     return fail(DEFT_ERR_HIP, "weight table failed");
   *out = p;

@@ -165,6 +165,7 @@ extern "C" int deft_functional_grad_noalloc(int nx, int ny, int nz, const double
     v.wtab = c.take<double>(2 * sizeof(double) * (size_t)v.ks);
     v.vext64 = vext_dev ? const_cast<double *>(vext_dev) : c.take<double>(sizeof(double) * (size_t)N);
   } else {
+    v.wtab32 = c.take<float>(2 * sizeof(float) * (size_t)v.ks);
     v.vext = c.take<float>(sizeof(float) * (size_t)N);
   }
   v.partial = c.take<double>(sizeof(double) * NUM_SLOTS * (size_t)v.pstride);
@@ -186,9 +187,10 @@ extern "C" int deft_functional_grad_noalloc(int nx, int ny, int nz, const double
   hipLaunchKernelGGL(k_noalloc_twiddles, dim3((ntw + kThreads - 1) / kThreads), tb, 0, v.stream, nx, ny, nz, twd,
                      twf);
   if (f64) {
-    hipLaunchKernelGGL(k_weight_table, eb, tb, 0, v.stream, grid_args(&v), v.wtab);
+    hipLaunchKernelGGL(k_weight_table, eb, tb, 0, v.stream, grid_args(&v), v.wtab, nullptr);
     if (!vext_dev) hipLaunchKernelGGL(k_noalloc_vext, eb, tb, 0, v.stream, N, nullptr, nullptr, v.vext64);
   } else {
+    hipLaunchKernelGGL(k_weight_table, eb, tb, 0, v.stream, grid_args(&v), nullptr, v.wtab32);
     hipLaunchKernelGGL(k_noalloc_vext, eb, tb, 0, v.stream, N, vext_dev, v.vext, nullptr);
   }
   {

@@ -46,6 +46,7 @@ enum Slot : int {
 
 struct GridArgs {
   const double* wtab;  // FP64 weight table: (w3/N, w2/N) per spectral element, rows padded to ldk
+  const float* wtab32;  // the same table rounded to FP32, for the FP32 passes
   const double* stab;  // soft FMT: the seven radial weights /N per spectral element (8 doubles), rows padded to ldk
   int Nx, Ny, Nz, Nkz;
   int ldk;        // padded half-spectrum row stride (complex values)
@@ -130,19 +131,20 @@ static __device__ __forceinline__ long long col_offset(const GridArgs& g, long l
   return y * g.ldk + (v - y * g.Nkz);
 }
 
-// Weights (scaled by 1/N) of spectral element (x, y, kz): w3, w2 and the
-// vector weight's i-coefficient times k_j, which is -w3 k_j.  FP32 evaluates
-// them in registers; FP64 reads the plan's precomputed table (FP64 sincos/exp
-// per element had dominated the x passes' VALU time).
 // Component d of the k-vector (mx, my, mz) . b in R, with contraction off so
 // that its rounding does not depend on the kernel it is inlined into (the
-// x-pass variants recompute it and must agree bit for bit).
+// x-pass variants recompute it).
 template <typename R>
 __device__ __forceinline__ R kvec_component(const GridArgs& g, R mx, R my, R mz, int d) {
 #pragma clang fp contract(off)
   return mx * (R)g.b[d] + my * (R)g.b[3 + d] + mz * (R)g.b[6 + d];
 }
 
+// Weights (scaled by 1/N) of spectral element (x, y, kz): w3, w2 and the
+// vector weight's i-coefficient times k_j, which is -w3 k_j.  w3 and w2 come
+// from the plan's table, computed once in FP64 (FP64 reads it as is, FP32 its
+// FP32 rounding): evaluated per element, their sines, cosines and
+// exponentials had dominated the x passes' VALU time in both precisions.
 template <typename T>
 __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y, int kz, T& w3, T& w2, T (&wk)[3]) {
   if constexpr (sizeof(T) <= 4) {
@@ -150,12 +152,10 @@ __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y,
     const float kx = kvec_component<float>(g, mx, my, mz, 0);
     const float ky = kvec_component<float>(g, mx, my, mz, 1);
     const float kz_ = kvec_component<float>(g, mx, my, mz, 2);
-    const float k = sqrtf(kx * kx + ky * ky + kz_ * kz_);
-    float a3, a2;
-    fm_weights_f32(k, (float)g.R, (float)g.six_dr2, a3, a2);
-    const float invN = (float)g.invN;
-    w3 = a3 * invN;
-    w2 = a2 * invN;
+    const long long idx = ((long long)x * g.Ny + y) * g.ldk + kz;
+    const float2 w = reinterpret_cast<const float2*>(g.wtab32)[idx];
+    w3 = w.x;
+    w2 = w.y;
     wk[0] = -w3 * kx;
     wk[1] = -w3 * ky;
     wk[2] = -w3 * kz_;
@@ -175,8 +175,10 @@ __device__ __forceinline__ void element_weights(const GridArgs& g, int x, int y,
   }
 }
 
-// FP64 weight table (w3/N, w2/N) of every valid spectral element, built once per plan.
-static __global__ __launch_bounds__(kThreads) void k_weight_table(GridArgs g, double* __restrict__ tab) {
+// FP64 weight table (w3/N, w2/N) of every valid spectral element, built once per
+// plan, and its FP32 rounding (either output may be null).
+static __global__ __launch_bounds__(kThreads) void k_weight_table(GridArgs g, double* __restrict__ tab,
+                                                                  float* __restrict__ tab32) {
   const long long nk = (long long)g.Nx * g.Ny * g.Nkz;
   for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < nk; i += (long long)gridDim.x * kThreads) {
     const int kz = (int)(i % g.Nkz);
@@ -189,8 +191,14 @@ static __global__ __launch_bounds__(kThreads) void k_weight_table(GridArgs g, do
     double a3, a2, av;
     fm_weights(sqrt(kx * kx + ky * ky + kz_ * kz_), g.R, g.six_dr2, a3, a2, av);
     const long long j = r * g.ldk + kz;  // padded rows, (w3, w2) interleaved
-    tab[2 * j] = a3 * g.invN;
-    tab[2 * j + 1] = a2 * g.invN;
+    if (tab) {
+      tab[2 * j] = a3 * g.invN;
+      tab[2 * j + 1] = a2 * g.invN;
+    }
+    if (tab32) {
+      tab32[2 * j] = (float)(a3 * g.invN);
+      tab32[2 * j + 1] = (float)(a2 * g.invN);
+    }
   }
 }
 
@@ -674,6 +682,23 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
   const int ncol = x_tile_column<C>(g, bx, myc, cy, ckz, cok);  // cok == (myc < ncol)
   (void)cok;
   const long long off = (long long)cy * g.ldk + ckz;
+  // FP32: the next field's columns are loaded into registers while the
+  // current field is transformed and accumulated (two fields' loads in
+  // flight; 10% faster at 128^3).  FP64 loads each field in turn: there the
+  // extra registers halve the occupancy and the pass ran 10% slower.
+  constexpr bool kPrefetch = sizeof(T) == 4;
+  cx<T> pre[PT];
+  auto fetch = [&](int f) {
+    const cx<T>* src = in + f * g.ks;
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int e = threadIdx.x + i * kThreads;  // e % C == myc
+      pre[i] = load_or_zero(src, (long long)(e / C) * plane + off, 0, e < C * L && myc < ncol);
+    }
+  };
+  // the first field's loads go out before the weight-table reads, so their
+  // latencies overlap
+  if constexpr (kPrefetch) fetch(0);
   T w3[PT], w2[PT], wk[PT][3], wt[PT][6];
   cx<T> acc[PT];
   {
@@ -691,21 +716,6 @@ __device__ __forceinline__ void xconv_bwd_body(int bx, const GridArgs& g, const 
       }
     }
   }
-  // FP32: the next field's columns are loaded into registers while the
-  // current field is transformed and accumulated (two fields' loads in
-  // flight; 10% faster at 128^3).  FP64 loads each field in turn: there the
-  // extra registers halve the occupancy and the pass ran 10% slower.
-  constexpr bool kPrefetch = sizeof(T) == 4;
-  cx<T> pre[PT];
-  auto fetch = [&](int f) {
-    const cx<T>* src = in + f * g.ks;
-#pragma unroll
-    for (int i = 0; i < PT; ++i) {
-      const int e = threadIdx.x + i * kThreads;  // e % C == myc
-      pre[i] = load_or_zero(src, (long long)(e / C) * plane + off, 0, e < C * L && myc < ncol);
-    }
-  };
-  if constexpr (kPrefetch) fetch(0);
 #pragma unroll
   for (int f = 0; f < NW; ++f) {
     if constexpr (kPrefetch) {

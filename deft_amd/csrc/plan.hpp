@@ -109,6 +109,7 @@ struct deft_plan_s {
   deft::cx<float> *twf[3] = {nullptr, nullptr, nullptr};
   deft::cx<double> *twd[3] = {nullptr, nullptr, nullptr};
   double *wtab = nullptr;  // FP64 weight table, (w3, w2) per element of the padded half spectrum (2 ks doubles)
+  float *wtab32 = nullptr;  // its FP32 rounding (2 ks floats)
   double *tmp = nullptr;   // scratch for host-seam calls (grows on demand)
   size_t tmp_bytes = 0;
 
@@ -194,6 +195,7 @@ inline void reciprocal(const double *c, double det, double *b) {
 inline GridArgs grid_args(const deft_plan_s *p) {
   GridArgs a{};
   a.wtab = p->wtab;
+  a.wtab32 = p->wtab32;
   a.stab = p->stab;
   a.Nx = p->Nx;
   a.Ny = p->Ny;

@@ -366,39 +366,4 @@ __device__ __forceinline__ void fm_weights(double k, double R, double six_dr2, d
   }
 }
 
-// FP32 evaluation of w3 and w2 for the FP32 FFT path.  The vector weight is
-// wv = s 4 pi (kR cos kR - sin kR)/k^3 = -w3 (also term by term in the series
-// of src/ReciprocalOperators.h:172-233), so only w3 and w2 are computed.  For
-// 1e-3 < kR < 1, (sin x - x cos x)/x^3 = sum_n (-1)^(n+1) 2n x^(2n-2)/(2n+1)!
-// replaces the closed form, which loses digits to cancellation in FP32.
-__device__ __forceinline__ void fm_weights_f32(float k, float R, float six_dr2, float& w3, float& w2) {
-  const float x = k * R;
-  const float fp = 12.566370614359172f;
-  const float R3 = R * R * R;
-  if (x <= 1e-3f) {
-    const float x2 = x * x;
-    w3 = (fp / 3.0f) * R3 * (x2 * x2 * x2 * (-1.0f / 15120) + x2 * x2 * (1.0f / 280) + x2 * (-1.0f / 10) + 1);
-    w2 = fp * (R * R) * (-x2 * x2 * x2 * (1.0f / 120 / 6 / 7) + x2 * x2 * (1.0f / 120) - x2 * (1.0f / 6) + 1);
-    return;
-  }
-  const float s = expf(-six_dr2 * k * k);
-  float sn, cs;
-  sincosf(x, &sn, &cs);
-  if (x < 1.0f) {
-    const float x2 = x * x;
-    const float poly =
-        ((((((x2 * (1.0f / 93405312000.0f) - 1.0f / 518918400.0f) * x2 + 1.0f / 3991680.0f) * x2 - 1.0f / 45360.0f) *
-               x2 +
-           1.0f / 840.0f) *
-              x2 -
-          1.0f / 30.0f) *
-             x2 +
-         1.0f / 3.0f);
-    w3 = s * fp * R3 * poly;
-  } else {
-    w3 = s * fp * (sn - x * cs) / (k * k * k);
-  }
-  w2 = s * fp * R * sn / k;
-}
-
 }  // namespace deft
