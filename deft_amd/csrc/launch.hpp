@@ -1107,21 +1107,14 @@ int Launch<T>::autotune(deft_plan_s *p) {
   const bool was_prof = p->prof;
   p->prof = false;
   float best = 1e30f;
-  const int nzw = (p->functional == DEFT_WHITE_BEAR && wave_fft_length(p->Nz)) ? 2 : 1;
-  int best_zw = 0;
-  for (int zw = 0; zw < nzw; ++zw) {
-    for (int zb : kZBlocks) {
-      if (zw == 1 && zb != 512) continue;  // the wave variant exists at 512-value blocks
-      p->zw[ti] = zw;
-      const float t = time_launches(p, [&] { launch_zphi<T>(p, g, true, zb); });
-      if (t < best) {
-        best = t;
-        p->tz[ti] = zb;
-        best_zw = zw;
-      }
+  p->zw[ti] = p->xw[ti] = 0;
+  for (int zb : kZBlocks) {
+    const float t = time_launches(p, [&] { launch_zphi<T>(p, g, true, zb); });
+    if (t < best) {
+      best = t;
+      p->tz[ti] = zb;
     }
   }
-  p->zw[ti] = best_zw;
   best = 1e30f;
   for (int tb : kTileBytes) {
     const float t = time_launches(p, [&] { launch_ypass<T>(p, g, spec_ptr<T>(p, 1), false, 5, tb); });
@@ -1141,12 +1134,32 @@ int Launch<T>::autotune(deft_plan_s *p) {
       p->tx[ti] = tb;
     }
   }
-  p->xw[ti] = 0;
-  if (plan_fields(p) == 5 && wave_fft_length(p->Nx)) {
-    const float t0 = time_launches(p, [&] { launch_xconv_fwd<T>(p, g, p->tx[ti]); });
-    p->xw[ti] = 1;
-    const float t1 = time_launches(p, [&] { launch_xconv_fwd<T>(p, g, p->tx[ti]); });
-    if (t1 >= t0) p->xw[ti] = 0;
+  // The one-per-wave Phi and forward x passes are chosen by timing whole
+  // energy+gradient evaluations: launched back to back on their own they time
+  // up to 10% faster than the block passes, inside an evaluation they did not
+  // (at 128^3 FP64 the x pass measured 35.5 against 35.0 us).
+  const bool zok = p->functional == DEFT_WHITE_BEAR && wave_fft_length(p->Nz);
+  const bool xok = plan_fields(p) == 5 && wave_fft_length(p->Nx);
+  if (zok || xok) {
+    const int tz0 = p->tz[ti];
+    int bz = 0, bx = 0;
+    best = 1e30f;
+    for (int zw = 0; zw <= (zok ? 1 : 0); ++zw) {
+      for (int xw = 0; xw <= (xok ? 1 : 0); ++xw) {
+        p->zw[ti] = zw;
+        p->xw[ti] = xw;
+        p->tz[ti] = zw ? 512 : tz0;  // the wave Phi pass exists at 512-value blocks
+        const float t = time_evaluation(p);
+        if (t < best) {
+          best = t;
+          bz = zw;
+          bx = xw;
+        }
+      }
+    }
+    p->zw[ti] = bz;
+    p->xw[ti] = bx;
+    p->tz[ti] = bz ? 512 : tz0;
   }
   p->prof = was_prof;
   DEFT_HIP_CHECK(hipGetLastError());
