@@ -202,15 +202,16 @@ int deft_plan_set_tuning(deft_plan plan, int fft_precision, int zblock, int ytil
 /* The White-Bear Phi pass variant for one FFT precision: 0 transforms a
  * block's lines with block-wide FFT stages, 1 gives each field to one wave
  * (fused plans of the White-Bear functional with a power-of-two z edge,
- * 16..512); both compute the same values.  Set by deft_plan_autotune.  No
- * counterpart in the reference. */
+ * 16..512; it runs at zblock 512, with zblock 1024 the block-wide pass runs);
+ * both return identical bits.  Set by deft_plan_autotune.  No counterpart in
+ * the reference. */
 int deft_plan_zphi_variant(deft_plan plan, int fft_precision, int *variant);
 int deft_plan_set_zphi_variant(deft_plan plan, int fft_precision, int variant);
 /* The forward x pass variant (the transform of the density and the five
  * weighted densities): 0 block-wide FFT stages, 1 one tile of columns per wave
  * (fused plans of the five-field functionals with a power-of-two x edge);
- * both compute the same values.  Set by deft_plan_autotune.  No counterpart
- * in the reference. */
+ * both compute the same values, to rounding.  Set by deft_plan_autotune.  No
+ * counterpart in the reference. */
 int deft_plan_xfwd_variant(deft_plan plan, int fft_precision, int *variant);
 int deft_plan_set_xfwd_variant(deft_plan plan, int fft_precision, int variant);
 /* Radix schedule of one axis (0 x, 1 y, 2 z) for one FFT precision, in
