@@ -328,6 +328,8 @@ def test_every_tuning_variant_matches_oracle(gpu, prec, etol, gtol):
                 assert _rel(gr, gro) < gtol, (zb, yt, xt)
     t = f.autotune()
     assert t[prec]["zblock"] in (512, 1024) and t[prec]["xtile"] in (8192, 16384)
+    assert t[prec]["zphi"] in (0, 1) and t[prec]["xfwd"] in (0, 1)
+    assert t[prec]["zphi"] == 0 or t[prec]["zblock"] == 512  # the wave Phi pass runs at 512
     E, gr, pg = f.energy_grad_and_precond()
     assert abs(E - Eo) <= etol * abs(Eo)
     assert np.array_equal(f.Veff, of.x)  # tuning never touches the state
